@@ -1,0 +1,89 @@
+"""ctypes binding of ``libbeamz_amd.so`` (declarations: include/beamz_amd.h).
+
+The library is built in-tree by ``beamz_amd.build``. Loading never falls back
+to anything else: a missing library raises ``RuntimeError`` naming the build
+command, and every failing entry point raises with the library's own message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbeamz_amd.so")
+
+i32, i64, f32, f64, vp = C.c_int, C.c_int64, C.c_float, C.c_double, C.c_void_p
+P6 = vp * 6
+
+
+class Shape(C.Structure):
+    _fields_ = [("kind", i32), ("perm", f64), ("permb", f64), ("cond", f64), ("bbox", f64 * 6),
+                ("px", f64), ("py", f64), ("w", f64), ("h", f64), ("cx", f64), ("cy", f64), ("r", f64),
+                ("r_in", f64), ("r_out", f64), ("nv", i32), ("v", C.POINTER(f64)), ("nholes", i32),
+                ("hole_nv", C.POINTER(i32)), ("hv", C.POINTER(f64))]
+
+
+class Pml(C.Structure):
+    _fields_ = [("kind", i32), ("orient", i32), ("px", f64), ("py", f64), ("w", f64), ("h", f64), ("r", f64),
+                ("order", f64), ("sigma_factor", f64), ("alpha_max", f64)]
+
+
+# name -> argtypes (restype is int unless listed in _RESTYPE)
+SIGNATURES = {
+    "bz_last_error": [],
+    "bz_version": [],
+    "bz_device_count": [C.POINTER(i32)],
+    "bz_update_h_2d": [vp, i32, vp, i32, vp, i32, vp, i32, i32, i32, f64, f64, f64, f64, f64, vp],
+    "bz_update_e_2d": [vp, i32, vp, i32, vp, i32, vp, i32, vp, i32, i32, i32, f64, f64, f64, f64, vp],
+    "bz_update_3d": [P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, vp],
+    "bz_add_points": [vp, vp, vp, i32, f32, vp],
+    "bz_affine_points": [vp, vp, vp, vp, i32, f32, vp],
+    "bz_gather": [P6, vp, vp, i32, vp, i32, vp, vp, vp],
+    "bz_grid_power_2d": [vp, vp, vp, vp, i32, i32, i32, vp],
+    "bz_grid_power_3d": [vp, P6, i32, i32, i32, i32, vp],
+    "bz_raster_2d": [C.POINTER(Shape), i32, C.POINTER(f64), C.POINTER(Pml), i32, f64, f64, f64, i32, i32,
+                     vp, vp, vp],
+    "bz_raster_3d": [C.POINTER(Shape), i32, C.POINTER(f64), C.POINTER(Pml), i32, f64, f64, f64, f64, f64,
+                     i32, i32, i32, vp, vp, vp],
+}
+_RESTYPE = {"bz_last_error": C.c_char_p}
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load (once) and return the library with argtypes set."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m beamz_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPE.get(name, i32)
+    _lib = L
+    return L
+
+
+def call(name, *args):
+    """Call an entry point; raise ValueError/RuntimeError on a negative status."""
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        msg = lib().bz_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(msg)
+        raise NativeError(msg or f"{name} failed with status {rc}")
+    return rc
+
+
+def ptrs6(seq):
+    arr = P6()
+    for q, p in enumerate(seq):
+        arr[q] = p
+    return arr

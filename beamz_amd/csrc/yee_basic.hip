@@ -1,0 +1,261 @@
+// One-cell-per-thread Yee kernels: the plugin-level 2D updates, the two-pass
+// 3D update, point sources, monitor gathers with FP64 DFT, grid power.
+//
+// These are the direct MI355X counterparts of the reference's array-op groups
+// (numpy_backend.py:34-88, fdtd.py:200-310, helper.py:5-256). The fused
+// engine kernels (yee3d_fused.hip, yee2d_tiled.hip) are checked against them.
+#include "common.h"
+
+namespace {
+
+constexpr int TX = 64, TY = 4;  // one wave per x-row segment, 4 rows per workgroup
+
+// numpy_backend.py:34-61. Hx[j][i] (i < nx-1), Hy[j][i] (j < ny-1). Each array has its own row pitch.
+struct Pitch2 {
+    int hx, hy, ez, sig, eps;
+};
+
+__global__ __launch_bounds__(TX* TY) void k_h2d(float* __restrict__ hx, float* __restrict__ hy,
+                                                 const float* __restrict__ ez, const float* __restrict__ sig,
+                                                 int nx, int ny, Pitch2 ld, Coef c, float rdx, float rdy) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    if (i >= nx || j >= ny) return;
+    const long oe = (long)j * ld.ez + i;
+    float fa, fb;
+    h_coef(sig[(long)j * ld.sig + i], c, fa, fb);
+    const float e0 = ez[oe];
+    if (i < nx - 1) {
+        const long o = (long)j * ld.hx + i;
+        hx[o] = fa * hx[o] - fb * ((ez[oe + 1] - e0) * rdy);   // x-difference over dy (reference)
+    }
+    if (j < ny - 1) {
+        const long o = (long)j * ld.hy + i;
+        hy[o] = fa * hy[o] + fb * ((ez[oe + ld.ez] - e0) * rdx);
+    }
+}
+
+// numpy_backend.py:63-88. Ez interior 1..ny-2 x 1..nx-2.
+__global__ __launch_bounds__(TX* TY) void k_e2d(float* __restrict__ ez, const float* __restrict__ hx,
+                                                 const float* __restrict__ hy, const float* __restrict__ sig,
+                                                 const float* __restrict__ eps, int nx, int ny, Pitch2 ld, Coef c,
+                                                 float rdx, float rdy) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    if (i < 1 || i >= nx - 1 || j < 1 || j >= ny - 1) return;
+    float ca, cb;
+    e_coef(sig[(long)j * ld.sig + i], eps[(long)j * ld.eps + i], c, ca, cb);
+    const long ox = (long)j * ld.hx + i, oy = (long)j * ld.hy + i, oe = (long)j * ld.ez + i;
+    const float curl = -((hx[ox] - hx[ox - 1]) * rdy) + (hy[oy] - hy[oy - ld.hy]) * rdx;
+    ez[oe] = ca * ez[oe] + cb * curl;
+}
+
+// fdtd.py:210-247. All H entries; components use sigma of the same index.
+__global__ __launch_bounds__(TX* TY) void k_h3d(float* __restrict__ Ex, float* __restrict__ Ey,
+                                                 float* __restrict__ Ez, float* __restrict__ Hx,
+                                                 float* __restrict__ Hy, float* __restrict__ Hz,
+                                                 const float* __restrict__ sig, int nx, int ny, int nz, int ld,
+                                                 Coef c, float rdx, float rdy, float rdz) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    const long ps = (long)ny * ld;
+    const long o = (long)k * ps + (long)j * ld + i;
+    float fa, fb;
+    h_coef(sig[o], c, fa, fb);
+    const bool kx = k < nz - 1, jy = j < ny - 1, ix = i < nx - 1;
+    if (kx && jy) {
+        const float cx = (Ez[o + ld] - Ez[o]) * rdy - (Ey[o + ps] - Ey[o]) * rdz;
+        Hx[o] = fa * Hx[o] - fb * cx;
+    }
+    if (kx && ix) {
+        const float cy = (Ex[o + ps] - Ex[o]) * rdz - (Ez[o + 1] - Ez[o]) * rdx;
+        Hy[o] = fa * Hy[o] - fb * cy;
+    }
+    if (jy && ix) {
+        const float cz = (Ey[o + 1] - Ey[o]) * rdx - (Ex[o + ld] - Ex[o]) * rdy;
+        Hz[o] = fa * Hz[o] - fb * cz;
+    }
+}
+
+// fdtd.py:249-310. Interior E slices; all three use eps/sigma of the same index.
+__global__ __launch_bounds__(TX* TY) void k_e3d(float* __restrict__ Ex, float* __restrict__ Ey,
+                                                 float* __restrict__ Ez, const float* __restrict__ Hx,
+                                                 const float* __restrict__ Hy, const float* __restrict__ Hz,
+                                                 const float* __restrict__ sig, const float* __restrict__ eps,
+                                                 int nx, int ny, int nz, int ld, Coef c, float rdx, float rdy,
+                                                 float rdz) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    const long ps = (long)ny * ld;
+    const long o = (long)k * ps + (long)j * ld + i;
+    const bool kin = k >= 1 && k <= nz - 2, jin = j >= 1 && j <= ny - 2, iin = i >= 1 && i <= nx - 2;
+    const bool ux = kin && jin && i <= nx - 2;
+    const bool uy = kin && j <= ny - 2 && iin;
+    const bool uz = k <= nz - 2 && jin && iin;
+    if (!(ux || uy || uz)) return;
+    float ca, cb;
+    e_coef(sig[o], eps[o], c, ca, cb);
+    if (ux) Ex[o] = ca * Ex[o] + cb * ((Hz[o] - Hz[o - ld]) * rdy - (Hy[o] - Hy[o - ps]) * rdz);
+    if (uy) Ey[o] = ca * Ey[o] + cb * ((Hx[o] - Hx[o - ps]) * rdz - (Hz[o] - Hz[o - 1]) * rdx);
+    if (uz) Ez[o] = ca * Ez[o] + cb * ((Hy[o] - Hy[o - 1]) * rdx - (Hx[o] - Hx[o - ld]) * rdy);
+}
+
+__global__ void k_add_points(float* __restrict__ f, const int64_t* __restrict__ off, const float* __restrict__ w,
+                             int n, float amp) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) f[off[p]] += w[p] * amp;
+}
+
+__global__ void k_affine_points(float* __restrict__ f, const int64_t* __restrict__ off, const float* __restrict__ mul,
+                                const float* __restrict__ w, int n, float amp) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) f[off[p]] = mul[p] * f[off[p]] + w[p] * amp;
+}
+
+struct CompBase {
+    const float* p[6];
+};
+
+__global__ void k_gather(CompBase base, const int* __restrict__ comp, const int64_t* __restrict__ off, int n,
+                         float* __restrict__ series, int nfreq, const double* __restrict__ phasor,
+                         double* __restrict__ dft) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float v = base.p[comp[p]][off[p]];
+    if (series) series[p] = v;
+    const double dv = (double)v;
+    for (int k = 0; k < nfreq; ++k) {
+        const double re = phasor[2 * k], im = phasor[2 * k + 1];
+        double* a = dft + 2 * ((long)k * n + p);
+        a[0] += dv * re;
+        a[1] += dv * im;
+    }
+}
+
+// helper.py:152-187 (real fields)
+__global__ __launch_bounds__(TX* TY) void k_pow2d(double* __restrict__ acc, const float* __restrict__ ez,
+                                                   const float* __restrict__ hx, const float* __restrict__ hy,
+                                                   int nx, int ny, int ld) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    if (i >= nx || j >= ny) return;
+    const long o = (long)j * ld + i;
+    const double e = ez[o];
+    const double hxv = (i < nx - 1) ? (double)hx[o] : 0.0;
+    const double hyv = (j < ny - 1) ? (double)hy[o] : 0.0;
+    const double sx = -e * hyv, sy = e * hxv;
+    acc[o] += sx * sx + sy * sy;
+}
+
+// helper.py:124-151 (real fields) on the corner-aligned (nz-1, ny-1, nx-1) box
+__global__ __launch_bounds__(TX* TY) void k_pow3d(double* __restrict__ acc, const float* __restrict__ Ex,
+                                                   const float* __restrict__ Ey, const float* __restrict__ Ez,
+                                                   const float* __restrict__ Hx, const float* __restrict__ Hy,
+                                                   const float* __restrict__ Hz, int nx, int ny, int nz, int ld) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx - 1 || j >= ny - 1 || k >= nz - 1) return;
+    const long o = (long)k * ny * ld + (long)j * ld + i;
+    const double ex = Ex[o], ey = Ey[o], ez = Ez[o], hx = Hx[o], hy = Hy[o], hz = Hz[o];
+    const double sx = ey * hz - ez * hy, sy = ez * hx - ex * hz, sz = ex * hy - ey * hx;
+    acc[o] += sqrt(sx * sx + sy * sy + sz * sz);
+}
+
+}  // namespace
+
+extern "C" int bz_update_h_2d(float* Hx, int ld_hx, float* Hy, int ld_hy, const float* Ez, int ld_ez,
+                              const float* sigma, int ld_sig, int nx, int ny, double dx, double dy, double dt,
+                              double mu0, double eps0, void* stream) {
+    BZ_REQUIRE(Hx && Hy && Ez && sigma, "bz_update_h_2d: null pointer");
+    BZ_REQUIRE(nx >= 2 && ny >= 2 && ld_hx >= nx - 1 && ld_hy >= nx && ld_ez >= nx && ld_sig >= nx,
+               "bz_update_h_2d: bad shape");
+    const dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY), b(TX, TY);
+    const Pitch2 ld{ld_hx, ld_hy, ld_ez, ld_sig, 0};
+    hipLaunchKernelGGL(k_h2d, g, b, 0, (hipStream_t)stream, Hx, Hy, Ez, sigma, nx, ny, ld, make_coef(dt, mu0, eps0),
+                       (float)(1.0 / dx), (float)(1.0 / dy));
+    BZ_CHECK_LAUNCH("bz_update_h_2d");
+    return BZ_OK;
+}
+
+extern "C" int bz_update_e_2d(float* Ez, int ld_ez, const float* Hx, int ld_hx, const float* Hy, int ld_hy,
+                              const float* sigma, int ld_sig, const float* eps_r, int ld_eps, int nx, int ny,
+                              double dx, double dy, double dt, double eps0, void* stream) {
+    BZ_REQUIRE(Ez && Hx && Hy && sigma && eps_r, "bz_update_e_2d: null pointer");
+    BZ_REQUIRE(nx >= 2 && ny >= 2 && ld_hx >= nx - 1 && ld_hy >= nx && ld_ez >= nx && ld_sig >= nx && ld_eps >= nx,
+               "bz_update_e_2d: bad shape");
+    const dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY), b(TX, TY);
+    const Pitch2 ld{ld_hx, ld_hy, ld_ez, ld_sig, ld_eps};
+    hipLaunchKernelGGL(k_e2d, g, b, 0, (hipStream_t)stream, Ez, Hx, Hy, sigma, eps_r, nx, ny, ld,
+                       make_coef(dt, 1.0, eps0), (float)(1.0 / dx), (float)(1.0 / dy));
+    BZ_CHECK_LAUNCH("bz_update_e_2d");
+    return BZ_OK;
+}
+
+extern "C" int bz_update_3d(float* const f[6], const float* eps_r, const float* sigma, int nx, int ny, int nz, int ld,
+                            double dx, double dy, double dz, double dt, double mu0, double eps0, void* stream) {
+    BZ_REQUIRE(f && eps_r && sigma, "bz_update_3d: null pointer");
+    for (int q = 0; q < 6; ++q) BZ_REQUIRE(f[q], "bz_update_3d: null field pointer");
+    BZ_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2 && ld >= nx, "bz_update_3d: bad shape");
+    const Coef c = make_coef(dt, mu0, eps0);
+    const dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY, nz), b(TX, TY);
+    hipLaunchKernelGGL(k_h3d, g, b, 0, (hipStream_t)stream, f[0], f[1], f[2], f[3], f[4], f[5], sigma, nx, ny, nz, ld,
+                       c, (float)(1.0 / dx), (float)(1.0 / dy), (float)(1.0 / dz));
+    BZ_CHECK_LAUNCH("bz_update_3d(H)");
+    hipLaunchKernelGGL(k_e3d, g, b, 0, (hipStream_t)stream, f[0], f[1], f[2], f[3], f[4], f[5], sigma, eps_r, nx, ny,
+                       nz, ld, c, (float)(1.0 / dx), (float)(1.0 / dy), (float)(1.0 / dz));
+    BZ_CHECK_LAUNCH("bz_update_3d(E)");
+    return BZ_OK;
+}
+
+extern "C" int bz_add_points(float* field, const int64_t* off, const float* w, int n, float amp, void* stream) {
+    if (n == 0) return BZ_OK;
+    BZ_REQUIRE(field && off && w && n > 0, "bz_add_points: bad arguments");
+    hipLaunchKernelGGL(k_add_points, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, field, off, w, n, amp);
+    BZ_CHECK_LAUNCH("bz_add_points");
+    return BZ_OK;
+}
+
+extern "C" int bz_affine_points(float* field, const int64_t* off, const float* mul, const float* w, int n, float amp,
+                                void* stream) {
+    if (n == 0) return BZ_OK;
+    BZ_REQUIRE(field && off && mul && w && n > 0, "bz_affine_points: bad arguments");
+    hipLaunchKernelGGL(k_affine_points, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, field, off, mul, w,
+                       n, amp);
+    BZ_CHECK_LAUNCH("bz_affine_points");
+    return BZ_OK;
+}
+
+extern "C" int bz_gather(const float* const comp_base[6], const int* comp, const int64_t* off, int n, float* series,
+                         int nfreq, const double* phasor, double* dft, void* stream) {
+    if (n == 0) return BZ_OK;
+    BZ_REQUIRE(comp_base && comp && off && n > 0, "bz_gather: bad arguments");
+    BZ_REQUIRE(nfreq == 0 || (phasor && dft), "bz_gather: DFT requested without buffers");
+    CompBase cb;
+    for (int q = 0; q < 6; ++q) cb.p[q] = comp_base[q];
+    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, cb, comp, off, n, series,
+                       nfreq, phasor, dft);
+    BZ_CHECK_LAUNCH("bz_gather");
+    return BZ_OK;
+}
+
+extern "C" int bz_grid_power_2d(double* acc, const float* Ez, const float* Hx, const float* Hy, int nx, int ny, int ld,
+                                void* stream) {
+    BZ_REQUIRE(acc && Ez && Hx && Hy && nx >= 2 && ny >= 2 && ld >= nx, "bz_grid_power_2d: bad arguments");
+    const dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY), b(TX, TY);
+    hipLaunchKernelGGL(k_pow2d, g, b, 0, (hipStream_t)stream, acc, Ez, Hx, Hy, nx, ny, ld);
+    BZ_CHECK_LAUNCH("bz_grid_power_2d");
+    return BZ_OK;
+}
+
+extern "C" int bz_grid_power_3d(double* acc, float* const f[6], int nx, int ny, int nz, int ld, void* stream) {
+    BZ_REQUIRE(acc && f && nx >= 2 && ny >= 2 && nz >= 2 && ld >= nx, "bz_grid_power_3d: bad arguments");
+    const dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY, nz), b(TX, TY);
+    hipLaunchKernelGGL(k_pow3d, g, b, 0, (hipStream_t)stream, acc, f[0], f[1], f[2], f[3], f[4], f[5], nx, ny, nz, ld);
+    BZ_CHECK_LAUNCH("bz_grid_power_3d");
+    return BZ_OK;
+}

@@ -1,0 +1,189 @@
+"""Field monitors (beamz/devices/monitors.py:7-453).
+
+The reference samples fields on the host every step. Here the engine gathers
+the monitor's samples on the GPU every step into a device time-series buffer
+and, if frequencies are requested, into FP64 DFT accumulators; ``fields``,
+``power_history``, ``power_accumulated`` and ``power_timestamps`` are filled
+from the device buffers when read (same layout as the reference: one list per
+recorded step). Field values are the real fields (the reference's complex128
+fields have zero imaginary part for real source signals).
+
+New (no reference counterpart): ``dft_frequencies`` (Hz). After a run,
+``monitor.dft[name]`` holds sum_n f_n exp(-i w t_n) for each recorded
+component, shape (n_freq, n_points) in 2D or (n_freq, *plane_shape) in 3D.
+
+Plane monitors: z-normal planes follow the reference's slicing
+(monitors.py:222-273, 324-337). x- and y-normal planes are sampled as true
+planes here (the reference copies the whole volume for them); they are the
+flux planes for 3D ports.
+"""
+from __future__ import annotations
+
+from typing import Callable, Optional
+
+import numpy as np
+
+
+class Monitor:
+    def __init__(self, design=None, start=(0, 0), end=None, plane_normal=None, plane_position=0, size=None,
+                 record_fields=True, accumulate_power=True, live_update=False, record_interval=1,
+                 max_history_steps=None, objective_function: Optional[Callable[["Monitor"], float]] = None,
+                 name: Optional[str] = None, dft_frequencies=None):
+        self.design = design
+        self.should_record_fields = record_fields
+        self.accumulate_power = accumulate_power
+        self.live_update = live_update
+        self.record_interval = record_interval
+        self.max_history_steps = max_history_steps
+        self.objective_function = objective_function
+        self.objective_value = None
+        self.name = name
+        self.dft_frequencies = None if dft_frequencies is None else np.atleast_1d(np.asarray(dft_frequencies, float))
+        self.dft = {}
+        self.is_3d = self._determine_3d_mode(start, end, design)
+        names = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz") if self.is_3d else ("Ez", "Hx", "Hy")
+        self._fields = {k: [] for k in names}
+        self._fields["t"] = []
+        self._power_accumulated = None
+        self._power_history = []
+        self._power_timestamps = []
+        self.energy_history = []
+        self.power_accumulation_count = 0
+        self.step_count = 0
+        self.last_record_step = -1
+        self._pull = None          # set by the engine: callable that materialises device buffers
+        if self.is_3d:
+            self._init_3d_monitor(start, end, plane_normal, plane_position, size)
+        else:
+            self._init_2d_monitor(start, end)
+
+    # ---- host views filled from the device (lazy) ----
+    def _materialize(self):
+        if self._pull is not None:
+            pull, self._pull = self._pull, None
+            pull(self)
+
+    @property
+    def fields(self):
+        self._materialize()
+        return self._fields
+
+    @property
+    def power_history(self):
+        self._materialize()
+        return self._power_history
+
+    @property
+    def power_timestamps(self):
+        self._materialize()
+        return self._power_timestamps
+
+    @property
+    def power_accumulated(self):
+        self._materialize()
+        return self._power_accumulated
+
+    # ---- geometry (monitors.py:76-220) ----
+    def _determine_3d_mode(self, start, end, design):
+        if end is not None and len(end) == 3:
+            return True
+        if len(start) == 3:
+            return True
+        if end is not None and len(start) == 2 and len(end) == 2:
+            return False
+        return bool(design and getattr(design, "is_3d", False))
+
+    def _init_2d_monitor(self, start, end):
+        if end is None:
+            end = start
+        self.start, self.end = start, end
+        self.position = ((start[0] + end[0]) / 2, (start[1] + end[1]) / 2)
+        self.monitor_type = "line"
+
+    def _init_3d_monitor(self, start, end, plane_normal, plane_position, size):
+        if len(start) == 2:
+            start = (start[0], start[1], 0.0)
+        self.start = start
+        if end is not None:
+            if len(end) == 2:
+                end = (end[0], end[1], start[2])
+            self.end = end
+            self.size = (abs(end[0] - start[0]), abs(end[1] - start[1]))
+            self.plane_position = start[2]
+            self.plane_normal = "z"
+            self.start = (min(start[0], end[0]), min(start[1], end[1]), start[2])
+        else:
+            self.end = None
+            self.plane_normal = plane_normal or "z"
+            self.plane_position = plane_position
+            if size is None:
+                d = self.design
+                if d:
+                    size = {"z": (d.width, d.height), "y": (d.width, d.depth or d.width)}.get(
+                        self.plane_normal, (d.height, d.depth or d.height))
+                else:
+                    size = (1e-6, 1e-6)
+            self.size = size
+        self.monitor_type = "plane"
+        s, z = self.start, self.plane_position
+        if self.plane_normal == "z" or self.end is not None:
+            self.position = (s[0] + self.size[0] / 2, s[1] + self.size[1] / 2, z)
+        elif self.plane_normal == "y":
+            self.position = (s[0] + self.size[0] / 2, z, s[2] + self.size[1] / 2)
+        else:
+            self.position = (z, s[1] + self.size[0] / 2, s[2] + self.size[1] / 2)
+
+    def get_grid_points_2d(self, dx, dy):
+        """monitors.py:204-220: integer points from numpy.linspace(..., dtype=int)."""
+        sx, sy = int(round(self.start[0] / dx)), int(round(self.start[1] / dy))
+        ex, ey = int(round(self.end[0] / dx)), int(round(self.end[1] / dy))
+        n = (abs(ex - sx) if abs(ex - sx) > abs(ey - sy) else abs(ey - sy)) + 1
+        return list(zip(np.linspace(sx, ex, n, dtype=int), np.linspace(sy, ey, n, dtype=int)))
+
+    def get_grid_slice_3d(self, dx, dy, dz, field_shape=None):
+        """monitors.py:222-273."""
+        d = self.design
+        bnx = max(1, int(round(getattr(d, "width", 0.0) / dx)))
+        bny = max(1, int(round(getattr(d, "height", 0.0) / dy)))
+        bnz = max(1, int(round((getattr(d, "depth", 0.0) or 0.0) / dz)))
+        clamp = lambda v, hi: max(0, min(v, hi))
+        st, sz = self.start, self.size
+        if self.plane_normal == "z":
+            zi = clamp(int(round(self.plane_position / dz)), bnz - 1)
+            return (slice(clamp(int(round(st[1] / dy)), bny - 1), clamp(int(round((st[1] + sz[1]) / dy)), bny)),
+                    slice(clamp(int(round(st[0] / dx)), bnx - 1), clamp(int(round((st[0] + sz[0]) / dx)), bnx)), zi)
+        if self.plane_normal == "y":
+            yi = clamp(int(round(self.plane_position / dy)), bny - 1)
+            return (yi, slice(clamp(int(round(st[0] / dx)), bnx - 1), clamp(int(round((st[0] + sz[0]) / dx)), bnx)),
+                    slice(clamp(int(round(st[2] / dz)), bnz - 1), clamp(int(round((st[2] + sz[1]) / dz)), bnz)))
+        xi = clamp(int(round(self.plane_position / dx)), bnx - 1)
+        return (slice(clamp(int(round(st[1] / dy)), bny - 1), clamp(int(round((st[1] + sz[0]) / dy)), bny)), xi,
+                slice(clamp(int(round(st[2] / dz)), bnz - 1), clamp(int(round((st[2] + sz[1]) / dz)), bnz)))
+
+    def should_record(self, step):
+        return (step - self.last_record_step) >= self.record_interval
+
+    def evaluate_objective(self) -> Optional[float]:
+        """monitors.py:58-74."""
+        if self.objective_function is None:
+            return None
+        try:
+            v = self.objective_function(self)
+        except Exception as exc:  # the reference reports and continues
+            print(f"Warning: monitor objective evaluation failed: {exc}")
+            return None
+        if v is None:
+            return None
+        try:
+            self.objective_value = float(v)
+        except (TypeError, ValueError):
+            print(f"Warning: monitor objective returned non-numeric value: {v}")
+            return None
+        return self.objective_value
+
+    def copy(self):
+        m = Monitor.__new__(Monitor)
+        m.__dict__.update({k: v for k, v in self.__dict__.items()})
+        m._fields = {k: [] for k in self._fields}
+        m._power_history, m._power_timestamps, m._power_accumulated, m._pull, m.dft = [], [], None, None, {}
+        return m

@@ -1,0 +1,229 @@
+"""Device state and per-step work for one FDTD simulation.
+
+Fields live on the GPU as FP32 arrays on a padded box ([ny][ld] or
+[nz][ny][ld], ld a multiple of 64 floats so every row starts on a 256-byte
+boundary). Each component occupies the full box; entries outside the
+reference's staggered shape are never written and stay zero, which also
+provides the zero sentinel that out-of-range monitor points read
+(monitors.py:287-300 returns 0.0 for them).
+
+One step follows the reference order (beamz/simulation/fdtd.py:411-429):
+H update, E update, sources, monitor sampling (+ FP64 DFT), grid power.
+``Kernels`` selects the update kernels: the one-cell-per-thread pair
+(bz_update_*), or the fused engine kernels when available.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .. import _native as N
+from ..const import EPS_0, MU_0
+
+COMP3 = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+COMP2 = ("Ez", "Hx", "Hy")
+
+
+def align(n, a=64):
+    return (n + a - 1) // a * a
+
+
+def ref_shapes(nx, ny, nz=None):
+    """Reference field shapes (fdtd.py:151-186)."""
+    if nz is None:
+        return {"Ez": (ny, nx), "Hx": (ny, nx - 1), "Hy": (ny - 1, nx)}
+    return {"Ex": (nz, ny, nx - 1), "Ey": (nz, ny - 1, nx), "Ez": (nz - 1, ny, nx),
+            "Hx": (nz - 1, ny - 1, nx), "Hy": (nz - 1, ny, nx - 1), "Hz": (nz, ny - 1, nx - 1)}
+
+
+class _PointSource:
+    def __init__(self, comp, off, w, mul, signal):
+        self.comp, self.off, self.w, self.mul, self.signal = comp, off, w, mul, signal
+        self.n = int(off.numel())
+
+
+class _MonitorPlan:
+    pass
+
+
+class Grid:
+    """Device arrays of one simulation plus the work list of one step."""
+
+    def __init__(self, backend, eps, sigma, dx, dy, dt, dz=None):
+        self.backend = backend
+        torch = backend.torch
+        self.torch = torch
+        self.dev = backend.dev
+        self.is_3d = eps.ndim == 3
+        self.dx, self.dy, self.dz, self.dt = float(dx), float(dy), (None if dz is None else float(dz)), float(dt)
+        if self.is_3d:
+            self.nz, self.ny, self.nx = eps.shape
+            if min(self.nx, self.ny, self.nz) < 2:
+                raise ValueError("3D grids need at least 2 cells along every axis")
+        else:
+            self.ny, self.nx = eps.shape
+            self.nz = None
+            if min(self.nx, self.ny) < 2:
+                raise ValueError("2D grids need at least 2 cells along every axis")
+        self.ld = align(self.nx)
+        box = (self.nz, self.ny, self.ld) if self.is_3d else (self.ny, self.ld)
+        self.box = box
+        self.names = COMP3 if self.is_3d else COMP2
+        self.f = {k: torch.zeros(box, dtype=torch.float32, device=self.dev) for k in self.names}
+        self.eps = torch.ones(box, dtype=torch.float32, device=self.dev)
+        self.sig = torch.zeros(box, dtype=torch.float32, device=self.dev)
+        self.set_materials(eps, sigma)
+        self.shapes = ref_shapes(self.nx, self.ny, self.nz)
+        self.sources = []
+        self.monitors = []
+        self.power = None
+        self.power_count = 0
+        self.fused = None
+        self.kernel = "basic"
+
+    def kernel_name(self):
+        return "bz_update_3d (two-pass)" if self.is_3d else "bz_update_h_2d + bz_update_e_2d"
+
+    def launch_timer(self):
+        """Mean device time of one update launch over the last timed run, seconds (None if not measured)."""
+        return None
+
+    # ---------------- materials / views ----------------
+    def set_materials(self, eps, sigma):
+        e = np.asarray(eps, dtype=np.float32)
+        s = np.asarray(sigma, dtype=np.float32)
+        if self.is_3d:
+            self.eps[:, :, : self.nx] = self.torch.from_numpy(np.ascontiguousarray(e)).to(self.dev)
+            self.sig[:, :, : self.nx] = self.torch.from_numpy(np.ascontiguousarray(s)).to(self.dev)
+        else:
+            self.eps[:, : self.nx] = self.torch.from_numpy(np.ascontiguousarray(e)).to(self.dev)
+            self.sig[:, : self.nx] = self.torch.from_numpy(np.ascontiguousarray(s)).to(self.dev)
+
+    def view(self, name):
+        """Device view of a component in the reference's shape."""
+        s = self.shapes[name]
+        t = self.f[name]
+        return t[: s[0], : s[1], : s[2]] if self.is_3d else t[: s[0], : s[1]]
+
+    def offset(self, k, j, i):
+        if self.is_3d:
+            return (k * self.ny + j) * self.ld + i
+        return j * self.ld + i
+
+    @property
+    def stream(self):
+        return self.backend.stream
+
+    def reset_fields(self):
+        for t in self.f.values():
+            t.zero_()
+
+    # ---------------- sources ----------------
+    def add_point_source(self, comp, idx, w, mul, signal):
+        """idx: (m, ndim) integer cell indices; w, mul: (m,) FP64 host arrays."""
+        T = self.torch
+        off = np.array([self.offset(*(r if self.is_3d else (0, *r))) for r in idx], dtype=np.int64)
+        if len(np.unique(off)) != len(off):
+            raise ValueError("point source offsets must be unique")
+        src = _PointSource(comp, T.from_numpy(off).to(self.dev), T.from_numpy(np.asarray(w, np.float32)).to(self.dev),
+                           None if mul is None else T.from_numpy(np.asarray(mul, np.float32)).to(self.dev),
+                           np.asarray(signal))
+        self.sources.append(src)
+        return src
+
+    # ---------------- monitors ----------------
+    def add_gather(self, comps, offs, record_steps, t_all, dft_omegas, keep_series=True):
+        """Plan a monitor gather: comps/offs per sample, recorded at ``record_steps``."""
+        T = self.torch
+        p = _MonitorPlan()
+        p.n = len(offs)
+        p.comp = T.from_numpy(np.asarray(comps, dtype=np.int32)).to(self.dev)
+        p.off = T.from_numpy(np.asarray(offs, dtype=np.int64)).to(self.dev)
+        p.record_steps = np.asarray(record_steps, dtype=np.int64)
+        p.slot = {int(s): q for q, s in enumerate(p.record_steps)}
+        nrec = len(p.record_steps)
+        p.series = T.zeros((max(nrec, 1), p.n), dtype=T.float32, device=self.dev) if keep_series else None
+        p.nfreq = 0 if dft_omegas is None else len(dft_omegas)
+        if p.nfreq:
+            ts = np.asarray([t_all[s] for s in p.record_steps], dtype=np.float64)
+            ph = np.exp(-1j * np.outer(ts, np.asarray(dft_omegas, dtype=np.float64)))   # (nrec, nfreq)
+            p.phasor = T.from_numpy(np.ascontiguousarray(ph).view(np.float64)).to(self.dev)
+            p.dft = T.zeros((p.nfreq, p.n, 2), dtype=T.float64, device=self.dev)
+        else:
+            p.phasor = p.dft = None
+        self.monitors.append(p)
+        return p
+
+    def enable_power(self):
+        self.power = self.torch.zeros(self.box if not self.is_3d else self.box, dtype=self.torch.float64,
+                                      device=self.dev)
+        self.power_count = 0
+
+    # ---------------- one step ----------------
+    def _update(self):
+        st = self.stream
+        if self.is_3d:
+            ptr = N.ptrs6([self.f[k].data_ptr() for k in COMP3])
+            N.call("bz_update_3d", ptr, self.eps.data_ptr(), self.sig.data_ptr(), self.nx, self.ny, self.nz, self.ld,
+                   self.dx, self.dy, self.dz, self.dt, MU_0, EPS_0, st)
+        else:
+            ld = self.ld
+            f = self.f
+            N.call("bz_update_h_2d", f["Hx"].data_ptr(), ld, f["Hy"].data_ptr(), ld, f["Ez"].data_ptr(), ld,
+                   self.sig.data_ptr(), ld, self.nx, self.ny, self.dx, self.dy, self.dt, MU_0, EPS_0, st)
+            N.call("bz_update_e_2d", f["Ez"].data_ptr(), ld, f["Hx"].data_ptr(), ld, f["Hy"].data_ptr(), ld,
+                   self.sig.data_ptr(), ld, self.eps.data_ptr(), ld, self.nx, self.ny, self.dx, self.dy, self.dt,
+                   EPS_0, st)
+
+    def _sources(self, n):
+        st = self.stream
+        for s in self.sources:
+            amp = float(np.real(s.signal[n]))
+            f = self.f[s.comp]
+            if s.mul is None:
+                N.call("bz_add_points", f.data_ptr(), s.off.data_ptr(), s.w.data_ptr(), s.n, amp, st)
+            else:
+                N.call("bz_affine_points", f.data_ptr(), s.off.data_ptr(), s.mul.data_ptr(), s.w.data_ptr(), s.n,
+                       amp, st)
+
+    def _gathers(self, n):
+        st = self.stream
+        base = N.ptrs6([self.f[k].data_ptr() for k in self.names] + [0] * (6 - len(self.names)))
+        for p in self.monitors:
+            q = p.slot.get(n)
+            if q is None or p.n == 0:
+                continue
+            ser = 0 if p.series is None else p.series[q].data_ptr()
+            if p.nfreq:
+                N.call("bz_gather", base, p.comp.data_ptr(), p.off.data_ptr(), p.n, ser, p.nfreq,
+                       p.phasor[q].data_ptr(), p.dft.data_ptr(), st)
+            else:
+                N.call("bz_gather", base, p.comp.data_ptr(), p.off.data_ptr(), p.n, ser, 0, None, None, st)
+
+    def _power(self):
+        if self.power is None:
+            return
+        st = self.stream
+        if self.is_3d:
+            ptr = N.ptrs6([self.f[k].data_ptr() for k in COMP3])
+            N.call("bz_grid_power_3d", self.power.data_ptr(), ptr, self.nx, self.ny, self.nz, self.ld, st)
+        else:
+            f = self.f
+            N.call("bz_grid_power_2d", self.power.data_ptr(), f["Ez"].data_ptr(), f["Hx"].data_ptr(),
+                   f["Hy"].data_ptr(), self.nx, self.ny, self.ld, st)
+        self.power_count += 1
+
+    def step(self, n):
+        """Advance one step (step index n) in the reference order."""
+        self._update()
+        self._sources(n)
+        self._gathers(n)
+        self._power()
+
+    def run(self, n0, count):
+        if self.fused is not None:
+            done = self.fused.run(n0, count)
+            for n in range(n0 + done, n0 + count):
+                self.step(n)
+            return
+        for n in range(n0, n0 + count):
+            self.step(n)

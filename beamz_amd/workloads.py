@@ -1,0 +1,109 @@
+"""Builders for the benchmark and parity workloads, expressed with the public API.
+
+* ``waveguide_2d``: 2D TM straight Si waveguide in SiO2 (dx = lambda/20 at 1.55 um),
+  20-cell cubic-graded PML (R = 1e-6) on all sides, Courant 0.99/sqrt(2),
+  Gaussian-pulse point source, DFT line monitor. Default size 800 x 400 cells.
+* ``soi_mmi_3d``: 3D six-component SOI 1x2 MMI (Si 220 nm on SiO2, air above),
+  25 nm cells, seeded +-20% jitter of guide widths and body size, 12-cell PML
+  (x/y from the reference profile, z as the documented extension), dipole
+  Gaussian-pulse source, two x-normal DFT flux ports. Default 480 x 240 x 160.
+
+Gaussian pulse convention: carrier f0 = c/lambda0; spectral sigma
+df = c*dlambda/lambda0^2; time sigma 1/(2 pi df); centre 4 time sigmas.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .const import LIGHT_SPEED, µm
+from .design import Design, Material, PML, Rectangle
+from .devices import GaussianSource, Monitor
+from .devices.signals import gaussian_pulse
+from .simulation.layered import layered_grid_3d, pml_sigma_factor
+
+EPS_SI = 3.476 ** 2
+EPS_OX = 1.444 ** 2
+
+
+def pulse_signal(t, lam0=1.55 * µm, dlam=0.2 * µm):
+    f0 = LIGHT_SPEED / lam0
+    df = LIGHT_SPEED * dlam / lam0 ** 2
+    st = 1.0 / (2 * np.pi * df)
+    return gaussian_pulse(t, 1.0, 4 * st, st, f0, 0.0)
+
+
+def edge_pmls(W, H, cells, dx, order=3.0, reflection=1e-6):
+    s = pml_sigma_factor(cells, reflection, order)
+    L = cells * dx
+    return [PML("rect", (0, 0), (L, H), "left", polynomial_order=order, sigma_factor=s),
+            PML("rect", (W - L, 0), (L, H), "right", polynomial_order=order, sigma_factor=s),
+            PML("rect", (0, H - L), (W, L), "top", polynomial_order=order, sigma_factor=s),
+            PML("rect", (0, 0), (W, L), "bottom", polynomial_order=order, sigma_factor=s)]
+
+
+def waveguide_2d(nx=800, ny=400, steps=4000, npml=20, lam0=1.55 * µm, src_x=50, mon_x=700, n_freq=16):
+    """Returns (design, dx, time array, monitor). Sizes in cells."""
+    dx = lam0 / 20
+    W, H = nx * dx, ny * dx
+    d = Design(W, H, material=Material(EPS_OX), auto_pml=False)
+    d.boundaries += edge_pmls(W, H, npml, dx)
+    d += Rectangle(position=(0, H / 2 - 0.25 * µm), width=W, height=0.5 * µm, material=Material(EPS_SI))
+    dt = 0.99 * dx / (LIGHT_SPEED * np.sqrt(2))
+    t = np.arange(steps) * dt
+    d += GaussianSource(position=(src_x * dx, H / 2), width=dx / 4, signal=pulse_signal(t, lam0))
+    freqs = LIGHT_SPEED / (np.linspace(1.45, 1.65, n_freq) * µm)
+    mon = Monitor(design=d, start=(min(mon_x, nx - 1) * dx, npml * dx), end=(min(mon_x, nx - 1) * dx, H - npml * dx),
+                  dft_frequencies=freqs, name="line")
+    d += mon
+    return d, dx, t, mon
+
+
+def mmi_rectangles(W, H, rng, jitter=0.2):
+    """Input guide, 2.5 x 6 um body and two output guides, each size scaled by 1 + U(-jitter, jitter)."""
+    j = lambda: 1.0 + rng.uniform(-jitter, jitter)
+    gw_in, gw_o1, gw_o2 = 0.5 * µm * j(), 0.5 * µm * j(), 0.5 * µm * j()
+    body_w, body_l = 2.5 * µm * j(), 6.0 * µm * j()
+    x0 = (W - body_l) / 2
+    yc = H / 2
+    sep = body_w / 4 * j()
+    rects = [(0.0, yc - gw_in / 2, x0, gw_in),
+             (x0, yc - body_w / 2, body_l, body_w),
+             (x0 + body_l, yc + sep - gw_o1 / 2, W - (x0 + body_l), gw_o1),
+             (x0 + body_l, yc - sep - gw_o2 / 2, W - (x0 + body_l), gw_o2)]
+    return rects, (yc + sep, yc - sep), x0 + body_l
+
+
+def soi_mmi_3d(nx=480, ny=240, nz=160, steps=3000, npml=12, dx=25e-9, seed=0, port_freqs=8, z_pml=True):
+    """Returns (mesh, design3d, time array, monitors)."""
+    rng = np.random.default_rng(seed)
+    W, H, D = nx * dx, ny * dx, nz * dx
+    z_si0 = D / 2 - 0.11 * µm
+    z_si1 = z_si0 + 0.22 * µm
+    rects, port_y, body_end = mmi_rectangles(W, H, rng)
+
+    def layer(bg, with_mmi):
+        d2 = Design(W, H, material=Material(bg), auto_pml=False)
+        d2.boundaries += edge_pmls(W, H, npml, dx)
+        if with_mmi:
+            for (x, y, w, h) in rects:
+                d2 += Rectangle(position=(x, y), width=w, height=h, material=Material(EPS_SI))
+        return d2
+
+    d3 = Design(W, H, depth=D, material=Material(1.0), auto_pml=False)
+    mesh = layered_grid_3d(d3, [(0.0, z_si0, layer(EPS_OX, False)), (z_si0, z_si1, layer(1.0, True)),
+                                (z_si1, D, layer(1.0, False))], dx,
+                           z_pml_cells=npml if z_pml else 0, z_pml_sigma_factor=pml_sigma_factor(npml))
+    dt = 0.99 * dx / (LIGHT_SPEED * np.sqrt(3))
+    t = np.arange(steps) * dt
+    zc = (z_si0 + z_si1) / 2
+    d3 += GaussianSource(position=((npml + 20) * dx, H / 2, zc), width=dx, signal=pulse_signal(t))
+    freqs = LIGHT_SPEED / (np.linspace(1.5, 1.6, port_freqs) * µm)
+    mons = []
+    xp = W - (npml + 16) * dx
+    for q, yp in enumerate(port_y):
+        m = Monitor(design=d3, plane_normal="x", plane_position=xp, start=(xp, yp - 0.6 * µm, zc - 0.4 * µm),
+                    size=(1.2 * µm, 0.8 * µm), dft_frequencies=freqs, record_fields=False, accumulate_power=False,
+                    name=f"port{q + 1}")
+        d3 += m
+        mons.append(m)
+    return mesh, d3, t, mons

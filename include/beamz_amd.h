@@ -1,0 +1,124 @@
+/*
+ * beamz_amd C ABI: MI355X (gfx950) FDTD engine behind the beamz Python API.
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success, a negative BZ_E* code on failure; the message of
+ *     the last failure on the calling thread is returned by bz_last_error().
+ *   - device pointers are plain float / double pointers into memory the
+ *     caller owns (the Python layer allocates them as torch tensors); the
+ *     library never frees them.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream). All
+ *     launches are asynchronous on that stream.
+ *   - 2D arrays are row-major [ny][ld] with x fastest, ld >= nx; 3D arrays are
+ *     [nz][ny][ld]. Every field component is stored on the full padded box;
+ *     entries outside the reference's staggered shape stay zero.
+ *
+ * Reference interfaces replaced (paths relative to the beamz repository):
+ *   bz_update_h_2d ........ Backend.update_h_fields, beamz/simulation/backends/base.py:123-126,
+ *                           implemented by numpy_backend.py:34-61
+ *   bz_update_e_2d ........ Backend.update_e_field, base.py:128-131; numpy_backend.py:63-88
+ *   bz_update_3d .......... FDTD._update_3d_fields, beamz/simulation/fdtd.py:200-310
+ *   bz_add_points / bz_affine_points
+ *                           source injection, beamz/simulation/helper.py:5-118
+ *   bz_gather ............. monitor sampling, beamz/simulation/helper.py:231-256 with
+ *                           beamz/devices/monitors.py:279-397, plus the on-the-fly DFT
+ *                           (no reference counterpart)
+ *   bz_grid_power_2d/3d ... grid power accumulation, beamz/simulation/helper.py:120-187
+ *   bz_raster_2d/3d ....... RegularGrid.__rasterize__ / RegularGrid3D.__rasterize_3d__ +
+ *                           _process_3d_pml, beamz/simulation/meshing.py:92-536, 616-781,
+ *                           with PML.get_conductivity, beamz/design/pml.py:21-62
+ *   bz_yee3d_run .......... fused multi-step 3D engine: fdtd.py:411-429 step order
+ *                           (H, E, sources) for n steps in one call
+ *   bz_yee2d_run .......... fused multi-step 2D engine (batched, temporally blocked):
+ *                           fdtd.py:411-429 for many independent simulations
+ */
+#ifndef BEAMZ_AMD_H
+#define BEAMZ_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BZ_OK 0
+#define BZ_EINVAL (-1)   /* bad argument (shape, null pointer, out-of-range index) */
+#define BZ_EHIP (-2)     /* a HIP runtime call failed */
+#define BZ_ENOTSUP (-3)  /* unsupported configuration */
+
+const char* bz_last_error(void);
+int bz_version(void);
+int bz_device_count(int* count);
+
+/* ---------------- 2D (Ez, Hx, Hy) plugin updates ----------------
+ * Arrays in the reference's shapes, Ez/sigma/eps_r [ny][nx], Hx [ny][nx-1], Hy [ny-1][nx], each with its
+ * own row pitch ld_* (elements). Updated in place (the reference returns new H arrays). */
+int bz_update_h_2d(float* Hx, int ld_hx, float* Hy, int ld_hy, const float* Ez, int ld_ez,
+                   const float* sigma, int ld_sig, int nx, int ny, double dx, double dy, double dt,
+                   double mu0, double eps0, void* stream);
+int bz_update_e_2d(float* Ez, int ld_ez, const float* Hx, int ld_hx, const float* Hy, int ld_hy,
+                   const float* sigma, int ld_sig, const float* eps_r, int ld_eps, int nx, int ny,
+                   double dx, double dy, double dt, double eps0, void* stream);
+
+/* ---------------- 3D six-component update (H then E) ---------------- */
+/* f[0..5] = Ex, Ey, Ez, Hx, Hy, Hz, each [nz][ny][ld]. */
+int bz_update_3d(float* const f[6], const float* eps_r, const float* sigma, int nx, int ny, int nz, int ld,
+                 double dx, double dy, double dz, double dt, double mu0, double eps0, void* stream);
+
+/* ---------------- sources ---------------- */
+/* field[off[p]] += w[p] * amp for p < n (offsets unique within one call). */
+int bz_add_points(float* field, const int64_t* off, const float* w, int n, float amp, void* stream);
+/* field[off[p]] = mul[p] * field[off[p]] + w[p] * amp for p < n (offsets unique).
+ * A ModeSource's sequential "add, then zero the upstream cell" (helper.py:37-54) reduces to one
+ * such affine map per touched cell, precomputed on the host. */
+int bz_affine_points(float* field, const int64_t* off, const float* mul, const float* w, int n, float amp,
+                     void* stream);
+
+/* ---------------- monitors ---------------- */
+/* For p < n: v = comp_base[comp[p]][off[p]].
+ *   series (if not NULL): series[p] = v            (caller offsets it to the step's row)
+ *   dft (if nfreq > 0):   dft[k*n + p] += v * phasor[k]   for k < nfreq, FP64 complex
+ * comp_base holds up to 6 device pointers. */
+int bz_gather(const float* const comp_base[6], const int* comp, const int64_t* off, int n,
+              float* series, int nfreq, const double* phasor, double* dft, void* stream);
+
+/* Grid power: acc[j*ld+i] += Sx^2 + Sy^2 with Sx = -Ez*Hy, Sy = Ez*Hx (zero-padded H). */
+int bz_grid_power_2d(double* acc, const float* Ez, const float* Hx, const float* Hy, int nx, int ny, int ld,
+                     void* stream);
+/* Grid power 3D: acc (on the [nz-1][ny-1][nx-1] corner box, pitch ld) += |E x H|. */
+int bz_grid_power_3d(double* acc, float* const f[6], int nx, int ny, int nz, int ld, void* stream);
+
+/* ---------------- rasterizer (host, FP64, bit-exact with the reference) ---------------- */
+typedef struct bz_shape {
+    int kind;               /* 0 axis-aligned rectangle (fast path), 1 circle, 2 ring, 3 polygon */
+    double perm, permb, cond;
+    double bbox[6];         /* min_x, min_y, min_z, max_x, max_y, max_z */
+    double px, py, w, h;    /* kind 0 */
+    double cx, cy, r, r_in, r_out; /* kinds 1, 2 */
+    int nv;                 /* exterior vertex count; v = x0,y0,x1,y1,... */
+    const double* v;
+    int nholes;             /* holes: hole_nv[h] vertices each, packed in hv */
+    const int* hole_nv;
+    const double* hv;
+} bz_shape;
+
+typedef struct bz_pml {
+    int kind;               /* 0 rect, 1 corner */
+    int orient;             /* rect: 0 left 1 right 2 top 3 bottom; corner: 0 tl 1 tr 2 bl 3 br */
+    double px, py, w, h, r;
+    double order, sigma_factor, alpha_max;
+} bz_pml;
+
+/* bg = {permittivity, permeability, conductivity} of structures[0] (or NULL for vacuum).
+ * Outputs [gh][gw] (2D) or [gd][gh][gw] (3D), FP64. */
+int bz_raster_2d(const bz_shape* shapes, int nshapes, const double* bg, const bz_pml* pmls, int npml,
+                 double width, double height, double res, int gw, int gh,
+                 double* perm, double* permb, double* cond);
+int bz_raster_3d(const bz_shape* shapes, int nshapes, const double* bg, const bz_pml* pmls, int npml,
+                 double width, double height, double depth, double res_xy, double res_z,
+                 int gw, int gh, int gd, double* perm, double* permb, double* cond);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,103 @@
+"""GPU parity: the HIP engine (FP32) against the FP64 oracle on identical grids,
+sources and monitors. Tolerances are relative L2 errors stated per test."""
+import numpy as np
+import pytest
+
+import beamz_amd as bz
+from beamz_amd.devices.signals import gaussian_pulse
+from oracle import fdtd_ref as R
+
+pytestmark = pytest.mark.gpu
+
+um = 1e-6
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64); b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def waveguide_design_2d(nx=160, ny=80, npml=12, lam=1.55 * um):
+    dx = lam / 20
+    W, H = nx * dx, ny * dx
+    d = bz.Design(W, H, material=bz.Material(1.444 ** 2), auto_pml=False)
+    s = 4 * np.log(1e6) / (2 * npml * 1.2)
+    for pos, size, o in (((0, 0), (npml * dx, H), "left"), ((W - npml * dx, 0), (npml * dx, H), "right"),
+                         ((0, H - npml * dx), (W, npml * dx), "top"), ((0, 0), (W, npml * dx), "bottom")):
+        d.boundaries.append(bz.PML("rect", pos, size, o, polynomial_order=3.0, sigma_factor=s))
+    d += bz.Rectangle(position=(0, H / 2 - 0.25 * um), width=W, height=0.5 * um, material=bz.Material(3.476 ** 2))
+    return d, dx
+
+
+def pulse(t, lam=1.55 * um, dlam=0.2 * um):
+    f0 = bz.LIGHT_SPEED / lam
+    df = bz.LIGHT_SPEED * dlam / lam ** 2
+    sig_t = 1 / (2 * np.pi * df)
+    return gaussian_pulse(t, 1.0, 4 * sig_t, sig_t, f0, 0.0)
+
+
+def test_backend_plugin_updates_match_oracle(hip):
+    rng = np.random.default_rng(5)
+    ny, nx = 37, 53
+    dx, dy, dt = 5e-8, 5e-8, 1e-16
+    be = bz.get_backend("hip")
+    Ez = rng.standard_normal((ny, nx)); Hx = rng.standard_normal((ny, nx - 1)) / 377; Hy = rng.standard_normal((ny - 1, nx)) / 377
+    sig = rng.random((ny, nx)) * 5e4; eps = 1 + 10 * rng.random((ny, nx))
+    dEz, dHx, dHy, dsig, deps = (be.from_numpy(a) for a in (Ez, Hx, Hy, sig, eps))
+    for _ in range(20):
+        dHx, dHy = be.update_h_fields(dHx, dHy, dEz, dsig, dx, dy, dt, bz.MU_0, bz.EPS_0)
+        dEz = be.update_e_field(dEz, dHx, dHy, dsig, deps, dx, dy, dt, bz.EPS_0)
+        Hx, Hy = R.update_h_fields_2d(Hx, Hy, Ez, sig, dx, dy, dt)
+        Ez = R.update_e_field_2d(Ez, Hx, Hy, sig, eps, dx, dy, dt)
+    assert rel_l2(be.to_numpy(dEz), Ez) < 2e-6
+    assert rel_l2(be.to_numpy(dHx), Hx) < 2e-6
+    assert rel_l2(be.to_numpy(dHy), Hy) < 2e-6
+
+
+def test_fdtd_2d_waveguide_matches_oracle(hip):
+    d, dx = waveguide_design_2d()
+    nsteps = 1200
+    dt = 0.99 * dx / (bz.LIGHT_SPEED * np.sqrt(2))
+    t = np.arange(nsteps) * dt
+    sig = pulse(t)
+    d += bz.GaussianSource(position=(30 * dx, d.height / 2), width=dx / 4, signal=sig)
+    freqs = bz.LIGHT_SPEED / np.linspace(1.45e-6, 1.65e-6, 8)
+    mon = bz.Monitor(design=d, start=(120 * dx, 14 * dx), end=(120 * dx, d.height - 14 * dx), dft_frequencies=freqs)
+    d += mon
+    sim = bz.FDTD(d, time=t, resolution=dx)
+    sim.run(live=False, save=False, accumulate_power=True)
+    ref = R.RefSim(sim.epsilon_r, sim.sigma, dx, dx, dt, nsteps, sources=d.sources, monitors=[mon],
+                   accumulate_power=True).run()
+    for k in ("Ez", "Hx", "Hy"):
+        assert rel_l2(np.real(getattr(sim, k)), ref.F[k]) < 1e-4, k
+    ser = np.array(mon.fields["Ez"]); ref_ser = np.array(ref.mon_series[0]["Ez"])
+    assert ser.shape == ref_ser.shape
+    assert rel_l2(ser, ref_ser) < 1e-4
+    assert rel_l2(mon.power_history, ref.mon_series[0]["power"]) < 2e-4
+    F = R.dft_of_series(ref_ser, ref.mon_series[0]["t"], 2 * np.pi * freqs)
+    assert rel_l2(mon.dft["Ez"], F) < 1e-4
+    assert rel_l2(sim.power_accumulated, ref.power_accumulated / ref.power_count) < 2e-4
+
+
+def test_fdtd_3d_matches_oracle(hip):
+    dx = 50e-9
+    nx, ny, nz = 40, 32, 24
+    d = bz.Design(nx * dx, ny * dx, depth=nz * dx, material=bz.Material(1.444 ** 2), pml_size=6 * dx)
+    d += bz.Rectangle(position=(0, ny * dx / 2 - 0.25 * um, nz * dx / 2), width=nx * dx, height=0.5 * um,
+                      depth=0.22 * um, material=bz.Material(3.476 ** 2))
+    nsteps = 300
+    dt = 0.99 * dx / (bz.LIGHT_SPEED * np.sqrt(3))
+    t = np.arange(nsteps) * dt
+    d += bz.GaussianSource(position=(10 * dx, ny * dx / 2, nz * dx / 2), width=dx, signal=pulse(t))
+    mon = bz.Monitor(design=d, start=(0, 0, nz * dx / 2 + dx), end=(nx * dx, ny * dx, nz * dx / 2 + dx),
+                     dft_frequencies=[bz.LIGHT_SPEED / 1.55e-6])
+    d += mon
+    sim = bz.FDTD(d, time=t, resolution=dx)
+    sim.run(live=False, save=False)
+    ref = R.RefSim(sim.epsilon_r, sim.sigma, dx, dx, dt, nsteps, dz=dx, sources=d.sources, monitors=[mon],
+                   design_dims=(d.width, d.height, d.depth)).run()
+    for k in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        assert rel_l2(np.real(getattr(sim, k)), ref.F[k]) < 1e-4, k
+    ser = np.array(mon.fields["Ez"]); ref_ser = np.array(ref.mon_series[0]["Ez"])
+    assert rel_l2(ser, ref_ser) < 1e-4
+    assert rel_l2(mon.power_history, ref.mon_series[0]["power"]) < 2e-4
