@@ -28,6 +28,13 @@ class Pml(C.Structure):
                 ("order", f64), ("sigma_factor", f64), ("alpha_max", f64)]
 
 
+class BoxSource(C.Structure):
+    _fields_ = [("comp", i32), ("x0", i32), ("y0", i32), ("z0", i32), ("sx", i32), ("sy", i32), ("sz", i32),
+                ("add", vp), ("mul", vp), ("amp", f32)]
+
+
+MAX_BOX = 4
+
 # name -> argtypes (restype is int unless listed in _RESTYPE)
 SIGNATURES = {
     "bz_last_error": [],
@@ -36,6 +43,8 @@ SIGNATURES = {
     "bz_update_h_2d": [vp, i32, vp, i32, vp, i32, vp, i32, i32, i32, f64, f64, f64, f64, f64, vp],
     "bz_update_e_2d": [vp, i32, vp, i32, vp, i32, vp, i32, vp, i32, i32, i32, f64, f64, f64, f64, vp],
     "bz_update_3d": [P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, vp],
+    "bz_yee3d_step": [P6, P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, C.POINTER(BoxSource), i32,
+                      i32, vp],
     "bz_add_points": [vp, vp, vp, i32, f32, vp],
     "bz_affine_points": [vp, vp, vp, vp, i32, f32, vp],
     "bz_gather": [P6, vp, vp, i32, vp, i32, vp, vp, vp],

@@ -68,7 +68,8 @@ class Grid:
         box = (self.nz, self.ny, self.ld) if self.is_3d else (self.ny, self.ld)
         self.box = box
         self.names = COMP3 if self.is_3d else COMP2
-        self.f = {k: torch.zeros(box, dtype=torch.float32, device=self.dev) for k in self.names}
+        self._fb = [{k: torch.zeros(box, dtype=torch.float32, device=self.dev) for k in self.names}, None]
+        self._cur = 0
         self.eps = torch.ones(box, dtype=torch.float32, device=self.dev)
         self.sig = torch.zeros(box, dtype=torch.float32, device=self.dev)
         self.set_materials(eps, sigma)
@@ -78,14 +79,32 @@ class Grid:
         self.power = None
         self.power_count = 0
         self.fused = None
-        self.kernel = "basic"
+        self.kernel = "fused" if self.is_3d else "basic"
+        self.kseg = 16
+        self.timing = False
+        self._events = []
+
+    @property
+    def f(self):
+        """Current field buffers (the fused 3D kernel ping-pongs between two sets)."""
+        return self._fb[self._cur]
 
     def kernel_name(self):
-        return "bz_update_3d (two-pass)" if self.is_3d else "bz_update_h_2d + bz_update_e_2d"
+        if self.is_3d:
+            return "bz_yee3d_step (fused z-march)" if self.kernel == "fused" else "bz_update_3d (two-pass)"
+        return "bz_update_h_2d + bz_update_e_2d"
+
+    def start_timing(self):
+        self.timing = True
+        self._events = []
 
     def launch_timer(self):
-        """Mean device time of one update launch over the last timed run, seconds (None if not measured)."""
-        return None
+        """Mean device time of one update launch over the timed launches, seconds (None if not measured)."""
+        if not self._events:
+            return None
+        self.torch.cuda.synchronize(self.dev)
+        ms = [a.elapsed_time(b) for a, b in self._events]
+        return float(np.mean(ms)) / 1e3
 
     # ---------------- materials / views ----------------
     def set_materials(self, eps, sigma):
@@ -114,8 +133,11 @@ class Grid:
         return self.backend.stream
 
     def reset_fields(self):
-        for t in self.f.values():
-            t.zero_()
+        for fs in self._fb:
+            if fs is not None:
+                for t in fs.values():
+                    t.zero_()
+        self._cur = 0
 
     # ---------------- sources ----------------
     def add_point_source(self, comp, idx, w, mul, signal):
@@ -127,8 +149,34 @@ class Grid:
         src = _PointSource(comp, T.from_numpy(off).to(self.dev), T.from_numpy(np.asarray(w, np.float32)).to(self.dev),
                            None if mul is None else T.from_numpy(np.asarray(mul, np.float32)).to(self.dev),
                            np.asarray(signal))
+        src.box = self._as_box(comp, np.asarray(idx), np.asarray(w, np.float64), mul)
         self.sources.append(src)
         return src
+
+    def _as_box(self, comp, idx, w, mul, max_cells=1 << 22):
+        """Dense (add, mul) box equivalent of a point list, for the fused kernel's source epilogue."""
+        if not self.is_3d or len(idx) == 0:
+            return None
+        lo = idx.min(axis=0); hi = idx.max(axis=0) + 1
+        sz, sy, sx = (int(v) for v in (hi - lo))
+        if sz * sy * sx > max_cells:
+            return None
+        add = np.zeros((sz, sy, sx), np.float32)
+        mu = np.ones((sz, sy, sx), np.float32)
+        rel = idx - lo
+        add[rel[:, 0], rel[:, 1], rel[:, 2]] = w
+        if mul is not None:
+            mu[rel[:, 0], rel[:, 1], rel[:, 2]] = mul
+        T = self.torch
+        b = N.BoxSource()
+        b.comp = ("Ex", "Ey", "Ez").index(comp)
+        b.z0, b.y0, b.x0 = (int(v) for v in lo)
+        b.sz, b.sy, b.sx = sz, sy, sx
+        dadd = T.from_numpy(add.ravel()).to(self.dev)
+        dmul = None if mul is None else T.from_numpy(mu.ravel()).to(self.dev)
+        b.add = dadd.data_ptr()
+        b.mul = 0 if dmul is None else dmul.data_ptr()
+        return (b, dadd, dmul)
 
     # ---------------- monitors ----------------
     def add_gather(self, comps, offs, record_steps, t_all, dft_omegas, keep_series=True):
@@ -212,10 +260,42 @@ class Grid:
                    f["Hy"].data_ptr(), self.nx, self.ny, self.ld, st)
         self.power_count += 1
 
+    def _fused3d(self, n):
+        """One fused step; returns True if the sources were applied inside the kernel."""
+        if self._fb[1] is None:
+            self._fb[1] = {k: self.torch.zeros_like(v) for k, v in self._fb[0].items()}
+        boxed = len(self.sources) <= N.MAX_BOX and all(s.box is not None for s in self.sources)
+        arr = (N.BoxSource * N.MAX_BOX)()
+        nb = 0
+        if boxed:
+            for s in self.sources:
+                b = s.box[0]
+                b.amp = float(np.real(s.signal[n]))
+                arr[nb] = b
+                nb += 1
+        src, dst = self._fb[self._cur], self._fb[1 - self._cur]
+        ps = N.ptrs6([src[k].data_ptr() for k in COMP3])
+        pd = N.ptrs6([dst[k].data_ptr() for k in COMP3])
+        st = self.stream
+        if self.timing:
+            e0, e1 = self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True)
+            e0.record()
+        N.call("bz_yee3d_step", ps, pd, self.eps.data_ptr(), self.sig.data_ptr(), self.nx, self.ny, self.nz, self.ld,
+               self.dx, self.dy, self.dz, self.dt, MU_0, EPS_0, arr, nb, self.kseg, st)
+        if self.timing:
+            e1.record()
+            self._events.append((e0, e1))
+        self._cur = 1 - self._cur
+        return boxed
+
     def step(self, n):
         """Advance one step (step index n) in the reference order."""
-        self._update()
-        self._sources(n)
+        if self.is_3d and self.kernel == "fused":
+            if not self._fused3d(n):
+                self._sources(n)
+        else:
+            self._update()
+            self._sources(n)
         self._gathers(n)
         self._power()
 

@@ -65,6 +65,26 @@ int bz_update_e_2d(float* Ez, int ld_ez, const float* Hx, int ld_hx, const float
 int bz_update_3d(float* const f[6], const float* eps_r, const float* sigma, int nx, int ny, int nz, int ld,
                  double dx, double dy, double dz, double dt, double mu0, double eps0, void* stream);
 
+/* ---------------- fused 3D step ----------------
+ * One time step (H, E, sources) of fdtd.py:200-310 + helper.py:5-118 in a single pass:
+ * reads src[6], writes every cell of dst[6] (ping-pong buffers; src != dst).
+ * Box sources: inside box b, after the E update, E_comp = mul * E_comp + add * amp
+ * (mul NULL means 1). A GaussianSource patch is a box with mul NULL; a ModeSource
+ * line/plane is a box whose mul zeroes the upstream cells (helper.py:37-54). */
+#define BZ_MAX_BOX 4
+typedef struct bz_box_source {
+    int comp;               /* 3D: 0 Ex, 1 Ey, 2 Ez; 2D: 0 Ez */
+    int x0, y0, z0, sx, sy, sz;
+    const float* add;       /* device, sz*sy*sx, x fastest */
+    const float* mul;       /* device or NULL */
+    float amp;              /* signal value of this step */
+} bz_box_source;
+
+/* kseg: z-planes per workgroup segment (<= 0: default 16). */
+int bz_yee3d_step(float* const src[6], float* const dst[6], const float* eps_r, const float* sigma,
+                  int nx, int ny, int nz, int ld, double dx, double dy, double dz, double dt,
+                  double mu0, double eps0, const bz_box_source* boxes, int nbox, int kseg, void* stream);
+
 /* ---------------- sources ---------------- */
 /* field[off[p]] += w[p] * amp for p < n (offsets unique within one call). */
 int bz_add_points(float* field, const int64_t* off, const float* w, int n, float amp, void* stream);

@@ -101,3 +101,21 @@ def test_fdtd_3d_matches_oracle(hip):
     ser = np.array(mon.fields["Ez"]); ref_ser = np.array(ref.mon_series[0]["Ez"])
     assert rel_l2(ser, ref_ser) < 1e-4
     assert rel_l2(mon.power_history, ref.mon_series[0]["power"]) < 2e-4
+
+
+def test_fused_3d_kernel_matches_two_pass_kernel(hip):
+    """The fused z-march kernel (ping-pong, LDS halos, segment prologues) against the two-pass kernels,
+    on a grid whose sizes are not multiples of the tile (63 x 7) or the z-segment."""
+    from beamz_amd.workloads import soi_mmi_3d
+    res = {}
+    for kern, kseg in (("basic", 16), ("fused", 5), ("fused", 16)):
+        mesh, d3, t, mons = soi_mmi_3d(nx=131, ny=45, nz=37, steps=120, npml=6, dx=40e-9)
+        sim = bz.FDTD(mesh, time=t)
+        sim.grid.kernel, sim.grid.kseg = kern, kseg
+        sim.run(live=False, save=False)
+        res[(kern, kseg)] = {k: np.real(getattr(sim, k)) for k in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")}
+        res[(kern, kseg)]["dft"] = mons[0].dft["Ey"]
+    base = res[("basic", 16)]
+    for key in (("fused", 5), ("fused", 16)):
+        for k in base:
+            assert rel_l2(res[key][k], base[k]) < 1e-5, (key, k)
