@@ -44,7 +44,7 @@ SIGNATURES = {
     "bz_update_e_2d": [vp, i32, vp, i32, vp, i32, vp, i32, vp, i32, i32, i32, f64, f64, f64, f64, vp],
     "bz_update_3d": [P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, vp],
     "bz_yee3d_step": [P6, P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, C.POINTER(BoxSource), i32,
-                      i32, vp],
+                      i32, i32, vp],
     "bz_add_points": [vp, vp, vp, i32, f32, vp],
     "bz_affine_points": [vp, vp, vp, vp, i32, f32, vp],
     "bz_gather": [P6, vp, vp, i32, vp, i32, vp, vp, vp],

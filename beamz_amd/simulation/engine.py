@@ -81,6 +81,7 @@ class Grid:
         self.fused = None
         self.kernel = "fused" if self.is_3d else "basic"
         self.kseg = 16
+        self.rows = 4
         self.timing = False
         self._events = []
 
@@ -91,7 +92,7 @@ class Grid:
 
     def kernel_name(self):
         if self.is_3d:
-            return "bz_yee3d_step (fused z-march)" if self.kernel == "fused" else "bz_update_3d (two-pass)"
+            return f"bz_yee3d_step (fused z-march, {self.rows} rows/wave)" if self.kernel == "fused" else "bz_update_3d (two-pass)"
         return "bz_update_h_2d + bz_update_e_2d"
 
     def start_timing(self):
@@ -281,7 +282,7 @@ class Grid:
             e0, e1 = self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True)
             e0.record()
         N.call("bz_yee3d_step", ps, pd, self.eps.data_ptr(), self.sig.data_ptr(), self.nx, self.ny, self.nz, self.ld,
-               self.dx, self.dy, self.dz, self.dt, MU_0, EPS_0, arr, nb, self.kseg, st)
+               self.dx, self.dy, self.dz, self.dt, MU_0, EPS_0, arr, nb, self.kseg, self.rows, st)
         if self.timing:
             e1.record()
             self._events.append((e0, e1))

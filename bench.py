@@ -59,6 +59,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default=os.environ.get("BEAMZ_AMD_KERNEL3D", "fused"))
     ap.add_argument("--kseg", type=int, default=int(os.environ.get("BEAMZ_AMD_KSEG", "16")))
+    ap.add_argument("--rows", type=int, default=int(os.environ.get("BEAMZ_AMD_ROWS", "4")))
     ap.add_argument("--timing-steps", type=int, default=30)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -73,6 +74,7 @@ def main():
     sim = bz.FDTD(mesh, time=t)
     sim.grid.kernel = a.kernel
     sim.grid.kseg = a.kseg
+    sim.grid.rows = a.rows
     sim.initialize_simulation(save=False, live=False)
     sim._advance(a.warmup)
     torch.cuda.synchronize()

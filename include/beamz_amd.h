@@ -80,10 +80,10 @@ typedef struct bz_box_source {
     float amp;              /* signal value of this step */
 } bz_box_source;
 
-/* kseg: z-planes per workgroup segment (<= 0: default 16). */
+/* kseg: z-planes per wave segment (<= 0: default 16); rows: y-rows per wave (2, 4 or 8). */
 int bz_yee3d_step(float* const src[6], float* const dst[6], const float* eps_r, const float* sigma,
                   int nx, int ny, int nz, int ld, double dx, double dy, double dz, double dt,
-                  double mu0, double eps0, const bz_box_source* boxes, int nbox, int kseg, void* stream);
+                  double mu0, double eps0, const bz_box_source* boxes, int nbox, int kseg, int rows, void* stream);
 
 /* ---------------- sources ---------------- */
 /* field[off[p]] += w[p] * amp for p < n (offsets unique within one call). */
