@@ -35,6 +35,13 @@ class BoxSource(C.Structure):
 
 MAX_BOX = 4
 
+
+class Sim2D(C.Structure):
+    _fields_ = [("nx", i32), ("ny", i32), ("ld", i32), ("ez", vp * 2), ("hx", vp * 2), ("hy", vp * 2),
+                ("eps", vp), ("sig", vp), ("kh", f32), ("kbh", f32), ("ke", f32), ("kbe", f32),
+                ("rdx", f32), ("rdy", f32), ("src_x0", i32), ("src_y0", i32), ("src_sx", i32), ("src_sy", i32),
+                ("src_add", vp), ("src_mul", vp), ("signal", vp), ("power", vp)]
+
 # name -> argtypes (restype is int unless listed in _RESTYPE)
 SIGNATURES = {
     "bz_last_error": [],
@@ -45,6 +52,7 @@ SIGNATURES = {
     "bz_update_3d": [P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, vp],
     "bz_yee3d_step": [P6, P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, C.POINTER(BoxSource), i32,
                       i32, i32, vp],
+    "bz_yee2d_run": [vp, i32, vp, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp],
     "bz_add_points": [vp, vp, vp, i32, f32, vp],
     "bz_affine_points": [vp, vp, vp, vp, i32, f32, vp],
     "bz_gather": [P6, vp, vp, i32, vp, i32, vp, vp, vp],

@@ -10,15 +10,16 @@
 // Work decomposition (wave64, no LDS, no barriers):
 //   * a wave owns x-columns i0-1 .. i0+62 (lane l -> i = i0-1+l) and rows
 //     j0-1 .. j0+R-1 (register row r -> j = j0-1+r) and marches along z over a
-//     segment [k0, k1). Lane 0 and row 0 are halo owners: they compute H (the
-//     E update of lane 1 / row 1 needs H at i-1 / j-1) and write nothing, so a
-//     wave outputs 63 x R columns.
+//     segment [k0, k1). Lanes 0 and 63 and row 0 are halo owners: lane 0 and
+//     row 0 supply H at i-1 / j-1, lane 63 supplies E at i+1; they store
+//     nothing, so a wave outputs 62 x R columns. Row R+1 supplies E at j+1.
 //   * per plane every column reads its eight values (Ex/Ey of the next plane,
 //     Ez, Hx, Hy, Hz, sigma, eps) once; Ex/Ey of the current plane and Hx/Hy
-//     of the previous plane stay in registers; the +x / -x neighbours come from
-//     the adjacent lane by wave shuffles, the +y / -y neighbours from the
-//     adjacent register row. One extra row (j0+R: Ez and the Ex queue) and,
-//     for lane 63, the +x column are read for the halo.
+//     of the previous plane stay in registers; x neighbours move between
+//     lanes by wave shuffles, y neighbours between register rows.
+//   * the loads of plane k+1 are issued before the stores of plane k: on gfx9
+//     loads and stores share one in-order completion counter, so waiting for
+//     loads issued after the stores would also wait for the stores.
 //   * a segment starting at k0 > 0 first recomputes H of plane k0-1 for its
 //     own columns (bit-identical to what the previous segment computes).
 //   * workgroups are 4 waves stacked in y; hardware workgroup ids are remapped
@@ -31,7 +32,7 @@
 namespace {
 
 constexpr int LANES = 64;
-constexpr int OX = LANES - 1;
+constexpr int OX = LANES - 2;
 constexpr int WPG = 4;       // waves per workgroup
 constexpr int NXCD = 8;
 
@@ -44,12 +45,136 @@ struct BoxSources {
     bz_box_source b[BZ_MAX_BOX];
 };
 
-__device__ __forceinline__ float ldg(const float* __restrict__ p, long o, bool ok) { return ok ? p[o] : 0.0f; }
+// Branch-free guarded load: the address is clamped to element 0 when the cell is outside the grid, so
+// every lane issues the load and no exec-mask branch splits the memory stream. Out-of-grid lanes thus
+// hold a finite in-grid value instead of zero; such values only ever reach H/E entries that the
+// validity selects discard or that are not stored (halo lanes/rows, planes past nz), so no select is
+// applied here and the loaded value is not consumed before the plane that needs it.
+__device__ __forceinline__ float ldg(const float* __restrict__ p, long o, bool ok) { return p[ok ? o : 0]; }
 
-__device__ __forceinline__ float shfl_dn(float v) { return __shfl_down(v, 1, LANES); }
-__device__ __forceinline__ float shfl_up(float v) { return __shfl_up(v, 1, LANES); }
+// Per-plane register set of one wave column block: row r = 0..R+1.
+template <int R>
+struct Plane {
+    float ex1[R + 2], ey1[R + 1];   // Ex, Ey of plane k+1
+    float ez[R + 2];                // Ez of plane k
+    float hx[R + 1], hy[R + 1], hz[R + 1], sg[R + 1], ee[R + 1];
+};
+
+// Values carried from plane k-1 to plane k: Ex/Ey of plane k, Hx/Hy (new) of plane k-1.
+template <int R>
+struct Carry {
+    float ex0[R + 2], ey0[R + 1], hxp[R + 1], hyp[R + 1];
+};
+
+struct Geo {
+    int nx, ny, nz, ld, jb, i, k0;
+    long ps;
+    bool lane_out, xin, hyok;
+};
 
 template <int R>
+__device__ __forceinline__ void load_plane(Plane<R>& P, const Fields6& S, const float* __restrict__ eps,
+                                           const float* __restrict__ sig, int k, const Geo& g, const bool* col,
+                                           bool valid) {
+    // `valid` is false for the plane past the segment end: the loads are still issued (to element 0)
+    // so that both register planes are written on every path and the loop needs no register copies.
+    const long ok = (long)k * g.ps;
+    const bool nk = valid && k + 1 < g.nz;
+    const bool we = valid && k >= g.k0;
+#pragma unroll
+    for (int r = 0; r < R + 2; ++r) {
+        const bool cv = col[r] && valid;
+        const long o = ok + (long)(g.jb + r) * g.ld + g.i;
+        P.ex1[r] = ldg(S.p[0], o + g.ps, cv && nk);
+        P.ez[r] = ldg(S.p[2], o, cv);
+        if (r < R + 1) {
+            P.ey1[r] = ldg(S.p[1], o + g.ps, cv && nk);
+            P.hx[r] = ldg(S.p[3], o, cv);
+            P.hz[r] = ldg(S.p[5], o, cv);
+            P.sg[r] = ldg(sig, o, cv);
+            P.hy[r] = (r > 0) ? ldg(S.p[4], o, cv) : 0.0f;
+            P.ee[r] = (r > 0) ? ldg(eps, o, cv && we) : 1.0f;
+        }
+    }
+}
+
+// One z-plane: H of plane k for rows 0..R, then (if k >= k0) E of rows 1..R, sources, stores.
+template <int R>
+__device__ __forceinline__ void march_plane(const Plane<R>& P, Carry<R>& C, int k, const Geo& g, const bool* col,
+                                            const Fields6& D, const Coef& c, float rdx, float rdy, float rdz,
+                                            const BoxSources& src) {
+    const long ok = (long)k * g.ps;
+    float hxn[R + 1], hyn[R + 1], hzn[R + 1];
+    const bool kh = k < g.nz - 1;
+#pragma unroll
+    for (int r = 0; r < R + 1; ++r) {
+        const int j = g.jb + r;
+        const float ezR = __shfl_down(P.ez[r], 1, LANES);
+        const float eyR = __shfl_down(C.ey0[r], 1, LANES);
+        float fa, fb;
+        h_coef(P.sg[r], c, fa, fb);
+        const bool jh = j < g.ny - 1;
+        hxn[r] = (kh && jh) ? fa * P.hx[r] - fb * ((P.ez[r + 1] - P.ez[r]) * rdy - (P.ey1[r] - C.ey0[r]) * rdz)
+                            : P.hx[r];
+        hyn[r] = (kh && g.hyok) ? fa * P.hy[r] - fb * ((P.ex1[r] - C.ex0[r]) * rdz - (ezR - P.ez[r]) * rdx)
+                                : P.hy[r];
+        hzn[r] = (jh && g.hyok) ? fa * P.hz[r] - fb * ((eyR - C.ey0[r]) * rdx - (C.ex0[r + 1] - C.ex0[r]) * rdy)
+                                : P.hz[r];
+    }
+    if (k >= g.k0) {
+        const bool kin = k >= 1 && k <= g.nz - 2;
+        const int i = g.i;
+#pragma unroll
+        for (int r = 1; r < R + 1; ++r) {
+            const float hzL = __shfl_up(hzn[r], 1, LANES);
+            const float hyL = __shfl_up(hyn[r], 1, LANES);
+            const int j = g.jb + r;
+            if (g.lane_out && col[r]) {
+                const long o = ok + (long)j * g.ld + i;
+                const bool yin = j >= 1 && j <= g.ny - 2;
+                float ca, cb;
+                e_coef(P.sg[r], P.ee[r], c, ca, cb);
+                float exn = (kin && yin && i <= g.nx - 2)
+                                ? ca * C.ex0[r] + cb * ((hzn[r] - hzn[r - 1]) * rdy - (hyn[r] - C.hyp[r]) * rdz)
+                                : C.ex0[r];
+                float eyn = (kin && j <= g.ny - 2 && g.xin)
+                                ? ca * C.ey0[r] + cb * ((hxn[r] - C.hxp[r]) * rdz - (hzn[r] - hzL) * rdx)
+                                : C.ey0[r];
+                float ezn = (k <= g.nz - 2 && yin && g.xin)
+                                ? ca * P.ez[r] + cb * ((hyn[r] - hyL) * rdx - (hxn[r] - hxn[r - 1]) * rdy)
+                                : P.ez[r];
+                for (int q = 0; q < src.n; ++q) {
+                    const bz_box_source& b = src.b[q];
+                    const int a = i - b.x0, bb = j - b.y0, cc = k - b.z0;
+                    if (a >= 0 && a < b.sx && bb >= 0 && bb < b.sy && cc >= 0 && cc < b.sz) {
+                        const long rr = ((long)cc * b.sy + bb) * b.sx + a;
+                        const float m = b.mul ? b.mul[rr] : 1.0f;
+                        if (b.comp == 0) exn = m * exn + b.add[rr] * b.amp;
+                        else if (b.comp == 1) eyn = m * eyn + b.add[rr] * b.amp;
+                        else ezn = m * ezn + b.add[rr] * b.amp;
+                    }
+                }
+                D.p[0][o] = exn;
+                D.p[1][o] = eyn;
+                D.p[2][o] = ezn;
+                D.p[3][o] = hxn[r];
+                D.p[4][o] = hyn[r];
+                D.p[5][o] = hzn[r];
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R + 2; ++r) {
+        C.ex0[r] = P.ex1[r];
+        if (r < R + 1) {
+            C.ey0[r] = P.ey1[r];
+            C.hxp[r] = hxn[r];
+            C.hyp[r] = hyn[r];
+        }
+    }
+}
+
+template <int R, int PIPE>
 __global__ __launch_bounds__(LANES* WPG) void k_yee3d_fused(Fields6 S, Fields6 D, const float* __restrict__ eps,
                                                            const float* __restrict__ sig, int nx, int ny, int nz,
                                                            int ld, int kseg, int gx, int gy, int gz, int per_xcd,
@@ -62,141 +187,61 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee3d_fused(Fields6 S, Fields6 D
     const int ty = (tile / gx) % gy;
     const int tz = tile / (gx * gy);
 
+    Geo g;
+    g.nx = nx; g.ny = ny; g.nz = nz; g.ld = ld;
     const int l = threadIdx.x;
-    const int i = tx * OX - 1 + l;
-    const int jb = (ty * WPG + threadIdx.y) * R - 1;   // row 0 of this wave (halo row)
-    const int k0 = tz * kseg;
-    const int k1 = min(nz, k0 + kseg);
-    const bool ci = i >= 0 && i < nx;
-    const long ps = (long)ny * ld;
-
-    const float* __restrict__ gEx = S.p[0];
-    const float* __restrict__ gEy = S.p[1];
-    const float* __restrict__ gEz = S.p[2];
-    const float* __restrict__ gHx = S.p[3];
-    const float* __restrict__ gHy = S.p[4];
-    const float* __restrict__ gHz = S.p[5];
+    g.i = tx * OX - 1 + l;
+    g.jb = (ty * WPG + threadIdx.y) * R - 1;   // row 0 of this wave (halo row)
+    g.k0 = tz * kseg;
+    g.ps = (long)ny * ld;
+    const int k1 = min(nz, g.k0 + kseg);
+    const bool ci = g.i >= 0 && g.i < nx;
+    g.lane_out = l > 0 && l < LANES - 1 && ci;
+    g.xin = g.i >= 1 && g.i <= nx - 2;
+    g.hyok = g.i < nx - 1;
 
     bool col[R + 2];
 #pragma unroll
-    for (int r = 0; r < R + 2; ++r) col[r] = ci && (jb + r) >= 0 && (jb + r) < ny;
-    const bool rx = i + 1 < nx;                  // +x neighbour exists (used by lane 63)
-    const bool lane_out = l > 0 && ci;
-    const bool xin = i >= 1 && i <= nx - 2;
-    const bool hyok = i < nx - 1;
+    for (int r = 0; r < R + 2; ++r) col[r] = ci && (g.jb + r) >= 0 && (g.jb + r) < ny;
 
-    const int kstart = k0 > 0 ? k0 - 1 : k0;
-    float ex0[R + 2], ey0[R + 1], hxp[R + 1], hyp[R + 1];
+    const int kstart = g.k0 > 0 ? g.k0 - 1 : g.k0;
+    Carry<R> C;
 #pragma unroll
     for (int r = 0; r < R + 2; ++r) {
-        const long o = (long)kstart * ps + (long)(jb + r) * ld + i;
-        ex0[r] = ldg(gEx, o, col[r]);
+        const long o = (long)kstart * g.ps + (long)(g.jb + r) * ld + g.i;
+        C.ex0[r] = ldg(S.p[0], o, col[r]);
         if (r < R + 1) {
-            ey0[r] = ldg(gEy, o, col[r]);
-            hxp[r] = 0.0f;
-            hyp[r] = 0.0f;
+            C.ey0[r] = ldg(S.p[1], o, col[r]);
+            C.hxp[r] = 0.0f;
+            C.hyp[r] = 0.0f;
         }
     }
-
-    for (int k = kstart; k < k1; ++k) {
-        const long ok = (long)k * ps;
-        const bool nk = k + 1 < nz;
-        float ex1[R + 2], ey1[R + 1], ez[R + 2], hx[R + 1], hy[R + 1], hz[R + 1], sg[R + 1], ee[R + 1];
-#pragma unroll
-        for (int r = 0; r < R + 2; ++r) {
-            const long o = ok + (long)(jb + r) * ld + i;
-            ex1[r] = ldg(gEx, o + ps, col[r] && nk);
-            ez[r] = ldg(gEz, o, col[r]);
-            if (r < R + 1) {
-                ey1[r] = ldg(gEy, o + ps, col[r] && nk);
-                hx[r] = ldg(gHx, o, col[r]);
-                hz[r] = ldg(gHz, o, col[r]);
-                sg[r] = ldg(sig, o, col[r]);
-                hy[r] = (r > 0) ? ldg(gHy, o, col[r]) : 0.0f;
-                ee[r] = (r > 0) ? ldg(eps, o, col[r] && k >= k0) : 1.0f;
-            }
+    Plane<R> A, B;
+    if (PIPE == 0) {
+        // Two register planes alternate so that plane k+1 is in flight while plane k is computed and stored.
+        load_plane<R>(A, S, eps, sig, kstart, g, col, true);
+        for (int k = kstart; k < k1; k += 2) {
+            load_plane<R>(B, S, eps, sig, k + 1, g, col, k + 1 < k1);
+            march_plane<R>(A, C, k, g, col, D, c, rdx, rdy, rdz, src);
+            if (k + 1 >= k1) break;
+            load_plane<R>(A, S, eps, sig, k + 2, g, col, k + 2 < k1);
+            march_plane<R>(B, C, k + 1, g, col, D, c, rdx, rdy, rdz, src);
         }
-        // +x neighbours of Ez and Ey (lane 63 reads the next tile's first column itself)
-        float ezR[R + 1], eyR[R + 1];
-#pragma unroll
-        for (int r = 0; r < R + 1; ++r) {
-            ezR[r] = shfl_dn(ez[r]);
-            eyR[r] = shfl_dn(ey0[r]);
-        }
-        if (l == LANES - 1) {
-#pragma unroll
-            for (int r = 0; r < R + 1; ++r) {
-                const long o = ok + (long)(jb + r) * ld + i + 1;
-                ezR[r] = ldg(gEz, o, col[r] && rx);
-                eyR[r] = ldg(gEy, o, col[r] && rx);
-            }
-        }
-        float hxn[R + 1], hyn[R + 1], hzn[R + 1];
-        const bool kh = k < nz - 1;
-#pragma unroll
-        for (int r = 0; r < R + 1; ++r) {
-            const int j = jb + r;
-            float fa, fb;
-            h_coef(sg[r], c, fa, fb);
-            const bool jh = j < ny - 1;
-            hxn[r] = (kh && jh) ? fa * hx[r] - fb * ((ez[r + 1] - ez[r]) * rdy - (ey1[r] - ey0[r]) * rdz) : hx[r];
-            hyn[r] = (kh && hyok) ? fa * hy[r] - fb * ((ex1[r] - ex0[r]) * rdz - (ezR[r] - ez[r]) * rdx) : hy[r];
-            hzn[r] = (jh && hyok) ? fa * hz[r] - fb * ((eyR[r] - ey0[r]) * rdx - (ex0[r + 1] - ex0[r]) * rdy) : hz[r];
-        }
-        if (k >= k0) {
-            const bool kin = k >= 1 && k <= nz - 2;
-#pragma unroll
-            for (int r = 1; r < R + 1; ++r) {
-                const float hzL = shfl_up(hzn[r]);
-                const float hyL = shfl_up(hyn[r]);
-                const int j = jb + r;
-                if (lane_out && col[r]) {
-                    const long o = ok + (long)j * ld + i;
-                    const bool yin = j >= 1 && j <= ny - 2;
-                    float ca, cb;
-                    e_coef(sg[r], ee[r], c, ca, cb);
-                    float exn = (kin && yin && i <= nx - 2)
-                                    ? ca * ex0[r] + cb * ((hzn[r] - hzn[r - 1]) * rdy - (hyn[r] - hyp[r]) * rdz)
-                                    : ex0[r];
-                    float eyn = (kin && j <= ny - 2 && xin)
-                                    ? ca * ey0[r] + cb * ((hxn[r] - hxp[r]) * rdz - (hzn[r] - hzL) * rdx)
-                                    : ey0[r];
-                    float ezn = (k <= nz - 2 && yin && xin)
-                                    ? ca * ez[r] + cb * ((hyn[r] - hyL) * rdx - (hxn[r] - hxn[r - 1]) * rdy)
-                                    : ez[r];
-                    for (int q = 0; q < src.n; ++q) {
-                        const bz_box_source& b = src.b[q];
-                        const int a = i - b.x0, bb = j - b.y0, cc = k - b.z0;
-                        if (a >= 0 && a < b.sx && bb >= 0 && bb < b.sy && cc >= 0 && cc < b.sz) {
-                            const long rr = ((long)cc * b.sy + bb) * b.sx + a;
-                            const float m = b.mul ? b.mul[rr] : 1.0f;
-                            if (b.comp == 0) exn = m * exn + b.add[rr] * b.amp;
-                            else if (b.comp == 1) eyn = m * eyn + b.add[rr] * b.amp;
-                            else ezn = m * ezn + b.add[rr] * b.amp;
-                        }
-                    }
-                    D.p[0][o] = exn;
-                    D.p[1][o] = eyn;
-                    D.p[2][o] = ezn;
-                    D.p[3][o] = hxn[r];
-                    D.p[4][o] = hyn[r];
-                    D.p[5][o] = hzn[r];
-                }
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < R + 2; ++r) {
-            ex0[r] = ex1[r];
-            if (r < R + 1) {
-                ey0[r] = ey1[r];
-                hxp[r] = hxn[r];
-                hyp[r] = hyn[r];
-            }
+    } else {
+        // Both planes of an iteration are requested before any store of the iteration: the second
+        // plane's latency hides behind the first plane's compute, and no loaded plane is carried across
+        // the loop back-edge.
+        for (int k = kstart; k < k1; k += 2) {
+            load_plane<R>(A, S, eps, sig, k, g, col, true);
+            load_plane<R>(B, S, eps, sig, k + 1, g, col, k + 1 < k1);
+            march_plane<R>(A, C, k, g, col, D, c, rdx, rdy, rdz, src);
+            if (k + 1 >= k1) break;
+            march_plane<R>(B, C, k + 1, g, col, D, c, rdx, rdy, rdz, src);
         }
     }
 }
 
-template <int R>
+template <int R, int PIPE>
 int launch(const Fields6& S, const Fields6& D, const float* eps, const float* sig, int nx, int ny, int nz, int ld,
            int kseg, const Coef& c, float rdx, float rdy, float rdz, const BoxSources& bs, hipStream_t st) {
     const int gx = (nx + OX - 1) / OX;
@@ -204,7 +249,7 @@ int launch(const Fields6& S, const Fields6& D, const float* eps, const float* si
     const int gz = (nz + kseg - 1) / kseg;
     const int n = gx * gy * gz;
     const int per_xcd = (n + NXCD - 1) / NXCD;
-    hipLaunchKernelGGL(k_yee3d_fused<R>, dim3(per_xcd * NXCD), dim3(LANES, WPG), 0, st, S, D, eps, sig, nx, ny, nz,
+    hipLaunchKernelGGL(k_yee3d_fused<R, PIPE>, dim3(per_xcd * NXCD), dim3(LANES, WPG), 0, st, S, D, eps, sig, nx, ny, nz,
                        ld, kseg, gx, gy, gz, per_xcd, c, rdx, rdy, rdz, bs);
     BZ_CHECK_LAUNCH("bz_yee3d_step");
     return BZ_OK;
@@ -238,9 +283,10 @@ extern "C" int bz_yee3d_step(float* const src[6], float* const dst[6], const flo
     const float rx = (float)(1.0 / dx), ry = (float)(1.0 / dy), rz = (float)(1.0 / dz);
     const hipStream_t st = (hipStream_t)stream;
     switch (rows) {
-        case 2: return launch<2>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, st);
-        case 4: return launch<4>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, st);
-        case 8: return launch<8>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, st);
-        default: bz::set_error("bz_yee3d_step: rows must be 2, 4 or 8"); return BZ_EINVAL;
+        case 2: return launch<2, 0>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, st);
+        case 4: return launch<4, 0>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, st);
+        case 102: return launch<2, 1>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, st);
+        case 104: return launch<4, 1>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, st);
+        default: bz::set_error("bz_yee3d_step: rows must be 2 or 4 (+100: two-plane issue variant)"); return BZ_EINVAL;
     }
 }

@@ -79,7 +79,8 @@ class Grid:
         self.power = None
         self.power_count = 0
         self.fused = None
-        self.kernel = "fused" if self.is_3d else "basic"
+        self.kernel = "fused"
+        self._f2d = None
         self.kseg = 16
         self.rows = 4
         self.timing = False
@@ -93,7 +94,7 @@ class Grid:
     def kernel_name(self):
         if self.is_3d:
             return f"bz_yee3d_step (fused z-march, {self.rows} rows/wave)" if self.kernel == "fused" else "bz_update_3d (two-pass)"
-        return "bz_update_h_2d + bz_update_e_2d"
+        return "bz_yee2d_run (fused, batched)" if self.kernel == "fused" else "bz_update_h_2d + bz_update_e_2d"
 
     def start_timing(self):
         self.timing = True
@@ -104,8 +105,9 @@ class Grid:
         if not self._events:
             return None
         self.torch.cuda.synchronize(self.dev)
-        ms = [a.elapsed_time(b) for a, b in self._events]
-        return float(np.mean(ms)) / 1e3
+        ms = sum(e[0].elapsed_time(e[1]) for e in self._events)
+        launches = sum(e[2] if len(e) > 2 else 1 for e in self._events)
+        return ms / launches / 1e3
 
     # ---------------- materials / views ----------------
     def set_materials(self, eps, sigma):
@@ -151,7 +153,9 @@ class Grid:
                            None if mul is None else T.from_numpy(np.asarray(mul, np.float32)).to(self.dev),
                            np.asarray(signal))
         src.box = self._as_box(comp, np.asarray(idx), np.asarray(w, np.float64), mul)
+        src.idx, src.w_host, src.mul_host = np.asarray(idx), np.asarray(w, np.float64), mul
         self.sources.append(src)
+        self._f2d = None
         return src
 
     def _as_box(self, comp, idx, w, mul, max_cells=1 << 22):
@@ -200,9 +204,11 @@ class Grid:
         else:
             p.phasor = p.dft = None
         self.monitors.append(p)
+        self._f2d = None
         return p
 
     def enable_power(self):
+        self._f2d = None
         self.power = self.torch.zeros(self.box if not self.is_3d else self.box, dtype=self.torch.float64,
                                       device=self.dev)
         self.power_count = 0
@@ -291,6 +297,8 @@ class Grid:
 
     def step(self, n):
         """Advance one step (step index n) in the reference order."""
+        if not self.is_3d and self.kernel == "fused" and self._fused2d(n, 1):
+            return
         if self.is_3d and self.kernel == "fused":
             if not self._fused3d(n):
                 self._sources(n)
@@ -301,10 +309,126 @@ class Grid:
         self._power()
 
     def run(self, n0, count):
-        if self.fused is not None:
-            done = self.fused.run(n0, count)
-            for n in range(n0 + done, n0 + count):
-                self.step(n)
+        if not self.is_3d and self.kernel == "fused" and self._fused2d(n0, count):
             return
         for n in range(n0, n0 + count):
             self.step(n)
+
+    # ---------------- fused, batched 2D path ----------------
+    def invalidate(self):
+        self._f2d = None
+
+    def _setup_fused2d(self):
+        """Descriptor, tile list and combined monitor tables for bz_yee2d_run (None if unsupported)."""
+        T = self.torch
+        if len(self.sources) > 1:
+            return None
+        plans = [p for p in self.monitors if p.n > 0]
+        if plans:
+            p0 = plans[0]
+            for p in plans[1:]:
+                if not np.array_equal(p.record_steps, p0.record_steps) or p.nfreq != p0.nfreq:
+                    return None
+                if p.nfreq and not T.equal(p.phasor, p0.phasor):
+                    return None
+        if self._fb[1] is None:
+            self._fb[1] = {k: T.zeros_like(v) for k, v in self._fb[0].items()}
+        st = {}
+        d = N.Sim2D()
+        d.nx, d.ny, d.ld = self.nx, self.ny, self.ld
+        d.eps, d.sig = self.eps.data_ptr(), self.sig.data_ptr()
+        c = (self.dt, MU_0, EPS_0)
+        d.kh = (MU_0 / EPS_0) * self.dt / (2.0 * MU_0)
+        d.kbh, d.ke, d.kbe = self.dt / MU_0, self.dt / (2.0 * EPS_0), self.dt / EPS_0
+        d.rdx, d.rdy = 1.0 / self.dx, 1.0 / self.dy
+        d.src_sx = 0
+        if self.sources:
+            s = self.sources[0]
+            idx = s.idx
+            lo = idx.min(axis=0); hi = idx.max(axis=0) + 1
+            sy, sx = (int(v) for v in (hi - lo))
+            add = np.zeros((sy, sx), np.float32); mul = np.ones((sy, sx), np.float32)
+            rel = idx - lo
+            add[rel[:, 0], rel[:, 1]] = s.w_host
+            if s.mul_host is not None:
+                mul[rel[:, 0], rel[:, 1]] = s.mul_host
+            st["add"] = T.from_numpy(add.ravel()).to(self.dev)
+            st["mul"] = T.from_numpy(mul.ravel()).to(self.dev) if s.mul_host is not None else None
+            st["sig"] = T.from_numpy(np.real(s.signal).astype(np.float32)).to(self.dev)
+            d.src_y0, d.src_x0, d.src_sy, d.src_sx = int(lo[0]), int(lo[1]), sy, sx
+            d.src_add = st["add"].data_ptr()
+            d.src_mul = st["mul"].data_ptr() if st["mul"] is not None else None
+            d.signal = st["sig"].data_ptr()
+        d.power = self.power.data_ptr() if self.power is not None else None
+        st["desc"] = d
+        gx = (self.nx + 62) // 63
+        seg = 8
+        gy = (self.ny + seg - 1) // seg
+        tiles = np.array([(0, tx, ty) for ty in range(gy) for tx in range(gx)], dtype=np.int32)
+        st["tiles"] = T.from_numpy(tiles.ravel()).to(self.dev)
+        st["ntiles"], st["seg"] = len(tiles), seg
+        st["step"] = T.zeros(1, dtype=T.int32, device=self.dev)
+        st["npts"] = 0
+        if plans:
+            ntot = sum(p.n for p in plans)
+            comp = T.cat([p.comp for p in plans]).cpu().numpy()
+            off = T.cat([p.off for p in plans]).cpu().numpy()
+            pts = np.stack([np.zeros_like(comp), comp, off.astype(np.int32)], axis=1).astype(np.int32)
+            st["pts"] = T.from_numpy(pts.ravel()).to(self.dev)
+            slot = np.full(max(int(p0.record_steps.max()) + 1 if len(p0.record_steps) else 1, 1), -1, np.int32)
+            slot[p0.record_steps] = np.arange(len(p0.record_steps), dtype=np.int32)
+            st["nslot"] = len(slot)
+            st["slot"] = T.from_numpy(slot).to(self.dev)
+            nrec = max(len(p0.record_steps), 1)
+            keep = all(p.series is not None for p in plans)
+            st["series"] = T.zeros((nrec, ntot), dtype=T.float32, device=self.dev) if keep else None
+            st["dft"] = T.zeros((p0.nfreq, ntot, 2), dtype=T.float64, device=self.dev) if p0.nfreq else None
+            st["phasor"] = p0.phasor
+            c0 = 0
+            for p in plans:
+                if keep:
+                    p.series = st["series"][:, c0:c0 + p.n]
+                if p0.nfreq:
+                    p.dft = st["dft"][:, c0:c0 + p.n, :]
+                c0 += p.n
+            st["npts"], st["nfreq"] = ntot, p0.nfreq
+        st["shift"] = None
+        return st
+
+    def _fused2d(self, n0, count):
+        if getattr(self, "_f2d", None) is None:
+            self._f2d = self._setup_fused2d()
+            if self._f2d is None:
+                self._f2d = False
+        st = self._f2d
+        if st is False:
+            return False
+        T = self.torch
+        shift = (self._cur - (n0 & 1)) % 2
+        if st["shift"] != shift:
+            d = st["desc"]
+            for p in (0, 1):
+                b = self._fb[(p + shift) % 2]
+                d.ez[p], d.hx[p], d.hy[p] = b["Ez"].data_ptr(), b["Hx"].data_ptr(), b["Hy"].data_ptr()
+            st["dev"] = T.frombuffer(bytearray(bytes(d)), dtype=T.uint8).to(self.dev)
+            st["shift"] = shift
+        if st["npts"] and n0 + count > st["nslot"]:
+            pad = T.full((n0 + count - st["nslot"],), -1, dtype=T.int32, device=self.dev)
+            st["slot"] = T.cat([st["slot"], pad])
+            st["nslot"] = int(st["slot"].numel())
+        st["step"].fill_(n0)
+        strm = self.stream
+        if self.timing:
+            e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
+            e0.record()
+        sp = lambda k: st[k].data_ptr() if st.get(k) is not None else None
+        N.call("bz_yee2d_run", st["dev"].data_ptr(), 1, st["tiles"].data_ptr(), st["ntiles"], st["seg"],
+               st["step"].data_ptr(), count, sp("pts"), st["npts"], sp("slot"), sp("series"), st.get("nfreq", 0),
+               sp("phasor"), sp("dft"), strm)
+        if self.timing:
+            e1.record()
+            self._events.append((e0, e1, count))
+        self._cur = (self._cur + count) % 2
+        if self.power is not None:
+            self.power_count += count
+        return True

@@ -220,6 +220,7 @@ class FDTD:
     def _plan_sources(self):
         g = self.grid
         g.sources = []
+        g.invalidate()
         ez_nz = self.nz - 1 if self.is_3d else None
         for src in self.sources:
             sig = np.asarray(src.signal)
@@ -302,6 +303,7 @@ class FDTD:
     def _plan_monitors(self, t_all):
         g = self.grid
         g.monitors = []
+        g.invalidate()
         self._plans = []
         for mon in self.monitors:
             comps, offs, layout = self._monitor_samples(mon)
@@ -419,6 +421,7 @@ class FDTD:
             self.grid.enable_power()
         else:
             self.grid.power = None
+        self.grid.invalidate()
         self._t_all = self._time_table()
         self._plan_sources()
         self._plan_monitors(self._t_all)

@@ -85,6 +85,35 @@ int bz_yee3d_step(float* const src[6], float* const dst[6], const float* eps_r, 
                   int nx, int ny, int nz, int ld, double dx, double dy, double dz, double dt,
                   double mu0, double eps0, const bz_box_source* boxes, int nbox, int kseg, int rows, void* stream);
 
+/* ---------------- fused, batched 2D engine ----------------
+ * Runs nsteps full steps (numpy_backend.py:34-88, then the source box, then optional grid power,
+ * then monitor sampling) of every simulation described in `sims` (device array), starting at the
+ * step held in device memory `step` (incremented once per step). Field buffers are ping-ponged by
+ * step parity: step n reads buffer n & 1 and writes buffer (n + 1) & 1.
+ * tiles: device int[3*ntiles] = (sim, x-tile of 63 columns, y-tile of `seg` rows).
+ * pts: device int[3*npts] = (sim, comp 0 Ez / 1 Hx / 2 Hy, offset); slot: device int[steps],
+ * slot[n] = recorded row of step n or -1; series [rows][npts] FP32 (may be NULL);
+ * phasor [rows][nfreq] complex FP64; dft [nfreq][npts] complex FP64 accumulators. */
+typedef struct bz_sim2d {
+    int nx, ny, ld;
+    float* ez[2];
+    float* hx[2];
+    float* hy[2];
+    const float* eps;
+    const float* sig;
+    float kh, kbh, ke, kbe;   /* dt/(2 eps0), dt/mu0, dt/(2 eps0), dt/eps0 */
+    float rdx, rdy;
+    int src_x0, src_y0, src_sx, src_sy;   /* box source on Ez (src_sx = 0: none) */
+    const float* src_add;
+    const float* src_mul;                 /* NULL means 1 */
+    const float* signal;                  /* device FP32 amplitude per step (NULL: no source) */
+    double* power;                        /* NULL or [ny][ld] FP64 grid-power accumulator */
+} bz_sim2d;
+
+int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step, int nsteps,
+                 const int* pts, int npts, const int* slot, float* series, int nfreq, const double* phasor,
+                 double* dft, void* stream);
+
 /* ---------------- sources ---------------- */
 /* field[off[p]] += w[p] * amp for p < n (offsets unique within one call). */
 int bz_add_points(float* field, const int64_t* off, const float* w, int n, float amp, void* stream);

@@ -111,7 +111,7 @@ def test_fused_3d_kernel_segments_and_oracle(hip):
     from beamz_amd.workloads import soi_mmi_3d
     res = {}
     ref = None
-    for kern, kseg, rows in (("basic", 16, 4), ("fused", 5, 4), ("fused", 16, 4), ("fused", 7, 2), ("fused", 16, 8)):
+    for kern, kseg, rows in (("basic", 16, 4), ("fused", 5, 4), ("fused", 16, 4), ("fused", 7, 2), ("fused", 37, 2)):
         mesh, d3, t, mons = soi_mmi_3d(nx=131, ny=45, nz=37, steps=120, npml=6, dx=40e-9)
         sim = bz.FDTD(mesh, time=t)
         sim.grid.kernel, sim.grid.kseg, sim.grid.rows = kern, kseg, rows
@@ -126,3 +126,68 @@ def test_fused_3d_kernel_segments_and_oracle(hip):
     for key in res:
         for k in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
             assert rel_l2(res[key][k], ref.F[k]) < 1e-4, (key, k)
+
+
+def _wg_sim(kernel, nsteps=400, two_monitors=True, freqs_b=None):
+    d, dx = waveguide_design_2d(nx=150, ny=70, npml=10)
+    dt = 0.99 * dx / (bz.LIGHT_SPEED * np.sqrt(2))
+    t = np.arange(nsteps) * dt
+    d += bz.GaussianSource(position=(25 * dx, d.height / 2 + 0.3 * dx), width=0.6 * dx, signal=pulse(t))
+    fa = bz.LIGHT_SPEED / np.array([1.5e-6, 1.55e-6, 1.6e-6])
+    m1 = bz.Monitor(design=d, start=(100 * dx, 10 * dx), end=(100 * dx, d.height - 10 * dx), dft_frequencies=fa)
+    d += m1
+    mons = [m1]
+    if two_monitors:
+        m2 = bz.Monitor(design=d, start=(40 * dx, 20 * dx), end=(140 * dx, 50 * dx),
+                        dft_frequencies=fa if freqs_b is None else freqs_b)
+        d += m2
+        mons.append(m2)
+    sim = bz.FDTD(d, time=t, resolution=dx)
+    sim.grid.kernel = kernel
+    return sim, d, mons, t, dx, dt
+
+
+@pytest.mark.parametrize("freqs_b", [None, [2.0e14]])
+def test_fused_2d_matches_basic_and_oracle(hip, freqs_b):
+    out = {}
+    for kern in ("basic", "fused"):
+        sim, d, mons, t, dx, dt = _wg_sim(kern, freqs_b=freqs_b)
+        sim.initialize_simulation(save=False, live=False, accumulate_power=True)
+        for _ in range(7):          # single steps, then a long batched run
+            sim.step()
+        sim._advance(len(t))
+        sim.finalize_simulation()
+        out[kern] = (sim, mons)
+    sim_b, mons_b = out["basic"]
+    sim_f, mons_f = out["fused"]
+    ref = R.RefSim(sim_f.epsilon_r, sim_f.sigma, dx, dx, dt, len(t), sources=sim_f.sources, monitors=mons_f,
+                   accumulate_power=True).run()
+    for k in ("Ez", "Hx", "Hy"):
+        assert rel_l2(np.real(getattr(sim_f, k)), ref.F[k]) < 1e-4, k
+        assert rel_l2(np.real(getattr(sim_f, k)), np.real(getattr(sim_b, k))) < 1e-5, k
+    for q, (mf, mb) in enumerate(zip(mons_f, mons_b)):
+        ser = np.array(mf.fields["Ez"])
+        assert rel_l2(ser, np.array(ref.mon_series[q]["Ez"])) < 1e-4
+        assert rel_l2(ser, np.array(mb.fields["Ez"])) < 1e-5
+        for k in ("Ez", "Hx", "Hy"):
+            F = R.dft_of_series(np.array(ref.mon_series[q][k]), ref.mon_series[q]["t"], 2 * np.pi * mf.dft_frequencies)
+            assert rel_l2(mf.dft[k], F) < 1e-4, (q, k)
+    assert rel_l2(sim_f.power_accumulated, ref.power_accumulated / ref.power_count) < 2e-4
+
+
+def test_mode_source_2d_matches_oracle(hip):
+    um = 1e-6
+    lam = 1.55 * um
+    d, dx = waveguide_design_2d(nx=140, ny=70, npml=10)
+    dt = 0.99 * dx / (bz.LIGHT_SPEED * np.sqrt(2))
+    t = np.arange(300) * dt
+    sig = bz.ramped_cosine(t, 1.0, bz.LIGHT_SPEED / lam, 0.0, ramp_duration=20 * dt, t_max=t[-1])
+    src = bz.ModeSource(design=d, start=(20 * dx, d.height / 2 - 1.0 * um), end=(20 * dx, d.height / 2 + 1.0 * um),
+                        wavelength=lam, signal=sig, num_modes=1, grid_resolution=400)
+    d += src
+    for enforce in (True, False):
+        src.enforce_direction = enforce
+        sim = bz.FDTD(d, time=t, resolution=dx)
+        sim.run(live=False, save=False)
+        ref = R.RefSim(sim.epsilon_r, sim.sigma, dx, dx, dt, len(t), sources=[src]).run()
+        assert rel_l2(np.real(sim.Ez), ref.F["Ez"]) < 1e-4, enforce
