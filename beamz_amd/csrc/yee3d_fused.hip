@@ -17,9 +17,12 @@
 //     Ez, Hx, Hy, Hz, sigma, eps) once; Ex/Ey of the current plane and Hx/Hy
 //     of the previous plane stay in registers; x neighbours move between
 //     lanes by wave shuffles, y neighbours between register rows.
-//   * the loads of plane k+1 are issued before the stores of plane k: on gfx9
-//     loads and stores share one in-order completion counter, so waiting for
-//     loads issued after the stores would also wait for the stores.
+//   * both planes of a loop iteration are requested before any store of the
+//     iteration (gfx9 counts loads and stores on one in-order counter).
+//   * addressing is 32-bit: each lane keeps one clamped in-grid offset per
+//     register row, and a plane adds one wave-uniform offset; halo lanes/rows
+//     outside the grid read a clamped in-grid cell whose value only reaches
+//     entries that the validity selects discard or that are not stored.
 //   * a segment starting at k0 > 0 first recomputes H of plane k0-1 for its
 //     own columns (bit-identical to what the previous segment computes).
 //   * workgroups are 4 waves stacked in y; hardware workgroup ids are remapped
@@ -45,13 +48,6 @@ struct BoxSources {
     bz_box_source b[BZ_MAX_BOX];
 };
 
-// Branch-free guarded load: the address is clamped to element 0 when the cell is outside the grid, so
-// every lane issues the load and no exec-mask branch splits the memory stream. Out-of-grid lanes thus
-// hold a finite in-grid value instead of zero; such values only ever reach H/E entries that the
-// validity selects discard or that are not stored (halo lanes/rows, planes past nz), so no select is
-// applied here and the loaded value is not consumed before the plane that needs it.
-__device__ __forceinline__ float ldg(const float* __restrict__ p, long o, bool ok) { return p[ok ? o : 0]; }
-
 // Per-plane register set of one wave column block: row r = 0..R+1.
 template <int R>
 struct Plane {
@@ -66,44 +62,67 @@ struct Carry {
     float ex0[R + 2], ey0[R + 1], hxp[R + 1], hyp[R + 1];
 };
 
+template <int R>
 struct Geo {
-    int nx, ny, nz, ld, jb, i, k0;
-    long ps;
+    int nx, ny, nz, jb, i, k0;
+    uint32_t ps;
+    uint32_t ro[R + 2];      // clamped in-grid offset of each register row within a plane
     bool lane_out, xin, hyok;
 };
 
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Element at a 32-bit byte offset from a wave-uniform base: lets the compiler use the scalar-base +
+// 32-bit vector-offset addressing form instead of per-lane 64-bit address arithmetic.
+__device__ __forceinline__ float at(const float* __restrict__ base, uint32_t byte_off) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+__device__ __forceinline__ void put(float* __restrict__ base, uint32_t byte_off, float v) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
+__device__ __forceinline__ void h_coef_fast(float sig, const Coef& c, float& fa, float& fb) {
+    const float s = sig * c.kh;
+    const float r = fast_rcp(1.0f + s);
+    fa = (1.0f - s) * r;
+    fb = c.kbh * r;
+}
+
+__device__ __forceinline__ void e_coef_fast(float sig, float eps, const Coef& c, float& ca, float& cb) {
+    const float ie = fast_rcp(eps);
+    const float s = sig * c.ke * ie;
+    const float r = fast_rcp(1.0f + s);
+    ca = (1.0f - s) * r;
+    cb = c.kbe * ie * r;
+}
+
 template <int R>
 __device__ __forceinline__ void load_plane(Plane<R>& P, const Fields6& S, const float* __restrict__ eps,
-                                           const float* __restrict__ sig, int k, const Geo& g, const bool* col,
-                                           bool valid) {
-    // `valid` is false for the plane past the segment end: the loads are still issued (to element 0)
-    // so that both register planes are written on every path and the loop needs no register copies.
-    const long ok = (long)k * g.ps;
-    const bool nk = valid && k + 1 < g.nz;
-    const bool we = valid && k >= g.k0;
+                                           const float* __restrict__ sig, int k, const Geo<R>& g) {
+    // planes past the grid are clamped to the last one (their values feed only discarded entries)
+    const uint32_t kp = (uint32_t)min(k, g.nz - 1) * g.ps;
+    const uint32_t kq = (uint32_t)min(k + 1, g.nz - 1) * g.ps;
 #pragma unroll
     for (int r = 0; r < R + 2; ++r) {
-        const bool cv = col[r] && valid;
-        const long o = ok + (long)(g.jb + r) * g.ld + g.i;
-        P.ex1[r] = ldg(S.p[0], o + g.ps, cv && nk);
-        P.ez[r] = ldg(S.p[2], o, cv);
+        const uint32_t o = kp + g.ro[r], q = kq + g.ro[r];
+        P.ex1[r] = at(S.p[0], q);
+        P.ez[r] = at(S.p[2], o);
         if (r < R + 1) {
-            P.ey1[r] = ldg(S.p[1], o + g.ps, cv && nk);
-            P.hx[r] = ldg(S.p[3], o, cv);
-            P.hz[r] = ldg(S.p[5], o, cv);
-            P.sg[r] = ldg(sig, o, cv);
-            P.hy[r] = (r > 0) ? ldg(S.p[4], o, cv) : 0.0f;
-            P.ee[r] = (r > 0) ? ldg(eps, o, cv && we) : 1.0f;
+            P.ey1[r] = at(S.p[1], q);
+            P.hx[r] = at(S.p[3], o);
+            P.hz[r] = at(S.p[5], o);
+            P.sg[r] = at(sig, o);
+            P.hy[r] = (r > 0) ? at(S.p[4], o) : 0.0f;
+            P.ee[r] = (r > 0) ? at(eps, o) : 1.0f;
         }
     }
 }
 
 // One z-plane: H of plane k for rows 0..R, then (if k >= k0) E of rows 1..R, sources, stores.
 template <int R>
-__device__ __forceinline__ void march_plane(const Plane<R>& P, Carry<R>& C, int k, const Geo& g, const bool* col,
-                                            const Fields6& D, const Coef& c, float rdx, float rdy, float rdz,
-                                            const BoxSources& src) {
-    const long ok = (long)k * g.ps;
+__device__ __forceinline__ void march_plane(const Plane<R>& P, Carry<R>& C, int k, const Geo<R>& g,
+                                            const bool* col, const Fields6& D, const Coef& c, float rdx, float rdy,
+                                            float rdz, const BoxSources& src) {
     float hxn[R + 1], hyn[R + 1], hzn[R + 1];
     const bool kh = k < g.nz - 1;
 #pragma unroll
@@ -112,7 +131,7 @@ __device__ __forceinline__ void march_plane(const Plane<R>& P, Carry<R>& C, int 
         const float ezR = __shfl_down(P.ez[r], 1, LANES);
         const float eyR = __shfl_down(C.ey0[r], 1, LANES);
         float fa, fb;
-        h_coef(P.sg[r], c, fa, fb);
+        h_coef_fast(P.sg[r], c, fa, fb);
         const bool jh = j < g.ny - 1;
         hxn[r] = (kh && jh) ? fa * P.hx[r] - fb * ((P.ez[r + 1] - P.ez[r]) * rdy - (P.ey1[r] - C.ey0[r]) * rdz)
                             : P.hx[r];
@@ -124,16 +143,17 @@ __device__ __forceinline__ void march_plane(const Plane<R>& P, Carry<R>& C, int 
     if (k >= g.k0) {
         const bool kin = k >= 1 && k <= g.nz - 2;
         const int i = g.i;
+        const uint32_t kp = (uint32_t)k * g.ps;
 #pragma unroll
         for (int r = 1; r < R + 1; ++r) {
             const float hzL = __shfl_up(hzn[r], 1, LANES);
             const float hyL = __shfl_up(hyn[r], 1, LANES);
             const int j = g.jb + r;
             if (g.lane_out && col[r]) {
-                const long o = ok + (long)j * g.ld + i;
+                const uint32_t o = kp + g.ro[r];
                 const bool yin = j >= 1 && j <= g.ny - 2;
                 float ca, cb;
-                e_coef(P.sg[r], P.ee[r], c, ca, cb);
+                e_coef_fast(P.sg[r], P.ee[r], c, ca, cb);
                 float exn = (kin && yin && i <= g.nx - 2)
                                 ? ca * C.ex0[r] + cb * ((hzn[r] - hzn[r - 1]) * rdy - (hyn[r] - C.hyp[r]) * rdz)
                                 : C.ex0[r];
@@ -154,12 +174,12 @@ __device__ __forceinline__ void march_plane(const Plane<R>& P, Carry<R>& C, int 
                         else ezn = m * ezn + b.add[rr] * b.amp;
                     }
                 }
-                D.p[0][o] = exn;
-                D.p[1][o] = eyn;
-                D.p[2][o] = ezn;
-                D.p[3][o] = hxn[r];
-                D.p[4][o] = hyn[r];
-                D.p[5][o] = hzn[r];
+                put(D.p[0], o, exn);
+                put(D.p[1], o, eyn);
+                put(D.p[2], o, ezn);
+                put(D.p[3], o, hxn[r]);
+                put(D.p[4], o, hyn[r]);
+                put(D.p[5], o, hzn[r]);
             }
         }
     }
@@ -174,7 +194,7 @@ __device__ __forceinline__ void march_plane(const Plane<R>& P, Carry<R>& C, int 
     }
 }
 
-template <int R, int PIPE>
+template <int R>
 __global__ __launch_bounds__(LANES* WPG) void k_yee3d_fused(Fields6 S, Fields6 D, const float* __restrict__ eps,
                                                            const float* __restrict__ sig, int nx, int ny, int nz,
                                                            int ld, int kseg, int gx, int gy, int gz, int per_xcd,
@@ -187,61 +207,56 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee3d_fused(Fields6 S, Fields6 D
     const int ty = (tile / gx) % gy;
     const int tz = tile / (gx * gy);
 
-    Geo g;
-    g.nx = nx; g.ny = ny; g.nz = nz; g.ld = ld;
+    Geo<R> g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
     const int l = threadIdx.x;
     g.i = tx * OX - 1 + l;
     g.jb = (ty * WPG + threadIdx.y) * R - 1;   // row 0 of this wave (halo row)
     g.k0 = tz * kseg;
-    g.ps = (long)ny * ld;
+    g.ps = (uint32_t)ny * (uint32_t)ld * 4u;     // plane stride in bytes
     const int k1 = min(nz, g.k0 + kseg);
     const bool ci = g.i >= 0 && g.i < nx;
     g.lane_out = l > 0 && l < LANES - 1 && ci;
     g.xin = g.i >= 1 && g.i <= nx - 2;
     g.hyok = g.i < nx - 1;
-
+    const int ic = min(max(g.i, 0), nx - 1);
     bool col[R + 2];
 #pragma unroll
-    for (int r = 0; r < R + 2; ++r) col[r] = ci && (g.jb + r) >= 0 && (g.jb + r) < ny;
+    for (int r = 0; r < R + 2; ++r) {
+        const int j = g.jb + r;
+        col[r] = ci && j >= 0 && j < ny;
+        g.ro[r] = ((uint32_t)min(max(j, 0), ny - 1) * (uint32_t)ld + (uint32_t)ic) * 4u;
+    }
 
     const int kstart = g.k0 > 0 ? g.k0 - 1 : g.k0;
     Carry<R> C;
+    {
+        const uint32_t kp = (uint32_t)kstart * g.ps;
 #pragma unroll
-    for (int r = 0; r < R + 2; ++r) {
-        const long o = (long)kstart * g.ps + (long)(g.jb + r) * ld + g.i;
-        C.ex0[r] = ldg(S.p[0], o, col[r]);
-        if (r < R + 1) {
-            C.ey0[r] = ldg(S.p[1], o, col[r]);
-            C.hxp[r] = 0.0f;
-            C.hyp[r] = 0.0f;
+        for (int r = 0; r < R + 2; ++r) {
+            C.ex0[r] = at(S.p[0], kp + g.ro[r]);
+            if (r < R + 1) {
+                C.ey0[r] = at(S.p[1], kp + g.ro[r]);
+                C.hxp[r] = 0.0f;
+                C.hyp[r] = 0.0f;
+            }
         }
     }
+    // Both planes of an iteration are requested before any store of the iteration: the second plane's
+    // latency hides behind the first plane's compute, and no loaded plane crosses the loop back-edge.
     Plane<R> A, B;
-    if (PIPE == 0) {
-        // Two register planes alternate so that plane k+1 is in flight while plane k is computed and stored.
-        load_plane<R>(A, S, eps, sig, kstart, g, col, true);
-        for (int k = kstart; k < k1; k += 2) {
-            load_plane<R>(B, S, eps, sig, k + 1, g, col, k + 1 < k1);
-            march_plane<R>(A, C, k, g, col, D, c, rdx, rdy, rdz, src);
-            if (k + 1 >= k1) break;
-            load_plane<R>(A, S, eps, sig, k + 2, g, col, k + 2 < k1);
-            march_plane<R>(B, C, k + 1, g, col, D, c, rdx, rdy, rdz, src);
-        }
-    } else {
-        // Both planes of an iteration are requested before any store of the iteration: the second
-        // plane's latency hides behind the first plane's compute, and no loaded plane is carried across
-        // the loop back-edge.
-        for (int k = kstart; k < k1; k += 2) {
-            load_plane<R>(A, S, eps, sig, k, g, col, true);
-            load_plane<R>(B, S, eps, sig, k + 1, g, col, k + 1 < k1);
-            march_plane<R>(A, C, k, g, col, D, c, rdx, rdy, rdz, src);
-            if (k + 1 >= k1) break;
-            march_plane<R>(B, C, k + 1, g, col, D, c, rdx, rdy, rdz, src);
-        }
+    for (int k = kstart; k < k1; k += 2) {
+        load_plane<R>(A, S, eps, sig, k, g);
+        load_plane<R>(B, S, eps, sig, k + 1, g);
+        march_plane<R>(A, C, k, g, col, D, c, rdx, rdy, rdz, src);
+        if (k + 1 >= k1) break;
+        march_plane<R>(B, C, k + 1, g, col, D, c, rdx, rdy, rdz, src);
     }
 }
 
-template <int R, int PIPE>
+template <int R>
 int launch(const Fields6& S, const Fields6& D, const float* eps, const float* sig, int nx, int ny, int nz, int ld,
            int kseg, const Coef& c, float rdx, float rdy, float rdz, const BoxSources& bs, hipStream_t st) {
     const int gx = (nx + OX - 1) / OX;
@@ -249,8 +264,9 @@ int launch(const Fields6& S, const Fields6& D, const float* eps, const float* si
     const int gz = (nz + kseg - 1) / kseg;
     const int n = gx * gy * gz;
     const int per_xcd = (n + NXCD - 1) / NXCD;
-    hipLaunchKernelGGL(k_yee3d_fused<R, PIPE>, dim3(per_xcd * NXCD), dim3(LANES, WPG), 0, st, S, D, eps, sig, nx, ny, nz,
-                       ld, kseg, gx, gy, gz, per_xcd, c, rdx, rdy, rdz, bs);
+    auto kern = k_yee3d_fused<R>;
+    hipLaunchKernelGGL(kern, dim3(per_xcd * NXCD), dim3(LANES, WPG), 0, st, S, D, eps, sig, nx, ny, nz, ld, kseg, gx,
+                       gy, gz, per_xcd, c, rdx, rdy, rdz, bs);
     BZ_CHECK_LAUNCH("bz_yee3d_step");
     return BZ_OK;
 }
@@ -263,6 +279,7 @@ extern "C" int bz_yee3d_step(float* const src[6], float* const dst[6], const flo
     BZ_REQUIRE(src && dst && eps_r && sigma, "bz_yee3d_step: null pointer");
     for (int q = 0; q < 6; ++q) BZ_REQUIRE(src[q] && dst[q] && src[q] != dst[q], "bz_yee3d_step: bad field buffers");
     BZ_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2 && ld >= nx, "bz_yee3d_step: bad shape");
+    BZ_REQUIRE(4.0 * nz * ny * ld < 4294967295.0, "bz_yee3d_step: a field array exceeds 4 GiB (32-bit offsets)");
     BZ_REQUIRE(nbox >= 0 && nbox <= BZ_MAX_BOX && (nbox == 0 || boxes), "bz_yee3d_step: bad source list");
     BoxSources bs;
     bs.n = nbox;
@@ -283,10 +300,8 @@ extern "C" int bz_yee3d_step(float* const src[6], float* const dst[6], const flo
     const float rx = (float)(1.0 / dx), ry = (float)(1.0 / dy), rz = (float)(1.0 / dz);
     const hipStream_t st = (hipStream_t)stream;
     switch (rows) {
-        case 2: return launch<2, 0>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, st);
-        case 4: return launch<4, 0>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, st);
-        case 102: return launch<2, 1>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, st);
-        case 104: return launch<4, 1>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, st);
-        default: bz::set_error("bz_yee3d_step: rows must be 2 or 4 (+100: two-plane issue variant)"); return BZ_EINVAL;
+        case 2: return launch<2>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, st);
+        case 4: return launch<4>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, st);
+        default: bz::set_error("bz_yee3d_step: rows must be 2 or 4"); return BZ_EINVAL;
     }
 }

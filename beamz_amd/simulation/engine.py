@@ -48,6 +48,8 @@ class _MonitorPlan:
 class Grid:
     """Device arrays of one simulation plus the work list of one step."""
 
+    STAGGER = 1088      # floats between consecutive arrays beyond their size (4 KiB + 256 B)
+
     def __init__(self, backend, eps, sigma, dx, dy, dt, dz=None):
         self.backend = backend
         torch = backend.torch
@@ -68,10 +70,21 @@ class Grid:
         box = (self.nz, self.ny, self.ld) if self.is_3d else (self.ny, self.ld)
         self.box = box
         self.names = COMP3 if self.is_3d else COMP2
-        self._fb = [{k: torch.zeros(box, dtype=torch.float32, device=self.dev) for k in self.names}, None]
+        # All arrays of one simulation (two field sets for ping-pong, permittivity, conductivity) share
+        # one allocation, their bases staggered by 4 KiB + 256 B: the update kernels read every array at
+        # the same element offset at once, and equally aligned bases would put those concurrent streams
+        # on the same HBM channels.
+        nbox = int(np.prod(box))
+        stride = nbox + self.STAGGER
+        narr = 2 * len(self.names) + 2
+        self._flat = torch.zeros(narr * stride, dtype=torch.float32, device=self.dev)
+        views = [self._flat[q * stride: q * stride + nbox].view(box) for q in range(narr)]
+        nc = len(self.names)
+        self._fb = [dict(zip(self.names, views[:nc])), dict(zip(self.names, views[nc:2 * nc]))]
         self._cur = 0
-        self.eps = torch.ones(box, dtype=torch.float32, device=self.dev)
-        self.sig = torch.zeros(box, dtype=torch.float32, device=self.dev)
+        self.eps = views[2 * nc]
+        self.eps.fill_(1.0)
+        self.sig = views[2 * nc + 1]
         self.set_materials(eps, sigma)
         self.shapes = ref_shapes(self.nx, self.ny, self.nz)
         self.sources = []
@@ -269,8 +282,6 @@ class Grid:
 
     def _fused3d(self, n):
         """One fused step; returns True if the sources were applied inside the kernel."""
-        if self._fb[1] is None:
-            self._fb[1] = {k: self.torch.zeros_like(v) for k, v in self._fb[0].items()}
         boxed = len(self.sources) <= N.MAX_BOX and all(s.box is not None for s in self.sources)
         arr = (N.BoxSource * N.MAX_BOX)()
         nb = 0
@@ -331,8 +342,6 @@ class Grid:
                     return None
                 if p.nfreq and not T.equal(p.phasor, p0.phasor):
                     return None
-        if self._fb[1] is None:
-            self._fb[1] = {k: T.zeros_like(v) for k, v in self._fb[0].items()}
         st = {}
         d = N.Sim2D()
         d.nx, d.ny, d.ld = self.nx, self.ny, self.ld

@@ -148,6 +148,7 @@ class FDTD:
             self.nz, self.ny, self.nx = mesh_obj.permittivity.shape
         else:
             self.ny, self.nx = mesh_obj.permittivity.shape
+            self.nz = None
         self.epsilon_r = mesh_obj.permittivity
         self.mu_r = mesh_obj.permeability
         self.sigma = mesh_obj.conductivity

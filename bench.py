@@ -1,16 +1,20 @@
-"""Yee cell-update throughput of the 3D SOI 1x2 MMI workload on one MI355X.
+"""Yee cell-update throughput on one MI355X.
 
-Workload (default): 480 x 240 x 160 cells at 25 nm, six FP32 field components,
-SOI stack with a seeded MMI, 12-cell PML, dipole Gaussian-pulse source, two
-x-normal DFT flux ports (8 wavelengths). A step is one full Yee step (H and E
-updates, source injection, port DFT accumulation) over the whole grid, with
-all inputs resident in HBM.
+Default workload (the headline): the 3D SOI 1x2 MMI, 480 x 240 x 160 cells at
+25 nm, six FP32 field components, seeded MMI, 12-cell PML, dipole
+Gaussian-pulse source, two x-normal DFT flux ports (8 wavelengths). A step is
+one full Yee step (H and E updates, source injection, port DFT accumulation)
+over the whole grid, with all inputs resident in HBM.
 
-Prints one JSON line: value = cell-updates/s; roofline = the fused Yee
-kernel's algorithmic HBM bytes (56 B per cell-update: six FP32 components read
-and written, FP32 permittivity and conductivity read) over its measured mean
-launch time; cpu_baseline = the FP64 NumPy restatement of the reference's 3D
-step (complex128 fields, as the reference) timed on a bounded sample.
+``--workload wg2d`` runs the 2D TM straight-waveguide case instead (800 x 400
+cells, 20-cell PML, point source, 16-wavelength DFT line monitor).
+
+Prints one JSON line: value = cell-updates/s; roofline = the update kernel's
+algorithmic HBM bytes (3D: 56 B per cell-update, six FP32 components read and
+written plus FP32 permittivity and conductivity read; 2D: 32 B) over its mean
+launch time measured with HIP events on the launch stream; cpu_baseline = the
+FP64 NumPy restatement of the reference's step (complex128 fields, as the
+reference computes) timed on a bounded sample on one host core.
 """
 from __future__ import annotations
 
@@ -26,11 +30,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
-B_ALG_3D = 56              # bytes per 3D cell-update
-B_ALG_2D = 32
+B_ALG = {"mmi3d": 56, "wg2d": 32}
 
 
-def cpu_baseline(nx, ny, nz, dx, dt, seconds_target=15.0):
+def cpu_baseline_3d(nx, ny, nz, dx, dt, seconds_target=15.0):
     """FP64 NumPy restatement of fdtd.py:200-310 (complex128 fields), timed on 1 core."""
     from oracle import fdtd_ref as R
     rng = np.random.default_rng(0)
@@ -48,18 +51,35 @@ def cpu_baseline(nx, ny, nz, dx, dt, seconds_target=15.0):
     return nx * ny * nz * n / el, n, el
 
 
+def cpu_baseline_2d(sim, seconds_target=15.0):
+    """The reference-order FP64 step (oracle RefSim, complex128 fields, sources + monitors) on 1 core."""
+    from oracle import fdtd_ref as R
+    ref = R.RefSim(sim.epsilon_r, sim.sigma, sim.dx, sim.dy, sim.dt, sim.num_steps, sources=sim.sources,
+                   monitors=sim.monitors, dtype=np.complex128)
+    ref.step()
+    n, t0 = 0, time.perf_counter()
+    while n < sim.num_steps - 1:
+        ref.step()
+        n += 1
+        el = time.perf_counter() - t0
+        if el > seconds_target:
+            break
+    return sim.nx * sim.ny * n / el, n, el
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--nx", type=int, default=480)
-    ap.add_argument("--ny", type=int, default=240)
+    ap.add_argument("--workload", choices=sorted(B_ALG), default="mmi3d")
+    ap.add_argument("--nx", type=int, default=None)
+    ap.add_argument("--ny", type=int, default=None)
     ap.add_argument("--nz", type=int, default=160)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", default=os.environ.get("BEAMZ_AMD_KERNEL3D", "fused"))
+    ap.add_argument("--kernel", default=os.environ.get("BEAMZ_AMD_KERNEL", "fused"))
     ap.add_argument("--kseg", type=int, default=int(os.environ.get("BEAMZ_AMD_KSEG", "16")))
-    ap.add_argument("--rows", type=int, default=int(os.environ.get("BEAMZ_AMD_ROWS", "4")))
+    ap.add_argument("--rows", type=int, default=int(os.environ.get("BEAMZ_AMD_ROWS", "2")))
     ap.add_argument("--timing-steps", type=int, default=30)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -67,11 +87,22 @@ def main():
         sys.exit(2)
     import torch
     import beamz_amd as bz
-    from beamz_amd.workloads import soi_mmi_3d
+    from beamz_amd.workloads import soi_mmi_3d, waveguide_2d
 
     total = a.warmup + a.steps + a.timing_steps
-    mesh, d3, t, mons = soi_mmi_3d(a.nx, a.ny, a.nz, steps=max(total, 16))
-    sim = bz.FDTD(mesh, time=t)
+    if a.workload == "mmi3d":
+        nx, ny, nz = a.nx or 480, a.ny or 240, a.nz
+        mesh, d3, t, mons = soi_mmi_3d(nx, ny, nz, steps=max(total, 16))
+        sim = bz.FDTD(mesh, time=t)
+        cells = nx * ny * nz
+        desc = (f"3D SOI 1x2 MMI Yee step {nx}x{ny}x{nz} @25nm, 12-cell PML, dipole pulse, "
+                f"2 DFT flux ports x 8 wavelengths")
+    else:
+        nx, ny = a.nx or 800, a.ny or 400
+        d, dx, t, mon = waveguide_2d(nx, ny, steps=max(total, 16))
+        sim = bz.FDTD(d, time=t, resolution=dx)
+        cells = nx * ny
+        desc = f"2D TM Si waveguide Yee step {nx}x{ny} @lambda/20, 20-cell PML, point pulse, 16-wavelength DFT line"
     sim.grid.kernel = a.kernel
     sim.grid.kseg = a.kseg
     sim.grid.rows = a.rows
@@ -87,32 +118,33 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_s = ev0.elapsed_time(ev1) / 1e3
-    cells = a.nx * a.ny * a.nz
     value = cells * a.steps / wall
-    # kernel duration: HIP events around each update launch, on the launch stream, in a separate run
+    # update-kernel duration: HIP events around each launch, on the launch stream, in a separate run
     sim.grid.start_timing()
     sim._advance(min(a.timing_steps, max(0, sim.num_steps - sim.current_step)))
     lt = sim.grid.launch_timer()
     sim.grid.timing = False
     step_kernel_s = lt if lt else gpu_s / a.steps
-    achieved = cells * B_ALG_3D / step_kernel_s
+    b_alg = B_ALG[a.workload]
+    achieved = cells * b_alg / step_kernel_s
     out = {
-        "metric": "Yee cell-updates/s (3D six-component FP32 step)", "value": value, "unit": "cell-updates/s",
+        "metric": "Yee cell-updates/s", "value": value, "unit": "cell-updates/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"3D SOI 1x2 MMI Yee step {a.nx}x{a.ny}x{a.nz} @25nm, 12-cell PML, dipole pulse, "
-                               f"2 DFT flux ports x 8 wavelengths", "cells": cells,
-                   "kernel": sim.grid.kernel_name()},
+        "config": {"workload": desc, "cells": cells, "kernel": sim.grid.kernel_name(), "gpu_event_s": gpu_s},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel_ms": step_kernel_s * 1e3, "bytes_per_cell_update": B_ALG_3D},
+                     "kernel_ms": step_kernel_s * 1e3, "bytes_per_cell_update": b_alg},
     }
     if not a.no_cpu:
-        dt = float(t[1] - t[0])
-        v, n, el = cpu_baseline(a.nx, a.ny, max(4, a.nz // 8), 25e-9, dt)
-        out["cpu_baseline"] = {"value": v, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-                               "sample": f"{n} steps of the NumPy complex128 3D Yee update on "
-                                         f"{a.nx}x{a.ny}x{max(4, a.nz // 8)} cells ({el:.1f} s)"}
+        if a.workload == "mmi3d":
+            nzs = max(4, nz // 8)
+            v, n, el = cpu_baseline_3d(nx, ny, nzs, 25e-9, float(t[1] - t[0]))
+            sample = f"{n} steps of the NumPy complex128 3D Yee update on {nx}x{ny}x{nzs} cells ({el:.1f} s)"
+        else:
+            v, n, el = cpu_baseline_2d(sim)
+            sample = f"{n} reference-order complex128 steps (update, source, DFT-line samples) on {nx}x{ny} ({el:.1f} s)"
+        out["cpu_baseline"] = {"value": v, "unit": "cell-updates/s", "cores": 1, "kind": "port", "sample": sample}
     print(json.dumps(out))
 
 
