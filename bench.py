@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--kseg", type=int, default=int(os.environ.get("BEAMZ_AMD_KSEG", "16")))
     ap.add_argument("--rows", type=int, default=int(os.environ.get("BEAMZ_AMD_ROWS", "2")))
     ap.add_argument("--timing-steps", type=int, default=30)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (3 launches) as a counter calibration kernel")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -107,6 +109,14 @@ def main():
     sim.grid.kseg = a.kseg
     sim.grid.rows = a.rows
     sim.initialize_simulation(save=False, live=False)
+    if a.copy_mib > 0:
+        # known traffic: each launch reads and writes copy_mib MiB (FETCH_SIZE / WRITE_SIZE calibration)
+        src = torch.ones(a.copy_mib * (1 << 18), device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)          # one elementwise kernel: reads and writes copy_mib MiB
+        torch.cuda.synchronize()
+        del src, dst
     sim._advance(a.warmup)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
