@@ -32,6 +32,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 B_ALG = {"mmi3d": 56, "wg2d": 32}
 
+# HBM bytes per launch of the update kernel, from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of
+# this script with --copy-mib 512, each counter scaled by the factor that makes the 512 MiB calibration
+# copy read its known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0). Keyed by (workload, kernel, rows, kseg,
+# nx, ny, nz); a run with any other configuration reports traffic as null. Source: the counter summary
+# profiles/yee3d_fused_mmi_480x240x160_hbm_counters.md.
+MEASURED_TRAFFIC = {
+    ("mmi3d", "fused", 2, 16, 480, 240, 160): (874_454 + 476_068) * 1024,
+}
+
 
 def cpu_baseline_3d(nx, ny, nz, dx, dt, seconds_target=15.0):
     """FP64 NumPy restatement of fdtd.py:200-310 (complex128 fields), timed on 1 core."""
@@ -137,13 +146,15 @@ def main():
     step_kernel_s = lt if lt else gpu_s / a.steps
     b_alg = B_ALG[a.workload]
     achieved = cells * b_alg / step_kernel_s
+    key = (a.workload, a.kernel, a.rows, a.kseg, nx, ny, nz if a.workload == "mmi3d" else 1)
+    traffic = MEASURED_TRAFFIC.get(key)
     out = {
         "metric": "Yee cell-updates/s", "value": value, "unit": "cell-updates/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": desc, "cells": cells, "kernel": sim.grid.kernel_name(), "gpu_event_s": gpu_s},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel_ms": step_kernel_s * 1e3, "bytes_per_cell_update": b_alg},
     }
     if not a.no_cpu:
