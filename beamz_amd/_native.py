@@ -40,7 +40,17 @@ class Sim2D(C.Structure):
     _fields_ = [("nx", i32), ("ny", i32), ("ld", i32), ("ez", vp * 2), ("hx", vp * 2), ("hy", vp * 2),
                 ("eps", vp), ("sig", vp), ("kh", f32), ("kbh", f32), ("ke", f32), ("kbe", f32),
                 ("rdx", f32), ("rdy", f32), ("src_x0", i32), ("src_y0", i32), ("src_sx", i32), ("src_sy", i32),
-                ("src_add", vp), ("src_mul", vp), ("signal", vp), ("power", vp)]
+                ("src_add", vp), ("src_mul", vp), ("signal", vp), ("power", vp),
+                ("snap_x0", i32), ("snap_y0", i32), ("snap_sx", i32), ("snap_sy", i32), ("snap", vp)]
+
+
+class Adj2D(C.Structure):
+    _fields_ = [("nx", i32), ("ny", i32), ("ld", i32), ("le", vp * 2), ("lhx", vp * 2), ("lhy", vp * 2),
+                ("eps", vp), ("sig", vp), ("kh", f32), ("kbh", f32), ("ke", f32), ("kbe", f32),
+                ("rdx", f32), ("rdy", f32), ("src_x0", i32), ("src_y0", i32), ("src_sx", i32), ("src_sy", i32),
+                ("src_mul", vp), ("box_x0", i32), ("box_y0", i32), ("box_sx", i32), ("box_sy", i32),
+                ("snap", vp), ("snap_step0", i32), ("grad", vp),
+                ("inj_x0", i32), ("inj_y0", i32), ("inj_sx", i32), ("inj_sy", i32), ("inj_w", vp), ("phasor", vp)]
 
 # name -> argtypes (restype is int unless listed in _RESTYPE)
 SIGNATURES = {
@@ -53,6 +63,8 @@ SIGNATURES = {
     "bz_yee3d_step": [P6, P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, C.POINTER(BoxSource), i32,
                       i32, i32, vp],
     "bz_yee2d_run": [vp, i32, vp, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp],
+    "bz_adj2d_run": [vp, i32, vp, i32, i32, i32, i32, vp],
+    "bz_adj2d_inject": [vp, i32, i32, i32, vp],
     "bz_add_points": [vp, vp, vp, i32, f32, vp],
     "bz_affine_points": [vp, vp, vp, vp, i32, f32, vp],
     "bz_gather": [P6, vp, vp, i32, vp, i32, vp, vp, vp],

@@ -163,6 +163,30 @@ class Monitor:
     def should_record(self, step):
         return (step - self.last_record_step) >= self.record_interval
 
+    def dft_flux(self):
+        """Net spectral power through the monitor per DFT frequency (new, no reference counterpart).
+
+        Sums Re(S . n) over the monitor points from the DFT fields, with the
+        reference's own Poynting conventions: in 2D (monitors.py:407-421)
+        Sx = -Ez Hy and Sy = Ez Hx, here with conj(H) on the DFT amplitudes;
+        a vertical line is x-normal, a horizontal one y-normal. In 3D, the
+        plane's normal component of E x conj(H). Units are field-product sums
+        (no cell area factor): ratios between monitors of one grid are the
+        port transmissions.
+        """
+        self._materialize()
+        if not self.dft:
+            raise ValueError("dft_flux needs dft_frequencies and a finished run")
+        D = self.dft
+        if not self.is_3d:
+            vertical = abs(self.end[0] - self.start[0]) <= abs(self.end[1] - self.start[1])
+            s = -D["Ez"] * np.conj(D["Hy"]) if vertical else D["Ez"] * np.conj(D["Hx"])
+            return np.real(s).reshape(s.shape[0], -1).sum(axis=1)
+        n = self.plane_normal if self.end is None else "z"
+        a, b = {"x": ("y", "z"), "y": ("z", "x"), "z": ("x", "y")}[n]
+        s = D["E" + a] * np.conj(D["H" + b]) - D["E" + b] * np.conj(D["H" + a])
+        return np.real(s).reshape(s.shape[0], -1).sum(axis=1)
+
     def evaluate_objective(self) -> Optional[float]:
         """monitors.py:58-74."""
         if self.objective_function is None:

@@ -1,0 +1,258 @@
+"""Exact adjoint gradients of 2D mode-overlap objectives on the GPU.
+
+New API; the reference's topology loop (examples/4_topo.py:255-321) estimates
+the gradient by the time-reversed overlap of a forward and a separate
+"adjoint source" run (4_topo.py:282-285, kept here as
+``overlap_gradient_reference`` for comparison). Here the gradient is that of
+the discrete forward map itself:
+
+* objective: J = |a|^2 with a = sum_p c_p Ez_hat(p, w), the DFT at one
+  frequency of the Ez samples a line monitor records after every step,
+  projected on weights c_p (normally the conjugated, normalised mode profile
+  of the output guide, ``mode_weights``);
+* forward: the fused 2D engine (one launch per step for the whole batch of
+  designs), with the full field state checkpointed every K ~ sqrt(N) steps;
+* reverse: for each segment, newest first, the segment is recomputed from its
+  checkpoint by the same engine while it snapshots Ez, Hx, Hy inside the
+  design box, then the reverse-time kernel (``bz_adj2d_run``) walks the
+  segment backwards, transposing each step and accumulating dJ/d eps_r in
+  FP64 inside the design box.
+
+Memory: ceil(N/K) checkpoints of three field arrays plus K design-box
+snapshots per design. ``FDTDObjective`` wraps this as a torch.autograd
+Function of the design-box permittivity, so density filters and projections
+written as torch ops chain to it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from .. import _native as N
+from ..const import EPS_0, MU_0
+from ..simulation.engine import Fused2D
+
+
+def mode_weights(monitor, dx, dy, center, width, n_core, n_clad, wavelength):
+    """c_p = conj(m_p) / |m| for the fundamental slab mode of a guide of ``width`` centred at
+    ``center`` (the across-guide coordinate) sampled at the monitor's grid points, as the reference's
+    topology example builds its target (examples/4_topo.py:142-164, slab_mode_source)."""
+    from ..devices.mode import slab_mode_source
+    pts = monitor.get_grid_points_2d(dx, dy)
+    horizontal = abs(monitor.end[0] - monitor.start[0]) >= abs(monitor.end[1] - monitor.start[1])
+    pos = np.array([(p[0] * dx if horizontal else p[1] * dy) for p in pts], dtype=float)
+    prof, _ = slab_mode_source(x=pos - center, w=width, n_WG=n_core, n0=n_clad, wavelength=wavelength, ind_m=0, x0=0.0)
+    m = np.asarray(prof, dtype=complex)
+    nrm = np.linalg.norm(m)
+    return np.conj(m) / (nrm if nrm > 0 else 1.0)
+
+
+class AdjointProblem2D:
+    """Mode-overlap objective and its exact gradient w.r.t. eps_r in a design box, for a batch of
+    2D simulations that share geometry of the box, time axis and objective monitor layout.
+
+    sims: list of 2D ``FDTD`` objects (one additive or multiplicative source each); ``monitor`` of
+    sim b is ``monitors[b]``, a line monitor with exactly one DFT frequency; weights: complex c_p per
+    monitor point (shared); box: (x0, y0, sx, sy) design cells, inside the interior.
+    """
+
+    def __init__(self, sims, monitors, weights, box, checkpoint_every=None, seg=None):
+        self.sims = list(sims)
+        self.monitors = list(monitors)
+        if len(self.monitors) != len(self.sims):
+            raise ValueError("one objective monitor per simulation")
+        for m in self.monitors:
+            if m.dft_frequencies is None or len(m.dft_frequencies) != 1:
+                raise ValueError("the objective monitor needs exactly one DFT frequency")
+        self.c = np.asarray(weights, dtype=np.complex128)
+        self.box = tuple(int(v) for v in box)
+        x0, y0, sx, sy = self.box
+        for s in self.sims:
+            if s.is_3d:
+                raise ValueError("2D simulations only")
+            if not (x0 >= 1 and y0 >= 1 and x0 + sx <= s.nx - 1 and y0 + sy <= s.ny - 1):
+                raise ValueError("the design box must lie inside the grid interior")
+        self.N = self.sims[0].num_steps
+        self.K = int(checkpoint_every or max(1, math.ceil(math.sqrt(self.N))))
+        self.seg = seg
+        self.torch = self.sims[0].backend.torch
+        self.dev = self.sims[0].backend.dev
+        self._ready = False
+
+    # ---------------- forward ----------------
+    def _setup(self):
+        T = self.torch
+        for s in self.sims:
+            s.initialize_simulation(save=False, live=False)
+        grids = [s.grid for s in self.sims]
+        self.fwd = Fused2D(grids, seg=self.seg, max_steps=self.N)
+        x0, y0, sx, sy = self.box
+        self.snap = T.zeros((len(grids), self.K, 3, sy + 1, sx + 1), dtype=T.float32, device=self.dev)
+        snaps = [(x0 - 1, y0 - 1, sx + 1, sy + 1, self.snap[b]) for b in range(len(grids))]
+        seg = self.fwd.seg if self.fwd.seg <= 8 else 8
+        self.rec = Fused2D(grids, seg=seg, monitors=False, snapshots=snaps)
+        for g in grids:
+            g._f2d = False
+        self._ready = True
+
+    def set_eps(self, eps_box):
+        """eps_box: (B, sy, sx) relative permittivity of the design boxes (torch, on the device)."""
+        x0, y0, sx, sy = self.box
+        e = eps_box.to(device=self.dev, dtype=self.torch.float32)
+        for b, s in enumerate(self.sims):
+            s.grid.eps[y0:y0 + sy, x0:x0 + sx] = e[b]
+
+    def forward(self, eps_box):
+        """Run every simulation with the given design permittivity; returns J (B,) FP64 torch tensor."""
+        T = self.torch
+        self._setup()          # resets fields and monitor accumulators
+        self.set_eps(eps_box)
+        grids = [s.grid for s in self.sims]
+        self.ckpt = []
+        n = 0
+        while n < self.N:
+            self.ckpt.append([T.stack([g.f["Ez"], g.f["Hx"], g.f["Hy"]]) for g in grids])
+            cnt = min(self.K, self.N - n)
+            self.fwd.run(n, cnt)
+            n += cnt
+        for s in self.sims:
+            s.current_step = self.N
+        c = T.from_numpy(self.c).to(self.dev)
+        self.a = T.stack([self._overlap(s, m, c) for s, m in zip(self.sims, self.monitors)])
+        self.J = (self.a.real ** 2 + self.a.imag ** 2)
+        return self.J
+
+    def _overlap(self, sim, mon, c):
+        plan = next(p for p in sim._plans if p.monitor is mon)
+        npts = plan.layout[1]
+        z = plan.dft[0, :npts, :]                  # Ez entries come first
+        zc = self.torch.complex(z[:, 0], z[:, 1])
+        return (c * zc).sum()
+
+    # ---------------- reverse ----------------
+    def _adjoint_descs(self, lam, grad, inj):
+        T = self.torch
+        x0, y0, sx, sy = self.box
+        descs = []
+        for b, s in enumerate(self.sims):
+            g = s.grid
+            d = N.Adj2D()
+            d.nx, d.ny, d.ld = g.nx, g.ny, g.ld
+            for p in (0, 1):
+                d.le[p], d.lhx[p], d.lhy[p] = (lam[b][p][q].data_ptr() for q in range(3))
+            d.eps, d.sig = g.eps.data_ptr(), g.sig.data_ptr()
+            d.kh = (MU_0 / EPS_0) * g.dt / (2.0 * MU_0)
+            d.kbh, d.ke, d.kbe = g.dt / MU_0, g.dt / (2.0 * EPS_0), g.dt / EPS_0
+            d.rdx, d.rdy = 1.0 / g.dx, 1.0 / g.dy
+            src_desc = self.rec.descs[b]
+            d.src_x0, d.src_y0, d.src_sx, d.src_sy = src_desc.src_x0, src_desc.src_y0, src_desc.src_sx, src_desc.src_sy
+            d.src_mul = src_desc.src_mul
+            d.box_x0, d.box_y0, d.box_sx, d.box_sy = x0, y0, sx, sy
+            d.snap = self.snap[b].data_ptr()
+            d.grad = grad[b].data_ptr()
+            ix0, iy0, isx, isy, w = inj[b]
+            d.inj_x0, d.inj_y0, d.inj_sx, d.inj_sy = ix0, iy0, isx, isy
+            d.inj_w = w.data_ptr()
+            d.phasor = self._phasor[b].data_ptr()
+            descs.append(d)
+        return descs
+
+    def _injection(self, b):
+        """Dense box of dJ/dEz-sample weights 2 conj(a) c_p over the monitor's Ez points."""
+        s, m = self.sims[b], self.monitors[b]
+        pts = np.array(m.get_grid_points_2d(s.dx, s.dy), dtype=np.int64)
+        wp = 2.0 * np.conj(complex(self.a[b].item())) * self.c
+        xs, ys = pts[:, 0], pts[:, 1]
+        ok = (xs >= 0) & (xs < s.nx) & (ys >= 0) & (ys < s.ny)
+        xs, ys, wp = xs[ok], ys[ok], wp[ok]
+        ix0, iy0 = int(xs.min()), int(ys.min())
+        isx, isy = int(xs.max()) - ix0 + 1, int(ys.max()) - iy0 + 1
+        box = np.zeros((isy, isx), dtype=np.complex128)
+        np.add.at(box, (ys - iy0, xs - ix0), wp)
+        return ix0, iy0, isx, isy, self.torch.from_numpy(np.ascontiguousarray(box).view(np.float64).copy()).to(self.dev)
+
+    def gradient(self):
+        """dJ/d eps_r over the design boxes, (B, sy, sx) FP64 torch tensor, for the last forward."""
+        T = self.torch
+        if not self._ready or not hasattr(self, "ckpt"):
+            raise RuntimeError("call forward() first")
+        x0, y0, sx, sy = self.box
+        grids = [s.grid for s in self.sims]
+        B = len(grids)
+        lam = [[T.zeros((3,) + tuple(g.box), dtype=T.float32, device=self.dev) for _ in (0, 1)] for g in grids]
+        grad = T.zeros((B, sy, sx), dtype=T.float64, device=self.dev)
+        inj = [self._injection(b) for b in range(B)]
+        self._phasor = []
+        for b, s in enumerate(self.sims):
+            plan = next(p for p in s._plans if p.monitor is self.monitors[b])
+            if len(plan.record_steps) != self.N:
+                raise ValueError("the objective monitor must record every step")
+            self._phasor.append(plan.phasor[:, :2].contiguous())     # (N, 1 freq) complex -> (N, 2) FP64
+        descs = self._adjoint_descs(lam, grad, inj)
+        raw = b"".join(bytes(d) for d in descs)
+        ddev = T.frombuffer(bytearray(raw), dtype=T.uint8).to(self.dev)
+        max_inj = max(i[2] * i[3] for i in inj)
+        st = grids[0].stream
+        N.call("bz_adj2d_inject", ddev.data_ptr(), B, max_inj, self.N, st)
+        nseg = len(self.ckpt)
+        for k in range(nseg - 1, -1, -1):
+            n0 = k * self.K
+            cnt = min(self.K, self.N - n0)
+            for g, ck in zip(grids, self.ckpt[k]):
+                g.f["Ez"].copy_(ck[0]); g.f["Hx"].copy_(ck[1]); g.f["Hy"].copy_(ck[2])
+            self.rec.run(n0, cnt)
+            for d in descs:
+                d.snap_step0 = n0
+            raw = b"".join(bytes(d) for d in descs)
+            ddev.copy_(T.frombuffer(bytearray(raw), dtype=T.uint8))
+            N.call("bz_adj2d_run", ddev.data_ptr(), B, self.rec.tiles.data_ptr(), self.rec.ntiles, self.rec.seg,
+                   n0 + cnt, cnt, st)
+        self._keep = (lam, inj, ddev)
+        return grad
+
+
+class FDTDObjective:
+    """torch.autograd Function: J = problem.forward(eps_box); backward = problem.gradient()."""
+
+    @staticmethod
+    def apply(eps_box, problem):
+        import torch
+
+        class _F(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, e):
+                return problem.forward(e.detach()).to(e.dtype)
+
+            @staticmethod
+            def backward(ctx, gJ):
+                g = problem.gradient().to(gJ.dtype)
+                return gJ.view(-1, 1, 1) * g
+
+        return _F.apply(eps_box)
+
+
+def cone_filter_torch(rho, radius):
+    """Normalised cone filter with zero padding, as ``topology.cone_filter``, on (B, H, W) tensors."""
+    import torch
+    import torch.nn.functional as F
+    r = int(radius)
+    yy, xx = np.mgrid[-r:r + 1, -r:r + 1]
+    K = np.maximum(0.0, float(r) - np.hypot(xx, yy))
+    k = torch.from_numpy(K).to(device=rho.device, dtype=rho.dtype)[None, None]
+    num = F.conv2d(rho[:, None], k, padding=r)[:, 0]
+    den = F.conv2d(torch.ones_like(rho[:1, None]), k, padding=r)[:, 0]
+    return num / den
+
+
+def tanh_projection_torch(x, beta, eta=0.5):
+    """(tanh(beta eta) + tanh(beta (x - eta))) / (tanh(beta eta) + tanh(beta (1 - eta))) (4_topo.py:107-117)."""
+    import math as m
+    den = m.tanh(beta * eta) + m.tanh(beta * (1.0 - eta))
+    return (m.tanh(beta * eta) + (beta * (x - eta)).tanh()) / den
+
+
+def density_to_eps(rho, eps_clad, eps_core, radius=3, beta=8.0, eta=0.5):
+    """Design density (B, H, W) -> permittivity: cone filter, tanh projection, linear material map."""
+    return eps_clad + tanh_projection_torch(cone_filter_torch(rho, radius), beta, eta) * (eps_core - eps_clad)

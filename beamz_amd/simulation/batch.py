@@ -1,0 +1,102 @@
+"""Many independent 2D simulations advanced together on one GPU.
+
+New API (the reference runs one ``FDTD`` at a time, fdtd.py:411-429 per step):
+``FDTDBatch`` takes ordinary 2D ``FDTD`` objects (each built from its own
+design, time axis, source and monitors exactly as for a single run) and
+advances all of them with one launch per step of the fused 2D engine
+(``bz_yee2d_run``): every simulation is cut into tiles of 62 columns x ``seg``
+rows, the tiles of the whole batch form one work list, and the monitor samples
+and FP64 DFT sums of every member are gathered into one table. Each member's
+step is bit-for-bit the step it takes when run alone.
+
+The members must share their time axis, their monitor record schedule and
+their DFT frequencies (one phasor table serves the batch), and each may have at
+most one source. ``autotune`` picks the tile height by timing a few steps of
+each candidate on the batch itself (HIP events), then restores the initial
+state.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .engine import Fused2D
+
+
+class FDTDBatch:
+    SEGS = Fused2D.SEGS
+
+    def __init__(self, sims, seg=None, graph=False, chunk=100):
+        if not sims:
+            raise ValueError("FDTDBatch needs at least one simulation")
+        for s in sims:
+            if s.is_3d:
+                raise ValueError("FDTDBatch runs 2D simulations only")
+        t0 = np.asarray(sims[0].time)
+        for s in sims[1:]:
+            if len(s.time) != len(t0) or not np.array_equal(np.asarray(s.time), t0):
+                raise ValueError("all simulations of a batch must share one time axis")
+        self.sims = list(sims)
+        self.seg, self.graph, self.chunk = seg, bool(graph), int(chunk)
+        self.num_steps = len(t0)
+        self.current_step = 0
+        self.engine = None
+
+    @property
+    def cells(self):
+        return sum(s.nx * s.ny for s in self.sims)
+
+    def initialize(self):
+        for s in self.sims:
+            s.initialize_simulation(save=False, live=False)
+        self.engine = Fused2D([s.grid for s in self.sims], seg=self.seg, graph=self.graph, max_steps=self.num_steps)
+        self.seg = self.engine.seg
+        for s in self.sims:
+            s.grid._f2d = False       # members advance through the batch engine only
+        self.current_step = 0
+
+    def advance(self, count):
+        """Run ``count`` steps of every member (bounded by the time axis)."""
+        if self.engine is None:
+            self.initialize()
+        end = min(self.num_steps, self.current_step + int(count))
+        while self.current_step < end:
+            n = self.current_step
+            c = min(self.chunk, end - n)
+            self.engine.run(n, c)
+            self.current_step += c
+        for s in self.sims:
+            s.current_step = self.current_step
+            s.t = self.current_step * s.dt
+
+    def run(self, steps=None):
+        """Initialise, run ``steps`` (default: the whole time axis) and finalise every member."""
+        self.initialize()
+        self.advance(self.num_steps if steps is None else steps)
+        return [s.finalize_simulation() for s in self.sims]
+
+    def synchronize(self):
+        self.sims[0].synchronize()
+
+    def autotune(self, steps=20, segs=None):
+        """Pick the tile height with the lowest measured step time; leaves the batch re-initialised."""
+        T = self.sims[0].backend.torch
+        if self.engine is None:
+            self.initialize()
+        best = None
+        timings = {}
+        for seg in segs or self.SEGS:
+            self.engine.set_seg(seg)
+            self.engine.run(0, 2)                       # warm-up
+            e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
+            e0.record()
+            self.engine.run(2, steps)
+            e1.record()
+            T.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / steps
+            timings[seg] = ms
+            if best is None or ms < timings[best]:
+                best = seg
+        self.seg = best
+        self.initialize()
+        self.engine.set_seg(best)
+        return best, timings

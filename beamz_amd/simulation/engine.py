@@ -14,6 +14,8 @@ H update, E update, sources, monitor sampling (+ FP64 DFT), grid power.
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 
 from .. import _native as N
@@ -94,6 +96,8 @@ class Grid:
         self.fused = None
         self.kernel = "fused"
         self._f2d = None
+        self.seg2d = None       # rows per tile of the fused 2D engine (None: by grid size)
+        self.graph2d = False    # capture fused 2D runs as HIP graphs
         self.kseg = 16
         self.rows = 4
         self.timing = False
@@ -329,30 +333,104 @@ class Grid:
     def invalidate(self):
         self._f2d = None
 
-    def _setup_fused2d(self):
-        """Descriptor, tile list and combined monitor tables for bz_yee2d_run (None if unsupported)."""
-        T = self.torch
-        if len(self.sources) > 1:
-            return None
-        plans = [p for p in self.monitors if p.n > 0]
+    def _fused2d(self, n0, count):
+        if self._f2d is None:
+            try:
+                self._f2d = Fused2D([self], seg=self.seg2d, graph=self.graph2d)
+            except NotImplementedError:
+                self._f2d = False
+        if self._f2d is False:
+            return False
+        self._f2d.run(n0, count)
+        return True
+
+
+class Fused2D:
+    """Runner of bz_yee2d_run over one or several 2D grids advanced in lockstep.
+
+    Builds the per-simulation descriptors (device array), the tile work queue
+    (tiles of 62 columns x ``seg`` rows of every grid) and the combined monitor
+    tables (all monitors of all grids in one gather list, one FP64 DFT
+    accumulator, one series buffer; the monitors' own ``series`` / ``dft``
+    become views into them). With ``graph=True`` a run of ``count`` steps is
+    captured once as a HIP graph and replayed for later runs of the same length.
+    """
+
+    def __init__(self, grids, seg=8, graph=False, max_steps=None, monitors=True, snapshots=None):
+        """monitors=False runs the steps without sampling any monitor (a recomputation);
+        snapshots: per grid None or (x0, y0, sx, sy, tensor [steps][3][sy][sx]) for the snapshot box."""
+        if not grids:
+            raise ValueError("Fused2D needs at least one grid")
+        self.snapshots = list(snapshots) if snapshots is not None else [None] * len(grids)
+        g0 = grids[0]
+        T = g0.torch
+        self.torch, self.dev, self.grids = T, g0.dev, list(grids)
+        for g in grids:
+            if g.is_3d:
+                raise ValueError("Fused2D runs 2D grids only")
+            if g.dev != g0.dev:
+                raise ValueError("all grids of a batch must be on one device")
+            if len(g.sources) > 1:
+                raise NotImplementedError("the fused 2D engine takes at most one source per simulation")
+            if 4 * g.ny * g.ld >= 2 ** 32:
+                raise ValueError("a 2D array exceeds 4 GiB (the engine uses 32-bit byte offsets)")
+        plans = [(s, p) for s, g in enumerate(grids) for p in g.monitors if p.n > 0] if monitors else []
         if plans:
-            p0 = plans[0]
-            for p in plans[1:]:
+            p0 = plans[0][1]
+            for _, p in plans[1:]:
                 if not np.array_equal(p.record_steps, p0.record_steps) or p.nfreq != p0.nfreq:
-                    return None
+                    raise NotImplementedError("monitors of a fused 2D run must share their record steps and DFT")
                 if p.nfreq and not T.equal(p.phasor, p0.phasor):
-                    return None
-        st = {}
+                    raise NotImplementedError("monitors of a fused 2D run must share their DFT phasors")
+        self.keep = []
+        self.signal_len = 1 << 62
+        self.descs = [self._describe(g, sn) for g, sn in zip(grids, self.snapshots)]
+        self.desc_dev = T.zeros(len(grids) * C.sizeof(N.Sim2D), dtype=T.uint8, device=self.dev)
+        self.shift = None
+        self.set_seg(seg)
+        self.step = T.zeros(1, dtype=T.int32, device=self.dev)
+        self.npts, self.nfreq = 0, 0
+        self.pts = self.slot = self.series = self.dft = self.phasor = None
+        self.nslot = 0
+        if plans:
+            ntot = sum(p.n for _, p in plans)
+            sim = np.concatenate([np.full(p.n, s, np.int32) for s, p in plans])
+            comp = T.cat([p.comp for _, p in plans]).cpu().numpy()
+            off = T.cat([p.off for _, p in plans]).cpu().numpy()
+            if off.max(initial=0) >= 2 ** 31:
+                raise ValueError("monitor offsets exceed 32 bits")
+            pts = np.stack([sim, comp, off.astype(np.int32)], axis=1).astype(np.int32)
+            self.pts = T.from_numpy(pts.ravel()).to(self.dev)
+            rs = self.record_steps = p0.record_steps
+            need = max(int(rs.max()) + 1 if len(rs) else 1, int(max_steps or 0), 1)
+            self._make_slot(need)
+            nrec = max(len(rs), 1)
+            keep = all(p.series is not None for _, p in plans)
+            self.series = T.zeros((nrec, ntot), dtype=T.float32, device=self.dev) if keep else None
+            self.dft = T.zeros((p0.nfreq, ntot, 2), dtype=T.float64, device=self.dev) if p0.nfreq else None
+            self.phasor = p0.phasor
+            c0 = 0
+            for _, p in plans:
+                if keep:
+                    p.series = self.series[:, c0:c0 + p.n]
+                if p0.nfreq:
+                    p.dft = self.dft[:, c0:c0 + p.n, :]
+                c0 += p.n
+            self.npts, self.nfreq = ntot, p0.nfreq
+        self.use_graph = bool(graph)
+        self.graphs = {}
+
+    def _describe(self, g, snap=None):
+        T = self.torch
         d = N.Sim2D()
-        d.nx, d.ny, d.ld = self.nx, self.ny, self.ld
-        d.eps, d.sig = self.eps.data_ptr(), self.sig.data_ptr()
-        c = (self.dt, MU_0, EPS_0)
-        d.kh = (MU_0 / EPS_0) * self.dt / (2.0 * MU_0)
-        d.kbh, d.ke, d.kbe = self.dt / MU_0, self.dt / (2.0 * EPS_0), self.dt / EPS_0
-        d.rdx, d.rdy = 1.0 / self.dx, 1.0 / self.dy
+        d.nx, d.ny, d.ld = g.nx, g.ny, g.ld
+        d.eps, d.sig = g.eps.data_ptr(), g.sig.data_ptr()
+        d.kh = (MU_0 / EPS_0) * g.dt / (2.0 * MU_0)
+        d.kbh, d.ke, d.kbe = g.dt / MU_0, g.dt / (2.0 * EPS_0), g.dt / EPS_0
+        d.rdx, d.rdy = 1.0 / g.dx, 1.0 / g.dy
         d.src_sx = 0
-        if self.sources:
-            s = self.sources[0]
+        if g.sources:
+            s = g.sources[0]
             idx = s.idx
             lo = idx.min(axis=0); hi = idx.max(axis=0) + 1
             sy, sx = (int(v) for v in (hi - lo))
@@ -361,83 +439,117 @@ class Grid:
             add[rel[:, 0], rel[:, 1]] = s.w_host
             if s.mul_host is not None:
                 mul[rel[:, 0], rel[:, 1]] = s.mul_host
-            st["add"] = T.from_numpy(add.ravel()).to(self.dev)
-            st["mul"] = T.from_numpy(mul.ravel()).to(self.dev) if s.mul_host is not None else None
-            st["sig"] = T.from_numpy(np.real(s.signal).astype(np.float32)).to(self.dev)
+            dadd = T.from_numpy(add.ravel()).to(self.dev)
+            dmul = T.from_numpy(mul.ravel()).to(self.dev) if s.mul_host is not None else None
+            dsig = T.from_numpy(np.real(s.signal).astype(np.float32)).to(self.dev)
+            self.keep += [dadd, dmul, dsig]
             d.src_y0, d.src_x0, d.src_sy, d.src_sx = int(lo[0]), int(lo[1]), sy, sx
-            d.src_add = st["add"].data_ptr()
-            d.src_mul = st["mul"].data_ptr() if st["mul"] is not None else None
-            d.signal = st["sig"].data_ptr()
-        d.power = self.power.data_ptr() if self.power is not None else None
-        st["desc"] = d
-        gx = (self.nx + 62) // 63
-        seg = 8
-        gy = (self.ny + seg - 1) // seg
-        tiles = np.array([(0, tx, ty) for ty in range(gy) for tx in range(gx)], dtype=np.int32)
-        st["tiles"] = T.from_numpy(tiles.ravel()).to(self.dev)
-        st["ntiles"], st["seg"] = len(tiles), seg
-        st["step"] = T.zeros(1, dtype=T.int32, device=self.dev)
-        st["npts"] = 0
-        if plans:
-            ntot = sum(p.n for p in plans)
-            comp = T.cat([p.comp for p in plans]).cpu().numpy()
-            off = T.cat([p.off for p in plans]).cpu().numpy()
-            pts = np.stack([np.zeros_like(comp), comp, off.astype(np.int32)], axis=1).astype(np.int32)
-            st["pts"] = T.from_numpy(pts.ravel()).to(self.dev)
-            slot = np.full(max(int(p0.record_steps.max()) + 1 if len(p0.record_steps) else 1, 1), -1, np.int32)
-            slot[p0.record_steps] = np.arange(len(p0.record_steps), dtype=np.int32)
-            st["nslot"] = len(slot)
-            st["slot"] = T.from_numpy(slot).to(self.dev)
-            nrec = max(len(p0.record_steps), 1)
-            keep = all(p.series is not None for p in plans)
-            st["series"] = T.zeros((nrec, ntot), dtype=T.float32, device=self.dev) if keep else None
-            st["dft"] = T.zeros((p0.nfreq, ntot, 2), dtype=T.float64, device=self.dev) if p0.nfreq else None
-            st["phasor"] = p0.phasor
-            c0 = 0
-            for p in plans:
-                if keep:
-                    p.series = st["series"][:, c0:c0 + p.n]
-                if p0.nfreq:
-                    p.dft = st["dft"][:, c0:c0 + p.n, :]
-                c0 += p.n
-            st["npts"], st["nfreq"] = ntot, p0.nfreq
-        st["shift"] = None
-        return st
+            d.src_add = dadd.data_ptr()
+            d.src_mul = dmul.data_ptr() if dmul is not None else None
+            d.signal = dsig.data_ptr()
+            self.signal_len = min(self.signal_len, len(s.signal))
+        d.power = g.power.data_ptr() if g.power is not None else None
+        if snap is not None:
+            x0, y0, sx, sy, buf = snap
+            if not (0 <= x0 and 0 <= y0 and x0 + sx <= g.nx and y0 + sy <= g.ny):
+                raise ValueError("snapshot box outside the grid")
+            if buf.dtype != T.float32 or buf.shape[1:] != (3, sy, sx) or not buf.is_contiguous():
+                raise ValueError("snapshot buffer must be contiguous FP32 [steps][3][sy][sx]")
+            d.snap_x0, d.snap_y0, d.snap_sx, d.snap_sy = x0, y0, sx, sy
+            d.snap = buf.data_ptr()
+            self.snap_steps = min(getattr(self, "snap_steps", 1 << 62), buf.shape[0])
+        return d
 
-    def _fused2d(self, n0, count):
-        if getattr(self, "_f2d", None) is None:
-            self._f2d = self._setup_fused2d()
-            if self._f2d is None:
-                self._f2d = False
-        st = self._f2d
-        if st is False:
-            return False
+    SEGS = (2, 4, 8, 16)
+    TILE_X = 62
+
+    def auto_seg(self, min_waves=8192, max_seg=8):
+        """Tallest tile up to max_seg rows (fewest halo rows; 16-row tiles need 170 VGPRs, 2 waves per
+        SIMD) that still gives min_waves tiles (8 waves per SIMD), else 2."""
+        for seg in sorted((s for s in self.SEGS if s <= max_seg), reverse=True):
+            if sum(-(-g.nx // self.TILE_X) * -(-g.ny // seg) for g in self.grids) >= min_waves:
+                return seg
+        return min(self.SEGS)
+
+    def set_seg(self, seg):
+        """Rebuild the tile work queue with tiles of 62 columns x seg rows (seg None: auto_seg)."""
         T = self.torch
-        shift = (self._cur - (n0 & 1)) % 2
-        if st["shift"] != shift:
-            d = st["desc"]
+        seg = self.auto_seg() if seg is None else int(seg)
+        if seg not in self.SEGS:
+            raise ValueError(f"tile height must be one of {self.SEGS}")
+        self.seg = seg
+        rows = []
+        for s, g in enumerate(self.grids):
+            gx = (g.nx + self.TILE_X - 1) // self.TILE_X
+            gy = (g.ny + self.seg - 1) // self.seg
+            t = np.stack(np.meshgrid(np.arange(gx), np.arange(gy), indexing="xy"), -1).reshape(-1, 2)
+            rows.append(np.concatenate([np.full((len(t), 1), s), t], axis=1))
+        tiles = np.concatenate(rows).astype(np.int32)
+        self.tiles = T.from_numpy(tiles.ravel()).to(self.dev)
+        self.ntiles = len(tiles)
+        self.graphs = {}
+
+    def _make_slot(self, need):
+        T = self.torch
+        rs = self.record_steps
+        slot = np.full(need, -1, np.int32)
+        rs = rs[rs < need]
+        slot[rs] = np.arange(len(rs), dtype=np.int32)
+        self.slot = T.from_numpy(slot).to(self.dev)
+        self.nslot = need
+        self.graphs = {}
+
+    def _sync_descs(self, n0):
+        shifts = [(g._cur - (n0 & 1)) % 2 for g in self.grids]
+        if shifts == self.shift:
+            return
+        for d, g, sh in zip(self.descs, self.grids, shifts):
             for p in (0, 1):
-                b = self._fb[(p + shift) % 2]
+                b = g._fb[(p + sh) % 2]
                 d.ez[p], d.hx[p], d.hy[p] = b["Ez"].data_ptr(), b["Hx"].data_ptr(), b["Hy"].data_ptr()
-            st["dev"] = T.frombuffer(bytearray(bytes(d)), dtype=T.uint8).to(self.dev)
-            st["shift"] = shift
-        if st["npts"] and n0 + count > st["nslot"]:
-            pad = T.full((n0 + count - st["nslot"],), -1, dtype=T.int32, device=self.dev)
-            st["slot"] = T.cat([st["slot"], pad])
-            st["nslot"] = int(st["slot"].numel())
-        st["step"].fill_(n0)
-        strm = self.stream
-        if self.timing:
+        raw = b"".join(bytes(d) for d in self.descs)
+        self.desc_dev.copy_(self.torch.frombuffer(bytearray(raw), dtype=self.torch.uint8))
+        self.shift = shifts
+
+    def _launch(self, count):
+        ptr = lambda t: t.data_ptr() if t is not None else None
+        g0 = self.grids[0]
+        N.call("bz_yee2d_run", self.desc_dev.data_ptr(), len(self.grids), self.tiles.data_ptr(), self.ntiles,
+               self.seg, self.step.data_ptr(), count, ptr(self.pts), self.npts, ptr(self.slot), ptr(self.series),
+               self.nfreq, ptr(self.phasor), ptr(self.dft), g0.stream)
+
+    def run(self, n0, count):
+        """Advance every grid by ``count`` steps starting at step index n0."""
+        if count <= 0:
+            return
+        T = self.torch
+        if n0 < 0 or n0 + count > self.signal_len:
+            raise ValueError(f"steps {n0}..{n0 + count - 1} run past the source signal ({self.signal_len} samples)")
+        if count > getattr(self, "snap_steps", 1 << 62):
+            raise ValueError(f"a run of {count} steps overflows the snapshot buffers ({self.snap_steps} slots)")
+        if self.npts and n0 + count > self.nslot:
+            self._make_slot(n0 + count)
+        self._sync_descs(n0)
+        self.step.fill_(n0)
+        g0 = self.grids[0]
+        if g0.timing:
             e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
             e0.record()
-        sp = lambda k: st[k].data_ptr() if st.get(k) is not None else None
-        N.call("bz_yee2d_run", st["dev"].data_ptr(), 1, st["tiles"].data_ptr(), st["ntiles"], st["seg"],
-               st["step"].data_ptr(), count, sp("pts"), st["npts"], sp("slot"), sp("series"), st.get("nfreq", 0),
-               sp("phasor"), sp("dft"), strm)
-        if self.timing:
+        if self.use_graph and count > 1:
+            gr = self.graphs.get(count)
+            if gr is None:
+                gr = T.cuda.CUDAGraph()
+                T.cuda.synchronize(self.dev)
+                with T.cuda.graph(gr):
+                    self._launch(count)
+                self.graphs[count] = gr
+            gr.replay()
+        else:
+            self._launch(count)
+        if g0.timing:
             e1.record()
-            self._events.append((e0, e1, count))
-        self._cur = (self._cur + count) % 2
-        if self.power is not None:
-            self.power_count += count
-        return True
+            g0._events.append((e0, e1, count))
+        for g in self.grids:
+            g._cur = (g._cur + count) % 2
+            if g.power is not None:
+                g.power_count += count

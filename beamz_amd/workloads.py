@@ -58,6 +58,59 @@ def waveguide_2d(nx=800, ny=400, steps=4000, npml=20, lam0=1.55 * µm, src_x=50,
     return d, dx, t, mon
 
 
+def binary_density(n, rng, sigma_px=3.0, fill=0.5):
+    """Seeded Bernoulli(fill) pixel map, Gaussian-blurred (sigma in pixels), thresholded at 0.5."""
+    from scipy.ndimage import gaussian_filter
+    r = (rng.random((n, n)) < fill).astype(np.float64)
+    return gaussian_filter(r, sigma_px, mode="nearest") > 0.5
+
+
+def batch_member_2d(n, rng, steps=2000, npml=10, dx=25e-9, n_freq=8, design_um=2.0):
+    """One member of the batched 2D workload: an n x n cell domain (SiO2), a design_um x design_um
+    binary Si/SiO2 design region of one cell per pixel at the centre between 0.5 um input and output
+    Si guides, 10-cell PML, Gaussian-pulse point source in the input guide, input and output DFT line
+    monitors across the guide. Returns (FDTD-ready mesh, time array, (monitor_in, monitor_out))."""
+    from .simulation.meshing import RegularGrid
+    # the grid has int(W / dx) cells (meshing.py:74-75 truncates); n*dx / dx rounds below n for some n
+    W = H = (n + 1e-6) * dx
+    npx = int(round(design_um * µm / dx))
+    x0 = y0 = (n - npx) // 2
+    d = Design(W, H, material=Material(EPS_OX), auto_pml=False)
+    d.boundaries += edge_pmls(W, H, npml, dx)
+    d += Rectangle(position=(0, H / 2 - 0.25 * µm), width=x0 * dx, height=0.5 * µm, material=Material(EPS_SI))
+    d += Rectangle(position=((x0 + npx) * dx, H / 2 - 0.25 * µm), width=W - (x0 + npx) * dx, height=0.5 * µm,
+                   material=Material(EPS_SI))
+    dt = 0.99 * dx / (LIGHT_SPEED * np.sqrt(2))
+    t = np.arange(steps) * dt
+    d += GaussianSource(position=((npml + 10) * dx, H / 2), width=dx / 4, signal=pulse_signal(t))
+    freqs = LIGHT_SPEED / (np.linspace(1.50, 1.60, n_freq) * µm)
+    span = min(1.0 * µm, H / 2 - (npml + 1) * dx)
+    m_in = Monitor(design=d, start=((npml + 20) * dx, H / 2 - span), end=((npml + 20) * dx, H / 2 + span),
+                   dft_frequencies=freqs, record_fields=False, accumulate_power=False, name="in")
+    xo = (n - npml - 10) * dx
+    m_out = Monitor(design=d, start=(xo, H / 2 - span), end=(xo, H / 2 + span), dft_frequencies=freqs,
+                    record_fields=False, accumulate_power=False, name="out")
+    d += m_in
+    d += m_out
+    mesh = RegularGrid(d, dx)
+    if mesh.permittivity.shape != (n, n):
+        raise AssertionError(f"grid {mesh.permittivity.shape} != ({n}, {n})")
+    eps = np.array(mesh.permittivity)
+    dens = binary_density(npx, rng)
+    eps[y0:y0 + npx, x0:x0 + npx] = np.where(dens, EPS_SI, EPS_OX)
+    mesh = RegularGrid.from_arrays(d, eps, mesh.conductivity, dx)
+    return mesh, t, (m_in, m_out)
+
+
+def batch_2d(nsims=64, seed=0, nmin=256, nmax=1024, steps=2000, **kw):
+    """The batched 2D workload: nsims members with sides drawn uniformly from {nmin..nmax} cells
+    (side-uniform), each built by batch_member_2d with its own seeded density. Returns a list of
+    (mesh, time, monitors)."""
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(nmin, nmax + 1, size=nsims)
+    return [batch_member_2d(int(n), rng, steps=steps, **kw) for n in sides]
+
+
 def mmi_rectangles(W, H, rng, jitter=0.2):
     """Input guide, 2.5 x 6 um body and two output guides, each size scaled by 1 + U(-jitter, jitter)."""
     j = lambda: 1.0 + rng.uniform(-jitter, jitter)

@@ -6,15 +6,19 @@ Gaussian-pulse source, two x-normal DFT flux ports (8 wavelengths). A step is
 one full Yee step (H and E updates, source injection, port DFT accumulation)
 over the whole grid, with all inputs resident in HBM.
 
-``--workload wg2d`` runs the 2D TM straight-waveguide case instead (800 x 400
-cells, 20-cell PML, point source, 16-wavelength DFT line monitor).
+``--workload wg2d`` runs the 2D TM straight-waveguide case (800 x 400 cells,
+20-cell PML, point source, 16-wavelength DFT line monitor); ``--workload
+batch2d`` the batch of 64 independent 2D simulations (sides drawn from
+256..1024 cells by seed, binary 2 x 2 um design regions, 10-cell PML, input and
+output DFT ports at 8 wavelengths), all advanced by one launch per step.
 
-Prints one JSON line: value = cell-updates/s; roofline = the update kernel's
-algorithmic HBM bytes (3D: 56 B per cell-update, six FP32 components read and
-written plus FP32 permittivity and conductivity read; 2D: 32 B) over its mean
-launch time measured with HIP events on the launch stream; cpu_baseline = the
-FP64 NumPy restatement of the reference's step (complex128 fields, as the
-reference computes) timed on a bounded sample on one host core.
+Prints one JSON line: value = cell-updates/s (for the batch: summed over its
+members); roofline = the update kernel's algorithmic HBM bytes (3D: 56 B per
+cell-update, six FP32 components read and written plus FP32 permittivity and
+conductivity read; 2D: 32 B) over its mean launch time measured with HIP
+events on the launch stream; cpu_baseline = the FP64 NumPy restatement of the
+reference's step (complex128 fields, as the reference computes) timed on a
+bounded sample on one host core.
 """
 from __future__ import annotations
 
@@ -30,7 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
-B_ALG = {"mmi3d": 56, "wg2d": 32}
+B_ALG = {"mmi3d": 56, "wg2d": 32, "batch2d": 32}
 
 # HBM bytes per launch of the update kernel, from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of
 # this script with --copy-mib 512, each counter scaled by the factor that makes the 512 MiB calibration
@@ -76,6 +80,72 @@ def cpu_baseline_2d(sim, seconds_target=15.0):
     return sim.nx * sim.ny * n / el, n, el
 
 
+class Runner:
+    """What the timing loop needs from a workload: advance(k), the grid whose launches are timed."""
+
+    def __init__(self, advance, grid, cells, desc, kernel_name, cpu):
+        self.advance, self.grid, self.cells, self.desc = advance, grid, cells, desc
+        self.kernel_name, self.cpu = kernel_name, cpu
+
+
+def build_runner(a, total):
+    import beamz_amd as bz
+    from beamz_amd.workloads import batch_2d, soi_mmi_3d, waveguide_2d
+    if a.workload == "batch2d":
+        from beamz_amd.simulation.batch import FDTDBatch
+        members = batch_2d(nsims=a.nsims, steps=max(total + 4, 16))
+        sims = [bz.FDTD(m, time=t) for m, t, _ in members]
+        batch = FDTDBatch(sims, seg=a.seg2d, graph=a.graph, chunk=a.chunk)
+        batch.initialize()
+        tuned = None
+        if a.autotune:
+            tuned = batch.autotune(steps=10)
+        cells = batch.cells
+        sides = sorted(s.nx for s in sims)
+        desc = (f"{len(sims)} batched 2D TE sims, sides {sides[0]}..{sides[-1]} (seeded), 25 nm cells, "
+                f"binary 2x2 um designs, 10-cell PML, 2 DFT ports x 8 wavelengths, one launch per step")
+        kname = f"bz_yee2d_run (fused, batched, seg {batch.seg}{', hipGraph' if a.graph else ''})"
+
+        def cpu():
+            big = sims[int(np.argsort([s.nx for s in sims])[len(sims) // 2])]
+            v, n, el = cpu_baseline_2d(big)
+            return v, f"{n} reference-order complex128 steps of the median member ({big.nx}x{big.ny}) ({el:.1f} s)"
+        r = Runner(batch.advance, sims[0].grid, cells, desc, kname, cpu)
+        r.tuned = tuned
+        return r
+    if a.workload == "mmi3d":
+        nx, ny, nz = a.nx or 480, a.ny or 240, a.nz
+        mesh, d3, t, mons = soi_mmi_3d(nx, ny, nz, steps=max(total, 16))
+        sim = bz.FDTD(mesh, time=t)
+        cells = nx * ny * nz
+        desc = (f"3D SOI 1x2 MMI Yee step {nx}x{ny}x{nz} @25nm, 12-cell PML, dipole pulse, "
+                f"2 DFT flux ports x 8 wavelengths")
+
+        def cpu():
+            nzs = max(4, nz // 8)
+            v, n, el = cpu_baseline_3d(nx, ny, nzs, 25e-9, float(t[1] - t[0]))
+            return v, f"{n} steps of the NumPy complex128 3D Yee update on {nx}x{ny}x{nzs} cells ({el:.1f} s)"
+    else:
+        nx, ny = a.nx or 800, a.ny or 400
+        d, dx, t, mon = waveguide_2d(nx, ny, steps=max(total, 16))
+        sim = bz.FDTD(d, time=t, resolution=dx)
+        cells = nx * ny
+        desc = f"2D TM Si waveguide Yee step {nx}x{ny} @lambda/20, 20-cell PML, point pulse, 16-wavelength DFT line"
+
+        def cpu():
+            v, n, el = cpu_baseline_2d(sim)
+            return v, f"{n} reference-order complex128 steps (update, source, DFT-line samples) on {nx}x{ny} ({el:.1f} s)"
+    sim.grid.kernel = a.kernel
+    sim.grid.kseg = a.kseg
+    sim.grid.rows = a.rows
+    sim.grid.seg2d = a.seg2d
+    sim.grid.graph2d = a.graph
+    sim.initialize_simulation(save=False, live=False)
+    r = Runner(sim._advance, sim.grid, cells, desc, sim.grid.kernel_name(), cpu)
+    r.tuned = None
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,10 +155,15 @@ def main():
     ap.add_argument("--nx", type=int, default=None)
     ap.add_argument("--ny", type=int, default=None)
     ap.add_argument("--nz", type=int, default=160)
+    ap.add_argument("--nsims", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default=os.environ.get("BEAMZ_AMD_KERNEL", "fused"))
     ap.add_argument("--kseg", type=int, default=int(os.environ.get("BEAMZ_AMD_KSEG", "16")))
     ap.add_argument("--rows", type=int, default=int(os.environ.get("BEAMZ_AMD_ROWS", "2")))
+    ap.add_argument("--seg2d", type=int, default=None, help="rows per tile of the 2D engine (default: by size)")
+    ap.add_argument("--graph", action="store_true", help="capture 2D runs as HIP graphs")
+    ap.add_argument("--autotune", action="store_true", help="batch2d: pick the tile height by timing")
+    ap.add_argument("--chunk", type=int, default=100, help="batch2d: steps per engine call")
     ap.add_argument("--timing-steps", type=int, default=30)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (3 launches) as a counter calibration kernel")
@@ -97,27 +172,9 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
     import torch
-    import beamz_amd as bz
-    from beamz_amd.workloads import soi_mmi_3d, waveguide_2d
 
     total = a.warmup + a.steps + a.timing_steps
-    if a.workload == "mmi3d":
-        nx, ny, nz = a.nx or 480, a.ny or 240, a.nz
-        mesh, d3, t, mons = soi_mmi_3d(nx, ny, nz, steps=max(total, 16))
-        sim = bz.FDTD(mesh, time=t)
-        cells = nx * ny * nz
-        desc = (f"3D SOI 1x2 MMI Yee step {nx}x{ny}x{nz} @25nm, 12-cell PML, dipole pulse, "
-                f"2 DFT flux ports x 8 wavelengths")
-    else:
-        nx, ny = a.nx or 800, a.ny or 400
-        d, dx, t, mon = waveguide_2d(nx, ny, steps=max(total, 16))
-        sim = bz.FDTD(d, time=t, resolution=dx)
-        cells = nx * ny
-        desc = f"2D TM Si waveguide Yee step {nx}x{ny} @lambda/20, 20-cell PML, point pulse, 16-wavelength DFT line"
-    sim.grid.kernel = a.kernel
-    sim.grid.kseg = a.kseg
-    sim.grid.rows = a.rows
-    sim.initialize_simulation(save=False, live=False)
+    r = build_runner(a, total)
     if a.copy_mib > 0:
         # known traffic: each launch reads and writes copy_mib MiB (FETCH_SIZE / WRITE_SIZE calibration)
         src = torch.ones(a.copy_mib * (1 << 18), device="cuda")
@@ -126,45 +183,42 @@ def main():
             torch.mul(src, 1.0, out=dst)          # one elementwise kernel: reads and writes copy_mib MiB
         torch.cuda.synchronize()
         del src, dst
-    sim._advance(a.warmup)
+    r.advance(a.warmup)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record(stream)
-    sim._advance(a.steps)
+    r.advance(a.steps)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_s = ev0.elapsed_time(ev1) / 1e3
-    value = cells * a.steps / wall
-    # update-kernel duration: HIP events around each launch, on the launch stream, in a separate run
-    sim.grid.start_timing()
-    sim._advance(min(a.timing_steps, max(0, sim.num_steps - sim.current_step)))
-    lt = sim.grid.launch_timer()
-    sim.grid.timing = False
+    value = r.cells * a.steps / wall
+    # update-kernel duration: HIP events around each launch (or run of launches), on the launch stream
+    r.grid.start_timing()
+    r.advance(a.timing_steps)
+    lt = r.grid.launch_timer()
+    r.grid.timing = False
     step_kernel_s = lt if lt else gpu_s / a.steps
     b_alg = B_ALG[a.workload]
-    achieved = cells * b_alg / step_kernel_s
-    key = (a.workload, a.kernel, a.rows, a.kseg, nx, ny, nz if a.workload == "mmi3d" else 1)
+    achieved = r.cells * b_alg / step_kernel_s
+    nx, ny = r.grid.nx, r.grid.ny
+    key = (a.workload, a.kernel, a.rows, a.kseg, nx, ny, r.grid.nz if a.workload == "mmi3d" else 1)
     traffic = MEASURED_TRAFFIC.get(key)
     out = {
         "metric": "Yee cell-updates/s", "value": value, "unit": "cell-updates/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": desc, "cells": cells, "kernel": sim.grid.kernel_name(), "gpu_event_s": gpu_s},
+        "config": {"workload": r.desc, "cells": r.cells, "kernel": r.kernel_name, "gpu_event_s": gpu_s},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel_ms": step_kernel_s * 1e3, "bytes_per_cell_update": b_alg},
     }
+    if r.tuned is not None:
+        out["config"]["autotune_ms_per_step"] = {str(k): v for k, v in r.tuned[1].items()}
     if not a.no_cpu:
-        if a.workload == "mmi3d":
-            nzs = max(4, nz // 8)
-            v, n, el = cpu_baseline_3d(nx, ny, nzs, 25e-9, float(t[1] - t[0]))
-            sample = f"{n} steps of the NumPy complex128 3D Yee update on {nx}x{ny}x{nzs} cells ({el:.1f} s)"
-        else:
-            v, n, el = cpu_baseline_2d(sim)
-            sample = f"{n} reference-order complex128 steps (update, source, DFT-line samples) on {nx}x{ny} ({el:.1f} s)"
+        v, sample = r.cpu()
         out["cpu_baseline"] = {"value": v, "unit": "cell-updates/s", "cores": 1, "kind": "port", "sample": sample}
     print(json.dumps(out))
 

@@ -27,10 +27,12 @@
  *   bz_raster_2d/3d ....... RegularGrid.__rasterize__ / RegularGrid3D.__rasterize_3d__ +
  *                           _process_3d_pml, beamz/simulation/meshing.py:92-536, 616-781,
  *                           with PML.get_conductivity, beamz/design/pml.py:21-62
- *   bz_yee3d_run .......... fused multi-step 3D engine: fdtd.py:411-429 step order
- *                           (H, E, sources) for n steps in one call
- *   bz_yee2d_run .......... fused multi-step 2D engine (batched, temporally blocked):
- *                           fdtd.py:411-429 for many independent simulations
+ *   bz_yee3d_step ......... one fused 3D step: FDTD._update_3d_fields (fdtd.py:200-310) followed by
+ *                           the soft-source adds of helper.py:62-118, i.e. the H, E, sources part of
+ *                           the step order of fdtd.py:411-429, in one pass over HBM
+ *   bz_yee2d_run .......... fused multi-step 2D engine for many independent simulations at once:
+ *                           per step numpy_backend.py:34-88 (H, E), helper.py:96-118 (source),
+ *                           helper.py:231-256 (monitor samples + DFT), helper.py:120-187 (power)
  */
 #ifndef BEAMZ_AMD_H
 #define BEAMZ_AMD_H
@@ -80,7 +82,7 @@ typedef struct bz_box_source {
     float amp;              /* signal value of this step */
 } bz_box_source;
 
-/* kseg: z-planes per wave segment (<= 0: default 16); rows: y-rows per wave (2, 4 or 8). */
+/* kseg: z-planes per wave segment (<= 0: default 16); rows: y-rows per wave (2 or 4). */
 int bz_yee3d_step(float* const src[6], float* const dst[6], const float* eps_r, const float* sigma,
                   int nx, int ny, int nz, int ld, double dx, double dy, double dz, double dt,
                   double mu0, double eps0, const bz_box_source* boxes, int nbox, int kseg, int rows, void* stream);
@@ -88,12 +90,18 @@ int bz_yee3d_step(float* const src[6], float* const dst[6], const float* eps_r, 
 /* ---------------- fused, batched 2D engine ----------------
  * Runs nsteps full steps (numpy_backend.py:34-88, then the source box, then optional grid power,
  * then monitor sampling) of every simulation described in `sims` (device array), starting at the
- * step held in device memory `step` (incremented once per step). Field buffers are ping-ponged by
- * step parity: step n reads buffer n & 1 and writes buffer (n + 1) & 1.
- * tiles: device int[3*ntiles] = (sim, x-tile of 63 columns, y-tile of `seg` rows).
+ * step held in device memory `step` (advanced by nsteps when the run's work completes; the call
+ * only enqueues launches, so it can be captured in a hipGraph and replayed). Field buffers are
+ * ping-ponged by step parity: step n reads buffer n & 1 and writes buffer (n + 1) & 1.
+ * tiles: device int[3*ntiles] = (sim, x-tile of 62 columns, y-tile of `seg` rows), seg one of
+ * 2, 4, 8, 16; every array of a simulation must hold fewer than 2^31 elements.
  * pts: device int[3*npts] = (sim, comp 0 Ez / 1 Hx / 2 Hy, offset); slot: device int[steps],
  * slot[n] = recorded row of step n or -1; series [rows][npts] FP32 (may be NULL);
- * phasor [rows][nfreq] complex FP64; dft [nfreq][npts] complex FP64 accumulators. */
+ * phasor [rows][nfreq] complex FP64; dft [nfreq][npts] complex FP64 accumulators.
+ * Snapshot box (snap_sx > 0): the q-th step of the call (q = 0..nsteps-1) stores, for every cell
+ * (j, i) of the box, the Ez it read and the Hx, Hy it wrote, at
+ * snap[((q*3 + c)*snap_sy + j - snap_y0)*snap_sx + i - snap_x0], c = 0 Ez, 1 Hx, 2 Hy
+ * (the forward states the reverse-time adjoint needs inside a design region). */
 typedef struct bz_sim2d {
     int nx, ny, ld;
     float* ez[2];
@@ -108,11 +116,50 @@ typedef struct bz_sim2d {
     const float* src_mul;                 /* NULL means 1 */
     const float* signal;                  /* device FP32 amplitude per step (NULL: no source) */
     double* power;                        /* NULL or [ny][ld] FP64 grid-power accumulator */
+    int snap_x0, snap_y0, snap_sx, snap_sy;   /* snapshot box (snap_sx = 0: none) */
+    float* snap;                          /* [nsteps][3][snap_sy][snap_sx] */
 } bz_sim2d;
 
 int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step, int nsteps,
                  const int* pts, int npts, const int* slot, float* series, int nfreq, const double* phasor,
                  double* dft, void* stream);
+
+/* ---------------- reverse-time 2D adjoint ----------------
+ * No reference counterpart (the reference's gradient is the overlap heuristic of
+ * examples/4_topo.py:282-285). Exact transpose of the bz_yee2d_run step for the objective
+ * J = |a|^2, a = sum_n sum_p c_p Ez_{n+1}(p) phasor[n]: reverse step n maps the adjoint state
+ * lam_{n+1} (buffer parity (n+1)&1) to lam_n (parity n&1) and, inside the design box, adds
+ * dJ/d eps_r of step n to grad using the forward snapshot that bz_yee2d_run stored for the
+ * segment (snapshot box = design box grown by one cell on the low x and y sides, slot
+ * n - snap_step0). The design box must lie in the interior (1 <= x, x + sx <= nx - 1, same in y).
+ * inj_w: complex dJ/dEz-sample weight 2 conj(a) c_p per cell of the injection box. */
+typedef struct bz_adj2d {
+    int nx, ny, ld;
+    float* le[2];
+    float* lhx[2];
+    float* lhy[2];
+    const float* eps;
+    const float* sig;
+    float kh, kbh, ke, kbe;
+    float rdx, rdy;
+    int src_x0, src_y0, src_sx, src_sy;
+    const float* src_mul;                 /* the forward source box's multiplier (NULL: additive source) */
+    int box_x0, box_y0, box_sx, box_sy;   /* design box */
+    const float* snap;                    /* [K][3][box_sy+1][box_sx+1] forward snapshot of the segment */
+    int snap_step0;
+    double* grad;                         /* [box_sy][box_sx] FP64 accumulator (NULL: no gradient) */
+    int inj_x0, inj_y0, inj_sx, inj_sy;
+    const double* inj_w;                  /* [inj_sy][inj_sx] complex, or NULL */
+    const double* phasor;                 /* [num_steps] complex phasor of the sample after step n */
+} bz_adj2d;
+
+/* Reverse steps n = n_hi-1 down to n_hi-nsteps of every adjoint described in `adjs` (device
+ * array); tiles as for bz_yee2d_run (seg 2, 4 or 8). */
+int bz_adj2d_run(const bz_adj2d* adjs, int nadj, const int* tiles, int ntiles, int seg, int n_hi, int nsteps,
+                 void* stream);
+/* lE_n += Re(inj_w * phasor[n-1]) over each injection box (max_inj >= the largest box's cell count):
+ * the terminal condition lam_N = dJ/du_N before the first reverse step. */
+int bz_adj2d_inject(const bz_adj2d* adjs, int nadj, int max_inj, int n, void* stream);
 
 /* ---------------- sources ---------------- */
 /* field[off[p]] += w[p] * amp for p < n (offsets unique within one call). */

@@ -1,0 +1,129 @@
+"""GPU: the reverse-time adjoint kernel against the FP64 adjoint oracle (itself pinned by
+central differences), against FP32 finite differences on the GPU, across checkpoint
+intervals, and through the torch.autograd chain of the density filters.
+
+Tolerances: objective J within 1e-4 relative of the FP64 oracle; dJ/d eps_r within 1e-3
+rel-L2 of the FP64 oracle gradient (FP32 fields and adjoint, FP64 gradient sums)."""
+import numpy as np
+import pytest
+
+import beamz_amd as bz
+from beamz_amd.devices.signals import gaussian_pulse
+from beamz_amd.optimization.adjoint import AdjointProblem2D, FDTDObjective, density_to_eps
+from beamz_amd.simulation.meshing import RegularGrid
+from oracle import adjoint_ref as A
+from oracle import fdtd_ref as R
+from oracle import topo_ref as TR
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, DX = 70, 56, 50e-9
+BOX = (25, 18, 20, 20)
+F0 = bz.LIGHT_SPEED / 1.55e-6
+
+
+def build(eps, sig, nsteps):
+    d = bz.Design(NX * DX * (1 + 1e-9), NY * DX * (1 + 1e-9), material=bz.Material(1.0), auto_pml=False)
+    dt = 0.99 * DX / (bz.LIGHT_SPEED * np.sqrt(2))
+    t = np.arange(nsteps) * dt
+    st = 1 / (2 * np.pi * bz.LIGHT_SPEED * 0.3e-6 / 1.55e-6 ** 2)
+    sig_t = gaussian_pulse(t, 1.0, 4 * st, st, F0, 0.0)
+    src = bz.GaussianSource(position=(12 * DX, NY / 2 * DX), width=DX / 4, signal=sig_t)
+    mon = bz.Monitor(design=d, start=((NX - 12) * DX, 8 * DX), end=((NX - 12) * DX, (NY - 8) * DX),
+                     dft_frequencies=[F0], record_fields=False, accumulate_power=False)
+    d += src
+    d += mon
+    mesh = RegularGrid.from_arrays(d, eps, sig, DX)
+    assert mesh.permittivity.shape == (NY, NX)
+    return bz.FDTD(mesh, time=t), mon, src, t
+
+
+def materials(seed=0):
+    rng = np.random.default_rng(seed)
+    eps = np.full((NY, NX), 1.444 ** 2)
+    eps[NY // 2 - 3:NY // 2 + 3, :] = 3.476 ** 2
+    x0, y0, sx, sy = BOX
+    eps[y0:y0 + sy, x0:x0 + sx] = 1.444 ** 2 + rng.random((sy, sx)) * (3.476 ** 2 - 1.444 ** 2)
+    sig = np.zeros((NY, NX))
+    for k in range(6):
+        v = 4e4 * ((6 - k) / 6) ** 3
+        sig[:, k] = np.maximum(sig[:, k], v); sig[:, NX - 1 - k] = np.maximum(sig[:, NX - 1 - k], v)
+        sig[k, :] = np.maximum(sig[k, :], v); sig[NY - 1 - k, :] = np.maximum(sig[NY - 1 - k, :], v)
+    return eps, sig, rng
+
+
+def oracle_problem(sim, mon, src, eps, sig, c):
+    patch = R.gaussian_patch_2d(src.position, src.width, sim.dx, sim.dy, NX, NY)
+    pts = R.monitor_points_2d(mon.start, mon.end, sim.dx, sim.dy)
+    return A.Problem2D(eps, sig, sim.dx, sim.dy, sim.dt, sim.num_steps, patch, src.signal, pts, c, 2 * np.pi * F0)
+
+
+def test_adjoint_gradient_matches_fp64_oracle_and_checkpointing(hip):
+    torch = hip
+    eps, sig, rng = materials()
+    nsteps = 420
+    grads = {}
+    for K in (20, 23):
+        sim, mon, src, t = build(eps, sig, nsteps)
+        npts = len(mon.get_grid_points_2d(DX, DX))
+        c = np.random.default_rng(7).standard_normal(npts) + 1j * np.random.default_rng(8).standard_normal(npts)
+        prob = AdjointProblem2D([sim], [mon], c, BOX, checkpoint_every=K)
+        x0, y0, sx, sy = BOX
+        e_box = torch.from_numpy(eps[y0:y0 + sy, x0:x0 + sx]).float().cuda()[None]
+        J = prob.forward(e_box).cpu().numpy()
+        grads[K] = prob.gradient().cpu().numpy()[0]
+    P = oracle_problem(sim, mon, src, eps.astype(np.float32).astype(np.float64), sig.astype(np.float32).astype(np.float64), c)
+    g_ref = P.gradient()
+    assert abs(J[0] - P.J) <= 1e-4 * P.J, (J[0], P.J)
+    gb = g_ref[y0:y0 + sy, x0:x0 + sx]
+    err = np.linalg.norm(grads[20] - gb) / np.linalg.norm(gb)
+    assert err < 1e-3, err
+    assert np.array_equal(grads[20], grads[23])          # checkpoint interval changes nothing
+
+
+def test_adjoint_gradient_matches_gpu_finite_differences(hip):
+    torch = hip
+    eps, sig, rng = materials(seed=1)
+    sim, mon, src, t = build(eps, sig, 360)
+    npts = len(mon.get_grid_points_2d(DX, DX))
+    c = np.ones(npts, dtype=complex) / np.sqrt(npts)
+    prob = AdjointProblem2D([sim], [mon], c, BOX)
+    x0, y0, sx, sy = BOX
+    e0 = torch.from_numpy(eps[y0:y0 + sy, x0:x0 + sx]).float().cuda()[None]
+    prob.forward(e0)
+    g = prob.gradient().cpu().numpy()[0]
+    idx = np.argsort(np.abs(g).ravel())[-4:]
+    for flat in idx:
+        j, i = divmod(int(flat), sx)
+        h = 0.02 * float(e0[0, j, i])
+        ep = e0.clone(); ep[0, j, i] += h
+        em = e0.clone(); em[0, j, i] -= h
+        fd = (prob.forward(ep).item() - prob.forward(em).item()) / (2 * h)
+        assert abs(fd - g[j, i]) <= 0.02 * abs(g[j, i]), (j, i, fd, g[j, i])
+
+
+def test_autograd_chain_through_density_filters(hip):
+    torch = hip
+    eps, sig, rng = materials(seed=2)
+    sim, mon, src, t = build(eps, sig, 300)
+    npts = len(mon.get_grid_points_2d(DX, DX))
+    c = np.exp(1j * np.linspace(0, 2, npts)) / np.sqrt(npts)
+    prob = AdjointProblem2D([sim], [mon], c, BOX)
+    x0, y0, sx, sy = BOX
+    ec, eh = 1.444 ** 2, 3.476 ** 2
+    rho0 = np.clip(0.5 + 0.05 * np.random.default_rng(3).standard_normal((sy, sx)), 0, 1)
+    rho = torch.tensor(rho0[None], dtype=torch.float32, device="cuda", requires_grad=True)
+    e_box = density_to_eps(rho, ec, eh, radius=3, beta=8.0)
+    J = FDTDObjective.apply(e_box, prob)
+    J.sum().backward()
+    g_rho = rho.grad.cpu().numpy()[0]
+    # the same chain on the host with the oracle's filter transpose and projection derivative
+    filt = TR.cone_filter(rho0, 3)
+    proj, dproj = TR.tanh_projection(filt, 8.0, 0.5)
+    eps_full = eps.astype(np.float32).astype(np.float64)
+    eps_full[y0:y0 + sy, x0:x0 + sx] = (ec + proj * (eh - ec)).astype(np.float32)
+    P = oracle_problem(sim, mon, src, eps_full, sig.astype(np.float32).astype(np.float64), c)
+    g_eps = P.gradient()[y0:y0 + sy, x0:x0 + sx]
+    g_ref = TR.cone_filter_T(g_eps * (eh - ec) * dproj, 3)
+    err = np.linalg.norm(g_rho - g_ref) / np.linalg.norm(g_ref)
+    assert err < 2e-3, err

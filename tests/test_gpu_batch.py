@@ -1,0 +1,103 @@
+"""GPU: the batched 2D engine (many uneven simulations per launch).
+
+A member of a batch must take bit-for-bit the steps it takes alone, whatever
+the tile height; graph replays must equal eager launches after every replay;
+and a member must match the FP64 oracle within 1e-4 rel-L2 (fields, DFT)."""
+import numpy as np
+import pytest
+
+import beamz_amd as bz
+from beamz_amd.simulation.batch import FDTDBatch
+from beamz_amd.workloads import batch_member_2d
+from oracle import fdtd_ref as R
+
+pytestmark = pytest.mark.gpu
+
+SIDES = (96, 131, 170)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.complex128); b = np.asarray(b, dtype=np.complex128)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def members(steps=300, seed=3):
+    rng = np.random.default_rng(seed)
+    out = []
+    for n in SIDES:
+        mesh, t, mons = batch_member_2d(n, rng, steps=steps, design_um=1.0)
+        out.append((bz.FDTD(mesh, time=t), mons))
+    return out
+
+
+def state(sim, mons):
+    d = {k: np.real(getattr(sim, k)).copy() for k in ("Ez", "Hx", "Hy")}
+    for m in mons:
+        for k, v in m.dft.items():
+            d[m.name + k] = np.array(v)
+    return d
+
+
+def test_batch_equals_single_runs_and_oracle(hip):
+    single = []
+    for sim, mons in members():
+        sim.run(live=False, save=False)
+        single.append(state(sim, mons))
+    mem = members()
+    batch = FDTDBatch([s for s, _ in mem], seg=8, chunk=64)
+    batch.run()
+    for (sim, mons), ref in zip(mem, single):
+        got = state(sim, mons)
+        for k in ref:
+            assert np.array_equal(got[k], ref[k]), (sim.nx, k)
+    # one member against the FP64 oracle
+    sim, mons = mem[1]
+    ref = R.RefSim(sim.epsilon_r, sim.sigma, sim.dx, sim.dy, sim.dt, sim.num_steps, sources=sim.sources,
+                   monitors=list(mons)).run()
+    for k in ("Ez", "Hx", "Hy"):
+        assert rel_l2(np.real(getattr(sim, k)), ref.F[k]) < 1e-4, k
+    for q, m in enumerate(mons):
+        for k in ("Ez", "Hx", "Hy"):
+            F = R.dft_of_series(np.array(ref.mon_series[q][k]), ref.mon_series[q]["t"], 2 * np.pi * m.dft_frequencies)
+            assert rel_l2(m.dft[k], F) < 1e-4, (m.name, k)
+    T = mons[1].dft_flux() / mons[0].dft_flux()
+    assert T.shape == (8,) and np.all(np.isfinite(T))
+
+
+def test_batch_graph_replays_equal_eager(hip):
+    torch = hip
+    a = members(steps=320)
+    b = members(steps=320)
+    eager = FDTDBatch([s for s, _ in a], seg=8, graph=False, chunk=40)
+    graph = FDTDBatch([s for s, _ in b], seg=8, graph=True, chunk=40)
+    eager.initialize()
+    graph.initialize()
+    for r in range(8):                      # the 40-step graph is captured once and replayed 8 times
+        eager.advance(40)
+        graph.advance(40)
+        junk = torch.randn(1 << 20, device="cuda").sum()       # unrelated eager work between replays
+        torch.cuda.synchronize()
+        assert torch.isfinite(junk)
+        for (sa, _), (sb, _) in zip(a, b):
+            for k in ("Ez", "Hx", "Hy"):
+                assert torch.equal(sa.field(k), sb.field(k)), (r, sa.nx, k)
+    for (sa, ma), (sb, mb) in zip(a, b):
+        sa.finalize_simulation(); sb.finalize_simulation()
+        A, B = state(sa, ma), state(sb, mb)
+        for k in A:
+            assert np.array_equal(A[k], B[k]), k
+
+
+def test_batch_autotune_keeps_results(hip):
+    ref_mem = members(steps=200)
+    FDTDBatch([s for s, _ in ref_mem], seg=8).run()
+    mem = members(steps=200)
+    batch = FDTDBatch([s for s, _ in mem], seg=8)
+    best, timings = batch.autotune(steps=10)
+    assert best in FDTDBatch.SEGS and set(timings) == set(FDTDBatch.SEGS)
+    batch.advance(batch.num_steps)
+    for (s, m), (r, rm) in zip(mem, ref_mem):
+        s.finalize_simulation()
+        A, B = state(s, m), state(r, rm)
+        for k in A:
+            assert np.array_equal(A[k], B[k]), (best, k)
