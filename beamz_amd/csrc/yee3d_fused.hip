@@ -32,6 +32,10 @@
 // halo rows / lanes (mostly L2 hits) and one extra plane per segment.
 #include "common.h"
 
+// No multiply-add contraction in this file or in yee3d_tb2.hip: both kernels then evaluate every
+// update with the same rounded operations, which is what makes them bit-identical.
+#pragma clang fp contract(off)
+
 namespace {
 
 constexpr int LANES = 64;

@@ -100,6 +100,10 @@ class Grid:
         self.graph2d = False    # capture fused 2D runs as HIP graphs
         self.kseg = 16
         self.rows = 4
+        self.tb2 = True         # 3D: two steps per launch where the step list allows it
+        self.kseg2 = 32         # z-planes per segment of the two-step kernel
+        self.rows2 = 2          # y-rows per wave of the two-step kernel (2, 3 or 4)
+        self._stage = None
         self.timing = False
         self._events = []
 
@@ -110,6 +114,8 @@ class Grid:
 
     def kernel_name(self):
         if self.is_3d:
+            if self.kernel == "fused" and self.tb2:
+                return f"bz_yee3d_step2 (two steps per pass, {self.rows2} rows/wave, {self.kseg2}-plane segments)"
             return f"bz_yee3d_step (fused z-march, {self.rows} rows/wave)" if self.kernel == "fused" else "bz_update_3d (two-pass)"
         return "bz_yee2d_run (fused, batched)" if self.kernel == "fused" else "bz_update_h_2d + bz_update_e_2d"
 
@@ -222,6 +228,7 @@ class Grid:
             p.phasor = p.dft = None
         self.monitors.append(p)
         self._f2d = None
+        self._stage = None
         return p
 
     def enable_power(self):
@@ -326,12 +333,93 @@ class Grid:
     def run(self, n0, count):
         if not self.is_3d and self.kernel == "fused" and self._fused2d(n0, count):
             return
-        for n in range(n0, n0 + count):
-            self.step(n)
+        n, end = n0, n0 + count
+        while n < end:
+            if end - n >= 2 and self._pair_ok():
+                self._fused3d_pair(n)
+                n += 2
+            else:
+                self.step(n)
+                n += 1
+
+    # ---------------- two-step 3D path ----------------
+    def _pair_ok(self):
+        """The two-step kernel applies: 3D fused path, box sources only, no grid power (needs every step)."""
+        if not (self.is_3d and self.kernel == "fused" and self.tb2) or self.power is not None:
+            return False
+        if len(self.sources) > N.MAX_BOX or any(s.box is None for s in self.sources):
+            return False
+        if self._stage is None:
+            self._stage = self._setup_stage()
+        return self._stage is not False
+
+    def _setup_stage(self, max_cells=1 << 23):
+        """Staging box covering every monitor sample, for the samples of the intermediate step."""
+        T = self.torch
+        plans = [p for p in self.monitors if p.n > 0]
+        if not plans:
+            return {"plans": []}
+        off = np.concatenate([p.off.cpu().numpy() for p in plans])
+        plane = self.ny * self.ld
+        k, rem = off // plane, off % plane
+        j, i = rem // self.ld, rem % self.ld
+        x0, y0, z0 = int(i.min()), int(j.min()), int(k.min())
+        sx, sy, sz = int(i.max()) - x0 + 1, int(j.max()) - y0 + 1, int(k.max()) - z0 + 1
+        cells = sx * sy * sz
+        if cells > max_cells:
+            return False
+        buf = T.zeros(6 * cells, dtype=T.float32, device=self.dev)
+        for p in plans:
+            o = p.off.cpu().numpy()
+            pk, pr = o // plane, o % plane
+            pj, pi = pr // self.ld, pr % self.ld
+            p.stage_off = T.from_numpy(((pk - z0) * sy + (pj - y0)) * sx + (pi - x0)).to(self.dev)
+        return {"plans": plans, "buf": buf, "box": (C.c_int * 6)(x0, y0, z0, sx, sy, sz),
+                "base": N.ptrs6([buf.data_ptr() + 4 * q * cells for q in range(6)])}
+
+    def _fused3d_pair(self, n):
+        """Steps n and n+1 in one launch of the two-step kernel, then the monitor samples of both."""
+        arr = (N.BoxSource * N.MAX_BOX)()
+        amp1 = (C.c_float * N.MAX_BOX)()
+        for q, s in enumerate(self.sources):
+            b = s.box[0]
+            b.amp = float(np.real(s.signal[n]))
+            arr[q] = b
+            amp1[q] = float(np.real(s.signal[n + 1]))
+        stg = self._stage
+        need = any(p.slot.get(n) is not None for p in stg["plans"])
+        src, dst = self._fb[self._cur], self._fb[1 - self._cur]
+        ps = N.ptrs6([src[k].data_ptr() for k in COMP3])
+        pd = N.ptrs6([dst[k].data_ptr() for k in COMP3])
+        st = self.stream
+        if self.timing:
+            e0, e1 = self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True)
+            e0.record()
+        N.call("bz_yee3d_step2", ps, pd, self.eps.data_ptr(), self.sig.data_ptr(), self.nx, self.ny, self.nz,
+               self.ld, self.dx, self.dy, self.dz, self.dt, MU_0, EPS_0, arr, amp1, len(self.sources),
+               stg["buf"].data_ptr() if need else None, stg["box"] if need else None, self.kseg2, self.rows2, st)
+        if self.timing:
+            e1.record()
+            self._events.append((e0, e1, 2))
+        self._cur = 1 - self._cur
+        if need:
+            for p in stg["plans"]:
+                q = p.slot.get(n)
+                if q is None:
+                    continue
+                ser = 0 if p.series is None else p.series[q].data_ptr()
+                if p.nfreq:
+                    N.call("bz_gather", stg["base"], p.comp.data_ptr(), p.stage_off.data_ptr(), p.n, ser, p.nfreq,
+                           p.phasor[q].data_ptr(), p.dft.data_ptr(), st)
+                else:
+                    N.call("bz_gather", stg["base"], p.comp.data_ptr(), p.stage_off.data_ptr(), p.n, ser, 0, None,
+                           None, st)
+        self._gathers(n + 1)
 
     # ---------------- fused, batched 2D path ----------------
     def invalidate(self):
         self._f2d = None
+        self._stage = None
 
     def _fused2d(self, n0, count):
         if self._f2d is None:

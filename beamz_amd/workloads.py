@@ -111,6 +111,36 @@ def batch_2d(nsims=64, seed=0, nmin=256, nmax=1024, steps=2000, **kw):
     return [batch_member_2d(int(n), rng, steps=steps, **kw) for n in sides]
 
 
+def topo_bend_2d(n=600, design_px=120, steps=3000, npml=20, dx=20e-9, lam0=1.55 * µm):
+    """The adjoint topology-optimisation workload: an n x n cell domain (SiO2), a 0.5 um Si input guide
+    along +x at mid-height ending at the design box, a 0.5 um Si output guide along +y from the top of the
+    design box to the top edge (a 90-degree bend, as in the reference's topology example), a
+    design_px x design_px cell design box at the centre, 20-cell PML, Gaussian-pulse point source in the
+    input guide and a horizontal DFT line monitor across the output guide at lam0.
+    Returns (mesh, time, monitor, box, mode_weights)."""
+    from .optimization.adjoint import mode_weights
+    from .simulation.meshing import RegularGrid
+    W = H = (n + 1e-6) * dx
+    x0 = y0 = (n - design_px) // 2
+    d = Design(W, H, material=Material(EPS_OX), auto_pml=False)
+    d.boundaries += edge_pmls(W, H, npml, dx)
+    d += Rectangle(position=(0, H / 2 - 0.25 * µm), width=x0 * dx, height=0.5 * µm, material=Material(EPS_SI))
+    d += Rectangle(position=(W / 2 - 0.25 * µm, (y0 + design_px) * dx), width=0.5 * µm,
+                   height=H - (y0 + design_px) * dx, material=Material(EPS_SI))
+    dt = 0.99 * dx / (LIGHT_SPEED * np.sqrt(2))
+    t = np.arange(steps) * dt
+    d += GaussianSource(position=((npml + 20) * dx, H / 2), width=dx / 4, signal=pulse_signal(t, lam0))
+    ym = (n - npml - 20) * dx
+    mon = Monitor(design=d, start=(W / 2 - 1.0 * µm, ym), end=(W / 2 + 1.0 * µm, ym),
+                  dft_frequencies=[LIGHT_SPEED / lam0], record_fields=False, accumulate_power=False, name="out")
+    d += mon
+    mesh = RegularGrid(d, dx)
+    if mesh.permittivity.shape != (n, n):
+        raise AssertionError(f"grid {mesh.permittivity.shape} != ({n}, {n})")
+    c = mode_weights(mon, dx, dx, W / 2, 0.5 * µm, 3.476, 1.444, lam0)
+    return mesh, t, mon, (x0, y0, design_px, design_px), c
+
+
 def mmi_rectangles(W, H, rng, jitter=0.2):
     """Input guide, 2.5 x 6 um body and two output guides, each size scaled by 1 + U(-jitter, jitter)."""
     j = lambda: 1.0 + rng.uniform(-jitter, jitter)

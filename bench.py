@@ -34,7 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
-B_ALG = {"mmi3d": 56, "wg2d": 32, "batch2d": 32}
+B_ALG = {"mmi3d": 56, "wg2d": 32, "batch2d": 32, "adjoint2d": 32}
 
 # HBM bytes per launch of the update kernel, from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of
 # this script with --copy-mib 512, each counter scaled by the factor that makes the 512 MiB calibration
@@ -90,7 +90,44 @@ class Runner:
 
 def build_runner(a, total):
     import beamz_amd as bz
-    from beamz_amd.workloads import batch_2d, soi_mmi_3d, waveguide_2d
+    from beamz_amd.workloads import batch_2d, soi_mmi_3d, topo_bend_2d, waveguide_2d
+    if a.workload == "adjoint2d":
+        import torch
+        from beamz_amd.optimization.adjoint import AdjointProblem2D, FDTDObjective, density_to_eps
+        B = a.designs
+        members = [topo_bend_2d(steps=a.nt) for _ in range(B)]
+        sims = [bz.FDTD(m, time=t) for m, t, _, _, _ in members]
+        box, c = members[0][3], members[0][4]
+        prob = AdjointProblem2D(sims, [m[2] for m in members], c, box)
+        rng = np.random.default_rng(0)
+        rho0 = np.clip(0.5 + 0.05 * rng.standard_normal((B, box[3], box[2])), 0.0, 1.0)
+        rho = torch.tensor(rho0, dtype=torch.float32, device="cuda", requires_grad=True)
+        ec, eh = 1.444 ** 2, 3.476 ** 2
+
+        def iteration(k):
+            for _ in range(int(k)):
+                rho.grad = None
+                J = FDTDObjective.apply(density_to_eps(rho, ec, eh, radius=3, beta=8.0), prob)
+                J.sum().backward()
+        cells = 3 * B * sims[0].nx * sims[0].ny * a.nt        # forward, recompute, reverse sweeps per gradient
+        desc = (f"adjoint gradient of a mode-overlap FOM, batch of {B} designs, {sims[0].nx}x{sims[0].ny} 2D grid, "
+                f"{box[2]}x{box[3]} design px, {a.nt} steps, sqrt(N) checkpoints; a step = one batched gradient "
+                f"(forward + recompute + reverse sweeps)")
+
+        def cpu():
+            from oracle import fdtd_ref as R
+            s = sims[0]
+            ref = R.RefSim(s.epsilon_r, s.sigma, s.dx, s.dy, s.dt, s.num_steps, sources=s.sources, dtype=np.complex128)
+            n, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < 15.0 and n < s.num_steps:
+                ref.step(); n += 1
+            el = time.perf_counter() - t0
+            return s.nx * s.ny * n / el, (f"{n} reference-order complex128 steps of one {s.nx}x{s.ny} design "
+                                         f"({el:.1f} s); the reference's loop needs two such runs per gradient")
+        r = Runner(iteration, sims[0].grid, cells, desc, "bz_yee2d_run + bz_adj2d_run", cpu)
+        r.tuned = None
+        r.per_step_cells = True
+        return r
     if a.workload == "batch2d":
         from beamz_amd.simulation.batch import FDTDBatch
         members = batch_2d(nsims=a.nsims, steps=max(total + 4, 16))
@@ -138,6 +175,9 @@ def build_runner(a, total):
     sim.grid.kernel = a.kernel
     sim.grid.kseg = a.kseg
     sim.grid.rows = a.rows
+    sim.grid.tb2 = bool(a.tb2)
+    sim.grid.rows2 = a.rows2
+    sim.grid.kseg2 = a.kseg2
     sim.grid.seg2d = a.seg2d
     sim.grid.graph2d = a.graph
     sim.initialize_simulation(save=False, live=False)
@@ -156,10 +196,15 @@ def main():
     ap.add_argument("--ny", type=int, default=None)
     ap.add_argument("--nz", type=int, default=160)
     ap.add_argument("--nsims", type=int, default=64)
+    ap.add_argument("--designs", type=int, default=16, help="adjoint2d: designs per batched gradient")
+    ap.add_argument("--nt", type=int, default=3000, help="adjoint2d: time steps per simulation")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default=os.environ.get("BEAMZ_AMD_KERNEL", "fused"))
     ap.add_argument("--kseg", type=int, default=int(os.environ.get("BEAMZ_AMD_KSEG", "16")))
     ap.add_argument("--rows", type=int, default=int(os.environ.get("BEAMZ_AMD_ROWS", "2")))
+    ap.add_argument("--tb2", type=int, default=1, help="3D: two steps per launch (1) or one (0)")
+    ap.add_argument("--rows2", type=int, default=2, help="3D two-step kernel: y-rows per wave (2, 3, 4)")
+    ap.add_argument("--kseg2", type=int, default=32, help="3D two-step kernel: z-planes per segment")
     ap.add_argument("--seg2d", type=int, default=None, help="rows per tile of the 2D engine (default: by size)")
     ap.add_argument("--graph", action="store_true", help="capture 2D runs as HIP graphs")
     ap.add_argument("--autotune", action="store_true", help="batch2d: pick the tile height by timing")
@@ -202,9 +247,13 @@ def main():
     r.grid.timing = False
     step_kernel_s = lt if lt else gpu_s / a.steps
     b_alg = B_ALG[a.workload]
-    achieved = r.cells * b_alg / step_kernel_s
+    launch_cells = r.cells if not getattr(r, "per_step_cells", False) else r.cells // (3 * a.nt)
+    achieved = launch_cells * b_alg / step_kernel_s
     nx, ny = r.grid.nx, r.grid.ny
-    key = (a.workload, a.kernel, a.rows, a.kseg, nx, ny, r.grid.nz if a.workload == "mmi3d" else 1)
+    if a.workload == "mmi3d" and a.tb2:
+        key = (a.workload, "fused-tb2", a.rows2, a.kseg2, nx, ny, r.grid.nz)
+    else:
+        key = (a.workload, a.kernel, a.rows, a.kseg, nx, ny, r.grid.nz if a.workload == "mmi3d" else 1)
     traffic = MEASURED_TRAFFIC.get(key)
     out = {
         "metric": "Yee cell-updates/s", "value": value, "unit": "cell-updates/s",
