@@ -66,6 +66,19 @@ SIGNATURES = {
                        C.POINTER(f32), i32, vp, C.POINTER(i32), i32, i32, vp],
     "bz_yee2d_run": [vp, i32, vp, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp],
     "bz_adj2d_run": [vp, i32, vp, i32, i32, i32, i32, vp],
+    "bz_nn_conv3x3": [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp],
+    "bz_nn_conv3x3_dgrad": [vp, vp, vp, i32, i32, i32, i32, i32, vp],
+    "bz_nn_conv3x3_wgrad": [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp],
+    "bz_nn_conv_first": [vp, vp, vp, vp, i32, i32, i32, i32, vp],
+    "bz_nn_conv_first_bwd": [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp],
+    "bz_nn_relu_mask": [vp, vp, i64, vp],
+    "bz_nn_avgpool2": [vp, vp, i32, i32, i32, i32, vp],
+    "bz_nn_avgpool2_bwd": [vp, vp, i32, i32, i32, i32, vp],
+    "bz_nn_head": [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp],
+    "bz_nn_head_bwd": [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp],
+    "bz_nn_weights": [vp, vp, vp, i32, i32, vp],
+    "bz_nn_adam": [vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp],
+    "bz_nn_to_bf16": [vp, vp, i64, vp],
     "bz_adj2d_inject": [vp, i32, i32, i32, vp],
     "bz_add_points": [vp, vp, vp, i32, f32, vp],
     "bz_affine_points": [vp, vp, vp, vp, i32, f32, vp],

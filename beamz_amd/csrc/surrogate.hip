@@ -1,0 +1,471 @@
+// BF16-MFMA / FP32-accumulate kernels of the CNN surrogate (new; no reference
+// counterpart): 3x3 convolutions forward, backward-data and backward-weights
+// as implicit GEMMs on v_mfma_f32_32x32x16_bf16, plus the small layers
+// (first convolution with one input channel, ReLU mask, 2x2 average pooling,
+// global average pooling + fully connected head, Adam).
+//
+// Layout: activations NHWC in bfloat16 (stored as uint16 bit patterns),
+// convolution weights [Co][3][3][Ci] (K = tap*Ci + ci contiguous), FP32 master
+// weights, gradients and biases.
+//
+// GEMM tiling: a workgroup of 4 waves computes a 64 x 64 tile of C = A * B^T
+// with A [M][K] and B [N][K] both staged K-contiguous in LDS (32-wide K
+// slices), each wave one 32 x 32 block with two MFMAs per slice. The
+// operand / result lane layout of the 32x32x16 bf16 MFMA is: lane l supplies
+// A[l%32][8*(l/32)+t] and B[8*(l/32)+t][l%32], t = 0..7, and holds
+// C[8*(r/4) + 4*(l/32) + r%4][l%32] in accumulator r = 0..15.
+//   forward: M = pixels, N = Co, K = 9*Ci, A = im2col(x) built on the fly;
+//   dgrad:   M = pixels, N = Ci, K = 9*Co, A = im2col(gz) with mirrored taps,
+//            B = weights transposed to [Ci][3][3][Co];
+//   wgrad:   M = Co, N = 9*Ci, K = pixels (split over workgroups, FP32
+//            atomics), both operands transposed while being staged in LDS.
+#include "common.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;   // LDS rows padded by 16 bytes
+
+__device__ __forceinline__ float b2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t f2b(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);     // round to nearest even (finite values)
+    return (uint16_t)(u >> 16);
+}
+
+struct Geo {
+    int B, H, W;
+};
+
+__device__ __forceinline__ void pixel(int p, const Geo& g, int& b, int& y, int& x) {
+    x = p % g.W;
+    const int q = p / g.W;
+    y = q % g.H;
+    b = q / g.H;
+}
+
+// Waves 0..3: (wm, wn) = (wave & 1, wave >> 1).
+__device__ __forceinline__ void mma_slice(const uint16_t (*As)[LDK], const uint16_t (*Bs)[LDK], f32x16& acc) {
+    const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 16) {
+        const uint4 av = *reinterpret_cast<const uint4*>(&As[wm * 32 + (l & 31)][kk + 8 * (l >> 5)]);
+        const uint4 bv = *reinterpret_cast<const uint4*>(&Bs[wn * 32 + (l & 31)][kk + 8 * (l >> 5)]);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv),
+                                                       acc, 0, 0, 0);
+    }
+}
+
+// Forward (sign = +1) or backward-data (sign = -1) 3x3 convolution as an implicit GEMM.
+// src: [P][Cs] bf16 (x, or the masked upstream gradient), wk: [N][9][Cs] bf16, out: [P][N] bf16.
+__global__ __launch_bounds__(256) void k_conv_gemm(const uint16_t* __restrict__ src, const uint16_t* __restrict__ wk,
+                                                   const float* __restrict__ bias, uint16_t* __restrict__ out, Geo g,
+                                                   int Cs, int N, int sign, int relu) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[TM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][LDK];
+    const int P = g.B * g.H * g.W;
+    const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+    const int t = threadIdx.x;
+    const int pr = t >> 2, kq = (t & 3) * 8;
+    int pb, py, px;
+    const int p = m0 + pr;
+    pixel(min(p, P - 1), g, pb, py, px);
+    const int nrow = n0 + pr;
+    const int K = 9 * Cs;
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    for (int k0 = 0; k0 < K; k0 += TK) {
+        const int tap = k0 / Cs, c0 = k0 - tap * Cs;
+        const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+        const int yy = py + sign * (ky - 1), xx = px + sign * (kx - 1);
+        uint4 a = make_uint4(0, 0, 0, 0), bw = make_uint4(0, 0, 0, 0);
+        if (p < P && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W)
+            a = *reinterpret_cast<const uint4*>(src + (((long)pb * g.H + yy) * g.W + xx) * Cs + c0 + kq);
+        if (nrow < N) bw = *reinterpret_cast<const uint4*>(wk + (long)nrow * K + k0 + kq);
+        *reinterpret_cast<uint4*>(&As[pr][kq]) = a;
+        *reinterpret_cast<uint4*>(&Bs[pr][kq]) = bw;
+        __syncthreads();
+        mma_slice(As, Bs, acc);
+        __syncthreads();
+    }
+    const int l = t & 63, wave = t >> 6;
+    const int n = n0 + (wave >> 1) * 32 + (l & 31);
+    if (n >= N) return;
+    const float bn = bias ? bias[n] : 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int m = m0 + (wave & 1) * 32 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+        if (m < P) {
+            float v = acc[r] + bn;
+            if (relu) v = fmaxf(v, 0.0f);
+            out[(long)m * N + n] = f2b(v);
+        }
+    }
+}
+
+// Weight gradient gw[co][tap][ci] += sum_p gz[p][co] * x[p + tap][ci] over this workgroup's pixel range.
+__global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ gz,
+                                                    float* __restrict__ gw, Geo g, int Ci, int Co, int chunk) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[TM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][LDK];
+    const int P = g.B * g.H * g.W;
+    const int N = 9 * Ci;
+    const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+    const int pstart = blockIdx.z * chunk, pend = min(P, pstart + chunk);
+    const int t = threadIdx.x;
+    const int pr = t >> 3, q8 = (t & 7) * 8;     // pixel row of the slice, 8-wide channel group
+    const int nb = n0 + q8;
+    const int tap = nb / Ci, ci = nb - tap * Ci;
+    const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    for (int p0 = pstart; p0 < pend; p0 += TK) {
+        const int p = p0 + pr;
+        uint4 a = make_uint4(0, 0, 0, 0), bx = make_uint4(0, 0, 0, 0);
+        if (p < pend) {
+            if (m0 + q8 < Co) a = *reinterpret_cast<const uint4*>(gz + (long)p * Co + m0 + q8);
+            int pb, py, px;
+            pixel(p, g, pb, py, px);
+            const int yy = py + ky - 1, xx = px + kx - 1;
+            if (nb < N && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W)
+                bx = *reinterpret_cast<const uint4*>(x + (((long)pb * g.H + yy) * g.W + xx) * Ci + ci);
+        }
+        const uint16_t* ah = reinterpret_cast<const uint16_t*>(&a);
+        const uint16_t* bh = reinterpret_cast<const uint16_t*>(&bx);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            As[q8 + e][pr] = ah[e];
+            Bs[q8 + e][pr] = bh[e];
+        }
+        __syncthreads();
+        mma_slice(As, Bs, acc);
+        __syncthreads();
+    }
+    const int l = t & 63, wave = t >> 6;
+    const int n = n0 + (wave >> 1) * 32 + (l & 31);
+    if (n >= N) return;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int m = m0 + (wave & 1) * 32 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+        if (m < Co) atomicAdd(gw + (long)m * N + n, acc[r]);
+    }
+}
+
+// First convolution (one input channel): y[p][co] = relu(b[co] + sum_tap x[p+tap] w[co][tap]).
+__global__ void k_conv_first(const uint16_t* __restrict__ x, const float* __restrict__ w, const float* __restrict__ b,
+                             uint16_t* __restrict__ y, Geo g, int Co) {
+    const int P = g.B * g.H * g.W;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)P * Co) return;
+    const int p = (int)(idx / Co), co = (int)(idx - (long)p * Co);
+    int pb, py, px;
+    pixel(p, g, pb, py, px);
+    float s = 0.0f;
+    for (int tap = 0; tap < 9; ++tap) {
+        const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
+        if (yy >= 0 && yy < g.H && xx >= 0 && xx < g.W)
+            s += b2f(x[((long)pb * g.H + yy) * g.W + xx]) * b2f(f2b(w[co * 9 + tap]));
+    }
+    y[idx] = f2b(fmaxf(s + b[co], 0.0f));
+}
+
+// First convolution, backward: gw[co][tap] and gb[co] (block partial sums + atomics), gx (input gradient).
+__global__ void k_conv_first_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ gz,
+                                   float* __restrict__ gw, float* __restrict__ gb, Geo g, int Co, int chunk) {
+    // one workgroup per (co, pixel chunk): threads reduce over pixels
+    __shared__ float red[10][256];
+    const int co = blockIdx.y;
+    const int P = g.B * g.H * g.W;
+    const int pstart = blockIdx.x * chunk, pend = min(P, pstart + chunk);
+    float s[10];
+    for (int q = 0; q < 10; ++q) s[q] = 0.0f;
+    for (int p = pstart + threadIdx.x; p < pend; p += blockDim.x) {
+        const float gv = b2f(gz[(long)p * Co + co]);
+        int pb, py, px;
+        pixel(p, g, pb, py, px);
+        for (int tap = 0; tap < 9; ++tap) {
+            const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
+            if (yy >= 0 && yy < g.H && xx >= 0 && xx < g.W) s[tap] += gv * b2f(x[((long)pb * g.H + yy) * g.W + xx]);
+        }
+        s[9] += gv;
+    }
+    for (int q = 0; q < 10; ++q) red[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int q = 0; q < 10; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < 9) atomicAdd(gw + co * 9 + threadIdx.x, red[threadIdx.x][0]);
+    if (threadIdx.x == 9) atomicAdd(gb + co, red[9][0]);
+}
+
+__global__ void k_conv_first_dgrad(const uint16_t* __restrict__ gz, const float* __restrict__ w, float* __restrict__ gx,
+                                   Geo g, int Co) {
+    const int P = g.B * g.H * g.W;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    int pb, py, px;
+    pixel(p, g, pb, py, px);
+    float s = 0.0f;
+    for (int tap = 0; tap < 9; ++tap) {
+        const int yy = py - (tap / 3 - 1), xx = px - (tap % 3 - 1);
+        if (yy < 0 || yy >= g.H || xx < 0 || xx >= g.W) continue;
+        const uint16_t* gp = gz + (((long)pb * g.H + yy) * g.W + xx) * Co;
+        for (int co = 0; co < Co; ++co) s += b2f(gp[co]) * b2f(f2b(w[co * 9 + tap]));
+    }
+    gx[p] = s;
+}
+
+// Column sums (bias gradient): gb[c] += sum_p g[p][c].
+__global__ void k_colsum(const uint16_t* __restrict__ gz, float* __restrict__ gb, long P, int C, int chunk) {
+    const int c = blockIdx.y * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const long p0 = (long)blockIdx.x * chunk, p1 = min(P, p0 + chunk);
+    float s = 0.0f;
+    for (long p = p0; p < p1; ++p) s += b2f(gz[p * C + c]);
+    atomicAdd(gb + c, s);
+}
+
+__global__ void k_relu_mask(uint16_t* __restrict__ gz, const uint16_t* __restrict__ y, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !(b2f(y[i]) > 0.0f)) gz[i] = 0;
+}
+
+__global__ void k_avgpool2(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, Geo g, int C) {
+    const int Ho = g.H / 2, Wo = g.W / 2;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)g.B * Ho * Wo * C) return;
+    const int c = (int)(idx % C);
+    const long q = idx / C;
+    const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), b = (int)(q / ((long)Wo * Ho));
+    const long base = (((long)b * g.H + 2 * yo) * g.W + 2 * xo) * C + c;
+    const float s = b2f(x[base]) + b2f(x[base + C]) + b2f(x[base + (long)g.W * C]) + b2f(x[base + (long)g.W * C + C]);
+    y[idx] = f2b(0.25f * s);
+}
+
+// gx (H x W) from gy (H/2 x W/2): every input cell receives a quarter of its pool's gradient.
+__global__ void k_avgpool2_bwd(const uint16_t* __restrict__ gy, uint16_t* __restrict__ gx, Geo g, int C) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)g.B * g.H * g.W * C) return;
+    const int c = (int)(idx % C);
+    const long q = idx / C;
+    const int xx = (int)(q % g.W), yy = (int)((q / g.W) % g.H), b = (int)(q / ((long)g.W * g.H));
+    const int Ho = g.H / 2, Wo = g.W / 2;
+    gx[idx] = f2b(0.25f * b2f(gy[(((long)b * Ho + yy / 2) * Wo + xx / 2) * C + c]));
+}
+
+// Head: gap[b][c] = mean_p a[b][p][c]; out[b][j] = fb[j] + sum_c fw[j][c] gap[b][c]. One workgroup per sample.
+__global__ void k_head(const uint16_t* __restrict__ a, const float* __restrict__ fw, const float* __restrict__ fb,
+                       float* __restrict__ gap, float* __restrict__ out, int HW, int C, int nout) {
+    extern __shared__ float sg[];
+    const int b = blockIdx.x;
+    for (int c = threadIdx.x; c < C; c += blockDim.x) {
+        float s = 0.0f;
+        for (int p = 0; p < HW; ++p) s += b2f(a[((long)b * HW + p) * C + c]);
+        sg[c] = s / (float)HW;
+        gap[(long)b * C + c] = sg[c];
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < nout; j += blockDim.x) {
+        float s = fb[j];
+        for (int c = 0; c < C; ++c) s += fw[j * C + c] * sg[c];
+        out[(long)b * nout + j] = s;
+    }
+}
+
+// Head backward: gfw += gout^T gap, gfb += sum gout, ga[b][p][c] = (gout[b] . fw[:, c]) / HW.
+__global__ void k_head_bwd(const float* __restrict__ gout, const float* __restrict__ gap, const float* __restrict__ fw,
+                           float* __restrict__ gfw, float* __restrict__ gfb, uint16_t* __restrict__ ga, int HW, int C,
+                           int nout) {
+    const int b = blockIdx.x;
+    for (int c = threadIdx.x; c < C; c += blockDim.x) {
+        float s = 0.0f;
+        for (int j = 0; j < nout; ++j) {
+            s += gout[(long)b * nout + j] * fw[j * C + c];
+            atomicAdd(gfw + j * C + c, gout[(long)b * nout + j] * gap[(long)b * C + c]);
+        }
+        const uint16_t v = f2b(s / (float)HW);
+        for (int p = 0; p < HW; ++p) ga[((long)b * HW + p) * C + c] = v;
+    }
+    if (threadIdx.x < (unsigned)nout) atomicAdd(gfb + threadIdx.x, gout[(long)b * nout + threadIdx.x]);
+}
+
+// bf16 copies of FP32 weights [Co][9][Ci]: w16 in the same order, wt16 with the channel axes swapped,
+// [Ci][9][Co] (same tap order).
+__global__ void k_weights(const float* __restrict__ w, uint16_t* __restrict__ w16, uint16_t* __restrict__ wt16, int Co,
+                          int Ci) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)Co * 9 * Ci) return;
+    const int ci = (int)(idx % Ci);
+    const int tap = (int)((idx / Ci) % 9);
+    const int co = (int)(idx / (9L * Ci));
+    const uint16_t v = f2b(w[idx]);
+    w16[idx] = v;
+    if (wt16) wt16[((long)ci * 9 + tap) * Co + co] = v;   // the dgrad GEMM mirrors the tap offset itself
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       long n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ void k_f2b(const float* __restrict__ x, uint16_t* __restrict__ y, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = f2b(x[i]);
+}
+
+inline dim3 grid1(long n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
+
+}  // namespace
+
+extern "C" int bz_nn_conv3x3(const uint16_t* x, const uint16_t* w16, const float* bias, uint16_t* y, int B, int H,
+                             int W, int Ci, int Co, int relu, void* stream) {
+    BZ_REQUIRE(x && w16 && y && B > 0 && H > 0 && W > 0, "bz_nn_conv3x3: bad arguments");
+    BZ_REQUIRE(Ci % TK == 0 && Co % 32 == 0, "bz_nn_conv3x3: Ci must be a multiple of 32 and Co of 32");
+    const long P = (long)B * H * W;
+    BZ_REQUIRE(P * (long)(Ci > Co ? Ci : Co) < (1L << 31), "bz_nn_conv3x3: tensor too large");
+    Geo g{B, H, W};
+    hipLaunchKernelGGL(k_conv_gemm, dim3((unsigned)((P + TM - 1) / TM), (Co + TN - 1) / TN), dim3(256), 0,
+                       (hipStream_t)stream, x, w16, bias, y, g, Ci, Co, 1, relu);
+    BZ_CHECK_LAUNCH("bz_nn_conv3x3");
+    return BZ_OK;
+}
+
+extern "C" int bz_nn_conv3x3_dgrad(const uint16_t* gz, const uint16_t* wt16, uint16_t* gx, int B, int H, int W, int Ci,
+                                   int Co, void* stream) {
+    BZ_REQUIRE(gz && wt16 && gx && B > 0 && H > 0 && W > 0, "bz_nn_conv3x3_dgrad: bad arguments");
+    BZ_REQUIRE(Co % TK == 0 && Ci % 32 == 0, "bz_nn_conv3x3_dgrad: Co must be a multiple of 32 and Ci of 32");
+    const long P = (long)B * H * W;
+    Geo g{B, H, W};
+    hipLaunchKernelGGL(k_conv_gemm, dim3((unsigned)((P + TM - 1) / TM), (Ci + TN - 1) / TN), dim3(256), 0,
+                       (hipStream_t)stream, gz, wt16, (const float*)nullptr, gx, g, Co, Ci, -1, 0);
+    BZ_CHECK_LAUNCH("bz_nn_conv3x3_dgrad");
+    return BZ_OK;
+}
+
+extern "C" int bz_nn_conv3x3_wgrad(const uint16_t* x, const uint16_t* gz, float* gw, float* gb, int B, int H, int W,
+                                   int Ci, int Co, int splits, void* stream) {
+    BZ_REQUIRE(x && gz && gw && gb && B > 0 && H > 0 && W > 0 && splits > 0, "bz_nn_conv3x3_wgrad: bad arguments");
+    BZ_REQUIRE(Ci % 8 == 0 && Co % 8 == 0, "bz_nn_conv3x3_wgrad: channel counts must be multiples of 8");
+    const long P = (long)B * H * W;
+    const int chunk = (int)(((P + splits - 1) / splits + TK - 1) / TK * TK);
+    const int nz = (int)((P + chunk - 1) / chunk);
+    Geo g{B, H, W};
+    const hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_conv_wgrad, dim3((Co + TM - 1) / TM, (9 * Ci + TN - 1) / TN, nz), dim3(256), 0, st, x, gz, gw,
+                       g, Ci, Co, chunk);
+    BZ_CHECK_LAUNCH("bz_nn_conv3x3_wgrad");
+    const int cchunk = 4096;
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)((P + cchunk - 1) / cchunk), (Co + 255) / 256), dim3(256), 0, st, gz, gb,
+                       P, Co, cchunk);
+    BZ_CHECK_LAUNCH("bz_nn_conv3x3_wgrad(bias)");
+    return BZ_OK;
+}
+
+extern "C" int bz_nn_conv_first(const uint16_t* x, const float* w, const float* b, uint16_t* y, int B, int H, int W,
+                                int Co, void* stream) {
+    BZ_REQUIRE(x && w && b && y && B > 0 && H > 0 && W > 0 && Co > 0, "bz_nn_conv_first: bad arguments");
+    Geo g{B, H, W};
+    hipLaunchKernelGGL(k_conv_first, grid1((long)B * H * W * Co), dim3(256), 0, (hipStream_t)stream, x, w, b, y, g, Co);
+    BZ_CHECK_LAUNCH("bz_nn_conv_first");
+    return BZ_OK;
+}
+
+extern "C" int bz_nn_conv_first_bwd(const uint16_t* x, const uint16_t* gz, const float* w, float* gw, float* gb,
+                                    float* gx, int B, int H, int W, int Co, void* stream) {
+    BZ_REQUIRE(x && gz && w && gw && gb && B > 0 && H > 0 && W > 0 && Co > 0, "bz_nn_conv_first_bwd: bad arguments");
+    Geo g{B, H, W};
+    const long P = (long)B * H * W;
+    const int chunk = 8192;
+    const hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_conv_first_wgrad, dim3((unsigned)((P + chunk - 1) / chunk), Co), dim3(256), 0, st, x, gz, gw,
+                       gb, g, Co, chunk);
+    BZ_CHECK_LAUNCH("bz_nn_conv_first_bwd(w)");
+    if (gx) {
+        hipLaunchKernelGGL(k_conv_first_dgrad, grid1(P), dim3(256), 0, st, gz, w, gx, g, Co);
+        BZ_CHECK_LAUNCH("bz_nn_conv_first_bwd(x)");
+    }
+    return BZ_OK;
+}
+
+extern "C" int bz_nn_relu_mask(uint16_t* gz, const uint16_t* y, long n, void* stream) {
+    BZ_REQUIRE(gz && y && n >= 0, "bz_nn_relu_mask: bad arguments");
+    if (n == 0) return BZ_OK;
+    hipLaunchKernelGGL(k_relu_mask, grid1(n), dim3(256), 0, (hipStream_t)stream, gz, y, n);
+    BZ_CHECK_LAUNCH("bz_nn_relu_mask");
+    return BZ_OK;
+}
+
+extern "C" int bz_nn_avgpool2(const uint16_t* x, uint16_t* y, int B, int H, int W, int C, void* stream) {
+    BZ_REQUIRE(x && y && B > 0 && H % 2 == 0 && W % 2 == 0 && H > 0 && W > 0 && C > 0, "bz_nn_avgpool2: bad arguments");
+    Geo g{B, H, W};
+    hipLaunchKernelGGL(k_avgpool2, grid1((long)B * (H / 2) * (W / 2) * C), dim3(256), 0, (hipStream_t)stream, x, y, g, C);
+    BZ_CHECK_LAUNCH("bz_nn_avgpool2");
+    return BZ_OK;
+}
+
+extern "C" int bz_nn_avgpool2_bwd(const uint16_t* gy, uint16_t* gx, int B, int H, int W, int C, void* stream) {
+    BZ_REQUIRE(gy && gx && B > 0 && H % 2 == 0 && W % 2 == 0 && H > 0 && W > 0 && C > 0,
+               "bz_nn_avgpool2_bwd: bad arguments");
+    Geo g{B, H, W};
+    hipLaunchKernelGGL(k_avgpool2_bwd, grid1((long)B * H * W * C), dim3(256), 0, (hipStream_t)stream, gy, gx, g, C);
+    BZ_CHECK_LAUNCH("bz_nn_avgpool2_bwd");
+    return BZ_OK;
+}
+
+extern "C" int bz_nn_head(const uint16_t* a, const float* fw, const float* fb, float* gap, float* out, int B, int HW,
+                          int C, int nout, void* stream) {
+    BZ_REQUIRE(a && fw && fb && gap && out && B > 0 && HW > 0 && C > 0 && C <= 4096 && nout > 0,
+               "bz_nn_head: bad arguments");
+    hipLaunchKernelGGL(k_head, dim3(B), dim3(256), C * sizeof(float), (hipStream_t)stream, a, fw, fb, gap, out, HW, C,
+                       nout);
+    BZ_CHECK_LAUNCH("bz_nn_head");
+    return BZ_OK;
+}
+
+extern "C" int bz_nn_head_bwd(const float* gout, const float* gap, const float* fw, float* gfw, float* gfb,
+                              uint16_t* ga, int B, int HW, int C, int nout, void* stream) {
+    BZ_REQUIRE(gout && gap && fw && gfw && gfb && ga && B > 0 && HW > 0 && C > 0 && nout > 0 && nout <= 256,
+               "bz_nn_head_bwd: bad arguments");
+    hipLaunchKernelGGL(k_head_bwd, dim3(B), dim3(256), 0, (hipStream_t)stream, gout, gap, fw, gfw, gfb, ga, HW, C, nout);
+    BZ_CHECK_LAUNCH("bz_nn_head_bwd");
+    return BZ_OK;
+}
+
+extern "C" int bz_nn_weights(const float* w, uint16_t* w16, uint16_t* wt16, int Co, int Ci, void* stream) {
+    BZ_REQUIRE(w && w16 && Co > 0 && Ci > 0, "bz_nn_weights: bad arguments");
+    hipLaunchKernelGGL(k_weights, grid1((long)Co * 9 * Ci), dim3(256), 0, (hipStream_t)stream, w, w16, wt16, Co, Ci);
+    BZ_CHECK_LAUNCH("bz_nn_weights");
+    return BZ_OK;
+}
+
+extern "C" int bz_nn_adam(float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
+                          int step, void* stream) {
+    BZ_REQUIRE(p && g && m && v && n >= 0 && step >= 1, "bz_nn_adam: bad arguments");
+    if (n == 0) return BZ_OK;
+    const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, grid1(n), dim3(256), 0, (hipStream_t)stream, p, g, m, v, n, lr, b1, b2, eps, c1, c2);
+    BZ_CHECK_LAUNCH("bz_nn_adam");
+    return BZ_OK;
+}
+
+extern "C" int bz_nn_to_bf16(const float* x, uint16_t* y, long n, void* stream) {
+    BZ_REQUIRE(x && y && n >= 0, "bz_nn_to_bf16: bad arguments");
+    if (n == 0) return BZ_OK;
+    hipLaunchKernelGGL(k_f2b, grid1(n), dim3(256), 0, (hipStream_t)stream, x, y, n);
+    BZ_CHECK_LAUNCH("bz_nn_to_bf16");
+    return BZ_OK;
+}

@@ -1,0 +1,186 @@
+"""CNN surrogate of the batched FDTD (new; no reference counterpart).
+
+A 4-level encoder of 3x3 convolutions (32, 64, 128, 256 channels, two per
+level, ReLU, 2x2 average pooling between levels) with a global-average-pool +
+fully connected head maps a density map (B, H, W) in [0, 1] to the port
+transmission at 8 wavelengths; back-propagating a weighting of the outputs
+gives d(weighted transmission)/d(density). Every layer runs on the HIP kernels
+of ``csrc/surrogate.hip``: BF16 weights and activations (NHWC), FP32
+accumulation on the MFMA units, FP32 master weights and Adam state.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .. import _native as N
+
+LEVELS = ((1, 32), (32, 64), (64, 128), (128, 256))
+
+
+class Surrogate:
+    def __init__(self, nout=8, seed=0, device="cuda"):
+        import torch
+        self.torch, self.dev, self.nout = torch, torch.device(device), int(nout)
+        rng = np.random.default_rng(seed)
+        P = {}
+        for lvl, (ci, co) in enumerate(LEVELS):
+            for s, (a, o) in enumerate(((ci, co), (co, co))):
+                P[f"w{lvl}{s}"] = rng.standard_normal((o, 3, 3, a)) * np.sqrt(2.0 / (9 * a))   # He normal
+                P[f"b{lvl}{s}"] = np.zeros(o)
+        P["fw"] = rng.standard_normal((self.nout, LEVELS[-1][1])) * np.sqrt(1.0 / LEVELS[-1][1])
+        P["fb"] = np.zeros(self.nout)
+        self.params = {k: torch.tensor(v, dtype=torch.float32, device=self.dev).contiguous() for k, v in P.items()}
+        self.m = {k: torch.zeros_like(v) for k, v in self.params.items()}
+        self.v = {k: torch.zeros_like(v) for k, v in self.params.items()}
+        self.grads = {k: torch.zeros_like(v) for k, v in self.params.items()}
+        self.step_count = 0
+        self.w16, self.wt16 = {}, {}
+        for lvl, (ci, co) in enumerate(LEVELS):
+            for s, a in enumerate((ci, co)):
+                if lvl == 0 and s == 0:
+                    continue
+                k = f"w{lvl}{s}"
+                self.w16[k] = torch.empty(self.params[k].shape, dtype=torch.int16, device=self.dev)
+                self.wt16[k] = torch.empty((a, 3, 3, self.params[k].shape[0]), dtype=torch.int16, device=self.dev)
+        self.sync_weights()
+
+    # ---------------- helpers ----------------
+    @property
+    def stream(self):
+        return self.torch.cuda.current_stream(self.dev).cuda_stream
+
+    def sync_weights(self):
+        """bf16 copies (and channel-transposed copies for backward-data) of the FP32 conv weights."""
+        for k, w in self.w16.items():
+            co, _, _, ci = self.params[k].shape
+            N.call("bz_nn_weights", self.params[k].data_ptr(), w.data_ptr(), self.wt16[k].data_ptr(), co, ci, self.stream)
+
+    def load_state(self, params):
+        for k, v in params.items():
+            self.params[k].copy_(v)
+        self.sync_weights()
+
+    def _empty16(self, *shape):
+        return self.torch.empty(shape, dtype=self.torch.int16, device=self.dev)
+
+    # ---------------- forward ----------------
+    def forward(self, x):
+        """x: (B, H, W) density (float tensor on the device; H, W divisible by 8). Returns out (B, nout) FP32."""
+        T = self.torch
+        x = x.to(device=self.dev, dtype=T.float32).contiguous()
+        B, H, W = x.shape
+        if H % 8 or W % 8:
+            raise ValueError("input height and width must be multiples of 8")
+        st = self.stream
+        x16 = self._empty16(B, H, W, 1)
+        N.call("bz_nn_to_bf16", x.data_ptr(), x16.data_ptr(), x.numel(), st)
+        cache = {"x16": x16, "shape": (B, H, W), "acts": []}
+        a, h, w = x16, H, W
+        for lvl, (ci, co) in enumerate(LEVELS):
+            if lvl > 0:
+                p = self._empty16(B, h // 2, w // 2, ci)
+                N.call("bz_nn_avgpool2", a.data_ptr(), p.data_ptr(), B, h, w, ci, st)
+                a, h, w = p, h // 2, w // 2
+            y0 = self._empty16(B, h, w, co)
+            if lvl == 0:
+                N.call("bz_nn_conv_first", a.data_ptr(), self.params["w00"].data_ptr(), self.params["b00"].data_ptr(),
+                       y0.data_ptr(), B, h, w, co, st)
+            else:
+                N.call("bz_nn_conv3x3", a.data_ptr(), self.w16[f"w{lvl}0"].data_ptr(),
+                       self.params[f"b{lvl}0"].data_ptr(), y0.data_ptr(), B, h, w, ci, co, 1, st)
+            y1 = self._empty16(B, h, w, co)
+            N.call("bz_nn_conv3x3", y0.data_ptr(), self.w16[f"w{lvl}1"].data_ptr(), self.params[f"b{lvl}1"].data_ptr(),
+                   y1.data_ptr(), B, h, w, co, co, 1, st)
+            cache["acts"].append((a, y0, y1, h, w))
+            a = y1
+        C = LEVELS[-1][1]
+        gap = T.empty((B, C), dtype=T.float32, device=self.dev)
+        out = T.empty((B, self.nout), dtype=T.float32, device=self.dev)
+        N.call("bz_nn_head", a.data_ptr(), self.params["fw"].data_ptr(), self.params["fb"].data_ptr(), gap.data_ptr(),
+               out.data_ptr(), B, h * w, C, self.nout, st)
+        cache["gap"] = gap
+        self.cache = cache
+        return out
+
+    # ---------------- backward ----------------
+    def backward(self, gout, input_grad=False):
+        """Accumulates d(sum(out * gout))/d(param) into self.grads (zeroed first); returns the input gradient
+        (B, H, W) FP32 if input_grad."""
+        T = self.torch
+        cache = self.cache
+        st = self.stream
+        for g in self.grads.values():
+            g.zero_()
+        B, H, W = cache["shape"]
+        gout = gout.to(device=self.dev, dtype=T.float32).contiguous()
+        _, _, y1, h, w = cache["acts"][-1]
+        C = LEVELS[-1][1]
+        ga = self._empty16(B, h, w, C)
+        N.call("bz_nn_head_bwd", gout.data_ptr(), cache["gap"].data_ptr(), self.params["fw"].data_ptr(),
+               self.grads["fw"].data_ptr(), self.grads["fb"].data_ptr(), ga.data_ptr(), B, h * w, C, self.nout, st)
+        gx = None
+        for lvl in range(len(LEVELS) - 1, -1, -1):
+            ci, co = LEVELS[lvl]
+            a, y0, y1, h, w = cache["acts"][lvl]
+            if lvl < len(LEVELS) - 1:
+                up = self._empty16(B, h, w, co)
+                N.call("bz_nn_avgpool2_bwd", ga.data_ptr(), up.data_ptr(), B, h, w, co, st)
+                ga = up
+            splits = self._splits(B * h * w, co, co)
+            # second convolution of the level: input y0 (co channels)
+            N.call("bz_nn_relu_mask", ga.data_ptr(), y1.data_ptr(), ga.numel(), st)
+            N.call("bz_nn_conv3x3_wgrad", y0.data_ptr(), ga.data_ptr(), self.grads[f"w{lvl}1"].data_ptr(),
+                   self.grads[f"b{lvl}1"].data_ptr(), B, h, w, co, co, splits, st)
+            g0 = self._empty16(B, h, w, co)
+            N.call("bz_nn_conv3x3_dgrad", ga.data_ptr(), self.wt16[f"w{lvl}1"].data_ptr(), g0.data_ptr(), B, h, w,
+                   co, co, st)
+            # first convolution of the level: input a (ci channels)
+            N.call("bz_nn_relu_mask", g0.data_ptr(), y0.data_ptr(), g0.numel(), st)
+            if lvl == 0:
+                if input_grad:
+                    gx = T.empty((B, H, W), dtype=T.float32, device=self.dev)
+                N.call("bz_nn_conv_first_bwd", a.data_ptr(), g0.data_ptr(), self.params["w00"].data_ptr(),
+                       self.grads["w00"].data_ptr(), self.grads["b00"].data_ptr(),
+                       gx.data_ptr() if gx is not None else None, B, h, w, co, st)
+            else:
+                N.call("bz_nn_conv3x3_wgrad", a.data_ptr(), g0.data_ptr(), self.grads[f"w{lvl}0"].data_ptr(),
+                       self.grads[f"b{lvl}0"].data_ptr(), B, h, w, ci, co, self._splits(B * h * w, co, ci), st)
+                gi = self._empty16(B, h, w, ci)
+                N.call("bz_nn_conv3x3_dgrad", g0.data_ptr(), self.wt16[f"w{lvl}0"].data_ptr(), gi.data_ptr(), B, h,
+                       w, ci, co, st)
+                ga = gi
+        return gx
+
+    @staticmethod
+    def _splits(P, co, ci, target=2048):
+        tiles = max(1, -(-co // 64)) * max(1, -(-(9 * ci) // 64))
+        return int(max(1, min(-(-target // tiles), P // 256)))
+
+    # ---------------- training / inference ----------------
+    def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+        self.step_count += 1
+        st = self.stream
+        for k, p in self.params.items():
+            N.call("bz_nn_adam", p.data_ptr(), self.grads[k].data_ptr(), self.m[k].data_ptr(), self.v[k].data_ptr(),
+                   p.numel(), lr, b1, b2, eps, self.step_count, st)
+        self.sync_weights()
+
+    def train_step(self, x, target, lr=1e-3):
+        """One Adam step on the mean squared error; returns the loss (a device scalar)."""
+        out = self.forward(x)
+        t = target.to(device=self.dev, dtype=self.torch.float32)
+        diff = out - t
+        self.backward(2.0 * diff / diff.numel())
+        self.adam(lr)
+        return (diff * diff).mean()
+
+    def predict(self, x, batch=256):
+        outs = [self.forward(x[q:q + batch]) for q in range(0, x.shape[0], batch)]
+        return self.torch.cat(outs)
+
+    def input_gradient(self, x, weights):
+        """d(sum_j weights[j] out[b, j]) / d x[b] for every sample b, (B, H, W)."""
+        out = self.forward(x)
+        w = self.torch.as_tensor(weights, dtype=self.torch.float32, device=self.dev)
+        g = self.backward(w.expand_as(out).contiguous(), input_grad=True)
+        return out, g

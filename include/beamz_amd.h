@@ -172,6 +172,33 @@ int bz_adj2d_run(const bz_adj2d* adjs, int nadj, const int* tiles, int ntiles, i
  * the terminal condition lam_N = dJ/du_N before the first reverse step. */
 int bz_adj2d_inject(const bz_adj2d* adjs, int nadj, int max_inj, int n, void* stream);
 
+/* ---------------- CNN surrogate (BF16 MFMA, FP32 accumulate) ----------------
+ * No reference counterpart: the surrogate maps a density map to port transmission spectra and,
+ * by back-propagation, to d(spectrum)/d(density). Activations are NHWC bfloat16 (uint16 bit
+ * patterns), convolution weights [Co][3][3][Ci], gradients and master weights FP32.
+ * Gradient entry points accumulate (+=) into their FP32 outputs; the caller zeroes them. */
+int bz_nn_conv3x3(const uint16_t* x, const uint16_t* w16, const float* bias, uint16_t* y, int B, int H, int W,
+                  int Ci, int Co, int relu, void* stream);          /* Ci % 32 == 0, Co % 32 == 0 */
+int bz_nn_conv3x3_dgrad(const uint16_t* gz, const uint16_t* wt16, uint16_t* gx, int B, int H, int W, int Ci,
+                        int Co, void* stream);                      /* wt16 = [Ci][3][3][Co] */
+int bz_nn_conv3x3_wgrad(const uint16_t* x, const uint16_t* gz, float* gw, float* gb, int B, int H, int W,
+                        int Ci, int Co, int splits, void* stream);
+int bz_nn_conv_first(const uint16_t* x, const float* w, const float* b, uint16_t* y, int B, int H, int W, int Co,
+                     void* stream);                                 /* one input channel, ReLU */
+int bz_nn_conv_first_bwd(const uint16_t* x, const uint16_t* gz, const float* w, float* gw, float* gb, float* gx,
+                         int B, int H, int W, int Co, void* stream); /* gx (may be NULL) is written, not added */
+int bz_nn_relu_mask(uint16_t* gz, const uint16_t* y, long n, void* stream);
+int bz_nn_avgpool2(const uint16_t* x, uint16_t* y, int B, int H, int W, int C, void* stream);
+int bz_nn_avgpool2_bwd(const uint16_t* gy, uint16_t* gx, int B, int H, int W, int C, void* stream);
+int bz_nn_head(const uint16_t* a, const float* fw, const float* fb, float* gap, float* out, int B, int HW, int C,
+               int nout, void* stream);                             /* global average pool + fully connected */
+int bz_nn_head_bwd(const float* gout, const float* gap, const float* fw, float* gfw, float* gfb, uint16_t* ga,
+                   int B, int HW, int C, int nout, void* stream);
+int bz_nn_weights(const float* w, uint16_t* w16, uint16_t* wt16, int Co, int Ci, void* stream);
+int bz_nn_adam(float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
+               int step, void* stream);
+int bz_nn_to_bf16(const float* x, uint16_t* y, long n, void* stream);
+
 /* ---------------- sources ---------------- */
 /* field[off[p]] += w[p] * amp for p < n (offsets unique within one call). */
 int bz_add_points(float* field, const int64_t* off, const float* w, int n, float amp, void* stream);
