@@ -50,7 +50,7 @@ class Adj2D(C.Structure):
                 ("rdx", f32), ("rdy", f32), ("src_x0", i32), ("src_y0", i32), ("src_sx", i32), ("src_sy", i32),
                 ("src_mul", vp), ("box_x0", i32), ("box_y0", i32), ("box_sx", i32), ("box_sy", i32),
                 ("snap", vp), ("snap_step0", i32), ("grad", vp),
-                ("inj_x0", i32), ("inj_y0", i32), ("inj_sx", i32), ("inj_sy", i32), ("inj_w", vp), ("phasor", vp)]
+                ("inj_x0", i32), ("inj_y0", i32), ("inj_sx", i32), ("inj_sy", i32), ("inj_w", vp), ("phasor", vp), ("nf", i32)]
 
 # name -> argtypes (restype is int unless listed in _RESTYPE)
 SIGNATURES = {

@@ -15,7 +15,7 @@
 //   lE_n  = (interior ? ca : 1) lE'' + (fb muX(j,i) - fb muX(j,i-1)) / dy
 //                                    + (fb muY(j-1,i) - fb muY(j,i)) / dx
 //   lHx_n = fa muX,  lHy_n = fa muY
-//   lE_n(p) += Re(w_p phasor[n-1])  (objective: DFT of the Ez sample after step n-1)
+//   lE_n(p) += sum_f Re(w_fp phasor[n-1][f])  (objective: DFTs of the Ez sample after step n-1)
 //
 // and accumulates, inside the design box, dJ/deps += lE'' (dca/deps Ez_n +
 // dcb/deps curlH_{n+1}) from the forward snapshot that bz_yee2d_run stored
@@ -40,6 +40,18 @@ __device__ __forceinline__ float at(const float* __restrict__ base, uint32_t byt
 }
 __device__ __forceinline__ void put(float* __restrict__ base, uint32_t byte_off, float v) {
     *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
+// dJ/dEz of the sample after step n-1 at injection cell p: sum_f Re(inj_w[f][p] * phasor[n-1][f]).
+__device__ __forceinline__ double injection(const bz_adj2d& A, int p, int n) {
+    const long cells = (long)A.inj_sx * A.inj_sy;
+    const double* ph = A.phasor + 2 * (long)(n - 1) * A.nf;
+    double s = 0.0;
+    for (int f = 0; f < A.nf; ++f) {
+        const double* wp = A.inj_w + 2 * (f * cells + p);
+        s += wp[0] * ph[2 * f] - wp[1] * ph[2 * f + 1];
+    }
+    return s;
 }
 
 template <int SEG>
@@ -130,11 +142,7 @@ __global__ __launch_bounds__(LANES* WPG) void k_adj2d(const bz_adj2d* __restrict
             float lnew = (in ? ca * l2[r] : l2[r]) + (fmx[r] - fmxl[r]) * rdy + (fmy[r - 1] - fmy[r]) * rdx;
             if (A.inj_w && n >= 1) {
                 const int a = i - A.inj_x0, b = j - A.inj_y0;
-                if (a >= 0 && a < A.inj_sx && b >= 0 && b < A.inj_sy) {
-                    const double* wp = A.inj_w + 2 * ((long)b * A.inj_sx + a);
-                    const double* ph = A.phasor + 2 * (long)(n - 1);
-                    lnew += (float)(wp[0] * ph[0] - wp[1] * ph[1]);
-                }
+                if (a >= 0 && a < A.inj_sx && b >= 0 && b < A.inj_sy) lnew += (float)injection(A, b * A.inj_sx + a, n);
             }
             put(ole, ro[r], lnew);
             put(olx, ro[r], fa_[r] * mx[r]);
@@ -168,10 +176,8 @@ __global__ void k_adj_inject(const bz_adj2d* __restrict__ adjs, int n) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= A.inj_sx * A.inj_sy) return;
     const int a = p % A.inj_sx, b = p / A.inj_sx;
-    const double* wp = A.inj_w + 2 * (long)p;
-    const double* ph = A.phasor + 2 * (long)(n - 1);
     float* le = A.le[n & 1] + (long)(A.inj_y0 + b) * A.ld + (A.inj_x0 + a);
-    *le += (float)(wp[0] * ph[0] - wp[1] * ph[1]);
+    *le += (float)injection(A, p, n);
 }
 
 template <int SEG>

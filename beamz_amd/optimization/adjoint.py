@@ -50,23 +50,27 @@ def mode_weights(monitor, dx, dy, center, width, n_core, n_clad, wavelength):
 
 
 class AdjointProblem2D:
-    """Mode-overlap objective and its exact gradient w.r.t. eps_r in a design box, for a batch of
-    2D simulations that share geometry of the box, time axis and objective monitor layout.
+    """Mode-overlap objective J = sum_f s_f |a_f|^2, a_f = sum_p c_fp Ez_hat(p, w_f), and its exact
+    gradient w.r.t. eps_r in a design box, for a batch of 2D simulations that share geometry of the box,
+    time axis and objective monitor layout.
 
     sims: list of 2D ``FDTD`` objects (one additive or multiplicative source each); ``monitor`` of
-    sim b is ``monitors[b]``, a line monitor with exactly one DFT frequency; weights: complex c_p per
-    monitor point (shared); box: (x0, y0, sx, sy) design cells, inside the interior.
+    sim b is ``monitors[b]``, a line monitor with nf DFT frequencies; weights: complex c_fp, shape
+    (nf, npts) or (npts,) for nf = 1 (shared); fscale: s_f (default 1); box: (x0, y0, sx, sy) design
+    cells, inside the interior.
     """
 
-    def __init__(self, sims, monitors, weights, box, checkpoint_every=None, seg=None):
+    def __init__(self, sims, monitors, weights, box, checkpoint_every=None, seg=None, fscale=None):
         self.sims = list(sims)
         self.monitors = list(monitors)
         if len(self.monitors) != len(self.sims):
             raise ValueError("one objective monitor per simulation")
+        self.c = np.atleast_2d(np.asarray(weights, dtype=np.complex128))
+        self.nf = self.c.shape[0]
         for m in self.monitors:
-            if m.dft_frequencies is None or len(m.dft_frequencies) != 1:
-                raise ValueError("the objective monitor needs exactly one DFT frequency")
-        self.c = np.asarray(weights, dtype=np.complex128)
+            if m.dft_frequencies is None or len(m.dft_frequencies) != self.nf:
+                raise ValueError("the objective monitor needs one DFT frequency per row of weights")
+        self.fscale = np.ones(self.nf) if fscale is None else np.asarray(fscale, dtype=np.float64)
         self.box = tuple(int(v) for v in box)
         x0, y0, sx, sy = self.box
         for s in self.sims:
@@ -120,16 +124,18 @@ class AdjointProblem2D:
         for s in self.sims:
             s.current_step = self.N
         c = T.from_numpy(self.c).to(self.dev)
-        self.a = T.stack([self._overlap(s, m, c) for s, m in zip(self.sims, self.monitors)])
-        self.J = (self.a.real ** 2 + self.a.imag ** 2)
+        self.a = T.stack([self._overlap(s, m, c) for s, m in zip(self.sims, self.monitors)])   # (B, nf)
+        fs = T.from_numpy(self.fscale).to(self.dev)
+        self.Jf = self.a.real ** 2 + self.a.imag ** 2
+        self.J = (self.Jf * fs).sum(dim=1)
         return self.J
 
     def _overlap(self, sim, mon, c):
         plan = next(p for p in sim._plans if p.monitor is mon)
         npts = plan.layout[1]
-        z = plan.dft[0, :npts, :]                  # Ez entries come first
-        zc = self.torch.complex(z[:, 0], z[:, 1])
-        return (c * zc).sum()
+        z = plan.dft[:, :npts, :]                  # (nf, points, 2): Ez entries come first
+        zc = self.torch.complex(z[..., 0], z[..., 1])
+        return (c * zc).sum(dim=1)
 
     # ---------------- reverse ----------------
     def _adjoint_descs(self, lam, grad, inj):
@@ -156,21 +162,24 @@ class AdjointProblem2D:
             d.inj_x0, d.inj_y0, d.inj_sx, d.inj_sy = ix0, iy0, isx, isy
             d.inj_w = w.data_ptr()
             d.phasor = self._phasor[b].data_ptr()
+            d.nf = self.nf
             descs.append(d)
         return descs
 
     def _injection(self, b):
-        """Dense box of dJ/dEz-sample weights 2 conj(a) c_p over the monitor's Ez points."""
+        """Dense box of dJ/dEz-sample weights 2 s_f conj(a_f) c_fp per frequency over the monitor's Ez points."""
         s, m = self.sims[b], self.monitors[b]
         pts = np.array(m.get_grid_points_2d(s.dx, s.dy), dtype=np.int64)
-        wp = 2.0 * np.conj(complex(self.a[b].item())) * self.c
+        a = self.a[b].cpu().numpy()
+        wp = 2.0 * (self.fscale * np.conj(a))[:, None] * self.c          # (nf, npts)
         xs, ys = pts[:, 0], pts[:, 1]
         ok = (xs >= 0) & (xs < s.nx) & (ys >= 0) & (ys < s.ny)
-        xs, ys, wp = xs[ok], ys[ok], wp[ok]
+        xs, ys, wp = xs[ok], ys[ok], wp[:, ok]
         ix0, iy0 = int(xs.min()), int(ys.min())
         isx, isy = int(xs.max()) - ix0 + 1, int(ys.max()) - iy0 + 1
-        box = np.zeros((isy, isx), dtype=np.complex128)
-        np.add.at(box, (ys - iy0, xs - ix0), wp)
+        box = np.zeros((self.nf, isy, isx), dtype=np.complex128)
+        for f in range(self.nf):
+            np.add.at(box[f], (ys - iy0, xs - ix0), wp[f])
         return ix0, iy0, isx, isy, self.torch.from_numpy(np.ascontiguousarray(box).view(np.float64).copy()).to(self.dev)
 
     def gradient(self):
@@ -189,7 +198,7 @@ class AdjointProblem2D:
             plan = next(p for p in s._plans if p.monitor is self.monitors[b])
             if len(plan.record_steps) != self.N:
                 raise ValueError("the objective monitor must record every step")
-            self._phasor.append(plan.phasor[:, :2].contiguous())     # (N, 1 freq) complex -> (N, 2) FP64
+            self._phasor.append(plan.phasor.contiguous())     # (N, nf) complex as (N, 2 nf) FP64
         descs = self._adjoint_descs(lam, grad, inj)
         raw = b"".join(bytes(d) for d in descs)
         ddev = T.frombuffer(bytearray(raw), dtype=T.uint8).to(self.dev)

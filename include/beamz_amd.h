@@ -138,12 +138,12 @@ int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, 
 /* ---------------- reverse-time 2D adjoint ----------------
  * No reference counterpart (the reference's gradient is the overlap heuristic of
  * examples/4_topo.py:282-285). Exact transpose of the bz_yee2d_run step for the objective
- * J = |a|^2, a = sum_n sum_p c_p Ez_{n+1}(p) phasor[n]: reverse step n maps the adjoint state
+ * J = sum_f s_f |a_f|^2, a_f = sum_n sum_p c_fp Ez_{n+1}(p) phasor[n][f]: reverse step n maps the adjoint state
  * lam_{n+1} (buffer parity (n+1)&1) to lam_n (parity n&1) and, inside the design box, adds
  * dJ/d eps_r of step n to grad using the forward snapshot that bz_yee2d_run stored for the
  * segment (snapshot box = design box grown by one cell on the low x and y sides, slot
  * n - snap_step0). The design box must lie in the interior (1 <= x, x + sx <= nx - 1, same in y).
- * inj_w: complex dJ/dEz-sample weight 2 conj(a) c_p per cell of the injection box. */
+ * inj_w: complex dJ/dEz-sample weights 2 s_f conj(a_f) c_fp per frequency and cell of the injection box. */
 typedef struct bz_adj2d {
     int nx, ny, ld;
     float* le[2];
@@ -160,15 +160,16 @@ typedef struct bz_adj2d {
     int snap_step0;
     double* grad;                         /* [box_sy][box_sx] FP64 accumulator (NULL: no gradient) */
     int inj_x0, inj_y0, inj_sx, inj_sy;
-    const double* inj_w;                  /* [inj_sy][inj_sx] complex, or NULL */
-    const double* phasor;                 /* [num_steps] complex phasor of the sample after step n */
+    const double* inj_w;                  /* [nf][inj_sy][inj_sx] complex, or NULL */
+    const double* phasor;                 /* [num_steps][nf] complex phasors of the sample after step n */
+    int nf;                               /* number of objective frequencies */
 } bz_adj2d;
 
 /* Reverse steps n = n_hi-1 down to n_hi-nsteps of every adjoint described in `adjs` (device
  * array); tiles as for bz_yee2d_run (seg 2, 4 or 8). */
 int bz_adj2d_run(const bz_adj2d* adjs, int nadj, const int* tiles, int ntiles, int seg, int n_hi, int nsteps,
                  void* stream);
-/* lE_n += Re(inj_w * phasor[n-1]) over each injection box (max_inj >= the largest box's cell count):
+/* lE_n += sum_f Re(inj_w[f] * phasor[n-1][f]) over each injection box (max_inj >= the largest box's cell count):
  * the terminal condition lam_N = dJ/du_N before the first reverse step. */
 int bz_adj2d_inject(const bz_adj2d* adjs, int nadj, int max_inj, int n, void* stream);
 

@@ -9,7 +9,7 @@ shapes (numpy_backend.py:34-88 through ``fdtd_ref.update_h_fields_2d`` /
 gradient is the overlap heuristic of examples/4_topo.py:282-285, restated in
 ``topo_ref.overlap_gradient``):
 
-    a = sum_n sum_p c_p Ez_{n+1}(p) exp(-i w t_n),    J = |a|^2
+    a_f = sum_n sum_p c_fp Ez_{n+1}(p) exp(-i w_f t_n),    J = sum_f s_f |a_f|^2
 
 i.e. the DFT at w of the Ez samples a monitor records after each step,
 projected on complex weights c_p (a mode profile). ``Problem2D.gradient``
@@ -83,40 +83,43 @@ class Problem2D:
     """Forward run, objective and exact adjoint gradient on a small 2D grid (FP64, real fields).
 
     src: (y0, y1, x0, x1, weights) additive Ez patch; signal[n] its amplitude at step n.
-    pts: list of (x, y) Ez sample cells; c: complex weight per point; omega: angular frequency.
+    pts: list of (x, y) Ez sample cells; c: complex weights, (npts,) or (nf, npts); omega: angular
+    frequency, scalar or (nf,); fscale: s_f of J = sum_f s_f |a_f|^2 (default 1).
     """
 
-    def __init__(self, eps_r, sigma, dx, dy, dt, nsteps, src, signal, pts, c, omega):
+    def __init__(self, eps_r, sigma, dx, dy, dt, nsteps, src, signal, pts, c, omega, fscale=None):
         self.eps_r = np.asarray(eps_r, dtype=np.float64)
         self.sigma = np.asarray(sigma, dtype=np.float64)
         self.dx, self.dy, self.dt, self.nsteps = dx, dy, dt, int(nsteps)
         self.src, self.signal = src, np.asarray(signal, dtype=np.float64)
         self.pts = [(int(x), int(y)) for x, y in pts]
-        self.c = np.asarray(c, dtype=np.complex128)
-        self.omega = float(omega)
+        self.c = np.atleast_2d(np.asarray(c, dtype=np.complex128))
+        self.omega = np.atleast_1d(np.asarray(omega, dtype=np.float64))
+        self.fscale = np.ones(len(self.omega)) if fscale is None else np.asarray(fscale, dtype=np.float64)
         t = 0.0
         ts = []
         for _ in range(self.nsteps):      # fdtd.py accumulates t += dt
             ts.append(t)
             t += dt
-        self.phasor = np.exp(-1j * self.omega * np.asarray(ts))
+        self.phasor = np.exp(-1j * np.outer(np.asarray(ts), self.omega))     # (N, nf)
 
     def forward(self, eps_r=None, keep=False):
         eps = self.eps_r if eps_r is None else np.asarray(eps_r, dtype=np.float64)
         ny, nx = eps.shape
         E = np.zeros((ny, nx)); Hx = np.zeros((ny, nx - 1)); Hy = np.zeros((ny - 1, nx))
         y0, y1, x0, x1, wsrc = self.src
-        a = 0j
+        a = np.zeros(len(self.omega), dtype=np.complex128)
+        xs = np.array([p[0] for p in self.pts]); ys = np.array([p[1] for p in self.pts])
         states = [(E.copy(), Hx.copy(), Hy.copy())] if keep else None
         for n in range(self.nsteps):
             E, Hx, Hy = step_2d(E, Hx, Hy, eps, self.sigma, self.dx, self.dy, self.dt)
             E[y0:y1, x0:x1] += wsrc * self.signal[n]
-            a += self.phasor[n] * sum(cp * E[y, x] for cp, (x, y) in zip(self.c, self.pts))
+            a += self.phasor[n] * (self.c @ E[ys, xs])
             if keep:
                 states.append((E.copy(), Hx.copy(), Hy.copy()))
         self.a = a
         self.states = states
-        return float(abs(a) ** 2)
+        return float((self.fscale * np.abs(a) ** 2).sum())
 
     def gradient(self):
         """dJ/d eps_r on the full grid (zero outside the interior)."""
@@ -124,13 +127,14 @@ class Problem2D:
         S = self.states
         fa, fb, ca, cb, dca, dcb = coefficients_2d(self.eps_r, self.sigma, self.dt)
         ny, nx = self.eps_r.shape
-        wgt = 2.0 * np.conj(self.a) * self.c
+        wgt = 2.0 * (self.fscale * np.conj(self.a))[:, None] * self.c       # (nf, npts)
         lE = np.zeros((ny, nx)); lHx = np.zeros((ny, nx - 1)); lHy = np.zeros((ny - 1, nx))
         N = self.nsteps
 
         def inject(lE, n):          # the sample recorded after step n-1 is E_n
-            for wp, (x, y) in zip(wgt, self.pts):
-                lE[y, x] += np.real(wp * self.phasor[n - 1])
+            v = np.real(self.phasor[n - 1] @ wgt)                           # (npts,)
+            for q, (x, y) in enumerate(self.pts):
+                lE[y, x] += v[q]
 
         inject(lE, N)
         g = np.zeros((ny, nx))

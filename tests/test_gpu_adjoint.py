@@ -127,3 +127,34 @@ def test_autograd_chain_through_density_filters(hip):
     g_ref = TR.cone_filter_T(g_eps * (eh - ec) * dproj, 3)
     err = np.linalg.norm(g_rho - g_ref) / np.linalg.norm(g_ref)
     assert err < 2e-3, err
+
+
+def test_multi_frequency_adjoint_matches_fp64_oracle(hip):
+    torch = hip
+    eps, sig, rng = materials(seed=4)
+    d = bz.Design(NX * DX * (1 + 1e-9), NY * DX * (1 + 1e-9), material=bz.Material(1.0), auto_pml=False)
+    dt = 0.99 * DX / (bz.LIGHT_SPEED * np.sqrt(2))
+    t = np.arange(380) * dt
+    st = 1 / (2 * np.pi * bz.LIGHT_SPEED * 0.3e-6 / 1.55e-6 ** 2)
+    src = bz.GaussianSource(position=(12 * DX, NY / 2 * DX), width=DX / 4,
+                            signal=gaussian_pulse(t, 1.0, 4 * st, st, F0, 0.0))
+    freqs = F0 * np.array([0.96, 1.0, 1.04])
+    mon = bz.Monitor(design=d, start=((NX - 12) * DX, 8 * DX), end=((NX - 12) * DX, (NY - 8) * DX),
+                     dft_frequencies=freqs, record_fields=False, accumulate_power=False)
+    d += src
+    d += mon
+    sim = bz.FDTD(RegularGrid.from_arrays(d, eps, sig, DX), time=t)
+    npts = len(mon.get_grid_points_2d(DX, DX))
+    c = rng.standard_normal((3, npts)) + 1j * rng.standard_normal((3, npts))
+    fscale = [0.5, 1.0, 2.0]
+    prob = AdjointProblem2D([sim], [mon], c, BOX, fscale=fscale)
+    x0, y0, sx, sy = BOX
+    J = prob.forward(torch.from_numpy(eps[y0:y0 + sy, x0:x0 + sx]).float().cuda()[None]).item()
+    g = prob.gradient().cpu().numpy()[0]
+    patch = R.gaussian_patch_2d(src.position, src.width, sim.dx, sim.dy, NX, NY)
+    pts = R.monitor_points_2d(mon.start, mon.end, sim.dx, sim.dy)
+    P = A.Problem2D(eps.astype(np.float32).astype(np.float64), sig.astype(np.float32).astype(np.float64), sim.dx,
+                    sim.dy, sim.dt, sim.num_steps, patch, src.signal, pts, c, 2 * np.pi * freqs, fscale=fscale)
+    gr = P.gradient()[y0:y0 + sy, x0:x0 + sx]
+    assert abs(J - P.J) <= 1e-4 * P.J, (J, P.J)
+    assert np.linalg.norm(g - gr) / np.linalg.norm(gr) < 1e-3

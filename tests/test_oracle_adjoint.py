@@ -53,3 +53,20 @@ def test_adjoint_gradient_matches_central_differences():
         assert abs(fd - g[j, i]) <= 1e-6 * np.abs(g).max() + 1e-5 * abs(fd), (j, i, fd, g[j, i])
     # boundary cells are never updated, so eps there has no effect
     assert np.all(g[0, :] == 0) and np.all(g[:, 0] == 0) and np.all(g[-1, :] == 0) and np.all(g[:, -1] == 0)
+
+
+def test_multi_frequency_gradient_matches_central_differences():
+    P, rng = small_problem(seed=2, nsteps=140)
+    f0 = P.omega[0]
+    npts = len(P.pts)
+    c = rng.standard_normal((3, npts)) + 1j * rng.standard_normal((3, npts))
+    M = A.Problem2D(P.eps_r, P.sigma, P.dx, P.dy, P.dt, P.nsteps, P.src, P.signal, P.pts, c,
+                    f0 * np.array([0.95, 1.0, 1.05]), fscale=[0.5, 1.0, 2.0])
+    g = M.gradient()
+    ny, nx = M.eps_r.shape
+    for (j, i) in [(ny // 2, 12), (5, nx - 9), (ny - 4, 8)]:
+        h = 1e-4 * M.eps_r[j, i]
+        ep = M.eps_r.copy(); ep[j, i] += h
+        em = M.eps_r.copy(); em[j, i] -= h
+        fd = (M.forward(ep) - M.forward(em)) / (2 * h)
+        assert abs(fd - g[j, i]) <= 1e-6 * np.abs(g).max() + 1e-5 * abs(fd), (j, i, fd, g[j, i])
