@@ -63,7 +63,7 @@ SIGNATURES = {
     "bz_yee3d_step": [P6, P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, C.POINTER(BoxSource), i32,
                       i32, i32, vp],
     "bz_yee3d_step2": [P6, P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, C.POINTER(BoxSource),
-                       C.POINTER(f32), i32, vp, C.POINTER(i32), i32, i32, vp],
+                       C.POINTER(f32), i32, vp, C.POINTER(i32), i32, i32, i32, vp],
     "bz_yee2d_run": [vp, i32, vp, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp],
     "bz_adj2d_run": [vp, i32, vp, i32, i32, i32, i32, vp],
     "bz_nn_conv3x3": [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp],

@@ -374,7 +374,7 @@ int launch(const Fields6& S, const Fields6& D, const float* eps, const float* si
 extern "C" int bz_yee3d_step2(float* const src[6], float* const dst[6], const float* eps_r, const float* sigma,
                               int nx, int ny, int nz, int ld, double dx, double dy, double dz, double dt, double mu0,
                               double eps0, const bz_box_source* boxes, const float* amp1, int nbox, float* stage,
-                              const int* stage_box, int kseg, int rows, void* stream) {
+                              const int* stage_box, int kseg, int rows, int waves, void* stream) {
     BZ_REQUIRE(src && dst && eps_r && sigma, "bz_yee3d_step2: null pointer");
     for (int q = 0; q < 6; ++q)
         BZ_REQUIRE(src[q] && dst[q] && src[q] != dst[q], "bz_yee3d_step2: bad field buffers");
@@ -416,6 +416,8 @@ extern "C" int bz_yee3d_step2(float* const src[6], float* const dst[6], const fl
     const Coef c = make_coef(dt, mu0, eps0);
     const float rx = (float)(1.0 / dx), ry = (float)(1.0 / dy), rz = (float)(1.0 / dz);
     const hipStream_t st = (hipStream_t)stream;
+    if (waves == 16 && rows == 2) return launch<2, 16>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, stg, st);
+    BZ_REQUIRE(waves == 8, "bz_yee3d_step2: waves must be 8 (rows 2, 3, 4) or 16 (rows 2)");
     switch (rows) {
         case 2: return launch<2, 8>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, stg, st);
         case 3: return launch<3, 8>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, stg, st);

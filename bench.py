@@ -43,6 +43,8 @@ B_ALG = {"mmi3d": 56, "wg2d": 32, "batch2d": 32, "adjoint2d": 32}
 # profiles/yee3d_fused_mmi_480x240x160_hbm_counters.md.
 MEASURED_TRAFFIC = {
     ("mmi3d", "fused", 2, 16, 480, 240, 160): (874_454 + 476_068) * 1024,
+    # two-step kernel: one launch = two steps (profiles/yee3d_tb2_mmi_480x240x160.md)
+    ("mmi3d", "fused-tb2-w8", 2, 54, 480, 240, 160): (2 * 352_009 + 465_912) * 1024,
 }
 
 
@@ -178,6 +180,7 @@ def build_runner(a, total):
     sim.grid.tb2 = bool(a.tb2)
     sim.grid.rows2 = a.rows2
     sim.grid.kseg2 = a.kseg2
+    sim.grid.waves2 = a.waves2
     sim.grid.seg2d = a.seg2d
     sim.grid.graph2d = a.graph
     sim.initialize_simulation(save=False, live=False)
@@ -204,7 +207,8 @@ def main():
     ap.add_argument("--rows", type=int, default=int(os.environ.get("BEAMZ_AMD_ROWS", "2")))
     ap.add_argument("--tb2", type=int, default=1, help="3D: two steps per launch (1) or one (0)")
     ap.add_argument("--rows2", type=int, default=2, help="3D two-step kernel: y-rows per wave (2, 3, 4)")
-    ap.add_argument("--kseg2", type=int, default=32, help="3D two-step kernel: z-planes per segment")
+    ap.add_argument("--kseg2", type=int, default=0, help="3D two-step kernel: z-planes per segment (0: auto)")
+    ap.add_argument("--waves2", type=int, default=8, help="3D two-step kernel: waves per workgroup (8, 16)")
     ap.add_argument("--seg2d", type=int, default=None, help="rows per tile of the 2D engine (default: by size)")
     ap.add_argument("--graph", action="store_true", help="capture 2D runs as HIP graphs")
     ap.add_argument("--autotune", action="store_true", help="batch2d: pick the tile height by timing")
@@ -251,7 +255,7 @@ def main():
     achieved = launch_cells * b_alg / step_kernel_s
     nx, ny = r.grid.nx, r.grid.ny
     if a.workload == "mmi3d" and a.tb2:
-        key = (a.workload, "fused-tb2", a.rows2, a.kseg2, nx, ny, r.grid.nz)
+        key = (a.workload, f"fused-tb2-w{a.waves2}", a.rows2, r.grid.kseg2_eff(), nx, ny, r.grid.nz)
     else:
         key = (a.workload, a.kernel, a.rows, a.kseg, nx, ny, r.grid.nz if a.workload == "mmi3d" else 1)
     traffic = MEASURED_TRAFFIC.get(key)
@@ -262,7 +266,8 @@ def main():
         "config": {"workload": r.desc, "cells": r.cells, "kernel": r.kernel_name, "gpu_event_s": gpu_s},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel_ms": step_kernel_s * 1e3, "bytes_per_cell_update": b_alg},
+                     "kernel_ms": step_kernel_s * 1e3, "bytes_per_cell_update": b_alg,
+                     "steps_per_launch": 2 if (a.workload == "mmi3d" and a.tb2) else 1},
     }
     if r.tuned is not None:
         out["config"]["autotune_ms_per_step"] = {str(k): v for k, v in r.tuned[1].items()}

@@ -92,11 +92,12 @@ int bz_yee3d_step(float* const src[6], float* const dst[6], const float* eps_r, 
  * (step n+2); bit-identical to two bz_yee3d_step calls. boxes[q].amp is the amplitude of step n,
  * amp1[q] that of step n+1. If stage is not NULL, the fields of step n+1 inside the box
  * stage_box = {x0, y0, z0, sx, sy, sz} are written to stage[c][z][y][x] (c = Ex, Ey, Ez, Hx, Hy, Hz)
- * for monitor sampling. kseg: z-planes per segment (>= 2; <= 0: 32); rows: y-rows per wave (2, 3, 4). */
+ * for monitor sampling. kseg: z-planes per segment (>= 2; <= 0: 32); rows: y-rows per wave (2, 3, 4);
+ * waves: waves per workgroup stacked in y (8, or 16 with rows 2). */
 int bz_yee3d_step2(float* const src[6], float* const dst[6], const float* eps_r, const float* sigma,
                    int nx, int ny, int nz, int ld, double dx, double dy, double dz, double dt,
                    double mu0, double eps0, const bz_box_source* boxes, const float* amp1, int nbox,
-                   float* stage, const int* stage_box, int kseg, int rows, void* stream);
+                   float* stage, const int* stage_box, int kseg, int rows, int waves, void* stream);
 
 /* ---------------- fused, batched 2D engine ----------------
  * Runs nsteps full steps (numpy_backend.py:34-88, then the source box, then optional grid power,
