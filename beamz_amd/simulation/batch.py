@@ -54,6 +54,22 @@ class FDTDBatch:
             s.grid._f2d = False       # members advance through the batch engine only
         self.current_step = 0
 
+    def reset(self):
+        """Back to step 0 without re-planning: zero every member's fields and the monitor sums (series and
+        FP64 DFT accumulators) in place. Materials, sources and monitor layouts are kept, so a new design can
+        be written into the members' permittivity and run again at the cost of the steps alone."""
+        if self.engine is None:
+            self.initialize()
+            return
+        for s in self.sims:
+            s.grid.reset_fields()
+            s.t = 0
+            s.current_step = 0
+        for buf in (self.engine.series, self.engine.dft):
+            if buf is not None:
+                buf.zero_()
+        self.current_step = 0
+
     def advance(self, count):
         """Run ``count`` steps of every member (bounded by the time axis)."""
         if self.engine is None:

@@ -1,0 +1,98 @@
+"""Training data and objective of the surrogate, produced by the batched GPU FDTD.
+
+The device is the batched-2D member geometry at a fixed size: a 256 x 256 cell
+SiO2 domain (25 nm cells, 10-cell PML), 0.5 um Si input and output guides and a
+128 x 128 pixel (3.2 um) design region between them, a Gaussian-pulse point
+source in the input guide and a DFT line monitor across the output guide at 8
+wavelengths in 1.50-1.60 um, 2000 steps. The label of a density map rho is its
+mode-overlap transmission spectrum
+
+    T_f(rho) = |a_f(rho)|^2 / |a_f(straight guide)|^2,  a_f = sum_p c_fp Ez_hat(p, w_f)
+
+with c_f the conjugated, normalised fundamental slab mode of the output guide
+at wavelength f; the broadband figure of merit is mean_f T_f.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from ..const import µm
+from ..workloads import EPS_OX, EPS_SI, batch_member_2d, binary_density
+
+PX = 128
+SIDE = 256
+
+
+class TransmissionProblem:
+    """A fixed batch of `batch` simulations of the member geometry, re-used for every evaluation."""
+
+    def __init__(self, batch=256, steps=2000, seed=0, side=SIDE, px=PX):
+        import beamz_amd as bz
+        from ..optimization.adjoint import mode_weights
+        from ..simulation.batch import FDTDBatch
+        self.px, self.side, self.steps = px, side, steps
+        rng = np.random.default_rng(seed)
+        self.sims, self.mons = [], []
+        for _ in range(batch):
+            mesh, t, (m_in, m_out) = batch_member_2d(side, rng, steps=steps, design_um=px * 25e-9 / µm)
+            self.sims.append(bz.FDTD(mesh, time=t))
+            self.mons.append(m_out)
+        s0 = self.sims[0]
+        self.dx = s0.dx
+        self.x0 = self.y0 = (side - px) // 2
+        self.lams = bz.LIGHT_SPEED / self.mons[0].dft_frequencies
+        H = side * s0.dy
+        self.c = np.stack([mode_weights(self.mons[0], s0.dx, s0.dy, H / 2, 0.5 * µm, 3.476, 1.444, lam)
+                           for lam in self.lams])
+        self.batch = FDTDBatch(self.sims)
+        self.torch = s0.backend.torch
+        self.batch.initialize()
+        T = self.torch
+        self.plans = [next(p for p in s._plans if p.monitor is m) for s, m in zip(self.sims, self.mons)]
+        self.npts = self.plans[0].layout[1]
+        self.c_dev = T.from_numpy(self.c).to(s0.backend.dev)
+        self.ref = None
+        straight = np.zeros((1, px, px), dtype=np.float32)
+        band = (np.arange(px) + 0.5 - px / 2) * s0.dy
+        straight[0, np.abs(band) < 0.25 * µm, :] = 1.0
+        self.ref = self.raw_overlaps(self.torch.tensor(straight))[0]          # |a_f|^2 of the straight guide
+
+    @property
+    def box(self):
+        return (self.x0, self.y0, self.px, self.px)
+
+    def eps_of(self, rho):
+        return EPS_OX + rho * (EPS_SI - EPS_OX)
+
+    def raw_overlaps(self, rho):
+        """|a_f|^2 for each density map of rho (n, px, px) in [0, 1] (torch or numpy), in batches."""
+        T = self.torch
+        rho = T.as_tensor(rho, dtype=T.float32)
+        out = []
+        B = len(self.sims)
+        for q0 in range(0, rho.shape[0], B):
+            chunk = rho[q0:q0 + B].to(self.sims[0].backend.dev)
+            self.batch.reset()
+            for k, s in enumerate(self.sims):
+                e = self.eps_of(chunk[k] if k < chunk.shape[0] else chunk[0])
+                s.grid.eps[self.y0:self.y0 + self.px, self.x0:self.x0 + self.px] = e
+            self.batch.advance(self.batch.num_steps)
+            z = T.stack([p.dft[:, :self.npts, :] for p in self.plans[:chunk.shape[0]]])   # (n, nf, npts, 2)
+            a = (self.c_dev * T.complex(z[..., 0], z[..., 1])).sum(dim=2)
+            out.append((a.real ** 2 + a.imag ** 2).cpu().numpy())
+        return np.concatenate(out)
+
+    def transmission(self, rho):
+        return self.raw_overlaps(rho) / self.ref
+
+
+def random_densities(n, seed, px=PX):
+    """The batched workload's density generator: seeded Bernoulli(0.5), 3-pixel Gaussian blur, threshold."""
+    rng = np.random.default_rng(seed)
+    return np.stack([binary_density(px, rng).astype(np.float32) for _ in range(n)])
+
+
+def generate(problem, n, seed=0):
+    """n (density, transmission spectrum) pairs from the batched FDTD."""
+    rho = random_densities(n, seed, problem.px)
+    return rho, problem.transmission(rho).astype(np.float32)

@@ -158,3 +158,54 @@ def test_multi_frequency_adjoint_matches_fp64_oracle(hip):
     gr = P.gradient()[y0:y0 + sy, x0:x0 + sx]
     assert abs(J - P.J) <= 1e-4 * P.J, (J, P.J)
     assert np.linalg.norm(g - gr) / np.linalg.norm(gr) < 1e-3
+
+
+def test_c4_gradient_vs_central_differences_on_32_pixels(hip):
+    """The full-size topology workload (600 x 600 cells, 120 x 120 design pixels, 3000 steps, cone filter r=3,
+    tanh projection beta=8): d FOM / d rho from the adjoint + autograd chain against central differences
+    (h = 1e-3 in rho) on 32 seeded pixels, the 64 perturbed forward runs advanced as one batch. FP32 fields
+    limit the differences' accuracy, so the check is on the whole sampled vector: cosine > 0.99 and 2e-2
+    relative L2 over the pixels."""
+    torch = hip
+    from beamz_amd.simulation.batch import FDTDBatch
+    from beamz_amd.workloads import topo_bend_2d
+    mesh, t, mon, box, c = topo_bend_2d()
+    x0, y0, sx, sy = box
+    ec, eh = 1.444 ** 2, 3.476 ** 2
+    rng = np.random.default_rng(0)
+    rho0 = np.clip(0.5 + 0.05 * rng.standard_normal((sy, sx)), 0.0, 1.0)
+    sim = bz.FDTD(mesh, time=t)
+    prob = AdjointProblem2D([sim], [mon], c, box)
+    rho = torch.tensor(rho0[None], dtype=torch.float32, device="cuda", requires_grad=True)
+    J = FDTDObjective.apply(density_to_eps(rho, ec, eh, radius=3, beta=8.0), prob)
+    J.sum().backward()
+    g = rho.grad.cpu().numpy()[0]
+    pix = [(int(rng.integers(0, sy)), int(rng.integers(0, sx))) for _ in range(32)]
+    h = 1e-3
+    rhos = []
+    for (j, i) in pix:
+        for s in (1.0, -1.0):
+            r = rho0.copy(); r[j, i] += s * h
+            rhos.append(r)
+    eps_all = density_to_eps(torch.tensor(np.array(rhos), dtype=torch.float32, device="cuda"), ec, eh, 3, 8.0)
+    sims, mons = [], []
+    for q in range(len(rhos)):
+        m2, t2, mon2, _, _ = topo_bend_2d()
+        s2 = bz.FDTD(m2, time=t2)
+        sims.append(s2); mons.append(mon2)
+    batch = FDTDBatch(sims)
+    batch.initialize()
+    for q, s2 in enumerate(sims):
+        s2.grid.eps[y0:y0 + sy, x0:x0 + sx] = eps_all[q]
+    batch.advance(batch.num_steps)
+    Js = []
+    for s2, m2 in zip(sims, mons):
+        s2.finalize_simulation()
+        a = (c * m2.dft["Ez"][0]).sum()
+        Js.append(abs(a) ** 2)
+    fd = np.array([(Js[2 * k] - Js[2 * k + 1]) / (2 * h) for k in range(len(pix))])
+    ga = np.array([g[j, i] for (j, i) in pix])
+    cos = float(fd @ ga / (np.linalg.norm(fd) * np.linalg.norm(ga)))
+    err = float(np.linalg.norm(fd - ga) / np.linalg.norm(fd))
+    print(f"C4 FD check: cosine {cos:.6f}, rel-L2 {err:.3e}, J {float(J):.6e}")
+    assert cos > 0.99 and err < 2e-2, (cos, err)
