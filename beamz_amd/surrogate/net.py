@@ -30,6 +30,8 @@ class Surrogate:
         P["fw"] = rng.standard_normal((self.nout, LEVELS[-1][1])) * np.sqrt(1.0 / LEVELS[-1][1])
         P["fb"] = np.zeros(self.nout)
         self.params = {k: torch.tensor(v, dtype=torch.float32, device=self.dev).contiguous() for k, v in P.items()}
+        # outputs are head outputs times out_scale (set from the training targets, so the head works at O(1))
+        self.out_scale = torch.ones(self.nout, dtype=torch.float32, device=self.dev)
         self.m = {k: torch.zeros_like(v) for k, v in self.params.items()}
         self.v = {k: torch.zeros_like(v) for k, v in self.params.items()}
         self.grads = {k: torch.zeros_like(v) for k, v in self.params.items()}
@@ -100,7 +102,7 @@ class Surrogate:
                out.data_ptr(), B, h * w, C, self.nout, st)
         cache["gap"] = gap
         self.cache = cache
-        return out
+        return out * self.out_scale
 
     # ---------------- backward ----------------
     def backward(self, gout, input_grad=False):
@@ -112,7 +114,7 @@ class Surrogate:
         for g in self.grads.values():
             g.zero_()
         B, H, W = cache["shape"]
-        gout = gout.to(device=self.dev, dtype=T.float32).contiguous()
+        gout = (gout.to(device=self.dev, dtype=T.float32) * self.out_scale).contiguous()
         _, _, y1, h, w = cache["acts"][-1]
         C = LEVELS[-1][1]
         ga = self._empty16(B, h, w, C)
@@ -166,11 +168,12 @@ class Surrogate:
         self.sync_weights()
 
     def train_step(self, x, target, lr=1e-3):
-        """One Adam step on the mean squared error; returns the loss (a device scalar)."""
+        """One Adam step on the mean squared error of the outputs divided by out_scale; returns that loss
+        (a device scalar)."""
         out = self.forward(x)
         t = target.to(device=self.dev, dtype=self.torch.float32)
-        diff = out - t
-        self.backward(2.0 * diff / diff.numel())
+        diff = (out - t) / self.out_scale
+        self.backward(2.0 * diff / self.out_scale / diff.numel())
         self.adam(lr)
         return (diff * diff).mean()
 

@@ -19,11 +19,15 @@ import time
 import numpy as np
 
 
-def train(net, rho, T, epochs=10, batch=256, lr=1e-3, seed=0, log=None):
+def train(net, rho, T, epochs=10, batch=256, lr=1e-3, seed=0, log=None, set_scale=True):
+    """Mini-batch Adam on the spectra. With set_scale the network's output scale becomes the mean target
+    per wavelength (the loss is then the relative squared error, whatever the transmission level)."""
     torch = net.torch
     rng = np.random.default_rng(seed)
     x = torch.as_tensor(rho, dtype=torch.float32, device=net.dev)
     y = torch.as_tensor(T, dtype=torch.float32, device=net.dev)
+    if set_scale:
+        net.out_scale = y.mean(dim=0).clamp_min(1e-12).contiguous()
     n = x.shape[0]
     hist = []
     for ep in range(epochs):
@@ -39,17 +43,32 @@ def train(net, rho, T, epochs=10, batch=256, lr=1e-3, seed=0, log=None):
     return hist
 
 
-def relative_error(net, rho, T):
+LOG_FLOOR = 1e-4
+
+
+def to_log(T):
+    """Log-space targets: transmissions of random designs span orders of magnitude."""
+    return np.log(np.asarray(T, dtype=np.float64) + LOG_FLOOR).astype(np.float32)
+
+
+def from_log(y):
+    return np.exp(np.asarray(y, dtype=np.float64)) - LOG_FLOOR
+
+
+def predict_T(net, rho, log_space=False):
     torch = net.torch
     p = net.predict(torch.as_tensor(rho, dtype=torch.float32, device=net.dev)).cpu().numpy()
+    return from_log(p) if log_space else p
+
+
+def relative_error(net, rho, T, log_space=False):
+    p = predict_T(net, rho, log_space)
     return float(np.linalg.norm(p - T) / np.linalg.norm(T)), p
 
 
-def screen(net, problem, rho_candidates, top=16, seed=0):
+def screen(net, problem, rho_candidates, top=16, seed=0, log_space=False):
     """Verified mean transmission of the surrogate's top picks and of a random pick of the same size."""
-    torch = net.torch
-    x = torch.as_tensor(rho_candidates, dtype=torch.float32, device=net.dev)
-    score = net.predict(x).mean(dim=1).cpu().numpy()
+    score = predict_T(net, rho_candidates, log_space).mean(axis=1)
     best = np.argsort(-score)[:top]
     rand = np.random.default_rng(seed).choice(len(rho_candidates), top, replace=False)
     fom_best = problem.transmission(rho_candidates[best]).mean(axis=1)
@@ -57,11 +76,66 @@ def screen(net, problem, rho_candidates, top=16, seed=0):
     return fom_best, fom_rand, score[best]
 
 
+class PlainAdjointLoop:
+    """The method without the surrogate: batched exact-adjoint gradient ascent of the broadband figure of
+    merit mean_f T_f over `designs` density maps (cone filter r=3, tanh projection beta=8, Adam on the
+    latent densities), every iteration one batched forward + adjoint FDTD."""
+
+    def __init__(self, problem, designs=16, lr=0.05, radius=3, beta=8.0, seed=0):
+        import beamz_amd as bz
+        from ..optimization.adjoint import AdjointProblem2D
+        from ..workloads import batch_member_2d
+        from ..const import µm
+        torch = problem.torch
+        rng = np.random.default_rng(seed)
+        sims, mons = [], []
+        for _ in range(designs):
+            mesh, t, (m_in, m_out) = batch_member_2d(problem.side, rng, steps=problem.steps,
+                                                     design_um=problem.px * 25e-9 / µm)
+            sims.append(bz.FDTD(mesh, time=t))
+            mons.append(m_out)
+        fscale = 1.0 / (len(problem.ref) * problem.ref)
+        self.adj = AdjointProblem2D(sims, mons, problem.c, problem.box, fscale=fscale)
+        self.problem, self.radius, self.beta, self.lr = problem, radius, beta, lr
+        self.rho = torch.tensor(np.clip(0.5 + 0.05 * rng.standard_normal((designs, problem.px, problem.px)), 0, 1),
+                                dtype=torch.float32, device=problem.sims[0].backend.dev, requires_grad=True)
+        self.opt = torch.optim.Adam([self.rho], lr=lr)
+        self.history = []
+
+    def physical(self):
+        from ..optimization.adjoint import cone_filter_torch, tanh_projection_torch
+        return tanh_projection_torch(cone_filter_torch(self.rho, self.radius), self.beta)
+
+    def step(self):
+        from ..optimization.adjoint import FDTDObjective
+        torch = self.problem.torch
+        self.opt.zero_grad()
+        phys = self.physical()
+        J = FDTDObjective.apply(self.problem.eps_of(phys), self.adj)
+        (-J.sum()).backward()
+        self.opt.step()
+        with torch.no_grad():
+            self.rho.clamp_(0.0, 1.0)
+        return J.detach().cpu().numpy(), phys.detach()
+
+    def run(self, iterations=40, clock=None):
+        t0 = time.perf_counter() if clock is None else clock
+        best, best_rho = -1.0, None
+        for _ in range(iterations):
+            J, phys = self.step()
+            k = int(np.argmax(J))
+            if J[k] > best:
+                best, best_rho = float(J[k]), phys[k].cpu().numpy()
+            self.history.append((time.perf_counter() - t0, best))
+        return best, best_rho
+
+
 class SurrogateLoop:
     """Surrogate-assisted optimisation of the broadband figure of merit (mean transmission)."""
 
-    def __init__(self, net, verifier, candidates=1024, top=16, climb_steps=5, climb_rate=0.05, seed=0):
-        self.net, self.verifier = net, verifier
+    def __init__(self, net, verifier, candidates=1024, top=16, climb_steps=5, climb_rate=0.05, seed=0,
+                 log_space=False):
+        self.net, self.verifier, self.log_space = net, verifier, log_space
         self.candidates, self.top = candidates, top
         self.climb_steps, self.climb_rate = climb_steps, climb_rate
         self.rng = np.random.default_rng(seed)
@@ -84,8 +158,7 @@ class SurrogateLoop:
 
     def step(self, finetune_epochs=3, lr=5e-4):
         cand = self.propose()
-        torch = self.net.torch
-        score = self.net.predict(torch.as_tensor(cand, device=self.net.dev)).mean(dim=1).cpu().numpy()
+        score = predict_T(self.net, cand, self.log_space).mean(axis=1)
         pick = cand[np.argsort(-score)[:self.top]]
         T = self.verifier.transmission(pick).astype(np.float32)
         fom = T.mean(axis=1)
@@ -95,7 +168,8 @@ class SurrogateLoop:
         keep = np.argsort(-self.pool_fom)[:256]
         self.pool_rho, self.pool_T, self.pool_fom = self.pool_rho[keep], self.pool_T[keep], self.pool_fom[keep]
         if len(self.pool_rho) >= 64:
-            train(self.net, self.pool_rho, self.pool_T, epochs=finetune_epochs, batch=64, lr=lr)
+            y = to_log(self.pool_T) if self.log_space else self.pool_T
+            train(self.net, self.pool_rho, y, epochs=finetune_epochs, batch=64, lr=lr, set_scale=False)
         return float(fom.max())
 
     def run(self, seed_rho, seed_T, rounds=10, target=None, clock=None):

@@ -34,7 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
-B_ALG = {"mmi3d": 56, "wg2d": 32, "batch2d": 32, "adjoint2d": 32}
+B_ALG = {"mmi3d": 56, "wg2d": 32, "batch2d": 32, "adjoint2d": 32, "surrogate": 32}
 
 # HBM bytes per launch of the update kernel, from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of
 # this script with --copy-mib 512, each counter scaled by the factor that makes the 512 MiB calibration
@@ -80,6 +80,82 @@ def cpu_baseline_2d(sim, seconds_target=15.0):
         if el > seconds_target:
             break
     return sim.nx * sim.ny * n / el, n, el
+
+
+def surrogate_report(a):
+    """The surrogate workflow on one GPU: FDTD data generation, BF16-MFMA training (linear- and log-space
+    targets), held-out error against FDTD, screening gain, and the surrogate-assisted loop against the plain
+    adjoint loop (time to the plain loop's figure of merit)."""
+    import torch
+    from beamz_amd.surrogate import LEVELS, Surrogate
+    from beamz_amd.surrogate.data import TransmissionProblem, generate, random_densities
+    from beamz_amd.surrogate.optimize import (PlainAdjointLoop, SurrogateLoop, relative_error, screen, to_log,
+                                              train)
+    t0 = time.perf_counter()
+    prob = TransmissionProblem(batch=256)
+    verifier = TransmissionProblem(batch=16, seed=1)
+    setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rho, T = generate(prob, a.samples, seed=1)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    cells = prob.side * prob.side
+    gen_rate = -(-a.samples // 256) * 256 * cells * prob.steps / gen_s
+    rho_v, T_v = generate(prob, 1024, seed=2)
+    # FLOPs of one training sample (forward + both backward GEMMs) of the implemented network
+    px = prob.px
+    fwd = 0
+    for lvl, (ci, co) in enumerate(LEVELS):
+        hw = (px >> lvl) ** 2
+        fwd += 2 * hw * 9 * (ci * co + co * co)
+    res = {}
+    nets = {}
+    for mode in ("linear", "log"):
+        net = Surrogate(seed=0)
+        y = to_log(T) if mode == "log" else T
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        hist = train(net, rho, y, epochs=a.epochs, batch=128, lr=5e-4)
+        torch.cuda.synchronize()
+        tr = time.perf_counter() - t1
+        err, _ = relative_error(net, rho_v, T_v, log_space=(mode == "log"))
+        nets[mode] = net
+        res[mode] = {"train_s": tr, "loss_first": hist[0], "loss_last": hist[-1], "heldout_rel_l2": err,
+                     "train_tflops": 3 * fwd * a.epochs * (a.samples // 128) * 128 / tr / 1e12}
+    const_err = float(np.linalg.norm(T.mean(axis=0) - T_v) / np.linalg.norm(T_v))
+    cands = random_densities(1024, seed=3)
+    scr = {}
+    for mode, net in nets.items():
+        fb, fr, _ = screen(net, verifier, cands, top=16, seed=4, log_space=(mode == "log"))
+        scr[mode] = {"top16_verified_fom": float(fb.mean()), "random16_verified_fom": float(fr.mean())}
+    # plain adjoint loop vs surrogate-assisted loop on the broadband FOM
+    plain = PlainAdjointLoop(prob, designs=16)
+    t1 = time.perf_counter()
+    plain_best, _ = plain.run(iterations=a.iters)
+    plain_s = time.perf_counter() - t1
+    best_mode = "log" if res["log"]["heldout_rel_l2"] < res["linear"]["heldout_rel_l2"] else "linear"
+    loop = SurrogateLoop(nets[best_mode], verifier, log_space=(best_mode == "log"))
+    order = np.argsort(-T.mean(axis=1))[:256]
+    t1 = time.perf_counter()
+    surr_best = loop.run(rho[order], T[order], rounds=a.rounds, target=0.99 * plain_best)
+    surr_s = time.perf_counter() - t1
+    reach = next((t for t, b in loop.history if b >= 0.99 * plain_best), None)
+    out = {
+        "metric": "Yee cell-updates/s", "value": gen_rate, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": prob.steps, "warmup": 0, "ms_per_step": gen_s / (-(-a.samples // 256) * prob.steps) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16",
+        "data": "synthetic",
+        "config": {"workload": f"surrogate: {a.samples} FDTD samples ({prob.side}x{prob.side} cells, "
+                               f"{prob.px}x{prob.px} design px, {prob.steps} steps, 8 wavelengths), "
+                               f"4-level BF16-MFMA CNN, {a.epochs} epochs"},
+        "surrogate": {"setup_s": setup_s, "generation_s": gen_s, "train": res, "constant_predictor_rel_l2": const_err,
+                      "screen_1024": scr, "flop_per_sample_fwd": fwd,
+                      "loop": {"plain_adjoint_best_fom": plain_best, "plain_s": plain_s, "plain_iterations": a.iters,
+                               "surrogate_best_fom": surr_best, "surrogate_s": surr_s,
+                               "surrogate_time_to_99pct_of_plain_s": reach, "surrogate_model": best_mode,
+                               "one_time_cost_s": setup_s + gen_s + res[best_mode]["train_s"]}},
+    }
+    print(json.dumps(out))
 
 
 class Runner:
@@ -201,6 +277,10 @@ def main():
     ap.add_argument("--nsims", type=int, default=64)
     ap.add_argument("--designs", type=int, default=16, help="adjoint2d: designs per batched gradient")
     ap.add_argument("--nt", type=int, default=3000, help="adjoint2d: time steps per simulation")
+    ap.add_argument("--samples", type=int, default=20000, help="surrogate: FDTD training samples")
+    ap.add_argument("--epochs", type=int, default=8, help="surrogate: training epochs")
+    ap.add_argument("--iters", type=int, default=40, help="surrogate: plain adjoint loop iterations")
+    ap.add_argument("--rounds", type=int, default=12, help="surrogate: surrogate-assisted loop rounds")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default=os.environ.get("BEAMZ_AMD_KERNEL", "fused"))
     ap.add_argument("--kseg", type=int, default=int(os.environ.get("BEAMZ_AMD_KSEG", "16")))
@@ -222,6 +302,9 @@ def main():
         sys.exit(2)
     import torch
 
+    if a.workload == "surrogate":
+        surrogate_report(a)
+        return
     total = a.warmup + a.steps + a.timing_steps
     r = build_runner(a, total)
     if a.copy_mib > 0:
