@@ -20,14 +20,14 @@ import numpy as np
 
 
 def train(net, rho, T, epochs=10, batch=256, lr=1e-3, seed=0, log=None, set_scale=True):
-    """Mini-batch Adam on the spectra. With set_scale the network's output scale becomes the mean target
-    per wavelength (the loss is then the relative squared error, whatever the transmission level)."""
+    """Mini-batch Adam on the spectra. With set_scale the network's output scale becomes the mean magnitude
+    of the targets per wavelength (the loss is then a relative squared error, whatever their level)."""
     torch = net.torch
     rng = np.random.default_rng(seed)
     x = torch.as_tensor(rho, dtype=torch.float32, device=net.dev)
     y = torch.as_tensor(T, dtype=torch.float32, device=net.dev)
     if set_scale:
-        net.out_scale = y.mean(dim=0).clamp_min(1e-12).contiguous()
+        net.out_scale = y.abs().mean(dim=0).clamp_min(1e-12).contiguous()
     n = x.shape[0]
     hist = []
     for ep in range(epochs):

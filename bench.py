@@ -45,6 +45,8 @@ MEASURED_TRAFFIC = {
     ("mmi3d", "fused", 2, 16, 480, 240, 160): (874_454 + 476_068) * 1024,
     # two-step kernel: one launch = two steps (profiles/yee3d_tb2_mmi_480x240x160.md)
     ("mmi3d", "fused-tb2-w8", 2, 54, 480, 240, 160): (2 * 352_009 + 465_912) * 1024,
+    # batched 2D engine, 64 members, 2-row tiles (profiles/yee2d_fused_batch64_kernel_and_counters.md)
+    ("batch2d", "seg2", 64): (2 * 483_330 + 367_632) * 1024,
 }
 
 
@@ -227,6 +229,7 @@ def build_runner(a, total):
             return v, f"{n} reference-order complex128 steps of the median member ({big.nx}x{big.ny}) ({el:.1f} s)"
         r = Runner(batch.advance, sims[0].grid, cells, desc, kname, cpu)
         r.tuned = tuned
+        r.seg = batch.seg
         return r
     if a.workload == "mmi3d":
         nx, ny, nz = a.nx or 480, a.ny or 240, a.nz
@@ -339,6 +342,8 @@ def main():
     nx, ny = r.grid.nx, r.grid.ny
     if a.workload == "mmi3d" and a.tb2:
         key = (a.workload, f"fused-tb2-w{a.waves2}", a.rows2, r.grid.kseg2_eff(), nx, ny, r.grid.nz)
+    elif a.workload == "batch2d":
+        key = (a.workload, f"seg{r.seg}", a.nsims)
     else:
         key = (a.workload, a.kernel, a.rows, a.kseg, nx, ny, r.grid.nz if a.workload == "mmi3d" else 1)
     traffic = MEASURED_TRAFFIC.get(key)
