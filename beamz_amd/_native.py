@@ -41,7 +41,8 @@ class Sim2D(C.Structure):
                 ("eps", vp), ("sig", vp), ("kh", f32), ("kbh", f32), ("ke", f32), ("kbe", f32),
                 ("rdx", f32), ("rdy", f32), ("src_x0", i32), ("src_y0", i32), ("src_sx", i32), ("src_sy", i32),
                 ("src_add", vp), ("src_mul", vp), ("signal", vp), ("power", vp),
-                ("snap_x0", i32), ("snap_y0", i32), ("snap_sx", i32), ("snap_sy", i32), ("snap", vp)]
+                ("snap_x0", i32), ("snap_y0", i32), ("snap_sx", i32), ("snap_sy", i32), ("snap", vp),
+                ("stg", vp), ("stg_x0", i32), ("stg_y0", i32), ("stg_sx", i32), ("stg_sy", i32)]
 
 
 class Adj2D(C.Structure):
@@ -65,6 +66,7 @@ SIGNATURES = {
     "bz_yee3d_step2": [P6, P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, C.POINTER(BoxSource),
                        C.POINTER(f32), i32, vp, C.POINTER(i32), i32, i32, i32, vp],
     "bz_yee2d_run": [vp, i32, vp, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp],
+    "bz_yee2d_run2": [vp, i32, vp, i32, i32, vp, i32, i32, vp, vp, i32, vp, vp, i32, vp, vp, vp],
     "bz_adj2d_run": [vp, i32, vp, i32, i32, i32, i32, vp],
     "bz_nn_conv3x3": [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp],
     "bz_nn_conv3x3_dgrad": [vp, vp, vp, i32, i32, i32, i32, i32, vp],

@@ -29,6 +29,10 @@
 // so the whole run can be captured once in a hipGraph and replayed.
 #include "common.h"
 
+// No multiply-add contraction: the one-step and two-step kernels then round identically, so a run gives
+// the same bits whichever mix of them it uses.
+#pragma clang fp contract(off)
+
 namespace {
 
 constexpr int LANES = 64;
@@ -192,6 +196,161 @@ __global__ void k_tick(int* step, int nsteps) {
     if (threadIdx.x == 0) step[0] += nsteps;
 }
 
+// ---------------- two steps per pass ----------------
+// Steps n, n+1 of a tile of 60 columns x seg rows: the wave marches y computing step n+1 (H then E) on
+// row r and step n+2 on row r-1 (its E needs only the H of step n+2 on rows r-1 and r-2, carried), so
+// one pass reads the five arrays and writes the three fields once per two steps. Lanes 0, 1, 62, 63 are
+// halo lanes; a segment starting at j0 > 0 begins two rows early. Reads buffer rd, writes rd ^ 1.
+// Same rounded operations as k_yee2d_fused (no contraction in this file), so the results are identical.
+constexpr int OX2 = LANES - 4;
+
+__device__ __forceinline__ void box_src(float& e, const bz_sim2d& S, int i, int j, float amp) {
+    const int a = i - S.src_x0, b = j - S.src_y0;
+    if (a >= 0 && a < S.src_sx && b >= 0 && b < S.src_sy) {
+        const long r = (long)b * S.src_sx + a;
+        e = (S.src_mul ? S.src_mul[r] : 1.0f) * e + S.src_add[r] * amp;
+    }
+}
+
+__device__ __forceinline__ void add_power(const bz_sim2d& S, long o, float e, float hx, float hy) {
+    const double ed = e;
+    const double sx = -ed * (double)hy, sy = ed * (double)hx;
+    S.power[o] += sx * sx + sy * sy;
+}
+
+__global__ __launch_bounds__(LANES* WPG) void k_yee2d_tb2(const bz_sim2d* __restrict__ sims,
+                                                         const int* __restrict__ tiles, int ntiles, int seg,
+                                                         const int* __restrict__ step, int q, int rd) {
+    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * WPG + threadIdx.y);
+    if (t >= ntiles) return;
+    const int n = *step + 2 * q;
+    const int wr = rd ^ 1;
+    const bz_sim2d& S = sims[tiles[3 * t]];
+    const int tx = tiles[3 * t + 1], ty = tiles[3 * t + 2];
+    const int nx = S.nx, ny = S.ny, ld = S.ld;
+    const int l = threadIdx.x;
+    const int i = tx * OX2 - 2 + l;
+    const uint32_t ic = (uint32_t)min(max(i, 0), nx - 1);
+    const int j0 = ty * seg, j1 = min(ny, j0 + seg);
+    const float* __restrict__ gez = S.ez[rd];
+    const float* __restrict__ ghx = S.hx[rd];
+    const float* __restrict__ ghy = S.hy[rd];
+    float* __restrict__ oez = S.ez[wr];
+    float* __restrict__ ohx = S.hx[wr];
+    float* __restrict__ ohy = S.hy[wr];
+    Coef c;
+    c.kh = S.kh;
+    c.kbh = S.kbh;
+    c.ke = S.ke;
+    c.kbe = S.kbe;
+    const float rdx = S.rdx, rdy = S.rdy;
+    const bool hxok = i < nx - 1;
+    const bool xin = i >= 1 && i <= nx - 2;
+    const bool out = l >= 2 && l <= LANES - 3 && i < nx;
+    const float amp0 = S.signal ? S.signal[n] : 0.0f;
+    const float amp1 = S.signal ? S.signal[n + 1] : 0.0f;
+    const int rstart = j0 > 0 ? j0 - 2 : 0;
+    auto off = [&](int r) { return ((uint32_t)min(max(r, 0), ny - 1) * (uint32_t)ld + ic) * 4u; };
+
+    float ez0 = at(gez, off(rstart));
+    float e1p = 0.0f, h1xp = 0.0f, h1yp = 0.0f, sgp = 0.0f, eep = 1.0f, h2yp = 0.0f;
+    // software pipeline: the loads of row r+1 are in flight while row r is computed and stored
+    float nez = at(gez, off(rstart + 1)), nhx = at(ghx, off(rstart)), nhy = at(ghy, off(rstart));
+    float nsg = at(S.sig, off(rstart)), nee = at(S.eps, off(rstart));
+    for (int r = rstart; r <= j1; ++r) {
+        const float ez1 = nez, hx0 = nhx, hy0 = nhy, sg = nsg, ee = nee;
+        {
+            const uint32_t o1 = off(r + 1);
+            nez = at(gez, off(r + 2));
+            nhx = at(ghx, o1);
+            nhy = at(ghy, o1);
+            nsg = at(S.sig, o1);
+            nee = at(S.eps, o1);
+        }
+        // step n+1 on row r
+        const float ezR = __shfl_down(ez0, 1, LANES);
+        float fa, fb;
+        h_coef(sg, c, fa, fb);
+        const float h1x = hxok ? fa * hx0 - fb * ((ezR - ez0) * rdy) : hx0;
+        const float h1y = (r < ny - 1) ? fa * hy0 + fb * ((ez1 - ez0) * rdx) : hy0;
+        const float h1xL = __shfl_up(h1x, 1, LANES);
+        float ca, cb;
+        e_coef(sg, ee, c, ca, cb);
+        float e1 = (r >= 1 && r <= ny - 2 && xin) ? ca * ez0 + cb * (-((h1x - h1xL) * rdy) + (h1y - h1yp) * rdx) : ez0;
+        if (S.src_sx) box_src(e1, S, i, r, amp0);
+        // step n+2 on row m = r - 1 (step n+1 of that row is e1p, h1xp, h1yp)
+        const int m = r - 1;
+        const float e1pR = __shfl_down(e1p, 1, LANES);
+        float fa2, fb2;
+        h_coef(sgp, c, fa2, fb2);
+        const float h2x = hxok ? fa2 * h1xp - fb2 * ((e1pR - e1p) * rdy) : h1xp;
+        const float h2y = (m < ny - 1) ? fa2 * h1yp + fb2 * ((e1 - e1p) * rdx) : h1yp;
+        const float h2xL = __shfl_up(h2x, 1, LANES);
+        float ca2, cb2;
+        e_coef(sgp, eep, c, ca2, cb2);
+        float e2 = (m >= 1 && m <= ny - 2 && xin) ? ca2 * e1p + cb2 * (-((h2x - h2xL) * rdy) + (h2y - h2yp) * rdx)
+                                                  : e1p;
+        if (out && m >= j0 && m < j1) {
+            if (S.src_sx) box_src(e2, S, i, m, amp1);
+            const uint32_t om = off(m);
+            put(oez, om, e2);
+            put(ohx, om, h2x);
+            put(ohy, om, h2y);
+            if (S.power) add_power(S, (long)m * ld + i, e2, h2x, h2y);
+        }
+        if (out && r >= j0 && r < j1) {
+            if (S.power) add_power(S, (long)r * ld + i, e1, h1x, h1y);
+            if (S.stg) {
+                const int a = i - S.stg_x0, b = r - S.stg_y0;
+                if (a >= 0 && a < S.stg_sx && b >= 0 && b < S.stg_sy) {
+                    const long plane = (long)S.stg_sx * S.stg_sy;
+                    float* p = S.stg + (long)b * S.stg_sx + a;
+                    p[0] = e1;
+                    p[plane] = h1x;
+                    p[2 * plane] = h1y;
+                }
+            }
+        }
+        ez0 = ez1;
+        e1p = e1;
+        h1xp = h1x;
+        h1yp = h1y;
+        sgp = sg;
+        eep = ee;
+        h2yp = h2y;
+    }
+}
+
+// Monitor samples of both steps of pair q: step n from the staging box, step n+1 from buffer wr.
+__global__ void k_gather2d_pair(const bz_sim2d* __restrict__ sims, Gather2D G, const int* __restrict__ spts,
+                                const int* __restrict__ step, int q, int wr) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= G.npts) return;
+    const int n = *step + 2 * q;
+    const bz_sim2d& S = sims[G.pts[3 * p]];
+    const int comp = G.pts[3 * p + 1];
+    for (int s = 0; s < 2; ++s) {
+        const int row = G.slot[n + s];
+        if (row < 0) continue;
+        float v;
+        if (s == 0) {
+            const int so = spts[p];
+            v = so >= 0 ? S.stg[(long)comp * S.stg_sx * S.stg_sy + so] : 0.0f;
+        } else {
+            const float* f = comp == 0 ? S.ez[wr] : (comp == 1 ? S.hx[wr] : S.hy[wr]);
+            v = f[G.pts[3 * p + 2]];
+        }
+        if (G.series) G.series[(long)row * G.npts + p] = v;
+        const double dv = v;
+        const double* ph = G.phasor + 2 * (long)row * G.nfreq;
+        for (int k = 0; k < G.nfreq; ++k) {
+            double* a = G.dft + 2 * ((long)k * G.npts + p);
+            a[0] += dv * ph[2 * k];
+            a[1] += dv * ph[2 * k + 1];
+        }
+    }
+}
+
 template <int SEG>
 int run(const bz_sim2d* sims, const int* tiles, int ntiles, int* step, int nsteps, const Gather2D& G,
         hipStream_t st) {
@@ -238,4 +397,37 @@ extern "C" int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, i
         case 16: return run<16>(sims, tiles, ntiles, step, nsteps, G, st);
         default: bz::set_error("bz_yee2d_run: seg must be 2, 4, 8 or 16"); return BZ_EINVAL;
     }
+}
+
+extern "C" int bz_yee2d_run2(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step,
+                             int rd0, int npairs, const int* pts, const int* spts, int npts, const int* slot,
+                             float* series, int nfreq, const double* phasor, double* dft, void* stream) {
+    BZ_REQUIRE(sims && nsims > 0 && tiles && ntiles > 0 && step && npairs >= 0 && seg >= 2 && (rd0 == 0 || rd0 == 1),
+               "bz_yee2d_run2: bad arguments");
+    BZ_REQUIRE(npts >= 0 && (npts == 0 || (pts && spts && slot)), "bz_yee2d_run2: monitor points without tables");
+    BZ_REQUIRE(nfreq >= 0 && (nfreq == 0 || (phasor && dft)), "bz_yee2d_run2: DFT requested without buffers");
+    if (npairs == 0) return BZ_OK;
+    Gather2D G;
+    G.pts = pts;
+    G.npts = npts;
+    G.slot = slot;
+    G.series = series;
+    G.nfreq = nfreq;
+    G.phasor = phasor;
+    G.dft = dft;
+    const hipStream_t st = (hipStream_t)stream;
+    const dim3 g((ntiles + WPG - 1) / WPG), b(LANES, WPG);
+    for (int q = 0; q < npairs; ++q) {
+        const int rd = (rd0 + q) & 1;
+        hipLaunchKernelGGL(k_yee2d_tb2, g, b, 0, st, sims, tiles, ntiles, seg, step, q, rd);
+        BZ_CHECK_LAUNCH("bz_yee2d_run2(steps)");
+        if (npts > 0) {
+            hipLaunchKernelGGL(k_gather2d_pair, dim3((npts + GPB - 1) / GPB), dim3(GPB), 0, st, sims, G, spts, step, q,
+                               rd ^ 1);
+            BZ_CHECK_LAUNCH("bz_yee2d_run2(gather)");
+        }
+    }
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, step, 2 * npairs);
+    BZ_CHECK_LAUNCH("bz_yee2d_run2(tick)");
+    return BZ_OK;
 }

@@ -25,7 +25,7 @@ from .engine import Fused2D
 class FDTDBatch:
     SEGS = Fused2D.SEGS
 
-    def __init__(self, sims, seg=None, graph=False, chunk=100):
+    def __init__(self, sims, seg=None, graph=False, chunk=100, pairs=None, seg2=None):
         if not sims:
             raise ValueError("FDTDBatch needs at least one simulation")
         for s in sims:
@@ -37,6 +37,7 @@ class FDTDBatch:
                 raise ValueError("all simulations of a batch must share one time axis")
         self.sims = list(sims)
         self.seg, self.graph, self.chunk = seg, bool(graph), int(chunk)
+        self.pairs, self.seg2 = bool(pairs), seg2
         self.num_steps = len(t0)
         self.current_step = 0
         self.engine = None
@@ -48,7 +49,8 @@ class FDTDBatch:
     def initialize(self):
         for s in self.sims:
             s.initialize_simulation(save=False, live=False)
-        self.engine = Fused2D([s.grid for s in self.sims], seg=self.seg, graph=self.graph, max_steps=self.num_steps)
+        self.engine = Fused2D([s.grid for s in self.sims], seg=self.seg, graph=self.graph, max_steps=self.num_steps,
+                              pairs=self.pairs, seg2=self.seg2)
         self.seg = self.engine.seg
         for s in self.sims:
             s.grid._f2d = False       # members advance through the batch engine only
@@ -94,14 +96,17 @@ class FDTDBatch:
         self.sims[0].synchronize()
 
     def autotune(self, steps=20, segs=None):
-        """Pick the tile height with the lowest measured step time; leaves the batch re-initialised."""
+        """Pick the tile height with the lowest measured step time (the two-step kernel's tiles when the
+        engine runs pairs, else the one-step kernel's); leaves the batch re-initialised."""
         T = self.sims[0].backend.torch
         if self.engine is None:
             self.initialize()
+        pairs = self.engine.pairs
+        steps += steps % 2
         best = None
         timings = {}
-        for seg in segs or self.SEGS:
-            self.engine.set_seg(seg)
+        for seg in segs or (self.engine.SEGS2 if pairs else self.SEGS):
+            (self.engine.set_seg2 if pairs else self.engine.set_seg)(seg)
             self.engine.run(0, 2)                       # warm-up
             e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
             e0.record()
@@ -112,7 +117,9 @@ class FDTDBatch:
             timings[seg] = ms
             if best is None or ms < timings[best]:
                 best = seg
-        self.seg = best
+        if pairs:
+            self.seg2 = best
+        else:
+            self.seg = best
         self.initialize()
-        self.engine.set_seg(best)
         return best, timings

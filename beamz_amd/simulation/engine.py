@@ -496,9 +496,15 @@ class Fused2D:
     captured once as a HIP graph and replayed for later runs of the same length.
     """
 
-    def __init__(self, grids, seg=8, graph=False, max_steps=None, monitors=True, snapshots=None):
+    PAIR_MIN_CELLS = 1_000_000
+
+    def __init__(self, grids, seg=8, graph=False, max_steps=None, monitors=True, snapshots=None, pairs=None,
+                 seg2=None):
         """monitors=False runs the steps without sampling any monitor (a recomputation);
-        snapshots: per grid None or (x0, y0, sx, sy, tensor [steps][3][sy][sx]) for the snapshot box."""
+        snapshots: per grid None or (x0, y0, sx, sy, tensor [steps][3][sy][sx]) for the snapshot box;
+        pairs: advance two steps per launch (bz_yee2d_run2) where a run allows it (None: from PAIR_MIN_CELLS
+        cells on; never with snapshots or
+        graphs), with tiles of 60 columns x seg2 rows (None: by size)."""
         if not grids:
             raise ValueError("Fused2D needs at least one grid")
         self.snapshots = list(snapshots) if snapshots is not None else [None] * len(grids)
@@ -559,6 +565,65 @@ class Fused2D:
             self.npts, self.nfreq = ntot, p0.nfreq
         self.use_graph = bool(graph)
         self.graphs = {}
+        if pairs is None:
+            # small batches are launch-latency bound, and a pair's serial row march is slower there
+            # (800 x 400: 13.4 us/step in pairs against 10.2 with one step per launch)
+            pairs = sum(g.nx * g.ny for g in grids) >= self.PAIR_MIN_CELLS
+        self.pairs = bool(pairs) and not self.use_graph and all(s is None for s in self.snapshots)
+        self.spts = None
+        if self.pairs:
+            if self.npts:
+                self._setup_staging(pts)
+            self.set_seg2(seg2)
+
+    def _setup_staging(self, pts):
+        """Per simulation, a staging box around its monitor points for the first step of each pair."""
+        T = self.torch
+        spts = np.full(len(pts), -1, np.int32)
+        for s, g in enumerate(self.grids):
+            sel = np.nonzero(pts[:, 0] == s)[0]
+            if len(sel) == 0:
+                continue
+            j, i = np.divmod(pts[sel, 2].astype(np.int64), g.ld)
+            real = pts[sel, 2] != g.offset(0, 0, g.nx - 1)       # the zero sentinel of out-of-range points
+            if not real.any():
+                continue
+            x0, x1 = int(i[real].min()), int(i[real].max()) + 1
+            y0, y1 = int(j[real].min()), int(j[real].max()) + 1
+            sx, sy = x1 - x0, y1 - y0
+            buf = T.zeros(3 * sx * sy, dtype=T.float32, device=self.dev)
+            self.keep.append(buf)
+            d = self.descs[s]
+            d.stg, d.stg_x0, d.stg_y0, d.stg_sx, d.stg_sy = buf.data_ptr(), x0, y0, sx, sy
+            inside = real & (i >= x0) & (i < x1) & (j >= y0) & (j < y1)
+            spts[sel[inside]] = ((j[inside] - y0) * sx + (i[inside] - x0)).astype(np.int32)
+        self.spts = T.from_numpy(spts).to(self.dev)
+        self.shift = None            # descriptors changed: upload again
+
+    SEGS2 = (4, 8, 16, 32, 64)
+
+    def auto_seg2(self, min_waves=8192):
+        """Tallest pair tile (fewest prologue rows) that still gives min_waves tiles, else 4."""
+        for seg in sorted(self.SEGS2, reverse=True):
+            if sum(-(-g.nx // 60) * -(-g.ny // seg) for g in self.grids) >= min_waves:
+                return seg
+        return min(self.SEGS2)
+
+    def set_seg2(self, seg2):
+        """Tile list of the two-step kernel: tiles of 60 columns x seg2 rows."""
+        T = self.torch
+        self.seg2 = self.auto_seg2() if seg2 is None else int(seg2)
+        if self.seg2 < 2:
+            raise ValueError("pair tiles need at least 2 rows")
+        rows = []
+        for s, g in enumerate(self.grids):
+            gx = (g.nx + 59) // 60
+            gy = (g.ny + self.seg2 - 1) // self.seg2
+            t = np.stack(np.meshgrid(np.arange(gx), np.arange(gy), indexing="xy"), -1).reshape(-1, 2)
+            rows.append(np.concatenate([np.full((len(t), 1), s), t], axis=1))
+        tiles = np.concatenate(rows).astype(np.int32)
+        self.tiles2 = T.from_numpy(tiles.ravel()).to(self.dev)
+        self.ntiles2 = len(tiles)
 
     def _describe(self, g, snap=None):
         T = self.torch
@@ -675,21 +740,35 @@ class Fused2D:
         if g0.timing:
             e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
             e0.record()
-        if self.use_graph and count > 1:
-            gr = self.graphs.get(count)
+        rem = count
+        if self.pairs and count >= 2:
+            # pairs of steps; pair q reads the buffer holding step n0 + 2q and writes the other one
+            npairs = count // 2
+            ptr = lambda t: t.data_ptr() if t is not None else None
+            N.call("bz_yee2d_run2", self.desc_dev.data_ptr(), len(self.grids), self.tiles2.data_ptr(),
+                   self.ntiles2, self.seg2, self.step.data_ptr(), n0 & 1, npairs, ptr(self.pts), ptr(self.spts),
+                   self.npts, ptr(self.slot), ptr(self.series), self.nfreq, ptr(self.phasor), ptr(self.dft),
+                   g0.stream)
+            for g in self.grids:
+                g._cur = (g._cur + npairs) % 2
+            rem = count - 2 * npairs
+            if rem:
+                self._sync_descs(n0 + 2 * npairs)      # step counter is already there
+        if rem and self.use_graph and rem > 1:
+            gr = self.graphs.get(rem)
             if gr is None:
                 gr = T.cuda.CUDAGraph()
                 T.cuda.synchronize(self.dev)
                 with T.cuda.graph(gr):
-                    self._launch(count)
-                self.graphs[count] = gr
+                    self._launch(rem)
+                self.graphs[rem] = gr
             gr.replay()
-        else:
-            self._launch(count)
+        elif rem:
+            self._launch(rem)
         if g0.timing:
             e1.record()
             g0._events.append((e0, e1, count))
         for g in self.grids:
-            g._cur = (g._cur + count) % 2
+            g._cur = (g._cur + rem) % 2
             if g.power is not None:
                 g.power_count += count

@@ -221,7 +221,10 @@ def build_runner(a, total):
         sides = sorted(s.nx for s in sims)
         desc = (f"{len(sims)} batched 2D TE sims, sides {sides[0]}..{sides[-1]} (seeded), 25 nm cells, "
                 f"binary 2x2 um designs, 10-cell PML, 2 DFT ports x 8 wavelengths, one launch per step")
-        kname = f"bz_yee2d_run (fused, batched, seg {batch.seg}{', hipGraph' if a.graph else ''})"
+        if batch.engine.pairs:
+            kname = f"bz_yee2d_run2 (fused, batched, two steps per launch, 60x{batch.engine.seg2} tiles)"
+        else:
+            kname = f"bz_yee2d_run (fused, batched, seg {batch.engine.seg}{', hipGraph' if a.graph else ''})"
 
         def cpu():
             big = sims[int(np.argsort([s.nx for s in sims])[len(sims) // 2])]
@@ -229,7 +232,8 @@ def build_runner(a, total):
             return v, f"{n} reference-order complex128 steps of the median member ({big.nx}x{big.ny}) ({el:.1f} s)"
         r = Runner(batch.advance, sims[0].grid, cells, desc, kname, cpu)
         r.tuned = tuned
-        r.seg = batch.seg
+        r.seg_key = f"pairs-seg{batch.engine.seg2}" if batch.engine.pairs else f"seg{batch.engine.seg}"
+        r.steps_per_launch = 2 if batch.engine.pairs else 1
         return r
     if a.workload == "mmi3d":
         nx, ny, nz = a.nx or 480, a.ny or 240, a.nz
@@ -264,6 +268,7 @@ def build_runner(a, total):
     sim.grid.graph2d = a.graph
     sim.initialize_simulation(save=False, live=False)
     r = Runner(sim._advance, sim.grid, cells, desc, sim.grid.kernel_name(), cpu)
+    r.steps_per_launch = 2 if (a.workload == "mmi3d" and a.tb2) else 1
     r.tuned = None
     return r
 
@@ -343,7 +348,7 @@ def main():
     if a.workload == "mmi3d" and a.tb2:
         key = (a.workload, f"fused-tb2-w{a.waves2}", a.rows2, r.grid.kseg2_eff(), nx, ny, r.grid.nz)
     elif a.workload == "batch2d":
-        key = (a.workload, f"seg{r.seg}", a.nsims)
+        key = (a.workload, r.seg_key, a.nsims)
     else:
         key = (a.workload, a.kernel, a.rows, a.kseg, nx, ny, r.grid.nz if a.workload == "mmi3d" else 1)
     traffic = MEASURED_TRAFFIC.get(key)
@@ -355,7 +360,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel_ms": step_kernel_s * 1e3, "bytes_per_cell_update": b_alg,
-                     "steps_per_launch": 2 if (a.workload == "mmi3d" and a.tb2) else 1},
+                     "steps_per_launch": getattr(r, "steps_per_launch", 1)},
     }
     if r.tuned is not None:
         out["config"]["autotune_ms_per_step"] = {str(k): v for k, v in r.tuned[1].items()}

@@ -130,11 +130,22 @@ typedef struct bz_sim2d {
     double* power;                        /* NULL or [ny][ld] FP64 grid-power accumulator */
     int snap_x0, snap_y0, snap_sx, snap_sy;   /* snapshot box (snap_sx = 0: none) */
     float* snap;                          /* [nsteps][3][snap_sy][snap_sx] */
+    float* stg;                           /* bz_yee2d_run2: [3][stg_sy][stg_sx] staging of the first step's */
+    int stg_x0, stg_y0, stg_sx, stg_sy;   /* Ez, Hx, Hy for monitor samples (NULL: none) */
 } bz_sim2d;
 
 int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step, int nsteps,
                  const int* pts, int npts, const int* slot, float* series, int nfreq, const double* phasor,
                  double* dft, void* stream);
+
+/* Two steps per pass: npairs pairs of the bz_yee2d_run step (same results, bit for bit), pair q reading
+ * buffer (rd0 + q) & 1 and writing the other, so after the call the state sits in buffer
+ * (rd0 + npairs) & 1 and the step counter has advanced by 2 * npairs. tiles: (sim, x-tile of 60
+ * columns, y-tile of `seg` rows), seg >= 2. Monitor samples of the first step of a pair come from each
+ * simulation's staging box: spts[p] = offset of point p inside its box (-1: not in it, reads 0). */
+int bz_yee2d_run2(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step, int rd0,
+                  int npairs, const int* pts, const int* spts, int npts, const int* slot, float* series,
+                  int nfreq, const double* phasor, double* dft, void* stream);
 
 /* ---------------- reverse-time 2D adjoint ----------------
  * No reference counterpart (the reference's gradient is the overlap heuristic of

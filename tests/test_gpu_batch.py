@@ -94,10 +94,27 @@ def test_batch_autotune_keeps_results(hip):
     mem = members(steps=200)
     batch = FDTDBatch([s for s, _ in mem], seg=8)
     best, timings = batch.autotune(steps=10)
-    assert best in FDTDBatch.SEGS and set(timings) == set(FDTDBatch.SEGS)
+    segs = batch.engine.SEGS2 if batch.engine.pairs else FDTDBatch.SEGS
+    assert best in segs and set(timings) == set(segs)
     batch.advance(batch.num_steps)
     for (s, m), (r, rm) in zip(mem, ref_mem):
         s.finalize_simulation()
         A, B = state(s, m), state(r, rm)
         for k in A:
             assert np.array_equal(A[k], B[k]), (best, k)
+
+
+@pytest.mark.parametrize("seg2", [2, 7, 64])
+def test_two_step_pairs_equal_single_steps(hip, seg2):
+    """Two steps per launch (staged intermediate monitor samples) must give the bits of one step per launch,
+    with an odd step count (pairs plus a last single step) and any pair-tile height."""
+    single = members(steps=301)
+    FDTDBatch([s for s, _ in single], pairs=False).run()
+    pair = members(steps=301)
+    b = FDTDBatch([s for s, _ in pair], pairs=True, seg2=seg2, chunk=75)
+    b.run()
+    assert b.engine.pairs
+    for (s, m), (r, rm) in zip(pair, single):
+        A, B = state(s, m), state(r, rm)
+        for k in A:
+            assert np.array_equal(A[k], B[k]), (seg2, s.nx, k)
