@@ -37,7 +37,7 @@ class FDTDBatch:
                 raise ValueError("all simulations of a batch must share one time axis")
         self.sims = list(sims)
         self.seg, self.graph, self.chunk = seg, bool(graph), int(chunk)
-        self.pairs, self.seg2 = bool(pairs), seg2
+        self.pairs, self.seg2 = pairs, seg2      # pairs None: the engine decides by batch size
         self.num_steps = len(t0)
         self.current_step = 0
         self.engine = None
