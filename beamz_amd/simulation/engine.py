@@ -602,8 +602,12 @@ class Fused2D:
 
     SEGS2 = (4, 8, 16, 32, 64)
 
-    def auto_seg2(self, min_waves=8192):
-        """Tallest pair tile (fewest prologue rows) that still gives min_waves tiles, else 4."""
+    def auto_seg2(self, min_waves=49152):
+        """Tallest pair tile (fewest prologue rows) that still gives min_waves tiles, else 4.
+
+        One tile is one wave and about 8192 waves are resident at once, so 49152 tiles are six rounds:
+        fewer rounds leave the last one ragged. On 64 members of 256..1024 cells a side (30 M cells) the
+        measured step times were 0.150 / 0.139 / 0.146 / 0.159 / 0.177 ms for 4 / 8 / 16 / 32 / 64 rows."""
         for seg in sorted(self.SEGS2, reverse=True):
             if sum(-(-g.nx // 60) * -(-g.ny // seg) for g in self.grids) >= min_waves:
                 return seg

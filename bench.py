@@ -10,7 +10,7 @@ over the whole grid, with all inputs resident in HBM.
 20-cell PML, point source, 16-wavelength DFT line monitor); ``--workload
 batch2d`` the batch of 64 independent 2D simulations (sides drawn from
 256..1024 cells by seed, binary 2 x 2 um design regions, 10-cell PML, input and
-output DFT ports at 8 wavelengths), all advanced by one launch per step.
+output DFT ports at 8 wavelengths), all advanced together, two steps per launch.
 
 Prints one JSON line: value = cell-updates/s (for the batch: summed over its
 members); roofline = the update kernel's algorithmic HBM bytes (3D: 56 B per
@@ -47,6 +47,8 @@ MEASURED_TRAFFIC = {
     ("mmi3d", "fused-tb2-w8", 2, 54, 480, 240, 160): (2 * 352_009 + 465_912) * 1024,
     # batched 2D engine, 64 members, 2-row tiles (profiles/yee2d_fused_batch64_kernel_and_counters.md)
     ("batch2d", "seg2", 64): (2 * 483_330 + 367_632) * 1024,
+    # two steps per launch, 60 x 8 pair tiles (profiles/yee2d_tb2_batch64_kernel_and_counters.md)
+    ("batch2d", "pairs-seg8", 64): (2 * 523_987 + 403_190) * 1024,
 }
 
 
@@ -220,7 +222,8 @@ def build_runner(a, total):
         cells = batch.cells
         sides = sorted(s.nx for s in sims)
         desc = (f"{len(sims)} batched 2D TE sims, sides {sides[0]}..{sides[-1]} (seeded), 25 nm cells, "
-                f"binary 2x2 um designs, 10-cell PML, 2 DFT ports x 8 wavelengths, one launch per step")
+                f"binary 2x2 um designs, 10-cell PML, 2 DFT ports x 8 wavelengths, "
+                f"{'two steps per launch' if batch.engine.pairs else 'one launch per step'}")
         if batch.engine.pairs:
             kname = f"bz_yee2d_run2 (fused, batched, two steps per launch, 60x{batch.engine.seg2} tiles)"
         else:
