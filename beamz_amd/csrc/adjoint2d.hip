@@ -61,6 +61,7 @@ __global__ __launch_bounds__(LANES* WPG) void k_adj2d(const bz_adj2d* __restrict
     if (t >= ntiles) return;
     const int rd = (n + 1) & 1, wr = n & 1;
     const int s = tiles[3 * t], tx = tiles[3 * t + 1], ty = tiles[3 * t + 2];
+    if (s < 0) return;   // padding entry of a reordered tile list
     const bz_adj2d& A = adjs[s];
     const int nx = A.nx, ny = A.ny, ld = A.ld;
     const int l = threadIdx.x;

@@ -94,6 +94,7 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee2d_fused(const bz_sim2d* __re
     if (t >= ntiles) return;
     const int rd = n & 1, wr = rd ^ 1;
     const int s = tiles[3 * t], tx = tiles[3 * t + 1], ty = tiles[3 * t + 2];
+    if (s < 0) return;   // padding entry of a reordered tile list
     const bz_sim2d& S = sims[s];
     const int nx = S.nx, ny = S.ny, ld = S.ld;
     const int l = threadIdx.x;
@@ -225,7 +226,9 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee2d_tb2(const bz_sim2d* __rest
     if (t >= ntiles) return;
     const int n = *step + 2 * q;
     const int wr = rd ^ 1;
-    const bz_sim2d& S = sims[tiles[3 * t]];
+    const int s = tiles[3 * t];
+    if (s < 0) return;   // padding entry of a reordered tile list
+    const bz_sim2d& S = sims[s];
     const int tx = tiles[3 * t + 1], ty = tiles[3 * t + 2];
     const int nx = S.nx, ny = S.ny, ld = S.ld;
     const int l = threadIdx.x;
@@ -322,33 +325,45 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee2d_tb2(const bz_sim2d* __rest
 }
 
 // Monitor samples of both steps of pair q: step n from the staging box, step n+1 from buffer wr.
+// One thread per (point p, frequency k), p fastest, so the FP64 accumulators dft[k][p] are read and
+// written coalesced and the latency of the read-modify-write is spread over npts * nfreq threads. Each
+// accumulator still receives step n's term and then step n+1's, as two rounded adds in that order.
 __global__ void k_gather2d_pair(const bz_sim2d* __restrict__ sims, Gather2D G, const int* __restrict__ spts,
                                 const int* __restrict__ step, int q, int wr) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= G.npts) return;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = (int)(g % G.npts);
+    const int k = (int)(g / G.npts);
+    if (k >= max(G.nfreq, 1)) return;
     const int n = *step + 2 * q;
     const bz_sim2d& S = sims[G.pts[3 * p]];
     const int comp = G.pts[3 * p + 1];
+    int row[2];
+    float v[2];
     for (int s = 0; s < 2; ++s) {
-        const int row = G.slot[n + s];
-        if (row < 0) continue;
-        float v;
+        row[s] = G.slot[n + s];
+        v[s] = 0.0f;
+        if (row[s] < 0) continue;
         if (s == 0) {
             const int so = spts[p];
-            v = so >= 0 ? S.stg[(long)comp * S.stg_sx * S.stg_sy + so] : 0.0f;
+            v[s] = so >= 0 ? S.stg[(long)comp * S.stg_sx * S.stg_sy + so] : 0.0f;
         } else {
             const float* f = comp == 0 ? S.ez[wr] : (comp == 1 ? S.hx[wr] : S.hy[wr]);
-            v = f[G.pts[3 * p + 2]];
+            v[s] = f[G.pts[3 * p + 2]];
         }
-        if (G.series) G.series[(long)row * G.npts + p] = v;
-        const double dv = v;
-        const double* ph = G.phasor + 2 * (long)row * G.nfreq;
-        for (int k = 0; k < G.nfreq; ++k) {
-            double* a = G.dft + 2 * ((long)k * G.npts + p);
-            a[0] += dv * ph[2 * k];
-            a[1] += dv * ph[2 * k + 1];
-        }
+        if (k == 0 && G.series) G.series[(long)row[s] * G.npts + p] = v[s];
     }
+    if (k >= G.nfreq || (row[0] < 0 && row[1] < 0)) return;
+    double* a = G.dft + 2 * ((long)k * G.npts + p);
+    double re = a[0], im = a[1];
+    for (int s = 0; s < 2; ++s) {
+        if (row[s] < 0) continue;
+        const double dv = v[s];
+        const double* ph = G.phasor + 2 * ((long)row[s] * G.nfreq + k);
+        re += dv * ph[0];
+        im += dv * ph[1];
+    }
+    a[0] = re;
+    a[1] = im;
 }
 
 template <int SEG>
@@ -417,12 +432,15 @@ extern "C" int bz_yee2d_run2(const bz_sim2d* sims, int nsims, const int* tiles, 
     G.dft = dft;
     const hipStream_t st = (hipStream_t)stream;
     const dim3 g((ntiles + WPG - 1) / WPG), b(LANES, WPG);
+    const long gather_threads = (long)npts * (nfreq > 0 ? nfreq : 1);
+    BZ_REQUIRE(gather_threads / GPB < (1L << 31) - 1, "bz_yee2d_run2: too many monitor points x frequencies");
+    const int gather_blocks = (int)((gather_threads + GPB - 1) / GPB);
     for (int q = 0; q < npairs; ++q) {
         const int rd = (rd0 + q) & 1;
         hipLaunchKernelGGL(k_yee2d_tb2, g, b, 0, st, sims, tiles, ntiles, seg, step, q, rd);
         BZ_CHECK_LAUNCH("bz_yee2d_run2(steps)");
         if (npts > 0) {
-            hipLaunchKernelGGL(k_gather2d_pair, dim3((npts + GPB - 1) / GPB), dim3(GPB), 0, st, sims, G, spts, step, q,
+            hipLaunchKernelGGL(k_gather2d_pair, dim3(gather_blocks), dim3(GPB), 0, st, sims, G, spts, step, q,
                                rd ^ 1);
             BZ_CHECK_LAUNCH("bz_yee2d_run2(gather)");
         }

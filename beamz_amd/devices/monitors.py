@@ -83,6 +83,22 @@ class Monitor:
         self._materialize()
         return self._power_accumulated
 
+    # assignment replaces the host copy (load_data does this, as in the reference)
+    @fields.setter
+    def fields(self, value):
+        self._pull = None
+        self._fields = value
+
+    @power_history.setter
+    def power_history(self, value):
+        self._pull = None
+        self._power_history = value
+
+    @power_timestamps.setter
+    def power_timestamps(self, value):
+        self._pull = None
+        self._power_timestamps = value
+
     # ---- geometry (monitors.py:76-220) ----
     def _determine_3d_mode(self, start, end, design):
         if end is not None and len(end) == 3:
@@ -204,6 +220,92 @@ class Monitor:
             print(f"Warning: monitor objective returned non-numeric value: {v}")
             return None
         return self.objective_value
+
+    # ---- recorded-data access and files (monitors.py:549-589, 787-837) ----
+    def get_field_statistics(self):
+        """monitors.py:549-567: record count, time span, mean and peak power, and the monitor's geometry."""
+        f = self.fields
+        if not f["t"]:
+            return {}
+        ph = self.power_history
+        stats = {
+            "total_records": len(f["t"]),
+            "time_span": f["t"][-1] - f["t"][0] if len(f["t"]) > 1 else 0,
+            "avg_power": np.mean(ph) if ph else 0,
+            "max_power": np.max(ph) if ph else 0,
+            "monitor_type": self.monitor_type,
+            "is_3d": self.is_3d,
+        }
+        if self.is_3d:
+            stats["plane_normal"] = self.plane_normal
+            stats["plane_position"] = self.plane_position
+            stats["plane_size"] = self.size
+        else:
+            stats["line_start"] = self.start
+            stats["line_end"] = self.end
+        return stats
+
+    def get_field_at_time(self, field="Ez", time_value=None, time_index=None):
+        """monitors.py:787-814: the record at time_index, else the record nearest time_value, else the last."""
+        f = self.fields
+        if not f["t"] or field not in f:
+            return None
+        if time_index is not None:
+            return f[field][time_index] if 0 <= time_index < len(f[field]) else None
+        if time_value is not None:
+            return f[field][int(np.argmin(np.abs(np.array(f["t"]) - time_value)))]
+        return f[field][-1] if f[field] else None
+
+    def get_power_statistics(self):
+        """monitors.py:816-833."""
+        ph = self.power_history
+        if not ph:
+            return {}
+        a = np.array(ph)
+        m = np.mean(a)
+        return {"mean_power": m, "max_power": np.max(a), "min_power": np.min(a), "std_power": np.std(a),
+                "total_energy": np.sum(a), "peak_to_average_ratio": np.max(a) / m if m > 0 else 0}
+
+    def save_data(self, filename, format="npz"):
+        """Write the recorded data to an .npz file (monitors.py:569-579 names the same contents).
+
+        The reference stores ``fields`` and ``monitor_info`` as pickled dicts. Here every entry is a plain
+        array, so the file loads without unpickling: ``fields/<name>`` holds the stacked records of each
+        recorded component (records first), ``power_history`` and ``power_timestamps`` the power series,
+        and ``monitor_info`` a JSON string with the monitor type and dimensionality."""
+        import json
+        if format != "npz":
+            raise ValueError(f"Unsupported format: {format}")
+        f = self.fields
+        arrays = {f"fields/{k}": np.asarray(v) for k, v in f.items()}
+        np.savez(filename, power_history=np.asarray(self.power_history, dtype=float),
+                 power_timestamps=np.asarray(self.power_timestamps),
+                 monitor_info=np.array(json.dumps({"type": self.monitor_type, "is_3d": bool(self.is_3d)})),
+                 **arrays)
+
+    def load_data(self, filename, allow_pickle=False):
+        """Read a file written by save_data (monitors.py:581-589) back into fields / power_history /
+        power_timestamps; timestamps default to 0..len-1 when absent, as in the reference. A file in the
+        reference's own layout stores ``fields`` pickled: it is read only with allow_pickle=True, which
+        runs code from the file, so pass it only for files you trust."""
+        with np.load(filename, allow_pickle=allow_pickle) as data:
+            keys = set(data.files)
+            if "fields" in keys:
+                try:
+                    fields = data["fields"].item()
+                except ValueError as exc:
+                    raise ValueError(f"{filename} stores pickled fields (the reference's layout); "
+                                     "load it with allow_pickle=True if you trust the file") from exc
+            else:
+                fields = {k[len("fields/"):]: list(data[k]) for k in data.files if k.startswith("fields/")}
+            self.fields = fields
+            self.power_history = list(data["power_history"])
+            self.power_timestamps = (list(data["power_timestamps"]) if "power_timestamps" in keys
+                                     else list(range(len(self.power_history))))
+
+    def __str__(self):
+        stats = self.get_field_statistics()
+        return f"Monitor: {stats['monitor_type']} ({'3D' if stats['is_3d'] else '2D'}), {stats['total_records']} records"
 
     def copy(self):
         m = Monitor.__new__(Monitor)

@@ -485,6 +485,29 @@ class Grid:
         return True
 
 
+def xcd_order(tiles, per_wg=4, nxcd=8):
+    """Reorder a row-major tile list (rows of (sim, x-tile, y-tile)) for the XCDs of the GPU.
+
+    Workgroup w takes list entries per_wg*w .. per_wg*w + per_wg - 1 and is dispatched to XCD w % nxcd,
+    each XCD with its own L2. The reordered list gives XCD x one contiguous run of the original list, so
+    horizontally and vertically neighbouring tiles, whose halo rows and columns share cache lines, run at
+    about the same time on the same L2. Entries added to fill the last workgroups are (-1, 0, 0), which the
+    kernels skip. The result is a permutation of the input plus padding: which tiles exist is unchanged."""
+    tiles = np.asarray(tiles, np.int32).reshape(-1, 3)
+    n = len(tiles)
+    nw = -(-n // per_wg)
+    cnt = np.array([nw // nxcd + (x < nw % nxcd) for x in range(nxcd)])
+    off = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    w = np.arange(nw)
+    src = off[w % nxcd] + w // nxcd                       # a bijection of 0..nw-1
+    idx = (src[:, None] * per_wg + np.arange(per_wg)).ravel()
+    out = np.zeros((nw * per_wg, 3), np.int32)
+    out[:, 0] = -1
+    ok = idx < n
+    out[ok] = tiles[idx[ok]]
+    return out
+
+
 class Fused2D:
     """Runner of bz_yee2d_run over one or several 2D grids advanced in lockstep.
 
@@ -497,20 +520,23 @@ class Fused2D:
     """
 
     PAIR_MIN_CELLS = 1_000_000
+    XCD_ORDER = True        # tile lists in xcd_order (False: plain row-major order)
 
     def __init__(self, grids, seg=8, graph=False, max_steps=None, monitors=True, snapshots=None, pairs=None,
-                 seg2=None):
+                 seg2=None, xcd=None):
         """monitors=False runs the steps without sampling any monitor (a recomputation);
         snapshots: per grid None or (x0, y0, sx, sy, tensor [steps][3][sy][sx]) for the snapshot box;
         pairs: advance two steps per launch (bz_yee2d_run2) where a run allows it (None: from PAIR_MIN_CELLS
         cells on; never with snapshots or
-        graphs), with tiles of 60 columns x seg2 rows (None: by size)."""
+        graphs), with tiles of 60 columns x seg2 rows (None: by size); xcd: order the tile lists by
+        xcd_order (None: XCD_ORDER)."""
         if not grids:
             raise ValueError("Fused2D needs at least one grid")
         self.snapshots = list(snapshots) if snapshots is not None else [None] * len(grids)
         g0 = grids[0]
         T = g0.torch
         self.torch, self.dev, self.grids = T, g0.dev, list(grids)
+        self.xcd = self.XCD_ORDER if xcd is None else bool(xcd)
         for g in grids:
             if g.is_3d:
                 raise ValueError("Fused2D runs 2D grids only")
@@ -626,6 +652,8 @@ class Fused2D:
             t = np.stack(np.meshgrid(np.arange(gx), np.arange(gy), indexing="xy"), -1).reshape(-1, 2)
             rows.append(np.concatenate([np.full((len(t), 1), s), t], axis=1))
         tiles = np.concatenate(rows).astype(np.int32)
+        if self.xcd:
+            tiles = xcd_order(tiles)
         self.tiles2 = T.from_numpy(tiles.ravel()).to(self.dev)
         self.ntiles2 = len(tiles)
 
@@ -694,6 +722,8 @@ class Fused2D:
             t = np.stack(np.meshgrid(np.arange(gx), np.arange(gy), indexing="xy"), -1).reshape(-1, 2)
             rows.append(np.concatenate([np.full((len(t), 1), s), t], axis=1))
         tiles = np.concatenate(rows).astype(np.int32)
+        if self.xcd:
+            tiles = xcd_order(tiles)
         self.tiles = T.from_numpy(tiles.ravel()).to(self.dev)
         self.ntiles = len(tiles)
         self.graphs = {}

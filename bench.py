@@ -45,10 +45,13 @@ MEASURED_TRAFFIC = {
     ("mmi3d", "fused", 2, 16, 480, 240, 160): (874_454 + 476_068) * 1024,
     # two-step kernel: one launch = two steps (profiles/yee3d_tb2_mmi_480x240x160.md)
     ("mmi3d", "fused-tb2-w8", 2, 54, 480, 240, 160): (2 * 352_009 + 465_912) * 1024,
-    # batched 2D engine, 64 members, 2-row tiles (profiles/yee2d_fused_batch64_kernel_and_counters.md)
+    # batched 2D engine, 64 members, 2-row tiles in row-major order
+    # (profiles/yee2d_fused_batch64_kernel_and_counters.md)
     ("batch2d", "seg2", 64): (2 * 483_330 + 367_632) * 1024,
-    # two steps per launch, 60 x 8 pair tiles (profiles/yee2d_tb2_batch64_kernel_and_counters.md)
+    # two steps per launch, 60 x 8 pair tiles in row-major order (profiles/yee2d_tb2_batch64_kernel_and_counters.md)
     ("batch2d", "pairs-seg8", 64): (2 * 523_987 + 403_190) * 1024,
+    # the same in XCD order, the default (same profile)
+    ("batch2d", "pairs-seg8-xcd", 64): (2 * 461_652 + 398_545) * 1024,
 }
 
 
@@ -214,7 +217,8 @@ def build_runner(a, total):
         from beamz_amd.simulation.batch import FDTDBatch
         members = batch_2d(nsims=a.nsims, steps=max(total + 4, 16))
         sims = [bz.FDTD(m, time=t) for m, t, _ in members]
-        batch = FDTDBatch(sims, seg=a.seg2d, graph=a.graph, chunk=a.chunk)
+        batch = FDTDBatch(sims, seg=a.seg2d, graph=a.graph, chunk=a.chunk,
+                          pairs=None if a.pairs < 0 else bool(a.pairs))
         batch.initialize()
         tuned = None
         if a.autotune:
@@ -235,7 +239,8 @@ def build_runner(a, total):
             return v, f"{n} reference-order complex128 steps of the median member ({big.nx}x{big.ny}) ({el:.1f} s)"
         r = Runner(batch.advance, sims[0].grid, cells, desc, kname, cpu)
         r.tuned = tuned
-        r.seg_key = f"pairs-seg{batch.engine.seg2}" if batch.engine.pairs else f"seg{batch.engine.seg}"
+        r.seg_key = (f"pairs-seg{batch.engine.seg2}" if batch.engine.pairs else f"seg{batch.engine.seg}") + \
+            ("-xcd" if batch.engine.xcd else "")
         r.steps_per_launch = 2 if batch.engine.pairs else 1
         return r
     if a.workload == "mmi3d":
@@ -279,8 +284,9 @@ def build_runner(a, total):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 1000; adjoint2d: 5 gradients)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 20; adjoint2d: 1)")
     ap.add_argument("--workload", choices=sorted(B_ALG), default="mmi3d")
     ap.add_argument("--nx", type=int, default=None)
     ap.add_argument("--ny", type=int, default=None)
@@ -301,6 +307,8 @@ def main():
     ap.add_argument("--kseg2", type=int, default=0, help="3D two-step kernel: z-planes per segment (0: auto)")
     ap.add_argument("--waves2", type=int, default=8, help="3D two-step kernel: waves per workgroup (8, 16)")
     ap.add_argument("--seg2d", type=int, default=None, help="rows per tile of the 2D engine (default: by size)")
+    ap.add_argument("--xcd", type=int, default=1, help="2D engines: XCD-ordered tile lists (1) or row-major (0)")
+    ap.add_argument("--pairs", type=int, default=-1, help="batch2d: two steps per launch (1), one (0), by size (-1)")
     ap.add_argument("--graph", action="store_true", help="capture 2D runs as HIP graphs")
     ap.add_argument("--autotune", action="store_true", help="batch2d: pick the tile height by timing")
     ap.add_argument("--chunk", type=int, default=100, help="batch2d: steps per engine call")
@@ -308,6 +316,10 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (3 launches) as a counter calibration kernel")
     a = ap.parse_args()
+    if a.steps is None:
+        a.steps = 5 if a.workload == "adjoint2d" else 1000
+    if a.warmup is None:
+        a.warmup = 1 if a.workload == "adjoint2d" else 20
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
@@ -317,6 +329,8 @@ def main():
         surrogate_report(a)
         return
     total = a.warmup + a.steps + a.timing_steps
+    from beamz_amd.simulation.engine import Fused2D
+    Fused2D.XCD_ORDER = bool(a.xcd)
     r = build_runner(a, total)
     if a.copy_mib > 0:
         # known traffic: each launch reads and writes copy_mib MiB (FETCH_SIZE / WRITE_SIZE calibration)

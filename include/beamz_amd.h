@@ -106,7 +106,9 @@ int bz_yee3d_step2(float* const src[6], float* const dst[6], const float* eps_r,
  * only enqueues launches, so it can be captured in a hipGraph and replayed). Field buffers are
  * ping-ponged by step parity: step n reads buffer n & 1 and writes buffer (n + 1) & 1.
  * tiles: device int[3*ntiles] = (sim, x-tile of 62 columns, y-tile of `seg` rows), seg one of
- * 2, 4, 8, 16; every array of a simulation must hold fewer than 2^31 elements.
+ * 2, 4, 8, 16; an entry with sim < 0 is padding and is skipped (tiles are taken four to a workgroup,
+ * in list order, so the order decides which XCD runs which tile); every array of a simulation must
+ * hold fewer than 2^31 elements.
  * pts: device int[3*npts] = (sim, comp 0 Ez / 1 Hx / 2 Hy, offset); slot: device int[steps],
  * slot[n] = recorded row of step n or -1; series [rows][npts] FP32 (may be NULL);
  * phasor [rows][nfreq] complex FP64; dft [nfreq][npts] complex FP64 accumulators.
@@ -178,7 +180,7 @@ typedef struct bz_adj2d {
 } bz_adj2d;
 
 /* Reverse steps n = n_hi-1 down to n_hi-nsteps of every adjoint described in `adjs` (device
- * array); tiles as for bz_yee2d_run (seg 2, 4 or 8). */
+ * array); tiles as for bz_yee2d_run (seg 2, 4 or 8; sim < 0 marks padding). */
 int bz_adj2d_run(const bz_adj2d* adjs, int nadj, const int* tiles, int ntiles, int seg, int n_hi, int nsteps,
                  void* stream);
 /* lE_n += sum_f Re(inj_w[f] * phasor[n-1][f]) over each injection box (max_inj >= the largest box's cell count):

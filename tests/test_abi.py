@@ -35,3 +35,25 @@ def test_errors_are_reported_without_gpu():
     import pytest
     with pytest.raises(ValueError, match="bz_raster_2d"):
         N.call("bz_raster_2d", None, 0, None, None, 0, 1.0, 1.0, 0.1, 0, 0, None, None, None)
+
+
+def test_xcd_order_is_a_padded_permutation():
+    """The XCD tile order keeps every tile exactly once, pads with skipped (-1) entries only, and gives each
+    of the 8 XCDs (workgroup w -> XCD w % 8, 4 tiles per workgroup) one contiguous run of the input."""
+    import numpy as np
+    from beamz_amd.simulation.engine import xcd_order
+    for n in (1, 3, 4, 31, 32, 33, 1000, 66347):
+        tiles = np.stack([np.arange(n) // 7, np.arange(n) % 7, np.arange(n)], 1).astype(np.int32)
+        out = xcd_order(tiles)
+        assert len(out) % 4 == 0 and len(out) - n < 4
+        real = out[out[:, 0] >= 0]
+        assert len(real) == n
+        assert np.array_equal(np.sort(real[:, 2]), np.arange(n))
+        assert np.all(out[out[:, 0] < 0] == [-1, 0, 0])
+        per_xcd = {}
+        for w in range(len(out) // 4):
+            for e in out[4 * w: 4 * w + 4]:
+                if e[0] >= 0:
+                    per_xcd.setdefault(w % 8, []).append(int(e[2]))
+        for ids in per_xcd.values():
+            assert ids == list(range(ids[0], ids[0] + len(ids)))      # contiguous, in order
