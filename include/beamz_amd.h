@@ -33,6 +33,12 @@
  *   bz_yee2d_run .......... fused multi-step 2D engine for many independent simulations at once:
  *                           per step numpy_backend.py:34-88 (H, E), helper.py:96-118 (source),
  *                           helper.py:231-256 (monitor samples + DFT), helper.py:120-187 (power)
+ *   bz_yee3d_step2 ........ two consecutive bz_yee3d_step steps in one pass (same bits)
+ *   bz_yee2d_run2 ......... two consecutive bz_yee2d_run steps per pass (same bits, no grid power)
+ *   bz_adj2d_run / _inject  reverse-time adjoint of the 2D step for the topology gradient of
+ *                           examples/4_topo.py:255-321 (the reference differentiates with a second
+ *                           forward run, 4_topo.py:282-285; this is the exact discrete adjoint)
+ *   bz_nn_* ............... CNN surrogate kernels; no reference counterpart
  */
 #ifndef BEAMZ_AMD_H
 #define BEAMZ_AMD_H
