@@ -21,6 +21,9 @@
 //            atomics), both operands transposed while being staged in LDS.
 #include "common.h"
 
+#include <algorithm>
+#include <cstdint>
+
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -205,6 +208,92 @@ __global__ void k_conv_first_wgrad(const uint16_t* __restrict__ x, const uint16_
     if (threadIdx.x == 9) atomicAdd(gb + co, red[9][0]);
 }
 
+// First convolution, 8 output channels per thread (Co % 8 == 0): the 9 input taps are read once per pixel.
+// Same sum order as k_conv_first (taps 0..8, out-of-range taps add nothing).
+__global__ void k_conv_first8(const uint16_t* __restrict__ x, const float* __restrict__ w, const float* __restrict__ b,
+                              uint16_t* __restrict__ y, Geo g, int Co) {
+    const int P = g.B * g.H * g.W, C8 = Co / 8;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)P * C8) return;
+    const int p = (int)(idx / C8), c = (int)(idx - (long)p * C8) * 8;
+    int pb, py, px;
+    pixel(p, g, pb, py, px);
+    float s[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) s[e] = 0.0f;
+    for (int tap = 0; tap < 9; ++tap) {
+        const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
+        if (yy < 0 || yy >= g.H || xx < 0 || xx >= g.W) continue;
+        const float xv = b2f(x[((long)pb * g.H + yy) * g.W + xx]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s[e] += xv * b2f(f2b(w[(c + e) * 9 + tap]));
+    }
+    uint4 o;
+    uint16_t* oh = reinterpret_cast<uint16_t*>(&o);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) oh[e] = f2b(fmaxf(s[e] + b[c + e], 0.0f));
+    *reinterpret_cast<uint4*>(y + (long)p * Co + c) = o;
+}
+
+// First convolution, weight and bias gradients, 8 channels per thread: G = Co / 8 channel groups (a power of
+// two, at most 16), R = 256 / G pixel rows per pass. Each thread sums gz * x(tap) for its channels over every
+// R-th pixel of the workgroup's chunk; lanes of one wave with the same channel group are added by butterfly
+// shuffles, the 4 waves in LDS, and one atomic per weight and workgroup goes to gw / gb.
+__global__ __launch_bounds__(256) void k_conv_first_wgrad8(const uint16_t* __restrict__ x, const uint16_t* __restrict__ gz,
+                                                           float* __restrict__ gw, float* __restrict__ gb, Geo g, int Co,
+                                                           long chunk) {
+    __shared__ float red[4][16][80];
+    const int G = Co / 8, R = 256 / G;
+    const int t = threadIdx.x, gi = t % G, r = t / G;
+    const int c = 8 * gi;
+    const long P = (long)g.B * g.H * g.W;
+    const long p0 = (long)blockIdx.x * chunk, p1 = min(P, p0 + chunk);
+    float acc[10][8];
+#pragma unroll
+    for (int q = 0; q < 10; ++q)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[q][e] = 0.0f;
+    for (long p = p0 + r; p < p1; p += R) {
+        const uint4 v = *reinterpret_cast<const uint4*>(gz + p * Co + c);
+        const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+        float gv[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) gv[e] = b2f(h[e]);
+        int pb, py, px;
+        pixel((int)p, g, pb, py, px);
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
+            const float xv = (yy >= 0 && yy < g.H && xx >= 0 && xx < g.W) ? b2f(x[((long)pb * g.H + yy) * g.W + xx]) : 0.0f;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc[tap][e] += gv[e] * xv;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[9][e] += gv[e];
+    }
+    // lanes l and l ^ off (off a multiple of G) hold the same channel group
+    for (int off = G; off < 64; off <<= 1)
+#pragma unroll
+        for (int q = 0; q < 10; ++q)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc[q][e] += __shfl_xor(acc[q][e], off);
+    const int wave = t >> 6, lane = t & 63;
+    if (lane < G)
+#pragma unroll
+        for (int q = 0; q < 10; ++q)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) red[wave][lane][q * 8 + e] = acc[q][e];
+    __syncthreads();
+    for (int k = t; k < G * 80; k += 256) {
+        const int gg = k / 80, qe = k % 80, q = qe / 8, e = qe % 8;
+        const float sum = red[0][gg][qe] + red[1][gg][qe] + red[2][gg][qe] + red[3][gg][qe];
+        if (q < 9)
+            atomicAdd(gw + (8 * gg + e) * 9 + q, sum);
+        else
+            atomicAdd(gb + 8 * gg + e, sum);
+    }
+}
+
 __global__ void k_conv_first_dgrad(const uint16_t* __restrict__ gz, const float* __restrict__ w, float* __restrict__ gx,
                                    Geo g, int Co) {
     const int P = g.B * g.H * g.W;
@@ -222,19 +311,105 @@ __global__ void k_conv_first_dgrad(const uint16_t* __restrict__ gz, const float*
     gx[p] = s;
 }
 
-// Column sums (bias gradient): gb[c] += sum_p g[p][c].
-__global__ void k_colsum(const uint16_t* __restrict__ gz, float* __restrict__ gb, long P, int C, int chunk) {
-    const int c = blockIdx.y * blockDim.x + threadIdx.x;
-    if (c >= C) return;
-    const long p0 = (long)blockIdx.x * chunk, p1 = min(P, p0 + chunk);
-    float s = 0.0f;
-    for (long p = p0; p < p1; ++p) s += b2f(gz[p * C + c]);
-    atomicAdd(gb + c, s);
+// Column sums (bias gradient): gb[c] += sum_p g[p][c] over the workgroup's `chunk` pixel rows. Thread t
+// reads channels 8*(t % G) .. +7 (G = ceil(C/8) groups, 16-byte loads when C % 8 == 0) of every R-th row
+// (R = 256 / G), the R partial sums of each channel are added in LDS, and one atomic per channel and
+// workgroup goes to gb. C <= 2048.
+__global__ __launch_bounds__(256) void k_colsum(const uint16_t* __restrict__ gz, float* __restrict__ gb, long P, int C,
+                                                long chunk) {
+    __shared__ float red[2048];
+    const int G = (C + 7) / 8, R = 256 / G;
+    const int t = threadIdx.x, gi = t % G, r = t / G;
+    const int c0 = 8 * gi;
+    float s[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) s[e] = 0.0f;
+    if (r < R) {
+        const long p0 = (long)blockIdx.x * chunk, p1 = min(P, p0 + chunk);
+        if ((C & 7) == 0) {
+            for (long p = p0 + r; p < p1; p += R) {
+                const uint4 v = *reinterpret_cast<const uint4*>(gz + p * C + c0);
+                const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) s[e] += b2f(h[e]);
+            }
+        } else {
+            for (long p = p0 + r; p < p1; p += R)
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    if (c0 + e < C) s[e] += b2f(gz[p * C + c0 + e]);
+        }
+    }
+    // every slot red[q * 8G + c] with q < R is written (unused channels c >= C hold 0); the sums read q < R only
+    if (r < R)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) red[r * 8 * G + c0 + e] = s[e];
+    __syncthreads();
+    for (int c = t; c < C; c += 256) {
+        float a = 0.0f;
+        for (int q = 0; q < R; ++q) a += red[q * 8 * G + c];
+        atomicAdd(gb + c, a);
+    }
+}
+
+// 8 consecutive elements per thread (n % 8 == 0): zero the gradient where the ReLU output is not positive.
+__global__ void k_relu_mask8(uint16_t* __restrict__ gz, const uint16_t* __restrict__ y, long n8) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n8) return;
+    uint4 g = reinterpret_cast<const uint4*>(gz)[i];
+    const uint4 v = reinterpret_cast<const uint4*>(y)[i];
+    uint16_t* gh = reinterpret_cast<uint16_t*>(&g);
+    const uint16_t* vh = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+        if (!(b2f(vh[e]) > 0.0f)) gh[e] = 0;
+    reinterpret_cast<uint4*>(gz)[i] = g;
 }
 
 __global__ void k_relu_mask(uint16_t* __restrict__ gz, const uint16_t* __restrict__ y, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && !(b2f(y[i]) > 0.0f)) gz[i] = 0;
+}
+
+// 2x2 average pooling and its backward, 8 channels per thread (C % 8 == 0).
+__global__ void k_avgpool2_8(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, Geo g, int C) {
+    const int Ho = g.H / 2, Wo = g.W / 2, C8 = C / 8;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)g.B * Ho * Wo * C8) return;
+    const int c = (int)(idx % C8) * 8;
+    const long q = idx / C8;
+    const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), b = (int)(q / ((long)Wo * Ho));
+    const long base = (((long)b * g.H + 2 * yo) * g.W + 2 * xo) * C + c;
+    const uint4 v0 = *reinterpret_cast<const uint4*>(x + base);
+    const uint4 v1 = *reinterpret_cast<const uint4*>(x + base + C);
+    const uint4 v2 = *reinterpret_cast<const uint4*>(x + base + (long)g.W * C);
+    const uint4 v3 = *reinterpret_cast<const uint4*>(x + base + (long)g.W * C + C);
+    const uint16_t* h0 = reinterpret_cast<const uint16_t*>(&v0);
+    const uint16_t* h1 = reinterpret_cast<const uint16_t*>(&v1);
+    const uint16_t* h2 = reinterpret_cast<const uint16_t*>(&v2);
+    const uint16_t* h3 = reinterpret_cast<const uint16_t*>(&v3);
+    uint4 o;
+    uint16_t* oh = reinterpret_cast<uint16_t*>(&o);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) oh[e] = f2b(0.25f * (b2f(h0[e]) + b2f(h1[e]) + b2f(h2[e]) + b2f(h3[e])));
+    *reinterpret_cast<uint4*>(y + q * C + c) = o;
+}
+
+__global__ void k_avgpool2_bwd8(const uint16_t* __restrict__ gy, uint16_t* __restrict__ gx, Geo g, int C) {
+    const int C8 = C / 8;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)g.B * g.H * g.W * C8) return;
+    const int c = (int)(idx % C8) * 8;
+    const long q = idx / C8;
+    const int xx = (int)(q % g.W), yy = (int)((q / g.W) % g.H), b = (int)(q / ((long)g.W * g.H));
+    const int Ho = g.H / 2, Wo = g.W / 2;
+    const uint4 v = *reinterpret_cast<const uint4*>(gy + (((long)b * Ho + yy / 2) * Wo + xx / 2) * C + c);
+    const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+    uint4 o;
+    uint16_t* oh = reinterpret_cast<uint16_t*>(&o);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) oh[e] = f2b(0.25f * b2f(h[e]));
+    *reinterpret_cast<uint4*>(gx + q * C + c) = o;
 }
 
 __global__ void k_avgpool2(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, Geo g, int C) {
@@ -368,9 +543,11 @@ extern "C" int bz_nn_conv3x3_wgrad(const uint16_t* x, const uint16_t* gz, float*
     hipLaunchKernelGGL(k_conv_wgrad, dim3((Co + TM - 1) / TM, (9 * Ci + TN - 1) / TN, nz), dim3(256), 0, st, x, gz, gw,
                        g, Ci, Co, chunk);
     BZ_CHECK_LAUNCH("bz_nn_conv3x3_wgrad");
-    const int cchunk = 4096;
-    hipLaunchKernelGGL(k_colsum, dim3((unsigned)((P + cchunk - 1) / cchunk), (Co + 255) / 256), dim3(256), 0, st, gz, gb,
-                       P, Co, cchunk);
+    BZ_REQUIRE(Co <= 2048, "bz_nn_conv3x3_wgrad: at most 2048 output channels");
+    // about 2048 workgroups, each summing at least 16 rows per thread
+    const long rows = 256 / ((Co + 7) / 8);
+    const long cchunk = std::max(rows * 16, (P + 2047) / 2048);
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)((P + cchunk - 1) / cchunk)), dim3(256), 0, st, gz, gb, P, Co, cchunk);
     BZ_CHECK_LAUNCH("bz_nn_conv3x3_wgrad(bias)");
     return BZ_OK;
 }
@@ -379,7 +556,12 @@ extern "C" int bz_nn_conv_first(const uint16_t* x, const float* w, const float* 
                                 int Co, void* stream) {
     BZ_REQUIRE(x && w && b && y && B > 0 && H > 0 && W > 0 && Co > 0, "bz_nn_conv_first: bad arguments");
     Geo g{B, H, W};
-    hipLaunchKernelGGL(k_conv_first, grid1((long)B * H * W * Co), dim3(256), 0, (hipStream_t)stream, x, w, b, y, g, Co);
+    if (Co % 8 == 0 && ((uintptr_t)y & 15) == 0)
+        hipLaunchKernelGGL(k_conv_first8, grid1((long)B * H * W * (Co / 8)), dim3(256), 0, (hipStream_t)stream, x, w, b,
+                           y, g, Co);
+    else
+        hipLaunchKernelGGL(k_conv_first, grid1((long)B * H * W * Co), dim3(256), 0, (hipStream_t)stream, x, w, b, y, g,
+                           Co);
     BZ_CHECK_LAUNCH("bz_nn_conv_first");
     return BZ_OK;
 }
@@ -389,10 +571,17 @@ extern "C" int bz_nn_conv_first_bwd(const uint16_t* x, const uint16_t* gz, const
     BZ_REQUIRE(x && gz && w && gw && gb && B > 0 && H > 0 && W > 0 && Co > 0, "bz_nn_conv_first_bwd: bad arguments");
     Geo g{B, H, W};
     const long P = (long)B * H * W;
-    const int chunk = 8192;
     const hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_conv_first_wgrad, dim3((unsigned)((P + chunk - 1) / chunk), Co), dim3(256), 0, st, x, gz, gw,
-                       gb, g, Co, chunk);
+    const int G = Co / 8;
+    if (Co % 8 == 0 && G <= 16 && (G & (G - 1)) == 0 && ((uintptr_t)gz & 15) == 0) {
+        const long chunk = std::max((long)(256 / G) * 8, (P + 1023) / 1024);     // about 1024 workgroups
+        hipLaunchKernelGGL(k_conv_first_wgrad8, dim3((unsigned)((P + chunk - 1) / chunk)), dim3(256), 0, st, x, gz, gw,
+                           gb, g, Co, chunk);
+    } else {
+        const int chunk = 8192;
+        hipLaunchKernelGGL(k_conv_first_wgrad, dim3((unsigned)((P + chunk - 1) / chunk), Co), dim3(256), 0, st, x, gz,
+                           gw, gb, g, Co, chunk);
+    }
     BZ_CHECK_LAUNCH("bz_nn_conv_first_bwd(w)");
     if (gx) {
         hipLaunchKernelGGL(k_conv_first_dgrad, grid1(P), dim3(256), 0, st, gz, w, gx, g, Co);
@@ -404,7 +593,10 @@ extern "C" int bz_nn_conv_first_bwd(const uint16_t* x, const uint16_t* gz, const
 extern "C" int bz_nn_relu_mask(uint16_t* gz, const uint16_t* y, long n, void* stream) {
     BZ_REQUIRE(gz && y && n >= 0, "bz_nn_relu_mask: bad arguments");
     if (n == 0) return BZ_OK;
-    hipLaunchKernelGGL(k_relu_mask, grid1(n), dim3(256), 0, (hipStream_t)stream, gz, y, n);
+    if (n % 8 == 0 && ((uintptr_t)gz & 15) == 0 && ((uintptr_t)y & 15) == 0)
+        hipLaunchKernelGGL(k_relu_mask8, grid1(n / 8), dim3(256), 0, (hipStream_t)stream, gz, y, n / 8);
+    else
+        hipLaunchKernelGGL(k_relu_mask, grid1(n), dim3(256), 0, (hipStream_t)stream, gz, y, n);
     BZ_CHECK_LAUNCH("bz_nn_relu_mask");
     return BZ_OK;
 }
@@ -412,7 +604,12 @@ extern "C" int bz_nn_relu_mask(uint16_t* gz, const uint16_t* y, long n, void* st
 extern "C" int bz_nn_avgpool2(const uint16_t* x, uint16_t* y, int B, int H, int W, int C, void* stream) {
     BZ_REQUIRE(x && y && B > 0 && H % 2 == 0 && W % 2 == 0 && H > 0 && W > 0 && C > 0, "bz_nn_avgpool2: bad arguments");
     Geo g{B, H, W};
-    hipLaunchKernelGGL(k_avgpool2, grid1((long)B * (H / 2) * (W / 2) * C), dim3(256), 0, (hipStream_t)stream, x, y, g, C);
+    if (C % 8 == 0 && ((uintptr_t)x & 15) == 0 && ((uintptr_t)y & 15) == 0)
+        hipLaunchKernelGGL(k_avgpool2_8, grid1((long)B * (H / 2) * (W / 2) * (C / 8)), dim3(256), 0, (hipStream_t)stream,
+                           x, y, g, C);
+    else
+        hipLaunchKernelGGL(k_avgpool2, grid1((long)B * (H / 2) * (W / 2) * C), dim3(256), 0, (hipStream_t)stream, x, y,
+                           g, C);
     BZ_CHECK_LAUNCH("bz_nn_avgpool2");
     return BZ_OK;
 }
@@ -421,7 +618,11 @@ extern "C" int bz_nn_avgpool2_bwd(const uint16_t* gy, uint16_t* gx, int B, int H
     BZ_REQUIRE(gy && gx && B > 0 && H % 2 == 0 && W % 2 == 0 && H > 0 && W > 0 && C > 0,
                "bz_nn_avgpool2_bwd: bad arguments");
     Geo g{B, H, W};
-    hipLaunchKernelGGL(k_avgpool2_bwd, grid1((long)B * H * W * C), dim3(256), 0, (hipStream_t)stream, gy, gx, g, C);
+    if (C % 8 == 0 && ((uintptr_t)gy & 15) == 0 && ((uintptr_t)gx & 15) == 0)
+        hipLaunchKernelGGL(k_avgpool2_bwd8, grid1((long)B * H * W * (C / 8)), dim3(256), 0, (hipStream_t)stream, gy, gx,
+                           g, C);
+    else
+        hipLaunchKernelGGL(k_avgpool2_bwd, grid1((long)B * H * W * C), dim3(256), 0, (hipStream_t)stream, gy, gx, g, C);
     BZ_CHECK_LAUNCH("bz_nn_avgpool2_bwd");
     return BZ_OK;
 }

@@ -62,6 +62,50 @@ def test_conv_layers_match_oracle(hip):
     assert rel_l2(gb.cpu().numpy(), db) < 1e-5
 
 
+def test_first_layer_pooling_and_mask_match_oracle(hip):
+    """The one-input-channel first convolution (forward, weight and bias gradients), 2x2 average pooling and
+    its backward, and the ReLU mask, on shapes whose channel counts take the 8-wide kernels (32, 64) and on
+    one that takes the scalar ones (12 channels for pooling, an odd element count for the mask)."""
+    torch = hip
+    rng = np.random.default_rng(11)
+    st = torch.cuda.current_stream().cuda_stream
+    B, H, W, Co = 3, 10, 14, 32
+    x = S.bf16(rng.random((B, H, W, 1)))
+    w = S.bf16(rng.standard_normal((Co, 3, 3, 1)) * 0.3)
+    b = rng.standard_normal(Co).astype(np.float32).astype(np.float64) * 0.1
+    xd = to16(torch, x)
+    wd = torch.tensor(w.reshape(Co, 9), dtype=torch.float32).cuda()
+    bd = torch.tensor(b, dtype=torch.float32).cuda()
+    y = torch.empty((B, H, W, Co), dtype=torch.int16).cuda()
+    N.call("bz_nn_conv_first", xd.data_ptr(), wd.data_ptr(), bd.data_ptr(), y.data_ptr(), B, H, W, Co, st)
+    ref = np.maximum(S.conv3x3(x, w, b), 0.0)
+    assert np.all(np.abs(from16(y) - S.bf16(ref)) <= 2.0 ** -7 * np.abs(ref) + 1e-6)
+    gz = S.bf16(rng.standard_normal((B, H, W, Co)))
+    gw = torch.zeros((Co, 9), dtype=torch.float32).cuda()
+    gb = torch.zeros(Co, dtype=torch.float32).cuda()
+    N.call("bz_nn_conv_first_bwd", xd.data_ptr(), to16(torch, gz).data_ptr(), wd.data_ptr(), gw.data_ptr(),
+           gb.data_ptr(), None, B, H, W, Co, st)
+    _, dw, db = S.conv3x3_backward(x, w, gz)
+    assert rel_l2(gw.cpu().numpy(), dw.reshape(Co, 9)) < 1e-5
+    assert rel_l2(gb.cpu().numpy(), db) < 1e-5
+    for C in (64, 12):
+        a = S.bf16(rng.standard_normal((2, 6, 8, C)))
+        p = torch.empty((2, 3, 4, C), dtype=torch.int16).cuda()
+        N.call("bz_nn_avgpool2", to16(torch, a).data_ptr(), p.data_ptr(), 2, 6, 8, C, st)
+        pr = S.avgpool2(a)
+        assert np.all(np.abs(from16(p) - S.bf16(pr)) <= 2.0 ** -7 * np.abs(pr) + 1e-7), C
+        gy = S.bf16(rng.standard_normal((2, 3, 4, C)))
+        gx = torch.empty((2, 6, 8, C), dtype=torch.int16).cuda()
+        N.call("bz_nn_avgpool2_bwd", to16(torch, gy).data_ptr(), gx.data_ptr(), 2, 6, 8, C, st)
+        assert np.array_equal(from16(gx), S.avgpool2_backward(gy)), C      # a quarter is exact in bfloat16
+    for n in (4096, 4099):
+        g = S.bf16(rng.standard_normal(n))
+        v = S.bf16(rng.standard_normal(n))
+        gd = to16(torch, g)
+        N.call("bz_nn_relu_mask", gd.data_ptr(), to16(torch, v).data_ptr(), n, st)
+        assert np.array_equal(from16(gd), np.where(v > 0, g, 0.0)), n
+
+
 def test_network_forward_backward_match_oracle(hip):
     torch = hip
     net = Surrogate(seed=3)
@@ -114,3 +158,18 @@ def test_surrogate_trained_on_fdtd_data_beats_random_screening(hip):
     assert hist[-1] < hist[0]
     assert err < err_const
     assert fom_best.mean() > fom_rand.mean()
+
+
+def test_transmission_labels_follow_their_densities(hip):
+    """The batched label generator must give each density its own spectrum: permuting the densities (across
+    two batch chunks, the second one partly filled) permutes the spectra bit for bit, and a rerun after
+    reset reproduces them."""
+    from beamz_amd.surrogate.data import TransmissionProblem, random_densities
+    prob = TransmissionProblem(batch=4, steps=400)
+    rho = random_densities(6, seed=7)
+    T1 = prob.transmission(rho)
+    perm = [5, 2, 0, 4, 1, 3]
+    T2 = prob.transmission(rho[perm])
+    assert np.all(np.isfinite(T1)) and T1.std() > 0
+    assert np.array_equal(T2, T1[perm])
+    assert np.array_equal(prob.transmission(rho), T1)
