@@ -9,8 +9,10 @@
 // weights, gradients and biases.
 //
 // GEMM tiling: a workgroup of 4 waves computes a 64 x 64 tile of C = A * B^T
-// with A [M][K] and B [N][K] both staged K-contiguous in LDS (32-wide K
-// slices), each wave one 32 x 32 block with two MFMAs per slice. The
+// with A [M][K] and B [N][K] both staged K-contiguous in LDS, each wave one
+// 32 x 32 block: 64-wide K slices (4 MFMAs per wave and slice) for forward and
+// backward-data, 128-pixel slices (8 MFMAs) for the weight gradient; the next
+// slice's global loads are in flight during the current slice's MFMAs. The
 // operand / result lane layout of the 32x32x16 bf16 MFMA is: lane l supplies
 // A[l%32][8*(l/32)+t] and B[8*(l/32)+t][l%32], t = 0..7, and holds
 // C[8*(r/4) + 4*(l/32) + r%4][l%32] in accumulator r = 0..15.
@@ -18,7 +20,8 @@
 //   dgrad:   M = pixels, N = Ci, K = 9*Co, A = im2col(gz) with mirrored taps,
 //            B = weights transposed to [Ci][3][3][Co];
 //   wgrad:   M = Co, N = 9*Ci, K = pixels (split over workgroups, FP32
-//            atomics), both operands transposed while being staged in LDS.
+//            atomics), both operands transposed in registers (8 x 8 blocks)
+//            before they are stored in LDS.
 #include "common.h"
 
 #include <algorithm>
@@ -29,7 +32,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;   // LDS rows padded by 16 bytes
+constexpr int TM = 64, TN = 64;   // output tile of a workgroup (4 waves, one 32 x 32 block each)
 
 __device__ __forceinline__ float b2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ __forceinline__ uint16_t f2b(float f) {
@@ -49,26 +52,18 @@ __device__ __forceinline__ void pixel(int p, const Geo& g, int& b, int& y, int& 
     b = q / g.H;
 }
 
-// Waves 0..3: (wm, wn) = (wave & 1, wave >> 1).
-__device__ __forceinline__ void mma_slice(const uint16_t (*As)[LDK], const uint16_t (*Bs)[LDK], f32x16& acc) {
-    const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-#pragma unroll
-    for (int kk = 0; kk < TK; kk += 16) {
-        const uint4 av = *reinterpret_cast<const uint4*>(&As[wm * 32 + (l & 31)][kk + 8 * (l >> 5)]);
-        const uint4 bv = *reinterpret_cast<const uint4*>(&Bs[wn * 32 + (l & 31)][kk + 8 * (l >> 5)]);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv),
-                                                       acc, 0, 0, 0);
-    }
-}
-
 // Forward (sign = +1) or backward-data (sign = -1) 3x3 convolution as an implicit GEMM.
 // src: [P][Cs] bf16 (x, or the masked upstream gradient), wk: [N][9][Cs] bf16, out: [P][N] bf16.
+// K slices of FK = 32 or 64: thread t stages the 8-wide chunks kq + 32 h (h < FK / 32) of row t / 4 of both
+// tiles (the tap of a chunk is k / Cs; Cs % 8 == 0, so a chunk never straddles two taps, and chunks past K
+// are zero). The next slice's loads are issued before the current slice's MFMAs.
+template <int FK>
 __global__ __launch_bounds__(256) void k_conv_gemm(const uint16_t* __restrict__ src, const uint16_t* __restrict__ wk,
                                                    const float* __restrict__ bias, uint16_t* __restrict__ out, Geo g,
                                                    int Cs, int N, int sign, int relu) {
-    __shared__ __attribute__((aligned(16))) uint16_t As[TM][LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][LDK];
+    constexpr int FLD = FK + 8, NH = FK / 32;
+    __shared__ __attribute__((aligned(16))) uint16_t As[TM][FLD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][FLD];
     const int P = g.B * g.H * g.W;
     const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
     const int t = threadIdx.x;
@@ -78,30 +73,52 @@ __global__ __launch_bounds__(256) void k_conv_gemm(const uint16_t* __restrict__ 
     pixel(min(p, P - 1), g, pb, py, px);
     const int nrow = n0 + pr;
     const int K = 9 * Cs;
+    uint4 a[NH], bw[NH];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
+            const int k = k0 + kq + 32 * h;
+            a[h] = make_uint4(0, 0, 0, 0);
+            bw[h] = make_uint4(0, 0, 0, 0);
+            if (k < K) {
+                const int tap = k / Cs, c = k - tap * Cs;
+                const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+                const int yy = py + sign * (ky - 1), xx = px + sign * (kx - 1);
+                if (p < P && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W)
+                    a[h] = *reinterpret_cast<const uint4*>(src + (((long)pb * g.H + yy) * g.W + xx) * Cs + c);
+                if (nrow < N) bw[h] = *reinterpret_cast<const uint4*>(wk + (long)nrow * K + k);
+            }
+        }
+    };
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += TK) {
-        const int tap = k0 / Cs, c0 = k0 - tap * Cs;
-        const int ky = tap / 3, kx = tap - 3 * (tap / 3);
-        const int yy = py + sign * (ky - 1), xx = px + sign * (kx - 1);
-        uint4 a = make_uint4(0, 0, 0, 0), bw = make_uint4(0, 0, 0, 0);
-        if (p < P && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W)
-            a = *reinterpret_cast<const uint4*>(src + (((long)pb * g.H + yy) * g.W + xx) * Cs + c0 + kq);
-        if (nrow < N) bw = *reinterpret_cast<const uint4*>(wk + (long)nrow * K + k0 + kq);
-        *reinterpret_cast<uint4*>(&As[pr][kq]) = a;
-        *reinterpret_cast<uint4*>(&Bs[pr][kq]) = bw;
+    const int l = t & 63, wave = t >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += FK) {
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
+            *reinterpret_cast<uint4*>(&As[pr][kq + 32 * h]) = a[h];
+            *reinterpret_cast<uint4*>(&Bs[pr][kq + 32 * h]) = bw[h];
+        }
         __syncthreads();
-        mma_slice(As, Bs, acc);
+        if (k0 + FK < K) load(k0 + FK);
+#pragma unroll
+        for (int kk = 0; kk < FK; kk += 16) {
+            const uint4 av = *reinterpret_cast<const uint4*>(&As[wm * 32 + (l & 31)][kk + 8 * (l >> 5)]);
+            const uint4 bv = *reinterpret_cast<const uint4*>(&Bs[wn * 32 + (l & 31)][kk + 8 * (l >> 5)]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av),
+                                                           __builtin_bit_cast(bf16x8, bv), acc, 0, 0, 0);
+        }
         __syncthreads();
     }
-    const int l = t & 63, wave = t >> 6;
-    const int n = n0 + (wave >> 1) * 32 + (l & 31);
+    const int n = n0 + wn * 32 + (l & 31);
     if (n >= N) return;
     const float bn = bias ? bias[n] : 0.0f;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        const int m = m0 + (wave & 1) * 32 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+        const int m = m0 + wm * 32 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
         if (m < P) {
             float v = acc[r] + bn;
             if (relu) v = fmaxf(v, 0.0f);
@@ -110,43 +127,77 @@ __global__ __launch_bounds__(256) void k_conv_gemm(const uint16_t* __restrict__ 
     }
 }
 
-// Weight gradient gw[co][tap][ci] += sum_p gz[p][co] * x[p + tap][ci] over this workgroup's pixel range.
+// Weight gradient gw[co][tap][ci] += sum_p gz[p][co] * x[p + tap][ci] over this workgroup's pixel range
+// (a multiple of WK pixels, the last one clipped at P). GEMM with M = Co, N = 9*Ci, K = pixels.
+// Staging of one WK-pixel slice: threads 0..127 take the gz operand, 128..255 the im2col(x) operand; each
+// loads 8 pixels x 8 channels (eight 16-byte rows), transposes that block in registers and stores 8
+// channel rows of 8 pixels (16 bytes each) into the K-contiguous LDS tiles. The next slice's loads are
+// issued before the current slice's MFMAs.
+constexpr int WK = 128, WLD = WK + 8;
+
+__device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
+
 __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ gz,
                                                     float* __restrict__ gw, Geo g, int Ci, int Co, int chunk) {
-    __shared__ __attribute__((aligned(16))) uint16_t As[TM][LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t As[TM][WLD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][WLD];
     const int P = g.B * g.H * g.W;
     const int N = 9 * Ci;
     const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
     const int pstart = blockIdx.z * chunk, pend = min(P, pstart + chunk);
     const int t = threadIdx.x;
-    const int pr = t >> 3, q8 = (t & 7) * 8;     // pixel row of the slice, 8-wide channel group
+    const bool opb = t >= 128;
+    const int u = t & 127;
+    const int pg = u >> 3, q8 = (u & 7) * 8;      // 8-pixel group of the slice, 8-wide channel group
     const int nb = n0 + q8;
     const int tap = nb / Ci, ci = nb - tap * Ci;
     const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+    const bool colok = opb ? nb < N : m0 + q8 < Co;
+    uint4 v[8];
+    auto load = [&](int p0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int p = p0 + pg * 8 + j;
+            uint4 a = make_uint4(0, 0, 0, 0);
+            if (p < pend && colok) {
+                if (!opb) {
+                    a = *reinterpret_cast<const uint4*>(gz + (long)p * Co + m0 + q8);
+                } else {
+                    int pb, py, px;
+                    pixel(p, g, pb, py, px);
+                    const int yy = py + ky - 1, xx = px + kx - 1;
+                    if (yy >= 0 && yy < g.H && xx >= 0 && xx < g.W)
+                        a = *reinterpret_cast<const uint4*>(x + (((long)pb * g.H + yy) * g.W + xx) * Ci + ci);
+                }
+            }
+            v[j] = a;
+        }
+    };
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    for (int p0 = pstart; p0 < pend; p0 += TK) {
-        const int p = p0 + pr;
-        uint4 a = make_uint4(0, 0, 0, 0), bx = make_uint4(0, 0, 0, 0);
-        if (p < pend) {
-            if (m0 + q8 < Co) a = *reinterpret_cast<const uint4*>(gz + (long)p * Co + m0 + q8);
-            int pb, py, px;
-            pixel(p, g, pb, py, px);
-            const int yy = py + ky - 1, xx = px + kx - 1;
-            if (nb < N && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W)
-                bx = *reinterpret_cast<const uint4*>(x + (((long)pb * g.H + yy) * g.W + xx) * Ci + ci);
-        }
-        const uint16_t* ah = reinterpret_cast<const uint16_t*>(&a);
-        const uint16_t* bh = reinterpret_cast<const uint16_t*>(&bx);
+    uint16_t (*dst)[WLD] = opb ? Bs : As;
+    load(pstart);
+    for (int p0 = pstart; p0 < pend; p0 += WK) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            As[q8 + e][pr] = ah[e];
-            Bs[q8 + e][pr] = bh[e];
+            uint16_t h[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) h[j] = reinterpret_cast<const uint16_t*>(&v[j])[e];
+            *reinterpret_cast<uint4*>(&dst[q8 + e][pg * 8]) =
+                make_uint4(pack2(h[0], h[1]), pack2(h[2], h[3]), pack2(h[4], h[5]), pack2(h[6], h[7]));
         }
         __syncthreads();
-        mma_slice(As, Bs, acc);
+        if (p0 + WK < pend) load(p0 + WK);
+        const int l = t & 63, wave = t >> 6;
+        const int wm = wave & 1, wn = wave >> 1;
+#pragma unroll
+        for (int kk = 0; kk < WK; kk += 16) {
+            const uint4 av = *reinterpret_cast<const uint4*>(&As[wm * 32 + (l & 31)][kk + 8 * (l >> 5)]);
+            const uint4 bv = *reinterpret_cast<const uint4*>(&Bs[wn * 32 + (l & 31)][kk + 8 * (l >> 5)]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av),
+                                                           __builtin_bit_cast(bf16x8, bv), acc, 0, 0, 0);
+        }
         __syncthreads();
     }
     const int l = t & 63, wave = t >> 6;
@@ -208,31 +259,45 @@ __global__ void k_conv_first_wgrad(const uint16_t* __restrict__ x, const uint16_
     if (threadIdx.x == 9) atomicAdd(gb + co, red[9][0]);
 }
 
-// First convolution, 8 output channels per thread (Co % 8 == 0): the 9 input taps are read once per pixel.
-// Same sum order as k_conv_first (taps 0..8, out-of-range taps add nothing).
-__global__ void k_conv_first8(const uint16_t* __restrict__ x, const float* __restrict__ w, const float* __restrict__ b,
-                              uint16_t* __restrict__ y, Geo g, int Co) {
-    const int P = g.B * g.H * g.W, C8 = Co / 8;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)P * C8) return;
-    const int p = (int)(idx / C8), c = (int)(idx - (long)p * C8) * 8;
+// First convolution, one pixel per thread, all Co channels (Co % 8 == 0, Co <= 256): the 9 input taps are
+// read once, the bf16-rounded weights come from LDS (same address across the wave: broadcast), and the
+// outputs go out as 16-byte rows. Same sum order as k_conv_first (taps 0..8, out-of-range taps add nothing).
+__global__ __launch_bounds__(256) void k_conv_first8(const uint16_t* __restrict__ x, const float* __restrict__ w,
+                                                     const float* __restrict__ b, uint16_t* __restrict__ y, Geo g,
+                                                     int Co) {
+    __shared__ float ws[9 * 256];
+    __shared__ float bs[256];
+    for (int i = threadIdx.x; i < 9 * Co; i += blockDim.x) ws[i] = b2f(f2b(w[i]));
+    for (int i = threadIdx.x; i < Co; i += blockDim.x) bs[i] = b[i];
+    __syncthreads();
+    const int P = g.B * g.H * g.W;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
     int pb, py, px;
     pixel(p, g, pb, py, px);
-    float s[8];
+    float xv[9];
+    bool in[9];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) s[e] = 0.0f;
     for (int tap = 0; tap < 9; ++tap) {
         const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
-        if (yy < 0 || yy >= g.H || xx < 0 || xx >= g.W) continue;
-        const float xv = b2f(x[((long)pb * g.H + yy) * g.W + xx]);
-#pragma unroll
-        for (int e = 0; e < 8; ++e) s[e] += xv * b2f(f2b(w[(c + e) * 9 + tap]));
+        in[tap] = yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
+        xv[tap] = in[tap] ? b2f(x[((long)pb * g.H + yy) * g.W + xx]) : 0.0f;
     }
-    uint4 o;
-    uint16_t* oh = reinterpret_cast<uint16_t*>(&o);
+    for (int c = 0; c < Co; c += 8) {
+        float s[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) oh[e] = f2b(fmaxf(s[e] + b[c + e], 0.0f));
-    *reinterpret_cast<uint4*>(y + (long)p * Co + c) = o;
+        for (int e = 0; e < 8; ++e) s[e] = 0.0f;
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap)
+            if (in[tap])
+#pragma unroll
+                for (int e = 0; e < 8; ++e) s[e] += xv[tap] * ws[(c + e) * 9 + tap];
+        uint4 o;
+        uint16_t* oh = reinterpret_cast<uint16_t*>(&o);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) oh[e] = f2b(fmaxf(s[e] + bs[c + e], 0.0f));
+        *reinterpret_cast<uint4*>(y + (long)p * Co + c) = o;
+    }
 }
 
 // First convolution, weight and bias gradients, 8 channels per thread: G = Co / 8 channel groups (a power of
@@ -509,12 +574,13 @@ inline dim3 grid1(long n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)
 extern "C" int bz_nn_conv3x3(const uint16_t* x, const uint16_t* w16, const float* bias, uint16_t* y, int B, int H,
                              int W, int Ci, int Co, int relu, void* stream) {
     BZ_REQUIRE(x && w16 && y && B > 0 && H > 0 && W > 0, "bz_nn_conv3x3: bad arguments");
-    BZ_REQUIRE(Ci % TK == 0 && Co % 32 == 0, "bz_nn_conv3x3: Ci must be a multiple of 32 and Co of 32");
+    BZ_REQUIRE(Ci % 32 == 0 && Co % 32 == 0, "bz_nn_conv3x3: Ci must be a multiple of 32 and Co of 32");
     const long P = (long)B * H * W;
     BZ_REQUIRE(P * (long)(Ci > Co ? Ci : Co) < (1L << 31), "bz_nn_conv3x3: tensor too large");
     Geo g{B, H, W};
-    hipLaunchKernelGGL(k_conv_gemm, dim3((unsigned)((P + TM - 1) / TM), (Co + TN - 1) / TN), dim3(256), 0,
-                       (hipStream_t)stream, x, w16, bias, y, g, Ci, Co, 1, relu);
+    hipLaunchKernelGGL((9 * Ci) % 64 == 0 ? k_conv_gemm<64> : k_conv_gemm<32>,
+                       dim3((unsigned)((P + TM - 1) / TM), (Co + TN - 1) / TN), dim3(256), 0, (hipStream_t)stream, x,
+                       w16, bias, y, g, Ci, Co, 1, relu);
     BZ_CHECK_LAUNCH("bz_nn_conv3x3");
     return BZ_OK;
 }
@@ -522,11 +588,12 @@ extern "C" int bz_nn_conv3x3(const uint16_t* x, const uint16_t* w16, const float
 extern "C" int bz_nn_conv3x3_dgrad(const uint16_t* gz, const uint16_t* wt16, uint16_t* gx, int B, int H, int W, int Ci,
                                    int Co, void* stream) {
     BZ_REQUIRE(gz && wt16 && gx && B > 0 && H > 0 && W > 0, "bz_nn_conv3x3_dgrad: bad arguments");
-    BZ_REQUIRE(Co % TK == 0 && Ci % 32 == 0, "bz_nn_conv3x3_dgrad: Co must be a multiple of 32 and Ci of 32");
+    BZ_REQUIRE(Co % 32 == 0 && Ci % 32 == 0, "bz_nn_conv3x3_dgrad: Co must be a multiple of 32 and Ci of 32");
     const long P = (long)B * H * W;
     Geo g{B, H, W};
-    hipLaunchKernelGGL(k_conv_gemm, dim3((unsigned)((P + TM - 1) / TM), (Ci + TN - 1) / TN), dim3(256), 0,
-                       (hipStream_t)stream, gz, wt16, (const float*)nullptr, gx, g, Co, Ci, -1, 0);
+    hipLaunchKernelGGL((9 * Co) % 64 == 0 ? k_conv_gemm<64> : k_conv_gemm<32>,
+                       dim3((unsigned)((P + TM - 1) / TM), (Ci + TN - 1) / TN), dim3(256), 0, (hipStream_t)stream, gz,
+                       wt16, (const float*)nullptr, gx, g, Co, Ci, -1, 0);
     BZ_CHECK_LAUNCH("bz_nn_conv3x3_dgrad");
     return BZ_OK;
 }
@@ -536,7 +603,7 @@ extern "C" int bz_nn_conv3x3_wgrad(const uint16_t* x, const uint16_t* gz, float*
     BZ_REQUIRE(x && gz && gw && gb && B > 0 && H > 0 && W > 0 && splits > 0, "bz_nn_conv3x3_wgrad: bad arguments");
     BZ_REQUIRE(Ci % 8 == 0 && Co % 8 == 0, "bz_nn_conv3x3_wgrad: channel counts must be multiples of 8");
     const long P = (long)B * H * W;
-    const int chunk = (int)(((P + splits - 1) / splits + TK - 1) / TK * TK);
+    const int chunk = (int)(((P + splits - 1) / splits + WK - 1) / WK * WK);
     const int nz = (int)((P + chunk - 1) / chunk);
     Geo g{B, H, W};
     const hipStream_t st = (hipStream_t)stream;
@@ -556,9 +623,8 @@ extern "C" int bz_nn_conv_first(const uint16_t* x, const float* w, const float* 
                                 int Co, void* stream) {
     BZ_REQUIRE(x && w && b && y && B > 0 && H > 0 && W > 0 && Co > 0, "bz_nn_conv_first: bad arguments");
     Geo g{B, H, W};
-    if (Co % 8 == 0 && ((uintptr_t)y & 15) == 0)
-        hipLaunchKernelGGL(k_conv_first8, grid1((long)B * H * W * (Co / 8)), dim3(256), 0, (hipStream_t)stream, x, w, b,
-                           y, g, Co);
+    if (Co % 8 == 0 && Co <= 256 && ((uintptr_t)y & 15) == 0)
+        hipLaunchKernelGGL(k_conv_first8, grid1((long)B * H * W), dim3(256), 0, (hipStream_t)stream, x, w, b, y, g, Co);
     else
         hipLaunchKernelGGL(k_conv_first, grid1((long)B * H * W * Co), dim3(256), 0, (hipStream_t)stream, x, w, b, y, g,
                            Co);

@@ -47,6 +47,8 @@ class TransmissionProblem:
         self.batch = FDTDBatch(self.sims)
         self.torch = s0.backend.torch
         self.batch.initialize()
+        if len(self.sims) >= 64:
+            self.batch.autotune(steps=20)     # one problem object serves every evaluation: tune the tiles once
         T = self.torch
         self.plans = [next(p for p in s._plans if p.monitor is m) for s, m in zip(self.sims, self.mons)]
         self.npts = self.plans[0].layout[1]

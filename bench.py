@@ -12,6 +12,10 @@ batch2d`` the batch of 64 independent 2D simulations (sides drawn from
 256..1024 cells by seed, binary 2 x 2 um design regions, 10-cell PML, input and
 output DFT ports at 8 wavelengths), all advanced together, two steps per launch.
 
+``--workload surrogate_train`` times the surrogate's BF16-MFMA training step
+alone (synthetic binary 128 x 128 densities and targets, batch 128) and reports
+FLOP/s against the dense BF16 MFMA peak.
+
 Prints one JSON line: value = cell-updates/s (for the batch: summed over its
 members); roofline = the update kernel's algorithmic HBM bytes (3D: 56 B per
 cell-update, six FP32 components read and written plus FP32 permittivity and
@@ -34,7 +38,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
-B_ALG = {"mmi3d": 56, "wg2d": 32, "batch2d": 32, "adjoint2d": 32, "surrogate": 32}
+B_ALG = {"mmi3d": 56, "wg2d": 32, "batch2d": 32, "adjoint2d": 32, "surrogate": 32, "surrogate_train": 0}
+MFMA_BF16_PEAK = 2.5e15    # MI355X dense BF16 MFMA, FLOP/s
 
 # HBM bytes per launch of the update kernel, from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of
 # this script with --copy-mib 512, each counter scaled by the factor that makes the 512 MiB calibration
@@ -130,6 +135,17 @@ def surrogate_report(a):
         res[mode] = {"train_s": tr, "loss_first": hist[0], "loss_last": hist[-1], "heldout_rel_l2": err,
                      "train_tflops": 3 * fwd * a.epochs * (a.samples // 128) * 128 / tr / 1e12}
     const_err = float(np.linalg.norm(T.mean(axis=0) - T_v) / np.linalg.norm(T_v))
+    # a second baseline without the network: ridge regression on the density averaged over 8 x 8 pixel blocks
+    def pooled(r):
+        n = len(r)
+        f = r.reshape(n, px // 8, 8, px // 8, 8).mean(axis=(2, 4)).reshape(n, -1)
+        return np.hstack([f, np.ones((n, 1))]).astype(np.float64)
+    Xp = pooled(rho)
+    Wr = np.linalg.solve(Xp.T @ Xp + 10.0 * np.eye(Xp.shape[1]), Xp.T @ T)
+    ridge_err = float(np.linalg.norm(pooled(rho_v) @ Wr - T_v) / np.linalg.norm(T_v))
+    fom = T.mean(axis=1)
+    label_stats = {"fom_mean": float(fom.mean()), "fom_cv": float(fom.std() / fom.mean()),
+                   "wavelength_corr_first_last": float(np.corrcoef(T[:, 0], T[:, -1])[0, 1])}
     cands = random_densities(1024, seed=3)
     scr = {}
     for mode, net in nets.items():
@@ -155,13 +171,61 @@ def surrogate_report(a):
         "config": {"workload": f"surrogate: {a.samples} FDTD samples ({prob.side}x{prob.side} cells, "
                                f"{prob.px}x{prob.px} design px, {prob.steps} steps, 8 wavelengths), "
                                f"4-level BF16-MFMA CNN, {a.epochs} epochs"},
+        # generation (the FDTD batch) dominates this line's wall time: algorithmic bytes at the wall-clock rate
+        "roofline": {"bound": "hbm", "achieved": gen_rate * B_ALG["batch2d"] / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": gen_rate * B_ALG["batch2d"] / HBM_PEAK, "traffic": None,
+                     "timing": "wall clock of the whole generation, set-up of each batch included"},
         "surrogate": {"setup_s": setup_s, "generation_s": gen_s, "train": res, "constant_predictor_rel_l2": const_err,
+                      "train_roofline": {"bound": "mfma", "achieved": res[best_mode]["train_tflops"], "peak": 2500.0,
+                                         "unit": "TFLOP/s", "frac": res[best_mode]["train_tflops"] / 2500.0,
+                                         "flops": "forward + backward-data + backward-weight convolutions"},
+                      "ridge_pooled8_rel_l2": ridge_err, "labels": label_stats,
                       "screen_1024": scr, "flop_per_sample_fwd": fwd,
                       "loop": {"plain_adjoint_best_fom": plain_best, "plain_s": plain_s, "plain_iterations": a.iters,
+                               "plain_first_iteration_s": plain.history[0][0],
+                               "plain_s_per_later_iteration": (plain.history[-1][0] - plain.history[0][0]) /
+                               max(1, len(plain.history) - 1),
+                               "plain_time_to_99pct_s": next(t for t, b in plain.history if b >= 0.99 * plain_best),
                                "surrogate_best_fom": surr_best, "surrogate_s": surr_s,
                                "surrogate_time_to_99pct_of_plain_s": reach, "surrogate_model": best_mode,
                                "one_time_cost_s": setup_s + gen_s + res[best_mode]["train_s"]}},
     }
+    print(json.dumps(out))
+
+
+def surrogate_train_report(a):
+    """Training throughput of the surrogate's kernels: Adam steps over synthetic binary densities (batch 128),
+    timed with HIP events on the network's stream after one warm-up epoch of 256 samples."""
+    import torch
+    from beamz_amd.surrogate import LEVELS, Surrogate
+    from beamz_amd.surrogate.optimize import train
+    n, epochs, px = a.samples, a.epochs, 128
+    rng = np.random.default_rng(0)
+    rho = (rng.random((n, px, px)) > 0.5).astype(np.float32)
+    T = rng.random((n, 8)).astype(np.float32)
+    net = Surrogate(seed=0)
+    train(net, rho[:256], T[:256], epochs=1, batch=128)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    train(net, rho, T, epochs=epochs, batch=128)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu = e0.elapsed_time(e1) / 1e3
+    fwd = sum(2 * (px >> lvl) ** 2 * 9 * (ci * co + co * co) for lvl, (ci, co) in enumerate(LEVELS))
+    flops = 3 * fwd * (n // 128) * 128 * epochs
+    out = {"metric": "surrogate training FLOP/s", "value": flops / wall, "unit": "FLOP/s", "n_gpus": 1,
+           "steps": (n // 128) * epochs, "warmup": 2, "ms_per_step": wall / ((n // 128) * epochs) * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
+           "data": "synthetic",
+           "config": {"workload": f"surrogate training step: 4-level CNN {[c for _, c in LEVELS]}, 128x128 input, "
+                                  f"batch 128, {n} samples x {epochs} epochs", "flop_per_sample_fwd": fwd},
+           "roofline": {"bound": "mfma", "achieved": flops / gpu / 1e12, "peak": MFMA_BF16_PEAK / 1e12,
+                        "unit": "TFLOP/s", "frac": flops / gpu / MFMA_BF16_PEAK, "traffic": None,
+                        "timing": "HIP events around all training steps (every kernel of the step counted)"}}
     print(json.dumps(out))
 
 
@@ -327,6 +391,9 @@ def main():
 
     if a.workload == "surrogate":
         surrogate_report(a)
+        return
+    if a.workload == "surrogate_train":
+        surrogate_train_report(a)
         return
     total = a.warmup + a.steps + a.timing_steps
     from beamz_amd.simulation.engine import Fused2D
