@@ -66,22 +66,33 @@ class TransmissionProblem:
     def eps_of(self, rho):
         return EPS_OX + rho * (EPS_SI - EPS_OX)
 
+    def submit(self, rho):
+        """Start the evaluation of at most len(self.sims) density maps (m, px, px) in [0, 1]: reset, write the
+        permittivities and enqueue every step. Returns at once; collect(m) waits and reads the result."""
+        T = self.torch
+        chunk = T.as_tensor(rho, dtype=T.float32).to(self.sims[0].backend.dev)
+        if not 0 < chunk.shape[0] <= len(self.sims):
+            raise ValueError(f"a chunk holds 1..{len(self.sims)} density maps")
+        self.batch.reset()
+        for k, s in enumerate(self.sims):
+            e = self.eps_of(chunk[k] if k < chunk.shape[0] else chunk[0])
+            s.grid.eps[self.y0:self.y0 + self.px, self.x0:self.x0 + self.px] = e
+        self.batch.advance(self.batch.num_steps)
+
+    def collect(self, m):
+        """|a_f|^2 (m, nf) of the first m maps of the last submitted chunk."""
+        T = self.torch
+        z = T.stack([p.dft[:, :self.npts, :] for p in self.plans[:m]])   # (m, nf, npts, 2)
+        a = (self.c_dev * T.complex(z[..., 0], z[..., 1])).sum(dim=2)
+        return (a.real ** 2 + a.imag ** 2).cpu().numpy()
+
     def raw_overlaps(self, rho):
         """|a_f|^2 for each density map of rho (n, px, px) in [0, 1] (torch or numpy), in batches."""
-        T = self.torch
-        rho = T.as_tensor(rho, dtype=T.float32)
-        out = []
         B = len(self.sims)
-        for q0 in range(0, rho.shape[0], B):
-            chunk = rho[q0:q0 + B].to(self.sims[0].backend.dev)
-            self.batch.reset()
-            for k, s in enumerate(self.sims):
-                e = self.eps_of(chunk[k] if k < chunk.shape[0] else chunk[0])
-                s.grid.eps[self.y0:self.y0 + self.px, self.x0:self.x0 + self.px] = e
-            self.batch.advance(self.batch.num_steps)
-            z = T.stack([p.dft[:, :self.npts, :] for p in self.plans[:chunk.shape[0]]])   # (n, nf, npts, 2)
-            a = (self.c_dev * T.complex(z[..., 0], z[..., 1])).sum(dim=2)
-            out.append((a.real ** 2 + a.imag ** 2).cpu().numpy())
+        out = []
+        for q0 in range(0, len(rho), B):
+            self.submit(rho[q0:q0 + B])
+            out.append(self.collect(min(B, len(rho) - q0)))
         return np.concatenate(out)
 
     def transmission(self, rho):
@@ -91,10 +102,29 @@ class TransmissionProblem:
 def random_densities(n, seed, px=PX):
     """The batched workload's density generator: seeded Bernoulli(0.5), 3-pixel Gaussian blur, threshold."""
     rng = np.random.default_rng(seed)
-    return np.stack([binary_density(px, rng).astype(np.float32) for _ in range(n)])
+    return np.stack([binary_density(px, rng).astype(np.float32) for _ in range(n)]) if n else \
+        np.zeros((0, px, px), np.float32)
 
 
 def generate(problem, n, seed=0):
-    """n (density, transmission spectrum) pairs from the batched FDTD."""
-    rho = random_densities(n, seed, problem.px)
-    return rho, problem.transmission(rho).astype(np.float32)
+    """n (density, transmission spectrum) pairs from the batched FDTD: the densities of random_densities(n,
+    seed), each chunk's drawn on the host while the GPU runs the previous chunk."""
+    rng = np.random.default_rng(seed)
+    B = len(problem.sims)
+
+    def draw(m):
+        return np.stack([binary_density(problem.px, rng).astype(np.float32) for _ in range(m)])
+
+    rho, T = [], []
+    cur = draw(min(B, n)) if n else None
+    done = 0
+    while cur is not None:
+        problem.submit(cur)
+        done += len(cur)
+        nxt = draw(min(B, n - done)) if done < n else None
+        T.append(problem.collect(len(cur)) / problem.ref)
+        rho.append(cur)
+        cur = nxt
+    if not rho:
+        return np.zeros((0, problem.px, problem.px), np.float32), np.zeros((0, len(problem.ref)), np.float32)
+    return np.concatenate(rho), np.concatenate(T).astype(np.float32)
