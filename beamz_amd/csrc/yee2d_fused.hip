@@ -202,6 +202,19 @@ __global__ void k_tick(int* step, int nsteps) {
 // row r and step n+2 on row r-1 (its E needs only the H of step n+2 on rows r-1 and r-2, carried), so
 // one pass reads the five arrays and writes the three fields once per two steps. Lanes 0, 1, 62, 63 are
 // halo lanes; a segment starting at j0 > 0 begins two rows early. Reads buffer rd, writes rd ^ 1.
+// Snapshot of the step with call-relative index slot at cell (j, i), if inside the box: the Ez the step read
+// and the Hx, Hy it wrote (the layout of bz_yee2d_run's snapshots).
+__device__ __forceinline__ void snap_put(const bz_sim2d& S, int slot, int i, int j, float ez, float hx, float hy) {
+    const int sa = i - S.snap_x0, sb = j - S.snap_y0;
+    if (sa >= 0 && sa < S.snap_sx && sb >= 0 && sb < S.snap_sy) {
+        const long plane = (long)S.snap_sy * S.snap_sx;
+        float* p = S.snap + (long)slot * 3 * plane + (long)sb * S.snap_sx + sa;
+        p[0] = ez;
+        p[plane] = hx;
+        p[2 * plane] = hy;
+    }
+}
+
 // Same rounded operations as k_yee2d_fused (no contraction in this file), so the results are identical.
 constexpr int OX2 = LANES - 4;
 
@@ -300,9 +313,11 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee2d_tb2(const bz_sim2d* __rest
             put(ohx, om, h2x);
             put(ohy, om, h2y);
             if (S.power) add_power(S, (long)m * ld + i, e2, h2x, h2y);
+            if (S.snap) snap_put(S, 2 * q + 1, i, m, e1p, h2x, h2y);
         }
         if (out && r >= j0 && r < j1) {
             if (S.power) add_power(S, (long)r * ld + i, e1, h1x, h1y);
+            if (S.snap) snap_put(S, 2 * q, i, r, ez0, h1x, h1y);
             if (S.stg) {
                 const int a = i - S.stg_x0, b = r - S.stg_y0;
                 if (a >= 0 && a < S.stg_sx && b >= 0 && b < S.stg_sy) {

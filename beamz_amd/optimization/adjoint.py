@@ -60,7 +60,7 @@ class AdjointProblem2D:
     cells, inside the interior.
     """
 
-    def __init__(self, sims, monitors, weights, box, checkpoint_every=None, seg=None, fscale=None):
+    def __init__(self, sims, monitors, weights, box, checkpoint_every=None, seg=None, fscale=None, pairs=None):
         self.sims = list(sims)
         self.monitors = list(monitors)
         if len(self.monitors) != len(self.sims):
@@ -81,6 +81,7 @@ class AdjointProblem2D:
         self.N = self.sims[0].num_steps
         self.K = int(checkpoint_every or max(1, math.ceil(math.sqrt(self.N))))
         self.seg = seg
+        self.pairs = pairs            # two steps per launch in the forward and recompute sweeps (None: by size)
         self.torch = self.sims[0].backend.torch
         self.dev = self.sims[0].backend.dev
         self._ready = False
@@ -88,18 +89,34 @@ class AdjointProblem2D:
     # ---------------- forward ----------------
     def _setup(self):
         T = self.torch
+        grids = [s.grid for s in self.sims]
+        if self._ready:
+            # later evaluations reuse the engines: zero the fields and the monitor sums in place
+            self._reset()
+            return
         for s in self.sims:
             s.initialize_simulation(save=False, live=False)
-        grids = [s.grid for s in self.sims]
-        self.fwd = Fused2D(grids, seg=self.seg, max_steps=self.N)
+        self.fwd = Fused2D(grids, seg=self.seg, max_steps=self.N, pairs=self.pairs)
+        if self.fwd.pairs and self.N >= 24:
+            self.fwd.tune_seg2(steps=10)
+            self._reset()
         x0, y0, sx, sy = self.box
         self.snap = T.zeros((len(grids), self.K, 3, sy + 1, sx + 1), dtype=T.float32, device=self.dev)
         snaps = [(x0 - 1, y0 - 1, sx + 1, sy + 1, self.snap[b]) for b in range(len(grids))]
         seg = self.fwd.seg if self.fwd.seg <= 8 else 8
-        self.rec = Fused2D(grids, seg=seg, monitors=False, snapshots=snaps)
+        self.rec = Fused2D(grids, seg=seg, monitors=False, snapshots=snaps, pairs=self.fwd.pairs,
+                           seg2=self.fwd.seg2 if self.fwd.pairs else None)
         for g in grids:
             g._f2d = False
         self._ready = True
+
+    def _reset(self):
+        for s in self.sims:
+            s.grid.reset_fields()
+            s.t, s.current_step = 0, 0
+        for buf in (self.fwd.series, self.fwd.dft):
+            if buf is not None:
+                buf.zero_()
 
     def set_eps(self, eps_box):
         """eps_box: (B, sy, sx) relative permittivity of the design boxes (torch, on the device)."""
@@ -111,7 +128,7 @@ class AdjointProblem2D:
     def forward(self, eps_box):
         """Run every simulation with the given design permittivity; returns J (B,) FP64 torch tensor."""
         T = self.torch
-        self._setup()          # resets fields and monitor accumulators
+        self._setup()          # first call: builds the engines; later calls: resets fields and monitor sums
         self.set_eps(eps_box)
         grids = [s.grid for s in self.sims]
         self.ckpt = []

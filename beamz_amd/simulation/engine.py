@@ -527,8 +527,8 @@ class Fused2D:
         """monitors=False runs the steps without sampling any monitor (a recomputation);
         snapshots: per grid None or (x0, y0, sx, sy, tensor [steps][3][sy][sx]) for the snapshot box;
         pairs: advance two steps per launch (bz_yee2d_run2) where a run allows it (None: from PAIR_MIN_CELLS
-        cells on; never with snapshots or
-        graphs), with tiles of 60 columns x seg2 rows (None: by size); xcd: order the tile lists by
+        cells on; never with graphs), with tiles of 60 columns x seg2 rows (None: by size); xcd: order the tile
+        lists by
         xcd_order (None: XCD_ORDER)."""
         if not grids:
             raise ValueError("Fused2D needs at least one grid")
@@ -595,7 +595,7 @@ class Fused2D:
             # small batches are launch-latency bound, and a pair's serial row march is slower there
             # (800 x 400: 13.4 us/step in pairs against 10.2 with one step per launch)
             pairs = sum(g.nx * g.ny for g in grids) >= self.PAIR_MIN_CELLS
-        self.pairs = bool(pairs) and not self.use_graph and all(s is None for s in self.snapshots)
+        self.pairs = bool(pairs) and not self.use_graph
         self.spts = None
         if self.pairs:
             if self.npts:
@@ -638,6 +638,28 @@ class Fused2D:
             if sum(-(-g.nx // 60) * -(-g.ny // seg) for g in self.grids) >= min_waves:
                 return seg
         return min(self.SEGS2)
+
+    def tune_seg2(self, n0=0, steps=10):
+        """Time every pair-tile height of SEGS2 on `steps` steps from step n0 (HIP events on the launch stream)
+        and keep the fastest; returns (best, {seg2: ms per step}). The grids' fields and the monitor sums are
+        advanced by the timing runs: callers reset them afterwards."""
+        T = self.torch
+        if not self.pairs:
+            raise ValueError("tune_seg2 needs an engine that runs pairs")
+        steps += steps % 2
+        timings = {}
+        for seg in self.SEGS2:
+            self.set_seg2(seg)
+            self.run(n0, 2)                                   # warm-up
+            e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
+            e0.record()
+            self.run(n0 + 2, steps)
+            e1.record()
+            T.cuda.synchronize(self.dev)
+            timings[seg] = e0.elapsed_time(e1) / steps
+        best = min(timings, key=timings.get)
+        self.set_seg2(best)
+        return best, timings
 
     def set_seg2(self, seg2):
         """Tile list of the two-step kernel: tiles of 60 columns x seg2 rows."""
@@ -738,17 +760,23 @@ class Fused2D:
         self.nslot = need
         self.graphs = {}
 
-    def _sync_descs(self, n0):
+    def _sync_descs(self, n0, slot0=0):
+        """Upload the descriptors if the field buffers' parity or the first snapshot slot changed: step n0
+        must read buffer n0 & 1, and the next launch's step q writes snapshot slot slot0 + q."""
         shifts = [(g._cur - (n0 & 1)) % 2 for g in self.grids]
-        if shifts == self.shift:
+        key = (tuple(shifts), slot0)
+        if key == self.shift:
             return
-        for d, g, sh in zip(self.descs, self.grids, shifts):
+        for d, g, sh, sn in zip(self.descs, self.grids, shifts, self.snapshots):
             for p in (0, 1):
                 b = g._fb[(p + sh) % 2]
                 d.ez[p], d.hx[p], d.hy[p] = b["Ez"].data_ptr(), b["Hx"].data_ptr(), b["Hy"].data_ptr()
+            if sn is not None:
+                buf = sn[4]
+                d.snap = buf.data_ptr() + slot0 * buf[0].numel() * buf.element_size()
         raw = b"".join(bytes(d) for d in self.descs)
         self.desc_dev.copy_(self.torch.frombuffer(bytearray(raw), dtype=self.torch.uint8))
-        self.shift = shifts
+        self.shift = key
 
     def _launch(self, count):
         ptr = lambda t: t.data_ptr() if t is not None else None
@@ -787,7 +815,7 @@ class Fused2D:
                 g._cur = (g._cur + npairs) % 2
             rem = count - 2 * npairs
             if rem:
-                self._sync_descs(n0 + 2 * npairs)      # step counter is already there
+                self._sync_descs(n0 + 2 * npairs, 2 * npairs)      # step counter is already there
         if rem and self.use_graph and rem > 1:
             gr = self.graphs.get(rem)
             if gr is None:

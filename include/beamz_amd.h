@@ -150,7 +150,8 @@ int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, 
  * buffer (rd0 + q) & 1 and writing the other, so after the call the state sits in buffer
  * (rd0 + npairs) & 1 and the step counter has advanced by 2 * npairs. tiles: (sim, x-tile of 60
  * columns, y-tile of `seg` rows), seg >= 2. Monitor samples of the first step of a pair come from each
- * simulation's staging box: spts[p] = offset of point p inside its box (-1: not in it, reads 0). */
+ * simulation's staging box: spts[p] = offset of point p inside its box (-1: not in it, reads 0).
+ * Snapshot boxes as for bz_yee2d_run: pair q writes slots 2q and 2q+1. */
 int bz_yee2d_run2(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step, int rd0,
                   int npairs, const int* pts, const int* spts, int npts, const int* slot, float* series,
                   int nfreq, const double* phasor, double* dft, void* stream);
