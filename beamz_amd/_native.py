@@ -70,9 +70,11 @@ SIGNATURES = {
     "bz_adj2d_run": [vp, i32, vp, i32, i32, i32, i32, vp],
     "bz_nn_conv3x3": [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp],
     "bz_nn_conv3x3_dgrad": [vp, vp, vp, i32, i32, i32, i32, i32, vp],
-    "bz_nn_conv3x3_wgrad": [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp],
+    "bz_nn_wgrad_work": [i32, i32, i32, i32, i32, i32],
+    "bz_nn_conv3x3_wgrad": [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp, i64, vp],
+    "bz_nn_first_work": [i32, i32, i32, i32],
     "bz_nn_conv_first": [vp, vp, vp, vp, i32, i32, i32, i32, vp],
-    "bz_nn_conv_first_bwd": [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp],
+    "bz_nn_conv_first_bwd": [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, i64, vp],
     "bz_nn_relu_mask": [vp, vp, i64, vp],
     "bz_nn_avgpool2": [vp, vp, i32, i32, i32, i32, vp],
     "bz_nn_avgpool2_bwd": [vp, vp, i32, i32, i32, i32, vp],
@@ -92,7 +94,7 @@ SIGNATURES = {
     "bz_raster_3d": [C.POINTER(Shape), i32, C.POINTER(f64), C.POINTER(Pml), i32, f64, f64, f64, f64, f64,
                      i32, i32, i32, vp, vp, vp],
 }
-_RESTYPE = {"bz_last_error": C.c_char_p}
+_RESTYPE = {"bz_last_error": C.c_char_p, "bz_nn_wgrad_work": i64, "bz_nn_first_work": i64}
 
 _lib = None
 

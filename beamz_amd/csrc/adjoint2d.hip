@@ -96,8 +96,8 @@ __global__ __launch_bounds__(LANES* WPG) void k_adj2d(const bz_adj2d* __restrict
             lx[r] = r > 0 ? at(glx, ro[r]) : 0.0f;
         }
     }
-    // lE'' and w on rows j0-1 .. j0+SEG
-    float l2[SEG + 2], w[SEG + 2];
+    // lE'' and w on rows j0-1 .. j0+SEG (the E coefficients of rows 1..SEG are kept for the update below)
+    float l2[SEG + 2], w[SEG + 2], ca_[SEG + 2], cb_[SEG + 2];
 #pragma unroll
     for (int r = 0; r < SEG + 2; ++r) {
         const int j = j0 - 1 + r;
@@ -107,9 +107,8 @@ __global__ __launch_bounds__(LANES* WPG) void k_adj2d(const bz_adj2d* __restrict
             if (a >= 0 && a < A.src_sx && b >= 0 && b < A.src_sy) m = A.src_mul[(long)b * A.src_sx + a];
         }
         l2[r] = m * le[r];
-        float ca, cb;
-        e_coef(sg[r], ee[r], c, ca, cb);
-        w[r] = (xin && j >= 1 && j <= ny - 2) ? cb * l2[r] : 0.0f;
+        e_coef(sg[r], ee[r], c, ca_[r], cb_[r]);
+        w[r] = (xin && j >= 1 && j <= ny - 2) ? cb_[r] * l2[r] : 0.0f;
     }
     // fb muY on rows j0-1 .. j0+SEG-1, fb muX on rows j0 .. j0+SEG-1 (zero where the H entry does not exist)
     float fmy[SEG + 1], fmx[SEG + 1], fmxl[SEG + 1], my[SEG + 1], mx[SEG + 1], fa_[SEG + 1];
@@ -137,8 +136,7 @@ __global__ __launch_bounds__(LANES* WPG) void k_adj2d(const bz_adj2d* __restrict
     for (int r = 1; r < SEG + 1; ++r) {
         const int j = j0 - 1 + r;
         if (out && j < ny) {
-            float ca, cb;
-            e_coef(sg[r], ee[r], c, ca, cb);
+            const float ca = ca_[r], cb = cb_[r];
             const bool in = xin && j >= 1 && j <= ny - 2;
             float lnew = (in ? ca * l2[r] : l2[r]) + (fmx[r] - fmxl[r]) * rdy + (fmy[r - 1] - fmy[r]) * rdx;
             if (A.inj_w && n >= 1) {

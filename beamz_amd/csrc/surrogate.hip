@@ -138,7 +138,8 @@ constexpr int WK = 128, WLD = WK + 8;
 __device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
 
 __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ gz,
-                                                    float* __restrict__ gw, Geo g, int Ci, int Co, int chunk) {
+                                                    float* __restrict__ gw, float* __restrict__ part, Geo g, int Ci,
+                                                    int Co, int chunk) {
     __shared__ __attribute__((aligned(16))) uint16_t As[TM][WLD];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][WLD];
     const int P = g.B * g.H * g.W;
@@ -206,8 +207,38 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t* __restrict__
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int m = m0 + (wave & 1) * 32 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
-        if (m < Co) atomicAdd(gw + (long)m * N + n, acc[r]);
+        if (m < Co) {
+            if (part)
+                part[((long)blockIdx.z * Co + m) * N + n] = acc[r];   // summed in split order by k_sum_parts
+            else
+                atomicAdd(gw + (long)m * N + n, acc[r]);
+        }
     }
+}
+
+// Deterministic second pass of the split reductions: Z partial rows of `count` floats are added in a fixed
+// tree order. k_sum_group adds the rows 16g .. 16g+15 of `in` into row g of `outp`; k_sum_parts adds the
+// last (at most 16) rows into out, in row order.
+constexpr int SUM_FAN = 16;
+
+__global__ void k_sum_group(const float* __restrict__ in, int Z, long sin, long count, float* __restrict__ outp,
+                            long sout) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long g = idx / count, i = idx - g * count;
+    const long z0 = g * SUM_FAN;
+    if (z0 >= Z) return;
+    const long z1 = min((long)Z, z0 + SUM_FAN);
+    float s = in[z0 * sin + i];
+    for (long z = z0 + 1; z < z1; ++z) s += in[z * sin + i];
+    outp[g * sout + i] = s;
+}
+
+__global__ void k_sum_parts(const float* __restrict__ part, int Z, long stride, long count, float* __restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float s = out[i];
+    for (int z = 0; z < Z; ++z) s += part[(long)z * stride + i];
+    out[i] = s;
 }
 
 // First convolution (one input channel): y[p][co] = relu(b[co] + sum_tap x[p+tap] w[co][tap]).
@@ -305,8 +336,8 @@ __global__ __launch_bounds__(256) void k_conv_first8(const uint16_t* __restrict_
 // R-th pixel of the workgroup's chunk; lanes of one wave with the same channel group are added by butterfly
 // shuffles, the 4 waves in LDS, and one atomic per weight and workgroup goes to gw / gb.
 __global__ __launch_bounds__(256) void k_conv_first_wgrad8(const uint16_t* __restrict__ x, const uint16_t* __restrict__ gz,
-                                                           float* __restrict__ gw, float* __restrict__ gb, Geo g, int Co,
-                                                           long chunk) {
+                                                           float* __restrict__ gw, float* __restrict__ gb,
+                                                           float* __restrict__ part, Geo g, int Co, long chunk) {
     __shared__ float red[4][16][80];
     const int G = Co / 8, R = 256 / G;
     const int t = threadIdx.x, gi = t % G, r = t / G;
@@ -352,8 +383,12 @@ __global__ __launch_bounds__(256) void k_conv_first_wgrad8(const uint16_t* __res
     for (int k = t; k < G * 80; k += 256) {
         const int gg = k / 80, qe = k % 80, q = qe / 8, e = qe % 8;
         const float sum = red[0][gg][qe] + red[1][gg][qe] + red[2][gg][qe] + red[3][gg][qe];
-        if (q < 9)
-            atomicAdd(gw + (8 * gg + e) * 9 + q, sum);
+        // partials laid out as [workgroup][gw (Co x 9) | gb (Co)]
+        const long o = q < 9 ? (long)(8 * gg + e) * 9 + q : 9L * Co + 8 * gg + e;
+        if (part)
+            part[(long)blockIdx.x * 10 * Co + o] = sum;
+        else if (q < 9)
+            atomicAdd(gw + o, sum);
         else
             atomicAdd(gb + 8 * gg + e, sum);
     }
@@ -380,8 +415,8 @@ __global__ void k_conv_first_dgrad(const uint16_t* __restrict__ gz, const float*
 // reads channels 8*(t % G) .. +7 (G = ceil(C/8) groups, 16-byte loads when C % 8 == 0) of every R-th row
 // (R = 256 / G), the R partial sums of each channel are added in LDS, and one atomic per channel and
 // workgroup goes to gb. C <= 2048.
-__global__ __launch_bounds__(256) void k_colsum(const uint16_t* __restrict__ gz, float* __restrict__ gb, long P, int C,
-                                                long chunk) {
+__global__ __launch_bounds__(256) void k_colsum(const uint16_t* __restrict__ gz, float* __restrict__ gb,
+                                                float* __restrict__ part, long P, int C, long chunk) {
     __shared__ float red[2048];
     const int G = (C + 7) / 8, R = 256 / G;
     const int t = threadIdx.x, gi = t % G, r = t / G;
@@ -413,7 +448,10 @@ __global__ __launch_bounds__(256) void k_colsum(const uint16_t* __restrict__ gz,
     for (int c = t; c < C; c += 256) {
         float a = 0.0f;
         for (int q = 0; q < R; ++q) a += red[q * 8 * G + c];
-        atomicAdd(gb + c, a);
+        if (part)
+            part[(long)blockIdx.x * C + c] = a;
+        else
+            atomicAdd(gb + c, a);
     }
 }
 
@@ -526,14 +564,28 @@ __global__ void k_head_bwd(const float* __restrict__ gout, const float* __restri
     const int b = blockIdx.x;
     for (int c = threadIdx.x; c < C; c += blockDim.x) {
         float s = 0.0f;
-        for (int j = 0; j < nout; ++j) {
-            s += gout[(long)b * nout + j] * fw[j * C + c];
-            atomicAdd(gfw + j * C + c, gout[(long)b * nout + j] * gap[(long)b * C + c]);
-        }
+        for (int j = 0; j < nout; ++j) s += gout[(long)b * nout + j] * fw[j * C + c];
         const uint16_t v = f2b(s / (float)HW);
         for (int p = 0; p < HW; ++p) ga[((long)b * HW + p) * C + c] = v;
     }
-    if (threadIdx.x < (unsigned)nout) atomicAdd(gfb + threadIdx.x, gout[(long)b * nout + threadIdx.x]);
+}
+
+// Head parameter gradients, summed over the samples in order: gfw[j][c] += sum_b gout[b][j] gap[b][c],
+// gfb[j] += sum_b gout[b][j]. One thread per output (nout x C weights, then nout biases).
+__global__ void k_head_wgrad(const float* __restrict__ gout, const float* __restrict__ gap, float* __restrict__ gfw,
+                             float* __restrict__ gfb, int B, int C, int nout) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (long)nout * C) {
+        const int j = (int)(i / C), c = (int)(i % C);
+        float s = gfw[i];
+        for (int b = 0; b < B; ++b) s += gout[(long)b * nout + j] * gap[(long)b * C + c];
+        gfw[i] = s;
+    } else if (i < (long)nout * C + nout) {
+        const int j = (int)(i - (long)nout * C);
+        float s = gfb[j];
+        for (int b = 0; b < B; ++b) s += gout[(long)b * nout + j];
+        gfb[j] = s;
+    }
 }
 
 // bf16 copies of FP32 weights [Co][9][Ci]: w16 in the same order, wt16 with the channel axes swapped,
@@ -598,24 +650,77 @@ extern "C" int bz_nn_conv3x3_dgrad(const uint16_t* gz, const uint16_t* wt16, uin
     return BZ_OK;
 }
 
+// Launch geometry of bz_nn_conv3x3_wgrad: pixel chunk and number of splits of the weight GEMM, and pixel
+// chunk and workgroup count of the bias column sums (about 2048 workgroups of at least 16 rows per thread).
+struct WgradPlan {
+    int chunk, nz;
+    long cchunk, nblk;
+};
+inline WgradPlan wgrad_plan(long P, int Co, int splits) {
+    WgradPlan q;
+    q.chunk = (int)(((P + splits - 1) / splits + WK - 1) / WK * WK);
+    q.nz = (int)((P + q.chunk - 1) / q.chunk);
+    const long rows = 256 / ((Co + 7) / 8);
+    q.cchunk = std::max(rows * 16, (P + 2047) / 2048);
+    q.nblk = (P + q.cchunk - 1) / q.cchunk;
+    return q;
+}
+
+// Rows of the extra region the tree needs for Z partial rows (the first level's output, the largest).
+inline long tree_rows(long Z) { return Z > SUM_FAN ? (Z + SUM_FAN - 1) / SUM_FAN : 0; }
+
+// out[0 .. count) += the Z rows of part (rows `stride` apart) in a fixed tree order; scratch holds
+// tree_rows(Z) rows of `count` floats. Levels alternate between scratch and the head of part, which the
+// previous level has finished reading (stream order).
+static void tree_sum(float* part, long Z, long stride, long count, float* scratch, float* out, hipStream_t st) {
+    float* cur = part;
+    long sc = stride;
+    bool in_part = true;
+    while (Z > SUM_FAN) {
+        const long Zn = (Z + SUM_FAN - 1) / SUM_FAN;
+        float* dst = in_part ? scratch : part;
+        hipLaunchKernelGGL(k_sum_group, grid1(Zn * count), dim3(256), 0, st, cur, (int)Z, sc, count, dst, count);
+        cur = dst;
+        sc = count;
+        in_part = !in_part;
+        Z = Zn;
+    }
+    hipLaunchKernelGGL(k_sum_parts, grid1(count), dim3(256), 0, st, cur, (int)Z, sc, count, out);
+}
+
+extern "C" long bz_nn_wgrad_work(int B, int H, int W, int Ci, int Co, int splits) {
+    if (B <= 0 || H <= 0 || W <= 0 || Ci <= 0 || Co <= 0 || splits <= 0) return -1;
+    const WgradPlan q = wgrad_plan((long)B * H * W, Co, splits);
+    const long nw = (long)Co * 9 * Ci;
+    return q.nz * nw + q.nblk * Co + std::max(tree_rows(q.nz) * nw, tree_rows(q.nblk) * Co);
+}
+
 extern "C" int bz_nn_conv3x3_wgrad(const uint16_t* x, const uint16_t* gz, float* gw, float* gb, int B, int H, int W,
-                                   int Ci, int Co, int splits, void* stream) {
+                                   int Ci, int Co, int splits, float* work, long nwork, void* stream) {
     BZ_REQUIRE(x && gz && gw && gb && B > 0 && H > 0 && W > 0 && splits > 0, "bz_nn_conv3x3_wgrad: bad arguments");
     BZ_REQUIRE(Ci % 8 == 0 && Co % 8 == 0, "bz_nn_conv3x3_wgrad: channel counts must be multiples of 8");
+    BZ_REQUIRE(Co <= 2048, "bz_nn_conv3x3_wgrad: at most 2048 output channels");
     const long P = (long)B * H * W;
-    const int chunk = (int)(((P + splits - 1) / splits + WK - 1) / WK * WK);
-    const int nz = (int)((P + chunk - 1) / chunk);
+    const WgradPlan q = wgrad_plan(P, Co, splits);
+    const long nw = (long)Co * 9 * Ci;
+    const long scratch_rows = std::max(tree_rows(q.nz) * nw, tree_rows(q.nblk) * Co);
+    BZ_REQUIRE(!work || nwork >= q.nz * nw + q.nblk * Co + scratch_rows,
+               "bz_nn_conv3x3_wgrad: workspace smaller than bz_nn_wgrad_work");
     Geo g{B, H, W};
     const hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_conv_wgrad, dim3((Co + TM - 1) / TM, (9 * Ci + TN - 1) / TN, nz), dim3(256), 0, st, x, gz, gw,
-                       g, Ci, Co, chunk);
+    float* wpart = work;
+    float* bpart = work ? work + q.nz * nw : nullptr;
+    hipLaunchKernelGGL(k_conv_wgrad, dim3((Co + TM - 1) / TM, (9 * Ci + TN - 1) / TN, q.nz), dim3(256), 0, st, x, gz,
+                       gw, wpart, g, Ci, Co, q.chunk);
     BZ_CHECK_LAUNCH("bz_nn_conv3x3_wgrad");
-    BZ_REQUIRE(Co <= 2048, "bz_nn_conv3x3_wgrad: at most 2048 output channels");
-    // about 2048 workgroups, each summing at least 16 rows per thread
-    const long rows = 256 / ((Co + 7) / 8);
-    const long cchunk = std::max(rows * 16, (P + 2047) / 2048);
-    hipLaunchKernelGGL(k_colsum, dim3((unsigned)((P + cchunk - 1) / cchunk)), dim3(256), 0, st, gz, gb, P, Co, cchunk);
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)q.nblk), dim3(256), 0, st, gz, gb, bpart, P, Co, q.cchunk);
     BZ_CHECK_LAUNCH("bz_nn_conv3x3_wgrad(bias)");
+    if (work) {
+        float* scratch = bpart + q.nblk * Co;
+        tree_sum(wpart, q.nz, nw, nw, scratch, gw, st);
+        tree_sum(bpart, q.nblk, Co, Co, scratch, gb, st);
+        BZ_CHECK_LAUNCH("bz_nn_conv3x3_wgrad(sums)");
+    }
     return BZ_OK;
 }
 
@@ -632,17 +737,51 @@ extern "C" int bz_nn_conv_first(const uint16_t* x, const float* w, const float* 
     return BZ_OK;
 }
 
+// The 8-channel first-layer weight gradient applies (Co / 8 a power of two up to 16): about 1024 workgroups.
+inline bool first8(int Co) {
+    const int G = Co / 8;
+    return Co % 8 == 0 && G <= 16 && (G & (G - 1)) == 0;
+}
+inline long first8_chunk(long P, int Co) { return std::max((long)(256 / (Co / 8)) * 8, (P + 1023) / 1024); }
+
+extern "C" long bz_nn_first_work(int B, int H, int W, int Co) {
+    if (B <= 0 || H <= 0 || W <= 0 || Co <= 0) return -1;
+    if (!first8(Co)) return 0;
+    const long P = (long)B * H * W, chunk = first8_chunk(P, Co), nblk = (P + chunk - 1) / chunk;
+    return (nblk + tree_rows(nblk)) * 10L * Co;
+}
+
 extern "C" int bz_nn_conv_first_bwd(const uint16_t* x, const uint16_t* gz, const float* w, float* gw, float* gb,
-                                    float* gx, int B, int H, int W, int Co, void* stream) {
+                                    float* gx, int B, int H, int W, int Co, float* work, long nwork, void* stream) {
     BZ_REQUIRE(x && gz && w && gw && gb && B > 0 && H > 0 && W > 0 && Co > 0, "bz_nn_conv_first_bwd: bad arguments");
     Geo g{B, H, W};
     const long P = (long)B * H * W;
     const hipStream_t st = (hipStream_t)stream;
-    const int G = Co / 8;
-    if (Co % 8 == 0 && G <= 16 && (G & (G - 1)) == 0 && ((uintptr_t)gz & 15) == 0) {
-        const long chunk = std::max((long)(256 / G) * 8, (P + 1023) / 1024);     // about 1024 workgroups
-        hipLaunchKernelGGL(k_conv_first_wgrad8, dim3((unsigned)((P + chunk - 1) / chunk)), dim3(256), 0, st, x, gz, gw,
-                           gb, g, Co, chunk);
+    if (first8(Co) && ((uintptr_t)gz & 15) == 0) {
+        const long chunk = first8_chunk(P, Co);
+        const long nblk = (P + chunk - 1) / chunk;
+        BZ_REQUIRE(!work || nwork >= (nblk + tree_rows(nblk)) * 10 * Co,
+                   "bz_nn_conv_first_bwd: workspace smaller than bz_nn_first_work");
+        hipLaunchKernelGGL(k_conv_first_wgrad8, dim3((unsigned)nblk), dim3(256), 0, st, x, gz, gw, gb, work, g, Co,
+                           chunk);
+        if (work) {
+            // reduce all 10 Co columns to at most SUM_FAN rows, then add the gw and gb columns into their outputs
+            float* cur = work;
+            long Z = nblk;
+            bool in_part = true;
+            float* scratch = work + nblk * 10L * Co;
+            while (Z > SUM_FAN) {
+                const long Zn = (Z + SUM_FAN - 1) / SUM_FAN;
+                float* dst = in_part ? scratch : work;
+                hipLaunchKernelGGL(k_sum_group, grid1(Zn * 10L * Co), dim3(256), 0, st, cur, (int)Z, 10L * Co, 10L * Co,
+                                   dst, 10L * Co);
+                cur = dst;
+                in_part = !in_part;
+                Z = Zn;
+            }
+            hipLaunchKernelGGL(k_sum_parts, grid1(9L * Co), dim3(256), 0, st, cur, (int)Z, 10L * Co, 9L * Co, gw);
+            hipLaunchKernelGGL(k_sum_parts, grid1(Co), dim3(256), 0, st, cur + 9L * Co, (int)Z, 10L * Co, (long)Co, gb);
+        }
     } else {
         const int chunk = 8192;
         hipLaunchKernelGGL(k_conv_first_wgrad, dim3((unsigned)((P + chunk - 1) / chunk), Co), dim3(256), 0, st, x, gz,
@@ -709,6 +848,9 @@ extern "C" int bz_nn_head_bwd(const float* gout, const float* gap, const float* 
                "bz_nn_head_bwd: bad arguments");
     hipLaunchKernelGGL(k_head_bwd, dim3(B), dim3(256), 0, (hipStream_t)stream, gout, gap, fw, gfw, gfb, ga, HW, C, nout);
     BZ_CHECK_LAUNCH("bz_nn_head_bwd");
+    hipLaunchKernelGGL(k_head_wgrad, grid1((long)nout * C + nout), dim3(256), 0, (hipStream_t)stream, gout, gap, gfw,
+                       gfb, B, C, nout);
+    BZ_CHECK_LAUNCH("bz_nn_head_bwd(weights)");
     return BZ_OK;
 }
 

@@ -103,7 +103,10 @@ class AdjointProblem2D:
         x0, y0, sx, sy = self.box
         self.snap = T.zeros((len(grids), self.K, 3, sy + 1, sx + 1), dtype=T.float32, device=self.dev)
         snaps = [(x0 - 1, y0 - 1, sx + 1, sy + 1, self.snap[b]) for b in range(len(grids))]
-        seg = self.fwd.seg if self.fwd.seg <= 8 else 8
+        # one-step tiles of the recompute engine, shared with the reverse kernel: the caller's height, else 2
+        # rows (reverse sweep of 16 x 600^2 cells, 3000 steps: 107 / 110 / 128 ms at 2 / 4 / 8 rows; the
+        # 8-row kernel holds 144 VGPRs and runs 3 waves per SIMD)
+        seg = min(self.seg, 8) if self.seg is not None else 2
         self.rec = Fused2D(grids, seg=seg, monitors=False, snapshots=snaps, pairs=self.fwd.pairs,
                            seg2=self.fwd.seg2 if self.fwd.pairs else None)
         for g in grids:

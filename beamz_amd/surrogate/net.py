@@ -36,6 +36,7 @@ class Surrogate:
         self.v = {k: torch.zeros_like(v) for k, v in self.params.items()}
         self.grads = {k: torch.zeros_like(v) for k, v in self.params.items()}
         self.step_count = 0
+        self.work, self.work_for = None, None      # workspace of the ordered gradient reductions
         self.w16, self.wt16 = {}, {}
         for lvl, (ci, co) in enumerate(LEVELS):
             for s, a in enumerate((ci, co)):
@@ -61,6 +62,20 @@ class Surrogate:
         for k, v in params.items():
             self.params[k].copy_(v)
         self.sync_weights()
+
+    def _workspace(self, B, H, W):
+        """Device FP32 workspace large enough for every layer's partial sums at this input size, so that the
+        weight and bias gradients are added in a fixed order (deterministic training)."""
+        if self.work_for != (B, H, W):
+            L = N.lib()
+            need = L.bz_nn_first_work(B, H, W, LEVELS[0][1])
+            for lvl, (ci, co) in enumerate(LEVELS):
+                h, w = H >> lvl, W >> lvl
+                for a in ((ci, co) if lvl > 0 else (co,)):
+                    need = max(need, L.bz_nn_wgrad_work(B, h, w, a, co, self._splits(B * h * w, co, a)))
+            self.work = self.torch.empty(max(int(need), 1), dtype=self.torch.float32, device=self.dev)
+            self.work_for = (B, H, W)
+        return self.work
 
     def _empty16(self, *shape):
         return self.torch.empty(shape, dtype=self.torch.int16, device=self.dev)
@@ -114,6 +129,8 @@ class Surrogate:
         for g in self.grads.values():
             g.zero_()
         B, H, W = cache["shape"]
+        work = self._workspace(B, H, W)
+        wp, wn = work.data_ptr(), work.numel()
         gout = (gout.to(device=self.dev, dtype=T.float32) * self.out_scale).contiguous()
         _, _, y1, h, w = cache["acts"][-1]
         C = LEVELS[-1][1]
@@ -132,7 +149,7 @@ class Surrogate:
             # second convolution of the level: input y0 (co channels)
             N.call("bz_nn_relu_mask", ga.data_ptr(), y1.data_ptr(), ga.numel(), st)
             N.call("bz_nn_conv3x3_wgrad", y0.data_ptr(), ga.data_ptr(), self.grads[f"w{lvl}1"].data_ptr(),
-                   self.grads[f"b{lvl}1"].data_ptr(), B, h, w, co, co, splits, st)
+                   self.grads[f"b{lvl}1"].data_ptr(), B, h, w, co, co, splits, wp, wn, st)
             g0 = self._empty16(B, h, w, co)
             N.call("bz_nn_conv3x3_dgrad", ga.data_ptr(), self.wt16[f"w{lvl}1"].data_ptr(), g0.data_ptr(), B, h, w,
                    co, co, st)
@@ -143,10 +160,11 @@ class Surrogate:
                     gx = T.empty((B, H, W), dtype=T.float32, device=self.dev)
                 N.call("bz_nn_conv_first_bwd", a.data_ptr(), g0.data_ptr(), self.params["w00"].data_ptr(),
                        self.grads["w00"].data_ptr(), self.grads["b00"].data_ptr(),
-                       gx.data_ptr() if gx is not None else None, B, h, w, co, st)
+                       gx.data_ptr() if gx is not None else None, B, h, w, co, wp, wn, st)
             else:
                 N.call("bz_nn_conv3x3_wgrad", a.data_ptr(), g0.data_ptr(), self.grads[f"w{lvl}0"].data_ptr(),
-                       self.grads[f"b{lvl}0"].data_ptr(), B, h, w, ci, co, self._splits(B * h * w, co, ci), st)
+                       self.grads[f"b{lvl}0"].data_ptr(), B, h, w, ci, co, self._splits(B * h * w, co, ci), wp, wn,
+                       st)
                 gi = self._empty16(B, h, w, ci)
                 N.call("bz_nn_conv3x3_dgrad", g0.data_ptr(), self.wt16[f"w{lvl}0"].data_ptr(), gi.data_ptr(), B, h,
                        w, ci, co, st)

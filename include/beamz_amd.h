@@ -203,12 +203,18 @@ int bz_nn_conv3x3(const uint16_t* x, const uint16_t* w16, const float* bias, uin
                   int Ci, int Co, int relu, void* stream);          /* Ci % 32 == 0, Co % 32 == 0 */
 int bz_nn_conv3x3_dgrad(const uint16_t* gz, const uint16_t* wt16, uint16_t* gx, int B, int H, int W, int Ci,
                         int Co, void* stream);                      /* wt16 = [Ci][3][3][Co] */
+/* Weight and bias gradients. With a workspace (work != NULL, nwork >= bz_nn_wgrad_work(...) floats) every
+ * partial sum is written out and added in a fixed order: the result is deterministic. With work == NULL
+ * the partial sums go through FP32 atomics, whose order varies from run to run. */
+long bz_nn_wgrad_work(int B, int H, int W, int Ci, int Co, int splits);
 int bz_nn_conv3x3_wgrad(const uint16_t* x, const uint16_t* gz, float* gw, float* gb, int B, int H, int W,
-                        int Ci, int Co, int splits, void* stream);
+                        int Ci, int Co, int splits, float* work, long nwork, void* stream);
 int bz_nn_conv_first(const uint16_t* x, const float* w, const float* b, uint16_t* y, int B, int H, int W, int Co,
                      void* stream);                                 /* one input channel, ReLU */
+long bz_nn_first_work(int B, int H, int W, int Co);                 /* workspace floats, as bz_nn_wgrad_work */
 int bz_nn_conv_first_bwd(const uint16_t* x, const uint16_t* gz, const float* w, float* gw, float* gb, float* gx,
-                         int B, int H, int W, int Co, void* stream); /* gx (may be NULL) is written, not added */
+                         int B, int H, int W, int Co, float* work, long nwork,
+                         void* stream);                             /* gx (may be NULL) is written, not added */
 int bz_nn_relu_mask(uint16_t* gz, const uint16_t* y, long n, void* stream);
 int bz_nn_avgpool2(const uint16_t* x, uint16_t* y, int B, int H, int W, int C, void* stream);
 int bz_nn_avgpool2_bwd(const uint16_t* gy, uint16_t* gx, int B, int H, int W, int C, void* stream);

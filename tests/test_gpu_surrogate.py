@@ -57,9 +57,24 @@ def test_conv_layers_match_oracle(hip):
     assert np.all(np.abs(from16(gx) - S.bf16(dx)) <= 2.0 ** -7 * np.abs(dx) + 1e-5)
     gw = torch.zeros((Co, 3, 3, Ci), dtype=torch.float32).cuda()
     gb = torch.zeros(Co, dtype=torch.float32).cuda()
-    N.call("bz_nn_conv3x3_wgrad", xd.data_ptr(), gzd.data_ptr(), gw.data_ptr(), gb.data_ptr(), B, H, W, Ci, Co, 3, st)
+    N.call("bz_nn_conv3x3_wgrad", xd.data_ptr(), gzd.data_ptr(), gw.data_ptr(), gb.data_ptr(), B, H, W, Ci, Co, 3,
+           None, 0, st)
     assert rel_l2(gw.cpu().numpy(), dw) < 1e-4
     assert rel_l2(gb.cpu().numpy(), db) < 1e-5
+    # ordered reductions through a workspace: the same sums, and the same bits on every call
+    need = N.lib().bz_nn_wgrad_work(B, H, W, Ci, Co, 3)
+    work = torch.empty(need, dtype=torch.float32).cuda()
+    with pytest.raises(ValueError, match="workspace"):
+        N.call("bz_nn_conv3x3_wgrad", xd.data_ptr(), gzd.data_ptr(), gw.data_ptr(), gb.data_ptr(), B, H, W, Ci, Co, 3,
+               work.data_ptr(), need - 1, st)
+    runs = []
+    for _ in range(2):
+        gw.zero_(); gb.zero_()
+        N.call("bz_nn_conv3x3_wgrad", xd.data_ptr(), gzd.data_ptr(), gw.data_ptr(), gb.data_ptr(), B, H, W, Ci, Co,
+               3, work.data_ptr(), need, st)
+        runs.append((gw.cpu().numpy(), gb.cpu().numpy()))
+    assert rel_l2(runs[0][0], dw) < 1e-4 and rel_l2(runs[0][1], db) < 1e-5
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
 
 
 def test_first_layer_pooling_and_mask_match_oracle(hip):
@@ -83,11 +98,15 @@ def test_first_layer_pooling_and_mask_match_oracle(hip):
     gz = S.bf16(rng.standard_normal((B, H, W, Co)))
     gw = torch.zeros((Co, 9), dtype=torch.float32).cuda()
     gb = torch.zeros(Co, dtype=torch.float32).cuda()
-    N.call("bz_nn_conv_first_bwd", xd.data_ptr(), to16(torch, gz).data_ptr(), wd.data_ptr(), gw.data_ptr(),
-           gb.data_ptr(), None, B, H, W, Co, st)
     _, dw, db = S.conv3x3_backward(x, w, gz)
-    assert rel_l2(gw.cpu().numpy(), dw.reshape(Co, 9)) < 1e-5
-    assert rel_l2(gb.cpu().numpy(), db) < 1e-5
+    need = N.lib().bz_nn_first_work(B, H, W, Co)
+    work = torch.empty(need, dtype=torch.float32).cuda()
+    for wk, nwk in ((None, 0), (work.data_ptr(), need)):           # atomics, then ordered partial sums
+        gw.zero_(); gb.zero_()
+        N.call("bz_nn_conv_first_bwd", xd.data_ptr(), to16(torch, gz).data_ptr(), wd.data_ptr(), gw.data_ptr(),
+               gb.data_ptr(), None, B, H, W, Co, wk, nwk, st)
+        assert rel_l2(gw.cpu().numpy(), dw.reshape(Co, 9)) < 1e-5
+        assert rel_l2(gb.cpu().numpy(), db) < 1e-5
     for C in (64, 12):
         a = S.bf16(rng.standard_normal((2, 6, 8, C)))
         p = torch.empty((2, 3, 4, C), dtype=torch.int16).cuda()
@@ -121,6 +140,23 @@ def test_network_forward_backward_match_oracle(hip):
     for k in ("fw", "fb", "w31", "w30", "w21", "w10", "w01", "w00", "b20", "b00"):
         assert rel_l2(net.grads[k].cpu().numpy(), G[k]) < 6e-2, k
     assert rel_l2(gx.cpu().numpy(), G["x"]) < 6e-2
+
+
+def test_training_is_deterministic(hip):
+    """Two networks from the same seed, trained on the same batches, hold the same bits: every gradient
+    reduction is added in a fixed order."""
+    torch = hip
+    g = torch.Generator(device="cuda").manual_seed(1)
+    x = (torch.rand((96, 32, 32), generator=g, device="cuda") > 0.5).float()
+    target = torch.rand((96, 8), generator=g, device="cuda")
+    params = []
+    for _ in range(2):
+        net = Surrogate(seed=4)
+        for q in range(0, 96, 32):
+            net.train_step(x[q:q + 32], target[q:q + 32], lr=1e-3)
+        params.append({k: v.cpu().numpy() for k, v in net.params.items()})
+    for k in params[0]:
+        assert np.array_equal(params[0][k], params[1][k]), k
 
 
 def test_training_reduces_loss(hip):
