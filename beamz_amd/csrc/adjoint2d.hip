@@ -29,10 +29,15 @@
 // the forward step.
 #include "common.h"
 
+// No multiply-add contraction: the one-step and the two-step kernels round every operation alike, so a
+// pair of reverse steps gives the bits of two single steps.
+#pragma clang fp contract(off)
+
 namespace {
 
 constexpr int LANES = 64;
 constexpr int OX = LANES - 2;
+constexpr int OX2 = LANES - 4;   // stored columns of a two-step tile (two halo lanes on each side)
 constexpr int WPG = 4;
 
 __device__ __forceinline__ float at(const float* __restrict__ base, uint32_t byte_off) {
@@ -54,12 +59,51 @@ __device__ __forceinline__ double injection(const bz_adj2d& A, int p, int n) {
     return s;
 }
 
+// Source-box multiplier of the forward step at cell (j, i) (1 outside the box or without one).
+__device__ __forceinline__ float src_mul_at(const bz_adj2d& A, int i, int j) {
+    if (A.src_mul) {
+        const int a = i - A.src_x0, b = j - A.src_y0;
+        if (a >= 0 && a < A.src_sx && b >= 0 && b < A.src_sy) return A.src_mul[(long)b * A.src_sx + a];
+    }
+    return 1.0f;
+}
+
+// Injection of the objective into lE at cell (j, i) before reverse step n's output is stored.
+__device__ __forceinline__ void inject_at(const bz_adj2d& A, float& lnew, int i, int j, int n) {
+    if (A.inj_w && n >= 1) {
+        const int a = i - A.inj_x0, b = j - A.inj_y0;
+        if (a >= 0 && a < A.inj_sx && b >= 0 && b < A.inj_sy) lnew += (float)injection(A, b * A.inj_sx + a, n);
+    }
+}
+
+// dJ/deps of reverse step n at an interior cell (j, i) of the design box: l2 (dca/deps Ez_n + dcb/deps curlH),
+// from the forward snapshot of step n (slot n - snap_step0; the snapshot box is the design box grown by one
+// cell on the low x and y sides).
+__device__ __forceinline__ void grad_at(const bz_adj2d& A, const Coef& c, int i, int j, int n, float l2, float ee,
+                                        float sg, float cb, float rdx, float rdy) {
+    const int a = i - A.box_x0, b = j - A.box_y0;
+    if (a >= 0 && a < A.box_sx && b >= 0 && b < A.box_sy) {
+        const int sx = A.box_sx + 1, sy = A.box_sy + 1;
+        const long plane = (long)sx * sy;
+        const float* p = A.snap + (long)(n - A.snap_step0) * 3 * plane + (long)(b + 1) * sx + (a + 1);
+        const float en = p[0];
+        const float* hx = p + plane;
+        const float* hy = p + 2 * plane;
+        const float curl = -((hx[0] - hx[-1]) * rdy) + (hy[0] - hy[-sx]) * rdx;
+        const float e = ee;
+        const float s = sg * c.ke / e;
+        const float dca = 2.0f * s / (e * (1.0f + s) * (1.0f + s));
+        const float dcb = -cb / (e * (1.0f + s));
+        A.grad[(long)b * A.box_sx + a] += (double)l2 * ((double)dca * en + (double)dcb * curl);
+    }
+}
+
 template <int SEG>
 __global__ __launch_bounds__(LANES* WPG) void k_adj2d(const bz_adj2d* __restrict__ adjs, const int* __restrict__ tiles,
-                                                     int ntiles, int n) {
+                                                     int ntiles, int n, int flip) {
     const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * WPG + threadIdx.y);
     if (t >= ntiles) return;
-    const int rd = (n + 1) & 1, wr = n & 1;
+    const int rd = (n + 1 + flip) & 1, wr = (n + flip) & 1;
     const int s = tiles[3 * t], tx = tiles[3 * t + 1], ty = tiles[3 * t + 2];
     if (s < 0) return;   // padding entry of a reordered tile list
     const bz_adj2d& A = adjs[s];
@@ -101,12 +145,7 @@ __global__ __launch_bounds__(LANES* WPG) void k_adj2d(const bz_adj2d* __restrict
 #pragma unroll
     for (int r = 0; r < SEG + 2; ++r) {
         const int j = j0 - 1 + r;
-        float m = 1.0f;
-        if (A.src_mul) {
-            const int a = i - A.src_x0, b = j - A.src_y0;
-            if (a >= 0 && a < A.src_sx && b >= 0 && b < A.src_sy) m = A.src_mul[(long)b * A.src_sx + a];
-        }
-        l2[r] = m * le[r];
+        l2[r] = src_mul_at(A, i, j) * le[r];
         e_coef(sg[r], ee[r], c, ca_[r], cb_[r]);
         w[r] = (xin && j >= 1 && j <= ny - 2) ? cb_[r] * l2[r] : 0.0f;
     }
@@ -139,32 +178,142 @@ __global__ __launch_bounds__(LANES* WPG) void k_adj2d(const bz_adj2d* __restrict
             const float ca = ca_[r], cb = cb_[r];
             const bool in = xin && j >= 1 && j <= ny - 2;
             float lnew = (in ? ca * l2[r] : l2[r]) + (fmx[r] - fmxl[r]) * rdy + (fmy[r - 1] - fmy[r]) * rdx;
-            if (A.inj_w && n >= 1) {
-                const int a = i - A.inj_x0, b = j - A.inj_y0;
-                if (a >= 0 && a < A.inj_sx && b >= 0 && b < A.inj_sy) lnew += (float)injection(A, b * A.inj_sx + a, n);
-            }
+            inject_at(A, lnew, i, j, n);
             put(ole, ro[r], lnew);
             put(olx, ro[r], fa_[r] * mx[r]);
             put(oly, ro[r], fa_[r] * my[r]);
-            if (A.grad && in) {
-                const int a = i - A.box_x0, b = j - A.box_y0;
-                if (a >= 0 && a < A.box_sx && b >= 0 && b < A.box_sy) {
-                    // snapshot box = design box grown by one cell on the low x and y sides
-                    const int sx = A.box_sx + 1, sy = A.box_sy + 1;
-                    const long plane = (long)sx * sy;
-                    const float* p = A.snap + (long)(n - A.snap_step0) * 3 * plane + (long)(b + 1) * sx + (a + 1);
-                    const float en = p[0];
-                    const float* hx = p + plane;
-                    const float* hy = p + 2 * plane;
-                    const float curl = -((hx[0] - hx[-1]) * rdy) + (hy[0] - hy[-sx]) * rdx;
-                    const float e = ee[r];
-                    const float s = sg[r] * c.ke / e;
-                    const float dca = 2.0f * s / (e * (1.0f + s) * (1.0f + s));
-                    const float dcb = -cb / (e * (1.0f + s));
-                    A.grad[(long)b * A.box_sx + a] += (double)l2[r] * ((double)dca * en + (double)dcb * curl);
-                }
-            }
+            if (A.grad && in) grad_at(A, c, i, j, n, l2[r], ee[r], sg[r], cb, rdx, rdy);
         }
+    }
+}
+
+// Reverse steps n and n-1 in one pass over a tile of 60 stored columns x seg rows, marching down the rows.
+// Row r of the march computes reverse step n on row r (it needs lam_{n+1} on rows r and r+1: the lE,
+// sigma and eps of row r+1 are loaded one iteration ahead) and reverse step n-1 on row r-1 (it needs
+// lam_n on rows r-2 .. r, i.e. this row's step-n output and the previous rows' ones carried in registers).
+// The march starts two rows above the tile. Step n is valid on lanes 1..62 and step n-1 on lanes 2..61,
+// the stored ones. The buffers: lam_{n+1} is read from buffer (n + 1 + flip) & 1 and lam_{n-1} written to
+// the other one (a pair moves the state across buffers, so the caller toggles flip after each pair). Same
+// operations in the same order as two k_adj2d steps, gradient contributions of step n before those of n-1.
+__global__ __launch_bounds__(LANES* WPG) void k_adj2d_tb2(const bz_adj2d* __restrict__ adjs,
+                                                         const int* __restrict__ tiles, int ntiles, int seg, int n,
+                                                         int flip) {
+    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * WPG + threadIdx.y);
+    if (t >= ntiles) return;
+    const int s = tiles[3 * t];
+    if (s < 0) return;   // padding entry of a reordered tile list
+    const bz_adj2d& A = adjs[s];
+    const int tx = tiles[3 * t + 1], ty = tiles[3 * t + 2];
+    const int nx = A.nx, ny = A.ny, ld = A.ld;
+    const int l = threadIdx.x;
+    const int i = tx * OX2 - 2 + l;
+    const uint32_t ic = (uint32_t)min(max(i, 0), nx - 1);
+    const int j0 = ty * seg, j1 = min(ny, j0 + seg);
+    const int rb = (n + 1 + flip) & 1, wb = rb ^ 1;
+    const float* __restrict__ gle = A.le[rb];
+    const float* __restrict__ glx = A.lhx[rb];
+    const float* __restrict__ gly = A.lhy[rb];
+    float* __restrict__ ole = A.le[wb];
+    float* __restrict__ olx = A.lhx[wb];
+    float* __restrict__ oly = A.lhy[wb];
+    Coef c;
+    c.kh = A.kh;
+    c.kbh = A.kbh;
+    c.ke = A.ke;
+    c.kbe = A.kbe;
+    const float rdx = A.rdx, rdy = A.rdy;
+    const bool xin = i >= 1 && i <= nx - 2;
+    const bool hxok = i <= nx - 2;
+    const bool out = l >= 2 && l <= LANES - 3 && i < nx;
+    const int rstart = j0 > 0 ? j0 - 2 : 0;
+    auto off = [&](int r) { return ((uint32_t)min(max(r, 0), ny - 1) * (uint32_t)ld + ic) * 4u; };
+    auto interior = [&](int j) { return xin && j >= 1 && j <= ny - 2; };
+
+    // lam_{n+1}-side quantities of the current row r: l2 = M lE, E coefficients, w = cb l2 on the interior
+    float sg0 = at(A.sig, off(rstart)), ee0 = at(A.eps, off(rstart));
+    float l2_0 = src_mul_at(A, i, rstart) * at(gle, off(rstart));
+    float ca0, cb0;
+    e_coef(sg0, ee0, c, ca0, cb0);
+    float w0 = interior(rstart) ? cb0 * l2_0 : 0.0f;
+    // loads one row ahead: lE, sigma, eps of row r+1 and lHx, lHy of row r
+    float nle = at(gle, off(rstart + 1)), nsg = at(A.sig, off(rstart + 1)), nee = at(A.eps, off(rstart + 1));
+    float nlx = at(glx, off(rstart)), nly = at(gly, off(rstart));
+    // carried from row r-1: fb muY of step n; step n's outputs there and its cell's coefficients for step n-1
+    float fmy_p = 0.0f;
+    float l2b_p = 0.0f, wb_p = 0.0f, lxn_p = 0.0f, lyn_p = 0.0f, fmyb_p = 0.0f;
+    float fa_p = 0.0f, fb_p = 0.0f, ca_p = 1.0f, cb_p = 0.0f, sg_p = 0.0f, ee_p = 1.0f;
+    for (int r = rstart; r <= j1; ++r) {
+        const float le1 = nle, sg1 = nsg, ee1 = nee, lx0 = nlx, ly0 = nly;
+        {
+            const uint32_t o2 = off(r + 2), o1 = off(r + 1);
+            nle = at(gle, o2);
+            nsg = at(A.sig, o2);
+            nee = at(A.eps, o2);
+            nlx = at(glx, o1);
+            nly = at(gly, o1);
+        }
+        // lam_{n+1}-side quantities of row r+1
+        const float l2_1 = src_mul_at(A, i, r + 1) * le1;
+        float ca1, cb1;
+        e_coef(sg1, ee1, c, ca1, cb1);
+        const float w1 = interior(r + 1) ? cb1 * l2_1 : 0.0f;
+
+        // ---- reverse step n on row r ----
+        float fa0, fb0;
+        h_coef(sg0, c, fa0, fb0);
+        const float wR = __shfl_down(w0, 1, LANES);
+        const float mx = hxok ? lx0 + (wR - w0) * rdy : 0.0f;
+        const float my = (r >= 0 && r <= ny - 2) ? ly0 + (w0 - w1) * rdx : 0.0f;
+        const float fmx = fb0 * mx, fmy = fb0 * my;
+        const float left = __shfl_up(fmx, 1, LANES);
+        const float fmxl = i >= 1 ? left : 0.0f;
+        const bool in0 = interior(r);
+        float lnew = (in0 ? ca0 * l2_0 : l2_0) + (fmx - fmxl) * rdy + (fmy_p - fmy) * rdx;
+        inject_at(A, lnew, i, r, n);
+        const float lxn = fa0 * mx, lyn = fa0 * my;
+        if (A.grad && out && in0 && r >= j0 && r < j1) grad_at(A, c, i, r, n, l2_0, ee0, sg0, cb0, rdx, rdy);
+        // lam_n-side quantities of row r for step n-1 (same cell coefficients)
+        const float l2b = src_mul_at(A, i, r) * lnew;
+        const float wbv = in0 ? cb0 * l2b : 0.0f;
+
+        // ---- reverse step n-1 on row m = r-1 ----
+        const int m = r - 1;
+        const float wbR = __shfl_down(wb_p, 1, LANES);
+        const float mx2 = hxok ? lxn_p + (wbR - wb_p) * rdy : 0.0f;
+        const float my2 = (m >= 0 && m <= ny - 2) ? lyn_p + (wb_p - wbv) * rdx : 0.0f;
+        const float fmx2 = fb_p * mx2, fmy2 = fb_p * my2;
+        const float left2 = __shfl_up(fmx2, 1, LANES);
+        const float fmxl2 = i >= 1 ? left2 : 0.0f;
+        const bool inm = interior(m);
+        float lnew2 = (inm ? ca_p * l2b_p : l2b_p) + (fmx2 - fmxl2) * rdy + (fmyb_p - fmy2) * rdx;
+        if (out && m >= j0 && m < j1) {
+            inject_at(A, lnew2, i, m, n - 1);
+            const uint32_t om = off(m);
+            put(ole, om, lnew2);
+            put(olx, om, fa_p * mx2);
+            put(oly, om, fa_p * my2);
+            if (A.grad && inm) grad_at(A, c, i, m, n - 1, l2b_p, ee_p, sg_p, cb_p, rdx, rdy);
+        }
+
+        // carry: row r's cell quantities and step-n outputs become the previous row's, row r+1's the current
+        ca_p = ca0;
+        cb_p = cb0;
+        fa_p = fa0;
+        fb_p = fb0;
+        sg_p = sg0;
+        ee_p = ee0;
+        l2b_p = l2b;
+        wb_p = wbv;
+        lxn_p = lxn;
+        lyn_p = lyn;
+        fmyb_p = fmy2;
+        fmy_p = fmy;
+        l2_0 = l2_1;
+        w0 = w1;
+        ca0 = ca1;
+        cb0 = cb1;
+        sg0 = sg1;
+        ee0 = ee1;
     }
 }
 
@@ -180,14 +329,24 @@ __global__ void k_adj_inject(const bz_adj2d* __restrict__ adjs, int n) {
 }
 
 template <int SEG>
-int run(const bz_adj2d* adjs, const int* tiles, int ntiles, int n_hi, int nsteps, hipStream_t st) {
+int run(const bz_adj2d* adjs, const int* tiles, int ntiles, int n_hi, int nsteps, int flip, hipStream_t st) {
     const dim3 g((ntiles + WPG - 1) / WPG), b(LANES, WPG);
     auto kern = k_adj2d<SEG>;
     for (int q = 0; q < nsteps; ++q) {
-        hipLaunchKernelGGL(kern, g, b, 0, st, adjs, tiles, ntiles, n_hi - 1 - q);
+        hipLaunchKernelGGL(kern, g, b, 0, st, adjs, tiles, ntiles, n_hi - 1 - q, flip);
         BZ_CHECK_LAUNCH("bz_adj2d_run");
     }
     return BZ_OK;
+}
+
+int run_seg(const bz_adj2d* adjs, const int* tiles, int ntiles, int seg, int n_hi, int nsteps, int flip,
+            hipStream_t st) {
+    switch (seg) {
+        case 2: return run<2>(adjs, tiles, ntiles, n_hi, nsteps, flip, st);
+        case 4: return run<4>(adjs, tiles, ntiles, n_hi, nsteps, flip, st);
+        case 8: return run<8>(adjs, tiles, ntiles, n_hi, nsteps, flip, st);
+        default: bz::set_error("bz_adj2d_run: seg must be 2, 4 or 8"); return BZ_EINVAL;
+    }
 }
 
 }  // namespace
@@ -196,13 +355,25 @@ extern "C" int bz_adj2d_run(const bz_adj2d* adjs, int nadj, const int* tiles, in
                             int nsteps, void* stream) {
     BZ_REQUIRE(adjs && nadj > 0 && tiles && ntiles > 0 && nsteps >= 0 && n_hi - nsteps >= 0,
                "bz_adj2d_run: bad arguments");
+    return run_seg(adjs, tiles, ntiles, seg, n_hi, nsteps, 0, (hipStream_t)stream);
+}
+
+extern "C" int bz_adj2d_run2(const bz_adj2d* adjs, int nadj, const int* tiles2, int ntiles2, int seg2,
+                             const int* tiles, int ntiles, int seg, int n_hi, int nsteps, int flip, void* stream) {
+    BZ_REQUIRE(adjs && nadj > 0 && tiles2 && ntiles2 > 0 && seg2 >= 2 && nsteps >= 0 && n_hi - nsteps >= 0 &&
+                   (flip == 0 || flip == 1),
+               "bz_adj2d_run2: bad arguments");
+    BZ_REQUIRE(nsteps % 2 == 0 || (tiles && ntiles > 0), "bz_adj2d_run2: an odd step count needs one-step tiles");
     const hipStream_t st = (hipStream_t)stream;
-    switch (seg) {
-        case 2: return run<2>(adjs, tiles, ntiles, n_hi, nsteps, st);
-        case 4: return run<4>(adjs, tiles, ntiles, n_hi, nsteps, st);
-        case 8: return run<8>(adjs, tiles, ntiles, n_hi, nsteps, st);
-        default: bz::set_error("bz_adj2d_run: seg must be 2, 4 or 8"); return BZ_EINVAL;
+    const dim3 g((ntiles2 + WPG - 1) / WPG), b(LANES, WPG);
+    const int npairs = nsteps / 2;
+    for (int q = 0; q < npairs; ++q) {
+        hipLaunchKernelGGL(k_adj2d_tb2, g, b, 0, st, adjs, tiles2, ntiles2, seg2, n_hi - 1 - 2 * q, flip);
+        BZ_CHECK_LAUNCH("bz_adj2d_run2");
+        flip ^= 1;
     }
+    if (nsteps % 2) return run_seg(adjs, tiles, ntiles, seg, n_hi - 2 * npairs, 1, flip, st);
+    return BZ_OK;
 }
 
 extern "C" int bz_adj2d_inject(const bz_adj2d* adjs, int nadj, int max_inj, int n, void* stream) {

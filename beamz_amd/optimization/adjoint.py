@@ -10,13 +10,15 @@ the discrete forward map itself:
   frequency of the Ez samples a line monitor records after every step,
   projected on weights c_p (normally the conjugated, normalised mode profile
   of the output guide, ``mode_weights``);
-* forward: the fused 2D engine (one launch per step for the whole batch of
-  designs), with the full field state checkpointed every K ~ sqrt(N) steps;
+* forward: the fused 2D engine over the whole batch of designs (two steps
+  per launch from 1 M cells on), with the full field state checkpointed
+  every K ~ sqrt(N) steps;
 * reverse: for each segment, newest first, the segment is recomputed from its
   checkpoint by the same engine while it snapshots Ez, Hx, Hy inside the
-  design box, then the reverse-time kernel (``bz_adj2d_run``) walks the
-  segment backwards, transposing each step and accumulating dJ/d eps_r in
-  FP64 inside the design box.
+  design box, then the reverse-time kernel (``bz_adj2d_run``, or two steps
+  per launch with ``bz_adj2d_run2``) walks the segment backwards,
+  transposing each step and accumulating dJ/d eps_r in FP64 inside the
+  design box. Pairs and single steps give the same bits.
 
 Memory: ceil(N/K) checkpoints of three field arrays plus K design-box
 snapshots per design. ``FDTDObjective`` wraps this as a torch.autograd
@@ -226,6 +228,7 @@ class AdjointProblem2D:
         st = grids[0].stream
         N.call("bz_adj2d_inject", ddev.data_ptr(), B, max_inj, self.N, st)
         nseg = len(self.ckpt)
+        flip = 0                    # pairs of reverse steps move the adjoint state across its two buffers
         for k in range(nseg - 1, -1, -1):
             n0 = k * self.K
             cnt = min(self.K, self.N - n0)
@@ -236,8 +239,13 @@ class AdjointProblem2D:
                 d.snap_step0 = n0
             raw = b"".join(bytes(d) for d in descs)
             ddev.copy_(T.frombuffer(bytearray(raw), dtype=T.uint8))
-            N.call("bz_adj2d_run", ddev.data_ptr(), B, self.rec.tiles.data_ptr(), self.rec.ntiles, self.rec.seg,
-                   n0 + cnt, cnt, st)
+            if self.rec.pairs:
+                N.call("bz_adj2d_run2", ddev.data_ptr(), B, self.rec.tiles2.data_ptr(), self.rec.ntiles2,
+                       self.rec.seg2, self.rec.tiles.data_ptr(), self.rec.ntiles, self.rec.seg, n0 + cnt, cnt, flip, st)
+                flip ^= (cnt // 2) & 1
+            else:
+                N.call("bz_adj2d_run", ddev.data_ptr(), B, self.rec.tiles.data_ptr(), self.rec.ntiles, self.rec.seg,
+                       n0 + cnt, cnt, st)
         self._keep = (lam, inj, ddev)
         return grad
 

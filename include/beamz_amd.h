@@ -190,6 +190,13 @@ typedef struct bz_adj2d {
  * array); tiles as for bz_yee2d_run (seg 2, 4 or 8; sim < 0 marks padding). */
 int bz_adj2d_run(const bz_adj2d* adjs, int nadj, const int* tiles, int ntiles, int seg, int n_hi, int nsteps,
                  void* stream);
+/* The same reverse steps two per launch (same bits), on tiles2 of 60 columns x seg2 rows (the layout of
+ * bz_yee2d_run2's tiles), with a last single step on `tiles` (seg 2, 4 or 8) when nsteps is odd. A pair
+ * reads lam_{n+1} from buffer (n + 1 + flip) & 1 and writes lam_{n-1} to the other buffer, so the caller
+ * passes flip = 0 for the first call after bz_adj2d_inject and afterwards flip ^ ((nsteps / 2) & 1) of
+ * the previous call; bz_adj2d_run is this with flip = 0 throughout. */
+int bz_adj2d_run2(const bz_adj2d* adjs, int nadj, const int* tiles2, int ntiles2, int seg2, const int* tiles,
+                  int ntiles, int seg, int n_hi, int nsteps, int flip, void* stream);
 /* lE_n += sum_f Re(inj_w[f] * phasor[n-1][f]) over each injection box (max_inj >= the largest box's cell count):
  * the terminal condition lam_N = dJ/du_N before the first reverse step. */
 int bz_adj2d_inject(const bz_adj2d* adjs, int nadj, int max_inj, int n, void* stream);

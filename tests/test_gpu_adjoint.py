@@ -212,8 +212,9 @@ def test_c4_gradient_vs_central_differences_on_32_pixels(hip):
 
 
 def test_pair_sweeps_give_the_same_gradient_and_engines_are_reused(hip):
-    """Forward and recompute sweeps in two-step launches (checkpoint interval 23, so every segment ends with a
-    single step whose snapshot slot follows the pairs') give the one-step gradient bit for bit; later
+    """Forward, recompute and reverse sweeps in two-step launches (checkpoint interval 23, so every segment ends
+    with a single step: its snapshot slot follows the pairs', and the reverse state has moved across the two
+    adjoint buffers an odd number of times) give the one-step gradient bit for bit; later
     evaluations on the same problem (engines reused, fields and monitor sums reset in place) repeat it, also
     after a different design was evaluated in between."""
     torch = hip
