@@ -62,6 +62,8 @@ class AdjointProblem2D:
     cells, inside the interior.
     """
 
+    REVERSE_SEGS2 = (4, 8, 16)      # pair-tile heights tried for the recompute and reverse sweeps
+
     def __init__(self, sims, monitors, weights, box, checkpoint_every=None, seg=None, fscale=None, pairs=None):
         self.sims = list(sims)
         self.monitors = list(monitors)
@@ -83,7 +85,8 @@ class AdjointProblem2D:
         self.N = self.sims[0].num_steps
         self.K = int(checkpoint_every or max(1, math.ceil(math.sqrt(self.N))))
         self.seg = seg
-        self.pairs = pairs            # two steps per launch in the forward and recompute sweeps (None: by size)
+        self.pairs = pairs            # two steps per launch in all three sweeps (None: by size)
+        self._rev_tuned = False
         self.torch = self.sims[0].backend.torch
         self.dev = self.sims[0].backend.dev
         self._ready = False
@@ -226,10 +229,9 @@ class AdjointProblem2D:
         ddev = T.frombuffer(bytearray(raw), dtype=T.uint8).to(self.dev)
         max_inj = max(i[2] * i[3] for i in inj)
         st = grids[0].stream
-        N.call("bz_adj2d_inject", ddev.data_ptr(), B, max_inj, self.N, st)
-        nseg = len(self.ckpt)
-        flip = 0                    # pairs of reverse steps move the adjoint state across its two buffers
-        for k in range(nseg - 1, -1, -1):
+
+        def segment(k, flip):
+            """Recompute segment k from its checkpoint (snapshots on), then walk it backwards; returns flip."""
             n0 = k * self.K
             cnt = min(self.K, self.N - n0)
             for g, ck in zip(grids, self.ckpt[k]):
@@ -242,10 +244,35 @@ class AdjointProblem2D:
             if self.rec.pairs:
                 N.call("bz_adj2d_run2", ddev.data_ptr(), B, self.rec.tiles2.data_ptr(), self.rec.ntiles2,
                        self.rec.seg2, self.rec.tiles.data_ptr(), self.rec.ntiles, self.rec.seg, n0 + cnt, cnt, flip, st)
-                flip ^= (cnt // 2) & 1
-            else:
-                N.call("bz_adj2d_run", ddev.data_ptr(), B, self.rec.tiles.data_ptr(), self.rec.ntiles, self.rec.seg,
-                       n0 + cnt, cnt, st)
+                return flip ^ ((cnt // 2) & 1)
+            N.call("bz_adj2d_run", ddev.data_ptr(), B, self.rec.tiles.data_ptr(), self.rec.ntiles, self.rec.seg,
+                   n0 + cnt, cnt, st)
+            return flip
+
+        nseg = len(self.ckpt)
+        if self.rec.pairs and not self._rev_tuned and nseg > 1:
+            # the recompute and reverse sweeps share the pair tiles: time one segment of both per tile height
+            # and keep the fastest (the adjoint state and gradient are cleared again below)
+            ms = {}
+            for seg2 in self.REVERSE_SEGS2:
+                self.rec.set_seg2(seg2)
+                segment(nseg - 1, 0)                                  # warm-up
+                e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
+                e0.record()
+                segment(nseg - 1, 0)
+                e1.record()
+                T.cuda.synchronize(self.dev)
+                ms[seg2] = e0.elapsed_time(e1)
+            self.rec.set_seg2(min(ms, key=ms.get))
+            self._rev_tuned = True
+            for pair in lam:
+                for t in pair:
+                    t.zero_()
+            grad.zero_()
+        N.call("bz_adj2d_inject", ddev.data_ptr(), B, max_inj, self.N, st)
+        flip = 0                    # pairs of reverse steps move the adjoint state across its two buffers
+        for k in range(nseg - 1, -1, -1):
+            flip = segment(k, flip)
         self._keep = (lam, inj, ddev)
         return grad
 
