@@ -520,6 +520,7 @@ class Fused2D:
     """
 
     PAIR_MIN_CELLS = 1_000_000
+    GRAPH_CHUNK = 64        # steps per captured graph (longer runs replay it)
     XCD_ORDER = True        # tile lists in xcd_order (False: plain row-major order)
 
     def __init__(self, grids, seg=8, graph=False, max_steps=None, monitors=True, snapshots=None, pairs=None,
@@ -817,14 +818,22 @@ class Fused2D:
             if rem:
                 self._sync_descs(n0 + 2 * npairs, 2 * npairs)      # step counter is already there
         if rem and self.use_graph and rem > 1:
-            gr = self.graphs.get(rem)
-            if gr is None:
-                gr = T.cuda.CUDAGraph()
-                T.cuda.synchronize(self.dev)
-                with T.cuda.graph(gr):
-                    self._launch(rem)
-                self.graphs[rem] = gr
-            gr.replay()
+            # replays of one captured GRAPH_CHUNK-step graph, then one graph for the rest: the step base lives
+            # in device memory, so each replay continues where the previous one stopped
+            left = rem
+            while left > 1:
+                c = min(left, self.GRAPH_CHUNK)
+                gr = self.graphs.get(c)
+                if gr is None:
+                    gr = T.cuda.CUDAGraph()
+                    T.cuda.synchronize(self.dev)
+                    with T.cuda.graph(gr):
+                        self._launch(c)
+                    self.graphs[c] = gr
+                gr.replay()
+                left -= c
+            if left:
+                self._launch(left)
         elif rem:
             self._launch(rem)
         if g0.timing:
