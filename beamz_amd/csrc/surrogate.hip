@@ -127,6 +127,91 @@ __global__ __launch_bounds__(256) void k_conv_gemm(const uint16_t* __restrict__ 
     }
 }
 
+// Forward / backward-data 3x3 convolution with the input halo staged once per channel chunk.
+// A workgroup computes 64 consecutive pixels (one image; a 64-pixel row segment if W >= 64, else 64 / W
+// whole rows) x 64 output channels. For each chunk of CC input channels it loads the (rows + 2) x (cols + 2)
+// pixel halo of the tile into LDS (zero outside the image), then forms all nine taps from it: the A
+// fragment of output pixel m at tap (ky, kx) is the halo slot of pixel m shifted by sign * (ky - 1, kx - 1).
+// Each activation is read from memory once per chunk instead of once per tap. The B fragments (weights,
+// [N][9][Cs], K-contiguous) come straight from global memory (L2), the next tap's while the current tap's
+// MFMAs run. Needs H*W % 64 == 0 and W % 64 == 0 or 64 % W == 0, Cs % CC == 0.
+// Used for 32-channel inputs (level-0 layers: 172 -> 139 us for batch 128 at 128 x 128). With 64 or more
+// channels the direct weight reads (1 KB per MFMA and wave from L2) cost more than the tap re-reads they
+// replace (92 -> 125 us), so those layers keep k_conv_gemm; staging the weights in LDS is the next step.
+template <int CC>
+__global__ __launch_bounds__(256) void k_conv_halo(const uint16_t* __restrict__ src, const uint16_t* __restrict__ wk,
+                                                   const float* __restrict__ bias, uint16_t* __restrict__ out, Geo g,
+                                                   int Cs, int N, int sign, int relu) {
+    constexpr int SLD = CC + 8;              // halfwords per halo slot (16-byte pad: conflict-free reads)
+    constexpr int KS = CC / 16;              // MFMA k-steps per tap and chunk
+    __shared__ __attribute__((aligned(16))) uint16_t halo[198][SLD];
+    const int HW = g.H * g.W;
+    const int p0 = blockIdx.x * 64, n0 = blockIdx.y * TN;
+    const int b = p0 / HW, rem = p0 - b * HW;
+    const int y0 = rem / g.W, x0 = rem - y0 * g.W;
+    const int ncols = g.W < 64 ? g.W : 64, nrows = 64 / ncols;
+    const int hc = ncols + 2, slots = (nrows + 2) * hc;
+    const int t = threadIdx.x, l = t & 63, wave = t >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int K = 9 * Cs;
+    // this lane's output pixel of the A fragment and its B row
+    const int m = wm * 32 + (l & 31);
+    const int my = m / ncols, mx = m - my * ncols;
+    const int n = n0 + wn * 32 + (l & 31);
+    const int kq = 8 * (l >> 5);
+    const uint16_t* wrow = wk + (long)min(n, N - 1) * K;
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    uint4 bq[2][KS];
+    auto load_b = [&](uint4* dst, int tap, int c0) {
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+            dst[s] = n < N ? *reinterpret_cast<const uint4*>(wrow + tap * Cs + c0 + 16 * s + kq) : make_uint4(0, 0, 0, 0);
+    };
+    for (int c0 = 0; c0 < Cs; c0 += CC) {
+        // halo of the tile for channels c0 .. c0 + CC - 1
+        for (int idx = t; idx < slots * (CC / 8); idx += 256) {
+            const int slot = idx / (CC / 8), cg = idx - slot * (CC / 8);
+            const int hy = slot / hc, hx = slot - hy * hc;
+            const int py = y0 + hy - 1, px = x0 + hx - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (py >= 0 && py < g.H && px >= 0 && px < g.W)
+                v = *reinterpret_cast<const uint4*>(src + (((long)b * g.H + py) * g.W + px) * Cs + c0 + 8 * cg);
+            *reinterpret_cast<uint4*>(&halo[slot][8 * cg]) = v;
+        }
+        load_b(bq[0], 0, c0);
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            if (tap < 8) load_b(bq[(tap + 1) & 1], tap + 1, c0);
+            const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+            const int slot = (my + 1 + sign * (ky - 1)) * hc + (mx + 1 + sign * (kx - 1));
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const uint4 av = *reinterpret_cast<const uint4*>(&halo[slot][16 * s + kq]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av),
+                                                               __builtin_bit_cast(bf16x8, bq[tap & 1][s]), acc, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    if (n >= N) return;
+    const float bn = bias ? bias[n] : 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int mm = p0 + wm * 32 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+        float v = acc[r] + bn;
+        if (relu) v = fmaxf(v, 0.0f);
+        out[(long)mm * N + n] = f2b(v);
+    }
+}
+
+inline bool halo_ok(const Geo& g) {
+    return (g.H * g.W) % 64 == 0 && (g.W % 64 == 0 || 64 % g.W == 0) && (64 / (g.W < 64 ? g.W : 64) + 2) *
+           ((g.W < 64 ? g.W : 64) + 2) <= 198;
+}
+
 // Weight gradient gw[co][tap][ci] += sum_p gz[p][co] * x[p + tap][ci] over this workgroup's pixel range
 // (a multiple of WK pixels, the last one clipped at P). GEMM with M = Co, N = 9*Ci, K = pixels.
 // Staging of one WK-pixel slice: threads 0..127 take the gz operand, 128..255 the im2col(x) operand; each
@@ -630,9 +715,13 @@ extern "C" int bz_nn_conv3x3(const uint16_t* x, const uint16_t* w16, const float
     const long P = (long)B * H * W;
     BZ_REQUIRE(P * (long)(Ci > Co ? Ci : Co) < (1L << 31), "bz_nn_conv3x3: tensor too large");
     Geo g{B, H, W};
-    hipLaunchKernelGGL((9 * Ci) % 64 == 0 ? k_conv_gemm<64> : k_conv_gemm<32>,
-                       dim3((unsigned)((P + TM - 1) / TM), (Co + TN - 1) / TN), dim3(256), 0, (hipStream_t)stream, x,
-                       w16, bias, y, g, Ci, Co, 1, relu);
+    if (halo_ok(g) && Ci == 32)
+        hipLaunchKernelGGL(k_conv_halo<32>, dim3((unsigned)(P / 64), (Co + TN - 1) / TN), dim3(256), 0,
+                           (hipStream_t)stream, x, w16, bias, y, g, Ci, Co, 1, relu);
+    else
+        hipLaunchKernelGGL((9 * Ci) % 64 == 0 ? k_conv_gemm<64> : k_conv_gemm<32>,
+                           dim3((unsigned)((P + TM - 1) / TM), (Co + TN - 1) / TN), dim3(256), 0, (hipStream_t)stream,
+                           x, w16, bias, y, g, Ci, Co, 1, relu);
     BZ_CHECK_LAUNCH("bz_nn_conv3x3");
     return BZ_OK;
 }
@@ -643,9 +732,13 @@ extern "C" int bz_nn_conv3x3_dgrad(const uint16_t* gz, const uint16_t* wt16, uin
     BZ_REQUIRE(Co % 32 == 0 && Ci % 32 == 0, "bz_nn_conv3x3_dgrad: Co must be a multiple of 32 and Ci of 32");
     const long P = (long)B * H * W;
     Geo g{B, H, W};
-    hipLaunchKernelGGL((9 * Co) % 64 == 0 ? k_conv_gemm<64> : k_conv_gemm<32>,
-                       dim3((unsigned)((P + TM - 1) / TM), (Ci + TN - 1) / TN), dim3(256), 0, (hipStream_t)stream, gz,
-                       wt16, (const float*)nullptr, gx, g, Co, Ci, -1, 0);
+    if (halo_ok(g) && Co == 32)
+        hipLaunchKernelGGL(k_conv_halo<32>, dim3((unsigned)(P / 64), (Ci + TN - 1) / TN), dim3(256), 0,
+                           (hipStream_t)stream, gz, wt16, (const float*)nullptr, gx, g, Co, Ci, -1, 0);
+    else
+        hipLaunchKernelGGL((9 * Co) % 64 == 0 ? k_conv_gemm<64> : k_conv_gemm<32>,
+                           dim3((unsigned)((P + TM - 1) / TM), (Ci + TN - 1) / TN), dim3(256), 0, (hipStream_t)stream,
+                           gz, wt16, (const float*)nullptr, gx, g, Co, Ci, -1, 0);
     BZ_CHECK_LAUNCH("bz_nn_conv3x3_dgrad");
     return BZ_OK;
 }

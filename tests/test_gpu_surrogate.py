@@ -77,6 +77,37 @@ def test_conv_layers_match_oracle(hip):
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
 
 
+@pytest.mark.parametrize("B,H,W,Ci,Co", [(2, 8, 16, 32, 96), (1, 2, 128, 32, 64), (3, 4, 32, 32, 32),
+                                         (2, 8, 16, 64, 32)])
+def test_halo_convolutions_match_oracle(hip, B, H, W, Ci, Co):
+    """Forward and backward-data convolutions on shapes the halo-staged kernel takes (32-channel inputs on
+    64-pixel tiles of whole rows or row segments, an output count that is not a multiple of 64; the last
+    case takes it for backward-data only), within one bfloat16 step of the rounded FP64 oracle."""
+    torch = hip
+    rng = np.random.default_rng(B * 1000 + W)
+    x = S.bf16(rng.standard_normal((B, H, W, Ci)))
+    w = S.bf16(rng.standard_normal((Co, 3, 3, Ci)) * 0.05)
+    b = rng.standard_normal(Co).astype(np.float32).astype(np.float64)
+    w32 = torch.tensor(w, dtype=torch.float32).cuda()
+    w16 = torch.empty((Co, 3, 3, Ci), dtype=torch.int16).cuda()
+    wt16 = torch.empty((Ci, 3, 3, Co), dtype=torch.int16).cuda()
+    st = torch.cuda.current_stream().cuda_stream
+    N.call("bz_nn_weights", w32.data_ptr(), w16.data_ptr(), wt16.data_ptr(), Co, Ci, st)
+    y = torch.empty((B, H, W, Co), dtype=torch.int16).cuda()
+    # device inputs stay referenced until the kernels have run (a temporary's memory can be handed to the
+    # next allocation, whose upload is queued before the kernel)
+    xd, bd = to16(torch, x), torch.tensor(b, dtype=torch.float32).cuda()
+    N.call("bz_nn_conv3x3", xd.data_ptr(), w16.data_ptr(), bd.data_ptr(), y.data_ptr(), B, H, W, Ci, Co, 1, st)
+    ref = np.maximum(S.conv3x3(x, w, b), 0.0)
+    assert np.all(np.abs(from16(y) - S.bf16(ref)) <= 2.0 ** -7 * np.abs(ref) + 1e-6)
+    gz = S.bf16(rng.standard_normal((B, H, W, Co)))
+    gx = torch.empty((B, H, W, Ci), dtype=torch.int16).cuda()
+    gzd = to16(torch, gz)
+    N.call("bz_nn_conv3x3_dgrad", gzd.data_ptr(), wt16.data_ptr(), gx.data_ptr(), B, H, W, Ci, Co, st)
+    dx, _, _ = S.conv3x3_backward(x, w, gz)
+    assert np.all(np.abs(from16(gx) - S.bf16(dx)) <= 2.0 ** -7 * np.abs(dx) + 1e-5)
+
+
 def test_first_layer_pooling_and_mask_match_oracle(hip):
     """The one-input-channel first convolution (forward, weight and bias gradients), 2x2 average pooling and
     its backward, and the ReLU mask, on shapes whose channel counts take the 8-wide kernels (32, 64) and on
@@ -103,25 +134,29 @@ def test_first_layer_pooling_and_mask_match_oracle(hip):
     work = torch.empty(need, dtype=torch.float32).cuda()
     for wk, nwk in ((None, 0), (work.data_ptr(), need)):           # atomics, then ordered partial sums
         gw.zero_(); gb.zero_()
-        N.call("bz_nn_conv_first_bwd", xd.data_ptr(), to16(torch, gz).data_ptr(), wd.data_ptr(), gw.data_ptr(),
+        gzd = to16(torch, gz)
+        N.call("bz_nn_conv_first_bwd", xd.data_ptr(), gzd.data_ptr(), wd.data_ptr(), gw.data_ptr(),
                gb.data_ptr(), None, B, H, W, Co, wk, nwk, st)
         assert rel_l2(gw.cpu().numpy(), dw.reshape(Co, 9)) < 1e-5
         assert rel_l2(gb.cpu().numpy(), db) < 1e-5
     for C in (64, 12):
         a = S.bf16(rng.standard_normal((2, 6, 8, C)))
         p = torch.empty((2, 3, 4, C), dtype=torch.int16).cuda()
-        N.call("bz_nn_avgpool2", to16(torch, a).data_ptr(), p.data_ptr(), 2, 6, 8, C, st)
+        ad = to16(torch, a)
+        N.call("bz_nn_avgpool2", ad.data_ptr(), p.data_ptr(), 2, 6, 8, C, st)
         pr = S.avgpool2(a)
         assert np.all(np.abs(from16(p) - S.bf16(pr)) <= 2.0 ** -7 * np.abs(pr) + 1e-7), C
         gy = S.bf16(rng.standard_normal((2, 3, 4, C)))
         gx = torch.empty((2, 6, 8, C), dtype=torch.int16).cuda()
-        N.call("bz_nn_avgpool2_bwd", to16(torch, gy).data_ptr(), gx.data_ptr(), 2, 6, 8, C, st)
+        gyd = to16(torch, gy)
+        N.call("bz_nn_avgpool2_bwd", gyd.data_ptr(), gx.data_ptr(), 2, 6, 8, C, st)
         assert np.array_equal(from16(gx), S.avgpool2_backward(gy)), C      # a quarter is exact in bfloat16
     for n in (4096, 4099):
         g = S.bf16(rng.standard_normal(n))
         v = S.bf16(rng.standard_normal(n))
         gd = to16(torch, g)
-        N.call("bz_nn_relu_mask", gd.data_ptr(), to16(torch, v).data_ptr(), n, st)
+        vd = to16(torch, v)
+        N.call("bz_nn_relu_mask", gd.data_ptr(), vd.data_ptr(), n, st)
         assert np.array_equal(from16(gd), np.where(v > 0, g, 0.0)), n
 
 
