@@ -77,6 +77,7 @@ SIGNATURES = {
     "bz_nn_conv_first": [vp, vp, vp, vp, i32, i32, i32, i32, vp],
     "bz_nn_conv_first_bwd": [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, i64, vp],
     "bz_nn_relu_mask": [vp, vp, i64, vp],
+    "bz_nonfinite_count": [vp, i64, vp, vp],
     "bz_nn_avgpool2": [vp, vp, i32, i32, i32, i32, vp],
     "bz_nn_avgpool2_bwd": [vp, vp, i32, i32, i32, i32, vp],
     "bz_nn_head": [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp],

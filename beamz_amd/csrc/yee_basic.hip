@@ -259,3 +259,28 @@ extern "C" int bz_grid_power_3d(double* acc, float* const f[6], int nx, int ny, 
     BZ_CHECK_LAUNCH("bz_grid_power_3d");
     return BZ_OK;
 }
+
+namespace {
+// Non-finite (NaN or infinite) entries of p[0 .. n): per-thread counts over a grid-stride loop, summed in
+// LDS, one global atomic per workgroup.
+__global__ void k_nonfinite(const float* __restrict__ p, long n, unsigned int* __restrict__ count) {
+    __shared__ unsigned int c;
+    if (threadIdx.x == 0) c = 0;
+    __syncthreads();
+    unsigned int mine = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        mine += (__float_as_uint(p[i]) & 0x7f800000u) == 0x7f800000u ? 1u : 0u;
+    if (mine) atomicAdd(&c, mine);
+    __syncthreads();
+    if (threadIdx.x == 0 && c) atomicAdd(count, c);
+}
+}  // namespace
+
+extern "C" int bz_nonfinite_count(const float* p, long n, unsigned int* count, void* stream) {
+    BZ_REQUIRE(p && count && n >= 0, "bz_nonfinite_count: bad arguments");
+    if (n == 0) return BZ_OK;
+    const long blocks = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+    hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, p, n, count);
+    BZ_CHECK_LAUNCH("bz_nonfinite_count");
+    return BZ_OK;
+}

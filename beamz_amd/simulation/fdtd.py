@@ -512,7 +512,23 @@ class FDTD:
                                    accumulate_power=accumulate_power, save_memory_mode=save_memory_mode,
                                    fields_to_cache=fields_to_cache)
         self._advance(self.num_steps if steps is None else int(steps))
+        bad = self.check_fields()
+        if bad:
+            display_status(f"non-finite field values after {self.current_step} steps: {bad}", "warning")
         return self.finalize_simulation()
+
+    def check_fields(self):
+        """Numerical watchdog (no reference counterpart): {component: number of NaN or infinite entries} of
+        the current fields, empty when every entry is finite. One pass over the device fields."""
+        from .. import _native as N
+        T = self.backend.torch
+        names = list(self.grid.names)
+        count = T.zeros(len(names), dtype=T.int32, device=self.backend.dev)
+        for q, k in enumerate(names):
+            t = self.grid.f[k]
+            N.call("bz_nonfinite_count", t.data_ptr(), t.numel(), count.data_ptr() + 4 * q, self.grid.stream)
+        c = count.cpu().numpy()
+        return {k: int(v) for k, v in zip(names, c) if v}
 
     def synchronize(self):
         self.backend.torch.cuda.synchronize(self.backend.dev)

@@ -257,6 +257,11 @@ int bz_grid_power_2d(double* acc, const float* Ez, const float* Hx, const float*
 /* Grid power 3D: acc (on the [nz-1][ny-1][nx-1] corner box, pitch ld) += |E x H|. */
 int bz_grid_power_3d(double* acc, float* const f[6], int nx, int ny, int nz, int ld, void* stream);
 
+/* ---------------- numerical watchdog ----------------
+ * count (device, caller-zeroed) += number of NaN or infinite entries of p[0 .. n). No reference
+ * counterpart: the reference has no numerical-failure detection. */
+int bz_nonfinite_count(const float* p, long n, unsigned int* count, void* stream);
+
 /* ---------------- rasterizer (host, FP64, bit-exact with the reference) ---------------- */
 typedef struct bz_shape {
     int kind;               /* 0 axis-aligned rectangle (fast path), 1 circle, 2 ring, 3 polygon */

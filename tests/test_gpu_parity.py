@@ -221,3 +221,25 @@ def test_two_step_3d_kernel_bit_identical_to_single_steps(hip):
     for key, r in out.items():
         for k in base:
             assert np.array_equal(r[k], base[k]), (key, k)
+
+
+def test_watchdog_counts_nonfinite_entries(hip):
+    """bz_nonfinite_count counts NaN and infinite entries exactly (across several workgroups); a normal run
+    reports no component, and a NaN planted in a field is reported for that component only."""
+    torch = hip
+    from beamz_amd import _native as N
+    st = torch.cuda.current_stream().cuda_stream
+    a = torch.zeros(1_000_003, device="cuda")
+    a[[5, 77, 999_999, 1_000_002]] = float("nan")
+    a[123] = float("inf")
+    a[124] = -float("inf")
+    a[125] = 3.0e38                       # large but finite
+    c = torch.zeros(2, dtype=torch.int32, device="cuda")
+    N.call("bz_nonfinite_count", a.data_ptr(), a.numel(), c.data_ptr(), st)
+    N.call("bz_nonfinite_count", a.data_ptr(), 100, c.data_ptr() + 4, st)
+    assert c.cpu().tolist() == [6, 2]
+    sim = _wg_sim("fused", nsteps=60)[0]
+    sim.run(live=False, save=False)
+    assert sim.check_fields() == {}
+    sim.grid.f["Hx"][3, 4] = float("nan")
+    assert sim.check_fields() == {"Hx": 1}
