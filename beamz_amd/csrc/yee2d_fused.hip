@@ -202,6 +202,49 @@ __global__ void k_tick(int* step, int nsteps) {
 // row r and step n+2 on row r-1 (its E needs only the H of step n+2 on rows r-1 and r-2, carried), so
 // one pass reads the five arrays and writes the three fields once per two steps. Lanes 0, 1, 62, 63 are
 // halo lanes; a segment starting at j0 > 0 begins two rows early. Reads buffer rd, writes rd ^ 1.
+// Monitor samples of both steps of pair q: step n from staging slot q & 1, step n+1 from buffer wr.
+// Thread g handles (point p, frequency k), p fastest, so the FP64 accumulators dft[k][p] are read and
+// written coalesced and the latency of the read-modify-write is spread over npts * nfreq threads. Each
+// accumulator still receives step n's term and then step n+1's, as two rounded adds in that order.
+__device__ __forceinline__ void gather_pair(const bz_sim2d* __restrict__ sims, const Gather2D& G,
+                                            const int* __restrict__ spts, const int* __restrict__ step, int q, int wr,
+                                            long g) {
+    const int p = (int)(g % G.npts);
+    const int k = (int)(g / G.npts);
+    if (k >= max(G.nfreq, 1)) return;
+    const int n = *step + 2 * q;
+    const bz_sim2d& S = sims[G.pts[3 * p]];
+    const int comp = G.pts[3 * p + 1];
+    int row[2];
+    float v[2];
+    for (int s = 0; s < 2; ++s) {
+        row[s] = G.slot[n + s];
+        v[s] = 0.0f;
+        if (row[s] < 0) continue;
+        if (s == 0) {
+            const int so = spts[p];
+            const long plane = (long)S.stg_sx * S.stg_sy;
+            v[s] = so >= 0 ? S.stg[((long)(q & 1) * 3 + comp) * plane + so] : 0.0f;
+        } else {
+            const float* f = comp == 0 ? S.ez[wr] : (comp == 1 ? S.hx[wr] : S.hy[wr]);
+            v[s] = f[G.pts[3 * p + 2]];
+        }
+        if (k == 0 && G.series) G.series[(long)row[s] * G.npts + p] = v[s];
+    }
+    if (k >= G.nfreq || (row[0] < 0 && row[1] < 0)) return;
+    double* a = G.dft + 2 * ((long)k * G.npts + p);
+    double re = a[0], im = a[1];
+    for (int s = 0; s < 2; ++s) {
+        if (row[s] < 0) continue;
+        const double dv = v[s];
+        const double* ph = G.phasor + 2 * ((long)row[s] * G.nfreq + k);
+        re += dv * ph[0];
+        im += dv * ph[1];
+    }
+    a[0] = re;
+    a[1] = im;
+}
+
 // Snapshot of the step with call-relative index slot at cell (j, i), if inside the box: the Ez the step read
 // and the Hx, Hy it wrote (the layout of bz_yee2d_run's snapshots).
 __device__ __forceinline__ void snap_put(const bz_sim2d& S, int slot, int i, int j, float ez, float hx, float hy) {
@@ -234,7 +277,16 @@ __device__ __forceinline__ void add_power(const bz_sim2d& S, long o, float e, fl
 
 __global__ __launch_bounds__(LANES* WPG) void k_yee2d_tb2(const bz_sim2d* __restrict__ sims,
                                                          const int* __restrict__ tiles, int ntiles, int seg,
-                                                         const int* __restrict__ step, int q, int rd) {
+                                                         const int* __restrict__ step, int q, int rd, Gather2D G,
+                                                         const int* __restrict__ spts) {
+    const int tile_blocks = (ntiles + WPG - 1) / WPG;
+    if ((int)blockIdx.x >= tile_blocks) {
+        // monitor samples of the previous pair (launched only for q > 0): its first step sits in staging slot
+        // (q - 1) & 1, which this launch does not write, and its second step in buffer rd, which it only reads
+        gather_pair(sims, G, spts, step, q - 1, rd,
+                    (long)((int)blockIdx.x - tile_blocks) * GPB + threadIdx.y * LANES + threadIdx.x);
+        return;
+    }
     const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * WPG + threadIdx.y);
     if (t >= ntiles) return;
     const int n = *step + 2 * q;
@@ -322,7 +374,7 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee2d_tb2(const bz_sim2d* __rest
                 const int a = i - S.stg_x0, b = r - S.stg_y0;
                 if (a >= 0 && a < S.stg_sx && b >= 0 && b < S.stg_sy) {
                     const long plane = (long)S.stg_sx * S.stg_sy;
-                    float* p = S.stg + (long)b * S.stg_sx + a;
+                    float* p = S.stg + (long)(q & 1) * 3 * plane + (long)b * S.stg_sx + a;
                     p[0] = e1;
                     p[plane] = h1x;
                     p[2 * plane] = h1y;
@@ -339,46 +391,10 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee2d_tb2(const bz_sim2d* __rest
     }
 }
 
-// Monitor samples of both steps of pair q: step n from the staging box, step n+1 from buffer wr.
-// One thread per (point p, frequency k), p fastest, so the FP64 accumulators dft[k][p] are read and
-// written coalesced and the latency of the read-modify-write is spread over npts * nfreq threads. Each
-// accumulator still receives step n's term and then step n+1's, as two rounded adds in that order.
+// Gather of pair q as a launch of its own (the last pair of a call).
 __global__ void k_gather2d_pair(const bz_sim2d* __restrict__ sims, Gather2D G, const int* __restrict__ spts,
                                 const int* __restrict__ step, int q, int wr) {
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int p = (int)(g % G.npts);
-    const int k = (int)(g / G.npts);
-    if (k >= max(G.nfreq, 1)) return;
-    const int n = *step + 2 * q;
-    const bz_sim2d& S = sims[G.pts[3 * p]];
-    const int comp = G.pts[3 * p + 1];
-    int row[2];
-    float v[2];
-    for (int s = 0; s < 2; ++s) {
-        row[s] = G.slot[n + s];
-        v[s] = 0.0f;
-        if (row[s] < 0) continue;
-        if (s == 0) {
-            const int so = spts[p];
-            v[s] = so >= 0 ? S.stg[(long)comp * S.stg_sx * S.stg_sy + so] : 0.0f;
-        } else {
-            const float* f = comp == 0 ? S.ez[wr] : (comp == 1 ? S.hx[wr] : S.hy[wr]);
-            v[s] = f[G.pts[3 * p + 2]];
-        }
-        if (k == 0 && G.series) G.series[(long)row[s] * G.npts + p] = v[s];
-    }
-    if (k >= G.nfreq || (row[0] < 0 && row[1] < 0)) return;
-    double* a = G.dft + 2 * ((long)k * G.npts + p);
-    double re = a[0], im = a[1];
-    for (int s = 0; s < 2; ++s) {
-        if (row[s] < 0) continue;
-        const double dv = v[s];
-        const double* ph = G.phasor + 2 * ((long)row[s] * G.nfreq + k);
-        re += dv * ph[0];
-        im += dv * ph[1];
-    }
-    a[0] = re;
-    a[1] = im;
+    gather_pair(sims, G, spts, step, q, wr, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 template <int SEG>
@@ -446,19 +462,22 @@ extern "C" int bz_yee2d_run2(const bz_sim2d* sims, int nsims, const int* tiles, 
     G.phasor = phasor;
     G.dft = dft;
     const hipStream_t st = (hipStream_t)stream;
-    const dim3 g((ntiles + WPG - 1) / WPG), b(LANES, WPG);
+    const dim3 b(LANES, WPG);
     const long gather_threads = (long)npts * (nfreq > 0 ? nfreq : 1);
     BZ_REQUIRE(gather_threads / GPB < (1L << 31) - 1, "bz_yee2d_run2: too many monitor points x frequencies");
     const int gather_blocks = (int)((gather_threads + GPB - 1) / GPB);
+    // the gather of pair q-1 runs as extra workgroups of pair q's launch; the last pair's gets its own launch
+    const int tile_blocks = (ntiles + WPG - 1) / WPG;
     for (int q = 0; q < npairs; ++q) {
         const int rd = (rd0 + q) & 1;
-        hipLaunchKernelGGL(k_yee2d_tb2, g, b, 0, st, sims, tiles, ntiles, seg, step, q, rd);
+        const dim3 gq(tile_blocks + (q > 0 && npts > 0 ? gather_blocks : 0));
+        hipLaunchKernelGGL(k_yee2d_tb2, gq, b, 0, st, sims, tiles, ntiles, seg, step, q, rd, G, spts);
         BZ_CHECK_LAUNCH("bz_yee2d_run2(steps)");
-        if (npts > 0) {
-            hipLaunchKernelGGL(k_gather2d_pair, dim3(gather_blocks), dim3(GPB), 0, st, sims, G, spts, step, q,
-                               rd ^ 1);
-            BZ_CHECK_LAUNCH("bz_yee2d_run2(gather)");
-        }
+    }
+    if (npts > 0) {
+        hipLaunchKernelGGL(k_gather2d_pair, dim3(gather_blocks), dim3(GPB), 0, st, sims, G, spts, step, npairs - 1,
+                           ((rd0 + npairs - 1) & 1) ^ 1);
+        BZ_CHECK_LAUNCH("bz_yee2d_run2(gather)");
     }
     hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, step, 2 * npairs);
     BZ_CHECK_LAUNCH("bz_yee2d_run2(tick)");

@@ -618,7 +618,7 @@ class Fused2D:
             x0, x1 = int(i[real].min()), int(i[real].max()) + 1
             y0, y1 = int(j[real].min()), int(j[real].max()) + 1
             sx, sy = x1 - x0, y1 - y0
-            buf = T.zeros(3 * sx * sy, dtype=T.float32, device=self.dev)
+            buf = T.zeros(2 * 3 * sx * sy, dtype=T.float32, device=self.dev)     # two slots: pair q uses q & 1
             self.keep.append(buf)
             d = self.descs[s]
             d.stg, d.stg_x0, d.stg_y0, d.stg_sx, d.stg_sy = buf.data_ptr(), x0, y0, sx, sy

@@ -138,8 +138,8 @@ typedef struct bz_sim2d {
     double* power;                        /* NULL or [ny][ld] FP64 grid-power accumulator */
     int snap_x0, snap_y0, snap_sx, snap_sy;   /* snapshot box (snap_sx = 0: none) */
     float* snap;                          /* [nsteps][3][snap_sy][snap_sx] */
-    float* stg;                           /* bz_yee2d_run2: [3][stg_sy][stg_sx] staging of the first step's */
-    int stg_x0, stg_y0, stg_sx, stg_sy;   /* Ez, Hx, Hy for monitor samples (NULL: none) */
+    float* stg;                           /* bz_yee2d_run2: [2][3][stg_sy][stg_sx] staging of the first step's */
+    int stg_x0, stg_y0, stg_sx, stg_sy;   /* Ez, Hx, Hy for monitor samples, slot q & 1 for pair q (NULL: none) */
 } bz_sim2d;
 
 int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step, int nsteps,
