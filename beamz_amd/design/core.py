@@ -3,8 +3,12 @@
 ``structures[0]`` is the background rectangle; later structures paint over
 earlier ones. With ``auto_pml`` the reference's eight PML regions (four edge
 rectangles, four corner quarter-discs) and four display-only rectangles
-(``is_pml=True``, skipped by the rasterizer) are appended. Polygon union and
-GDS import/export are outside the FDTD hot path and are not provided.
+(``is_pml=True``, skipped by the rasterizer) are appended.
+
+GDS export and import (core.py:445-553) go through this package's own GDSII
+reader and writer (``design/gds.py``; the reference uses gdspy). Polygon union
+(core.py:159-200) needs shapely, which is not available: ``unify_polygons``
+reports that and returns False, as the reference does without shapely.
 """
 from __future__ import annotations
 
@@ -105,6 +109,72 @@ class Design:
             for b in self.boundaries:
                 extra += b.get_conductivity(x, y, dx=dx, dt=dt, eps_avg=eps)
         return [eps, mu, sig + extra]
+
+    def unify_polygons(self):
+        """core.py:159-200. Merging touching polygons of one material needs shapely, which is absent here: as
+        the reference does without it, report and return False, leaving the structures as they are."""
+        display_status("Shapely library is required for polygon unification. Please install with: pip install shapely",
+                       "error")
+        return False
+
+    def export_gds(self, output_file):
+        """core.py:477-553: the structures (not the PML display regions, sources or monitors) as GDSII boundaries,
+        one layer per material in order of first appearance (permittivity, permeability, conductivity), in
+        microns with a 1 nm database unit. A polygon with interiors is written as one keyhole boundary."""
+        from ..devices.monitors import Monitor
+        from ..devices.sources import GaussianSource, ModeSource
+        from .gds import write_gds
+        self.unify_polygons()
+        groups = {}
+        for s in self.structures:
+            if getattr(s, "is_pml", False) or isinstance(s, (ModeSource, GaussianSource, Monitor)):
+                continue
+            m = getattr(s, "material", None)
+            if m is None:
+                continue
+            key = (getattr(m, "permittivity", 1.0), getattr(m, "permeability", 1.0), getattr(m, "conductivity", 0.0))
+            groups.setdefault(key, []).append(s)
+        scale = 1e6
+        polys = []
+        for layer, members in enumerate(groups.values()):
+            for s in members:
+                verts = getattr(s, "vertices", None)
+                if not isinstance(s, Polygon) or not verts:
+                    continue
+                outline = [(v[0] * scale, v[1] * scale) for v in verts]
+                holes = [[(v[0] * scale, v[1] * scale) for v in h] for h in (getattr(s, "interiors", None) or [])]
+                polys.append((layer, 0, outline, holes))
+        write_gds(output_file, polys, unit=1e-6, precision=1e-9)
+        print(f"GDS file saved as '{output_file}' with {len(groups)} material-based layers")
+        for layer, key in enumerate(groups):
+            print(f"Layer {layer}: εᵣ={key[0]:.1f}, μᵣ={key[1]:.1f}, σ={key[2]:.2e} S/m")
+        display_status(f"Created design with size: {self.width:.2e} x {self.height:.2e} x {self.depth:.2e} m")
+
+    @staticmethod
+    def import_gds(gds_file, default_depth=1e-6):
+        """core.py:445-475: every structure's polygons (references flattened) by layer, as Polygons of the given
+        depth in metres, in ``layers[layer]`` and in ``structures`` of a new default Design."""
+        from .gds import read_gds
+        unit, cells, skipped = read_gds(gds_file)
+        design = Design()
+        n = 0
+        for specs in cells.values():
+            for (layer, _datatype), polys in specs.items():
+                design.layers.setdefault(layer, [])
+                for pts in polys:
+                    poly = Polygon(vertices=[(float(x) * unit, float(y) * unit) for x, y in pts], depth=default_depth)
+                    design.layers[layer].append(poly)
+                    design.structures.append(poly)
+                    n += 1
+        if default_depth > 0:
+            design.is_3d = True
+            design.depth = default_depth
+        print(f"Imported {n} polygons from '{gds_file}' into Design object.")
+        if skipped:
+            display_status(f"{skipped} PATH/TEXT elements of '{gds_file}' were not imported", "warning")
+        if design.is_3d:
+            print(f"3D design with depth: {design.depth:.2e} m")
+        return design
 
     def copy(self):
         bg = self.structures[0].material if self.structures and hasattr(self.structures[0], "material") else None
