@@ -418,6 +418,192 @@ int run(const bz_sim2d* sims, const int* tiles, int ntiles, int* step, int nstep
     return BZ_OK;
 }
 
+
+// ---------------- K steps per pass ----------------
+// A workgroup of KW waves takes one tile: a region of 64 columns x KW*B rows, wave w holding rows
+// w*B .. w*B+B-1 of it in registers (lane l = column), with the H and E coefficients of its cells computed
+// once. It advances the whole region K steps; the cells within K of the region's edge go stale one ring
+// per step (they lack neighbours outside the region), so after K steps the inner (64-2K) x (KW*B-2K) cells
+// hold step n+K and are stored. Tiles overlap by 2K cells, so one pass over HBM serves K steps.
+// Neighbours along a row come from wave shuffles, neighbours across band edges through LDS (two barriers
+// per step). Cells outside the grid compute garbage that never reaches a cell inside it: the update rules
+// read only in-grid neighbours for every updated in-grid entry. Monitor samples of each step are copied
+// from the owning tile to stage[k][point] and accumulated into the FP64 DFT by k_gather2d_k in step order.
+// Reads buffer rd, writes rd ^ 1.
+constexpr int KW = 4;
+
+template <int B>
+__global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restrict__ sims,
+                                                        const int* __restrict__ tiles, int ntiles, int K,
+                                                        const int* __restrict__ step, int q, int rd,
+                                                        const int* __restrict__ ent_start,
+                                                        const int* __restrict__ ents, float* __restrict__ stage,
+                                                        int npts) {
+    constexpr int RH = KW * B;
+    __shared__ float ez_top[KW][LANES];
+    __shared__ float hy_bot[KW][LANES];
+    __shared__ float smp[RH][LANES];
+    const int t = blockIdx.x;
+    if (t >= ntiles) return;
+    const int s = tiles[3 * t];
+    if (s < 0) return;   // padding entry of a reordered tile list (uniform over the workgroup)
+    const bz_sim2d& S = sims[s];
+    const int tx = tiles[3 * t + 1], ty = tiles[3 * t + 2];
+    const int nx = S.nx, ny = S.ny, ld = S.ld;
+    const int l = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int i = tx * (LANES - 2 * K) - K + l;
+    const int jb = ty * (RH - 2 * K) - K + w * B;   // grid row of the wave's first register row
+    const uint32_t ic = (uint32_t)min(max(i, 0), nx - 1);
+    const int wr = rd ^ 1;
+    const int n0 = *step + K * q;
+    Coef c;
+    c.kh = S.kh;
+    c.kbh = S.kbh;
+    c.ke = S.ke;
+    c.kbe = S.kbe;
+    const float rdx = S.rdx, rdy = S.rdy;
+    const bool hxok = i < nx - 1;
+    const bool xin = i >= 1 && i <= nx - 2;
+
+    float ez[B], hx[B], hy[B], fa[B], fb[B], ca[B], cb[B];
+    {
+        const float* __restrict__ gez = S.ez[rd];
+        const float* __restrict__ ghx = S.hx[rd];
+        const float* __restrict__ ghy = S.hy[rd];
+        float sg[B], ee[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const uint32_t o = ((uint32_t)min(max(jb + r, 0), ny - 1) * (uint32_t)ld + ic) * 4u;
+            ez[r] = at(gez, o);
+            hx[r] = at(ghx, o);
+            hy[r] = at(ghy, o);
+            sg[r] = at(S.sig, o);
+            ee[r] = at(S.eps, o);
+        }
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            h_coef(sg[r], c, fa[r], fb[r]);
+            e_coef(sg[r], ee[r], c, ca[r], cb[r]);
+        }
+    }
+    const int e0 = ent_start ? ent_start[t] : 0, e1 = ent_start ? ent_start[t + 1] : 0;
+    const int tid = w * LANES + l;
+
+    for (int k = 0; k < K; ++k) {
+        // H of step n0+k+1: Hy of the band's last row needs Ez of the next band's first row
+        ez_top[w][l] = ez[0];
+        __syncthreads();
+        const float ezb = w < KW - 1 ? ez_top[w + 1][l] : ez[B - 1];
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const int j = jb + r;
+            const float ezR = __shfl_down(ez[r], 1, LANES);
+            const float ed = r < B - 1 ? ez[r + 1] : ezb;
+            hx[r] = hxok ? fa[r] * hx[r] - fb[r] * ((ezR - ez[r]) * rdy) : hx[r];
+            hy[r] = (j < ny - 1) ? fa[r] * hy[r] + fb[r] * ((ed - ez[r]) * rdx) : hy[r];
+        }
+        // E: Ez of the band's first row needs the new Hy of the previous band's last row
+        hy_bot[w][l] = hy[B - 1];
+        __syncthreads();
+        const float hya = w > 0 ? hy_bot[w - 1][l] : hy[0];
+        const float amp = S.signal ? S.signal[n0 + k] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const int j = jb + r;
+            const float hxl = __shfl_up(hx[r], 1, LANES);
+            const float hu = r > 0 ? hy[r - 1] : hya;
+            float e = (j >= 1 && j <= ny - 2 && xin) ? ca[r] * ez[r] + cb[r] * (-((hx[r] - hxl) * rdy) + (hy[r] - hu) * rdx)
+                                                     : ez[r];
+            if (S.src_sx && j >= S.src_y0 && j < S.src_y0 + S.src_sy) box_src(e, S, i, j, amp);
+            ez[r] = e;
+        }
+        // monitor samples of step n0+k owned by this tile: entries sorted by component
+        if (e1 > e0) {
+            for (int comp = 0; comp < 3; ++comp) {
+#pragma unroll
+                for (int r = 0; r < B; ++r) smp[w * B + r][l] = comp == 0 ? ez[r] : (comp == 1 ? hx[r] : hy[r]);
+                __syncthreads();
+                for (int e = e0 + tid; e < e1; e += LANES * KW) {
+                    const int code = ents[2 * e + 1];
+                    if ((code & 3) == comp) stage[(long)k * npts + ents[2 * e]] = smp[code >> 10][(code >> 2) & 255];
+                }
+                __syncthreads();
+            }
+        }
+    }
+
+    // the inner cells of the region hold step n0+K
+    if (l < K || l >= LANES - K || i >= nx) return;
+    float* __restrict__ oez = S.ez[wr];
+    float* __restrict__ ohx = S.hx[wr];
+    float* __restrict__ ohy = S.hy[wr];
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        const int rr = w * B + r, j = jb + r;
+        if (rr >= K && rr < RH - K && j < ny) {
+            const uint32_t o = ((uint32_t)j * (uint32_t)ld + (uint32_t)i) * 4u;
+            put(oez, o, ez[r]);
+            put(ohx, o, hx[r]);
+            put(ohy, o, hy[r]);
+        }
+    }
+}
+
+// Monitor samples of the K steps of launch q from stage[k][p]: thread g handles (point p, frequency f), and
+// each accumulator receives the K terms in step order, one rounded add each, as with one step per launch.
+__global__ void k_gather2d_k(Gather2D G, const float* __restrict__ stage, const int* __restrict__ step, int q, int K) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = (int)(g % G.npts);
+    const int f = (int)(g / G.npts);
+    if (f >= max(G.nfreq, 1)) return;
+    const int n0 = *step + K * q;
+    double re = 0.0, im = 0.0;
+    double* a = f < G.nfreq ? G.dft + 2 * ((long)f * G.npts + p) : nullptr;
+    if (a) {
+        re = a[0];
+        im = a[1];
+    }
+    bool any = false;
+    for (int k = 0; k < K; ++k) {
+        const int row = G.slot[n0 + k];
+        if (row < 0) continue;
+        any = true;
+        const float v = stage[(long)k * G.npts + p];
+        if (f == 0 && G.series) G.series[(long)row * G.npts + p] = v;
+        if (a) {
+            const double dv = v;
+            const double* ph = G.phasor + 2 * ((long)row * G.nfreq + f);
+            re += dv * ph[0];
+            im += dv * ph[1];
+        }
+    }
+    if (a && any) {
+        a[0] = re;
+        a[1] = im;
+    }
+}
+
+template <int B>
+int runk(const bz_sim2d* sims, const int* tiles, int ntiles, int K, int* step, int rd0, int nlaunch,
+         const int* ent_start, const int* ents, float* stage, const Gather2D& G, hipStream_t st) {
+    const dim3 b(LANES, KW);
+    const long gather_threads = (long)G.npts * (G.nfreq > 0 ? G.nfreq : 1);
+    const int gather_blocks = (int)((gather_threads + 255) / 256);
+    for (int q = 0; q < nlaunch; ++q) {
+        hipLaunchKernelGGL(k_yee2d_tbk<B>, dim3(ntiles), b, 0, st, sims, tiles, ntiles, K, step, q, (rd0 + q) & 1,
+                           ent_start, ents, stage, G.npts);
+        BZ_CHECK_LAUNCH("bz_yee2d_runk(steps)");
+        if (G.npts > 0) {
+            hipLaunchKernelGGL(k_gather2d_k, dim3(gather_blocks), dim3(256), 0, st, G, stage, step, q, K);
+            BZ_CHECK_LAUNCH("bz_yee2d_runk(gather)");
+        }
+    }
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, step, K * nlaunch);
+    BZ_CHECK_LAUNCH("bz_yee2d_runk(tick)");
+    return BZ_OK;
+}
+
 }  // namespace
 
 extern "C" int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step,
@@ -482,4 +668,33 @@ extern "C" int bz_yee2d_run2(const bz_sim2d* sims, int nsims, const int* tiles, 
     hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, step, 2 * npairs);
     BZ_CHECK_LAUNCH("bz_yee2d_run2(tick)");
     return BZ_OK;
+}
+
+extern "C" int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int rows, int ksteps,
+                             int* step, int rd0, int nlaunch, const int* ent_start, const int* ents, float* stage,
+                             int npts, const int* slot, float* series, int nfreq, const double* phasor, double* dft,
+                             void* stream) {
+    BZ_REQUIRE(sims && nsims > 0 && tiles && ntiles > 0 && step && nlaunch >= 0 && (rd0 == 0 || rd0 == 1),
+               "bz_yee2d_runk: bad arguments");
+    BZ_REQUIRE(rows == 8 || rows == 16, "bz_yee2d_runk: rows must be 8 or 16");
+    BZ_REQUIRE(ksteps >= 1 && 2 * ksteps < LANES && 2 * ksteps < KW * rows,
+               "bz_yee2d_runk: ksteps must leave inner cells (2 * ksteps < 64 and < 4 * rows)");
+    BZ_REQUIRE(npts >= 0 && (npts == 0 || (ent_start && ents && stage && slot)),
+               "bz_yee2d_runk: monitor points without tables");
+    BZ_REQUIRE(nfreq >= 0 && (nfreq == 0 || (phasor && dft)), "bz_yee2d_runk: DFT requested without buffers");
+    BZ_REQUIRE((long)npts * (nfreq > 0 ? nfreq : 1) / 256 < (1L << 31) - 1,
+               "bz_yee2d_runk: too many monitor points x frequencies");
+    if (nlaunch == 0) return BZ_OK;
+    Gather2D G;
+    G.pts = nullptr;
+    G.npts = npts;
+    G.slot = slot;
+    G.series = series;
+    G.nfreq = nfreq;
+    G.phasor = phasor;
+    G.dft = dft;
+    const hipStream_t st = (hipStream_t)stream;
+    if (npts == 0) ent_start = ents = nullptr;
+    return rows == 8 ? runk<8>(sims, tiles, ntiles, ksteps, step, rd0, nlaunch, ent_start, ents, stage, G, st)
+                     : runk<16>(sims, tiles, ntiles, ksteps, step, rd0, nlaunch, ent_start, ents, stage, G, st);
 }

@@ -25,7 +25,7 @@ from .engine import Fused2D
 class FDTDBatch:
     SEGS = Fused2D.SEGS
 
-    def __init__(self, sims, seg=None, graph=False, chunk=100, pairs=None, seg2=None):
+    def __init__(self, sims, seg=None, graph=False, chunk=100, pairs=None, seg2=None, kstep=None, krows=16):
         if not sims:
             raise ValueError("FDTDBatch needs at least one simulation")
         for s in sims:
@@ -38,6 +38,7 @@ class FDTDBatch:
         self.sims = list(sims)
         self.seg, self.graph, self.chunk = seg, bool(graph), int(chunk)
         self.pairs, self.seg2 = pairs, seg2      # pairs None: the engine decides by batch size
+        self.kstep, self.krows = kstep, krows    # K steps per pass (None or 0: off)
         self.num_steps = len(t0)
         self.current_step = 0
         self.engine = None
@@ -50,7 +51,7 @@ class FDTDBatch:
         for s in self.sims:
             s.initialize_simulation(save=False, live=False)
         self.engine = Fused2D([s.grid for s in self.sims], seg=self.seg, graph=self.graph, max_steps=self.num_steps,
-                              pairs=self.pairs, seg2=self.seg2)
+                              pairs=self.pairs, seg2=self.seg2, kstep=self.kstep, krows=self.krows)
         self.seg = self.engine.seg
         for s in self.sims:
             s.grid._f2d = False       # members advance through the batch engine only

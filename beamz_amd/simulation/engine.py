@@ -98,6 +98,8 @@ class Grid:
         self._f2d = None
         self.seg2d = None       # rows per tile of the fused 2D engine (None: by grid size)
         self.graph2d = False    # capture fused 2D runs as HIP graphs
+        self.kstep2d = 0        # fused 2D engine: K steps per pass (0: off)
+        self.krows2d = 16       # rows per wave band of the K-step kernel (8 or 16)
         self.kseg = 16
         self.rows = 4
         self.tb2 = True         # 3D: two steps per launch where the step list allows it
@@ -120,7 +122,12 @@ class Grid:
                 return (f"bz_yee3d_step2 (two steps per pass, {self.waves2} waves x {self.rows2} rows, "
                         f"{self.kseg2_eff()}-plane segments)")
             return f"bz_yee3d_step (fused z-march, {self.rows} rows/wave)" if self.kernel == "fused" else "bz_update_3d (two-pass)"
-        return "bz_yee2d_run (fused, batched)" if self.kernel == "fused" else "bz_update_h_2d + bz_update_e_2d"
+        if self.kernel != "fused":
+            return "bz_update_h_2d + bz_update_e_2d"
+        if self.kstep2d:
+            k, r = int(self.kstep2d), 4 * int(self.krows2d)
+            return f"bz_yee2d_runk ({k} steps per pass, 64x{r} regions, {64 - 2 * k}x{r - 2 * k} inner)"
+        return "bz_yee2d_run (fused, batched)"
 
     def start_timing(self):
         self.timing = True
@@ -476,7 +483,8 @@ class Grid:
     def _fused2d(self, n0, count):
         if self._f2d is None:
             try:
-                self._f2d = Fused2D([self], seg=self.seg2d, graph=self.graph2d)
+                self._f2d = Fused2D([self], seg=self.seg2d, graph=self.graph2d, kstep=self.kstep2d,
+                                    krows=self.krows2d)
             except NotImplementedError:
                 self._f2d = False
         if self._f2d is False:
@@ -524,13 +532,13 @@ class Fused2D:
     XCD_ORDER = True        # tile lists in xcd_order (False: plain row-major order)
 
     def __init__(self, grids, seg=8, graph=False, max_steps=None, monitors=True, snapshots=None, pairs=None,
-                 seg2=None, xcd=None):
+                 seg2=None, xcd=None, kstep=None, krows=16):
         """monitors=False runs the steps without sampling any monitor (a recomputation);
         snapshots: per grid None or (x0, y0, sx, sy, tensor [steps][3][sy][sx]) for the snapshot box;
         pairs: advance two steps per launch (bz_yee2d_run2) where a run allows it (None: from PAIR_MIN_CELLS
         cells on; never with graphs), with tiles of 60 columns x seg2 rows (None: by size); xcd: order the tile
-        lists by
-        xcd_order (None: XCD_ORDER)."""
+        lists by xcd_order (None: XCD_ORDER); kstep: advance K steps per pass (bz_yee2d_runk, regions of 64 x
+        4*krows cells) for runs of at least K steps, the rest by pairs or single steps (None or 0: off)."""
         if not grids:
             raise ValueError("Fused2D needs at least one grid")
         self.snapshots = list(snapshots) if snapshots is not None else [None] * len(grids)
@@ -602,6 +610,11 @@ class Fused2D:
             if self.npts:
                 self._setup_staging(pts)
             self.set_seg2(seg2)
+        self._pts_host = pts if plans else None
+        self.kstep, self.krows = 0, 16
+        self.tilesk = self.ent_start = self.ents = self.stagek = None
+        if kstep:
+            self.set_kstep(kstep, krows)
 
     def _setup_staging(self, pts):
         """Per simulation, a staging box around its monitor points for the first step of each pair."""
@@ -679,6 +692,60 @@ class Fused2D:
             tiles = xcd_order(tiles)
         self.tiles2 = T.from_numpy(tiles.ravel()).to(self.dev)
         self.ntiles2 = len(tiles)
+
+    KROWS = (8, 16)
+
+    def set_kstep(self, K, rows=16):
+        """Tile list, monitor-owner tables and sample staging of the K-steps-per-pass kernel (K 0: off).
+
+        A tile's region is 64 columns x 4*rows rows; its inner (64-2K) x (4*rows-2K) cells tile the grid, and
+        each monitor point is sampled by the tile whose inner cells hold it."""
+        T = self.torch
+        K = int(K or 0)
+        if not K:
+            self.kstep = 0
+            return
+        rows = int(rows)
+        if rows not in self.KROWS or not 2 * K < min(64, 4 * rows):
+            raise ValueError(f"K steps per pass need rows in {self.KROWS} and 2K < min(64, 4 rows)")
+        if any(g.power is not None for g in self.grids):
+            raise ValueError("K steps per pass do not accumulate grid power")
+        if any(sn is not None for sn in self.snapshots):
+            raise ValueError("K steps per pass do not record snapshots")
+        if self.use_graph:
+            raise ValueError("K steps per pass are not captured in graphs")
+        ix, iy = 64 - 2 * K, 4 * rows - 2 * K
+        parts = []
+        for s, g in enumerate(self.grids):
+            gx, gy = -(-g.nx // ix), -(-g.ny // iy)
+            t = np.stack(np.meshgrid(np.arange(gx), np.arange(gy), indexing="xy"), -1).reshape(-1, 2)
+            parts.append(np.concatenate([np.full((len(t), 1), s), t], axis=1))
+        tiles = np.concatenate(parts).astype(np.int32)
+        if self.xcd:
+            tiles = xcd_order(tiles, per_wg=1)
+        self.kstep, self.krows = K, rows
+        self.tilesk = T.from_numpy(tiles.ravel()).to(self.dev)
+        self.ntilesk = len(tiles)
+        self.ent_start = self.ents = self.stagek = None
+        if self.npts:
+            pts = self._pts_host
+            sim, comp = pts[:, 0].astype(np.int64), pts[:, 1].astype(np.int64)
+            lds = np.array([g.ld for g in self.grids], np.int64)[sim]
+            j, i = np.divmod(pts[:, 2].astype(np.int64), lds)
+            nxs = np.array([g.nx for g in self.grids], np.int64)[sim]
+            nys = np.array([g.ny for g in self.grids], np.int64)[sim]
+            if np.any(i >= nxs) or np.any(j >= nys):
+                raise ValueError("a monitor point lies outside its grid")
+            tx, ty = i // ix, j // iy
+            index = {(int(a), int(b), int(c)): q for q, (a, b, c) in enumerate(tiles) if a >= 0}
+            t = np.array([index[(int(a), int(b), int(c))] for a, b, c in zip(sim, tx, ty)], np.int64)
+            code = ((j - (ty * iy - K)) << 10) | ((i - (tx * ix - K)) << 2) | comp
+            order = np.lexsort((comp, t))
+            ents = np.stack([order, code[order]], axis=1).astype(np.int32)
+            start = np.searchsorted(t[order], np.arange(self.ntilesk + 1)).astype(np.int32)
+            self.ents = T.from_numpy(ents.ravel()).to(self.dev)
+            self.ent_start = T.from_numpy(start).to(self.dev)
+            self.stagek = T.zeros(K * self.npts, dtype=T.float32, device=self.dev)
 
     def _describe(self, g, snap=None):
         T = self.torch
@@ -803,6 +870,21 @@ class Fused2D:
         if g0.timing:
             e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
             e0.record()
+        total = count
+        if self.kstep and count >= self.kstep:
+            # launches of K steps each; launch q reads the buffer holding step n0 + qK and writes the other one
+            K = self.kstep
+            nl = count // K
+            ptr = lambda t: t.data_ptr() if t is not None else None
+            N.call("bz_yee2d_runk", self.desc_dev.data_ptr(), len(self.grids), self.tilesk.data_ptr(), self.ntilesk,
+                   self.krows, K, self.step.data_ptr(), n0 & 1, nl, ptr(self.ent_start), ptr(self.ents),
+                   ptr(self.stagek), self.npts, ptr(self.slot), ptr(self.series), self.nfreq, ptr(self.phasor),
+                   ptr(self.dft), g0.stream)
+            for g in self.grids:
+                g._cur = (g._cur + nl) % 2
+            n0, count = n0 + nl * K, count - nl * K
+            if count:
+                self._sync_descs(n0)                     # the step counter is already there
         rem = count
         if self.pairs and count >= 2:
             # pairs of steps; pair q reads the buffer holding step n0 + 2q and writes the other one
@@ -838,8 +920,8 @@ class Fused2D:
             self._launch(rem)
         if g0.timing:
             e1.record()
-            g0._events.append((e0, e1, count))
+            g0._events.append((e0, e1, total))
         for g in self.grids:
             g._cur = (g._cur + rem) % 2
             if g.power is not None:
-                g.power_count += count
+                g.power_count += total

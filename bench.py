@@ -286,7 +286,7 @@ def build_runner(a, total):
         members = batch_2d(nsims=a.nsims, steps=max(total + 4, 16))
         sims = [bz.FDTD(m, time=t) for m, t, _ in members]
         batch = FDTDBatch(sims, seg=a.seg2d, graph=a.graph, chunk=a.chunk,
-                          pairs=None if a.pairs < 0 else bool(a.pairs))
+                          pairs=None if a.pairs < 0 else bool(a.pairs), kstep=a.kstep, krows=a.krows)
         batch.initialize()
         tuned = None
         if a.autotune:
@@ -296,7 +296,11 @@ def build_runner(a, total):
         desc = (f"{len(sims)} batched 2D TE sims, sides {sides[0]}..{sides[-1]} (seeded), 25 nm cells, "
                 f"binary 2x2 um designs, 10-cell PML, 2 DFT ports x 8 wavelengths, "
                 f"{'two steps per launch' if batch.engine.pairs else 'one launch per step'}")
-        if batch.engine.pairs:
+        if batch.engine.kstep:
+            K, R = batch.engine.kstep, 4 * batch.engine.krows
+            desc = desc.rsplit(", ", 1)[0] + f", {K} steps per pass"
+            kname = f"bz_yee2d_runk (fused, batched, {K} steps per pass, 64x{R} regions, {64 - 2 * K}x{R - 2 * K} inner)"
+        elif batch.engine.pairs:
             kname = f"bz_yee2d_run2 (fused, batched, two steps per launch, 60x{batch.engine.seg2} tiles)"
         else:
             kname = f"bz_yee2d_run (fused, batched, seg {batch.engine.seg}{', hipGraph' if a.graph else ''})"
@@ -308,9 +312,12 @@ def build_runner(a, total):
         r = Runner(batch.advance, sims[0].grid, cells, desc, kname, cpu)
         r.grids = [s.grid for s in sims]
         r.tuned = tuned
-        r.seg_key = (f"pairs-seg{batch.engine.seg2}" if batch.engine.pairs else f"seg{batch.engine.seg}") + \
-            ("-xcd" if batch.engine.xcd else "")
-        r.steps_per_launch = 2 if batch.engine.pairs else 1
+        if batch.engine.kstep:
+            r.seg_key = f"k{batch.engine.kstep}-rows{batch.engine.krows}"
+        else:
+            r.seg_key = (f"pairs-seg{batch.engine.seg2}" if batch.engine.pairs else f"seg{batch.engine.seg}")
+        r.seg_key += "-xcd" if batch.engine.xcd else ""
+        r.steps_per_launch = batch.engine.kstep or (2 if batch.engine.pairs else 1)
         return r
     if a.workload == "mmi3d":
         nx, ny, nz = a.nx or 480, a.ny or 240, a.nz
@@ -343,9 +350,10 @@ def build_runner(a, total):
     sim.grid.waves2 = a.waves2
     sim.grid.seg2d = a.seg2d
     sim.grid.graph2d = a.graph
+    sim.grid.kstep2d, sim.grid.krows2d = a.kstep, a.krows
     sim.initialize_simulation(save=False, live=False)
     r = Runner(sim._advance, sim.grid, cells, desc, sim.grid.kernel_name(), cpu)
-    r.steps_per_launch = 2 if (a.workload == "mmi3d" and a.tb2) else 1
+    r.steps_per_launch = 2 if (a.workload == "mmi3d" and a.tb2) else (a.kstep or 1)
     r.tuned = None
     return r
 
@@ -378,6 +386,8 @@ def main():
     ap.add_argument("--seg2d", type=int, default=None, help="rows per tile of the 2D engine (default: by size)")
     ap.add_argument("--xcd", type=int, default=1, help="2D engines: XCD-ordered tile lists (1) or row-major (0)")
     ap.add_argument("--pairs", type=int, default=-1, help="batch2d: two steps per launch (1), one (0), by size (-1)")
+    ap.add_argument("--kstep", type=int, default=0, help="2D engines: K steps per pass (0: off)")
+    ap.add_argument("--krows", type=int, default=16, help="2D K-step kernel: rows per wave band (8 or 16)")
     ap.add_argument("--graph", action="store_true", help="capture 2D runs as HIP graphs")
     ap.add_argument("--autotune", action="store_true", help="batch2d: pick the tile height by timing")
     ap.add_argument("--chunk", type=int, default=100, help="batch2d: steps per engine call")

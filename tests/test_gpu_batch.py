@@ -147,3 +147,21 @@ def test_pair_snapshots_equal_one_step_snapshots(hip, count):
     for fa, fb in zip(got[False][1], got[True][1]):
         for k in fa:
             assert np.array_equal(fa[k], fb[k]), k
+
+
+@pytest.mark.parametrize("K,rows,pairs", [(1, 8, False), (3, 16, False), (8, 16, True), (15, 8, False),
+                                          (31, 16, False)])
+def test_k_steps_per_pass_equal_single_steps(hip, K, rows, pairs):
+    """K steps per pass (register bands, LDS edge exchange, staged monitor samples) must give the bits of one
+    step per launch: 301 steps in runs of 75 (K-step launches, then pairs or single steps for the rest), inner
+    tiles from 62 x 62 down to 2 x 2 cells, uneven members, sources and DFT monitors."""
+    single = members(steps=301)
+    FDTDBatch([s for s, _ in single], pairs=False).run()
+    mem = members(steps=301)
+    b = FDTDBatch([s for s, _ in mem], pairs=pairs, kstep=K, krows=rows, chunk=75)
+    b.run()
+    assert b.engine.kstep == K and b.engine.pairs == pairs
+    for (s, m), (r, rm) in zip(mem, single):
+        A, B = state(s, m), state(r, rm)
+        for k in A:
+            assert np.array_equal(A[k], B[k]), (K, rows, s.nx, k)

@@ -193,6 +193,50 @@ def test_mode_source_2d_matches_oracle(hip):
         assert rel_l2(np.real(sim.Ez), ref.F["Ez"]) < 1e-4, enforce
 
 
+def _k_case(kind):
+    """A single 2D grid with one source and one line monitor (series and DFT): a Gaussian point source, or a
+    ModeSource line that zeroes its upstream cells."""
+    from beamz_amd.workloads import waveguide_2d
+    if kind == "gauss":
+        d, dx, t, mon = waveguide_2d(nx=150, ny=90, steps=203, npml=10, src_x=20, mon_x=120, n_freq=4)
+        return d, dx, t, mon
+    lam = 1.55 * um
+    d, dx = waveguide_design_2d(nx=140, ny=70, npml=10)
+    dt = 0.99 * dx / (bz.LIGHT_SPEED * np.sqrt(2))
+    t = np.arange(203) * dt
+    sig = bz.ramped_cosine(t, 1.0, bz.LIGHT_SPEED / lam, 0.0, ramp_duration=20 * dt, t_max=t[-1])
+    d += bz.ModeSource(design=d, start=(20 * dx, d.height / 2 - 1.0 * um), end=(20 * dx, d.height / 2 + 1.0 * um),
+                       wavelength=lam, signal=sig, num_modes=1, grid_resolution=400)
+    mon = bz.Monitor(design=d, start=(100 * dx, 10 * dx), end=(100 * dx, d.height - 10 * dx),
+                     dft_frequencies=bz.LIGHT_SPEED / (np.array([1.5, 1.55, 1.6]) * um), name="m")
+    d += mon
+    return d, dx, t, mon
+
+
+@pytest.mark.parametrize("K,rows", [(5, 16), (12, 8)])
+def test_k_steps_per_pass_single_grid(hip, K, rows):
+    """One grid through the K-steps-per-pass kernel gives the bits of one step per launch: fields, the
+    monitor's time series and its FP64 DFT, with a Gaussian point source and with a ModeSource line."""
+    for kind in ("gauss", "mode"):
+        out = []
+        for k in (0, K):
+            d, dx, t, mon = _k_case(kind)
+            sim = bz.FDTD(d, time=t, resolution=dx)
+            sim.grid.kstep2d, sim.grid.krows2d = k, rows
+            sim.run(live=False, save=False)
+            r = {f: np.real(getattr(sim, f)).copy() for f in ("Ez", "Hx", "Hy")}
+            for f in ("Ez", "Hx", "Hy"):
+                if f in mon.fields:
+                    r["series" + f] = np.array(mon.fields[f])
+            for f, v in mon.dft.items():
+                r["dft" + f] = np.array(v)
+            out.append(r)
+        assert sim.grid._f2d.kstep == K
+        assert any(f.startswith("series") for f in out[0]) and any(f.startswith("dft") for f in out[0])
+        for f in out[0]:
+            assert np.array_equal(out[0][f], out[1][f]), (kind, K, rows, f)
+
+
 def test_two_step_3d_kernel_bit_identical_to_single_steps(hip):
     """The two-step kernel (steps n, n+1 in one pass, LDS row exchange, staged monitor samples) must give
     the very bits of two single-step launches, for every rows/segment shape, on a grid whose sizes are not
