@@ -67,7 +67,7 @@ SIGNATURES = {
                        C.POINTER(f32), i32, vp, C.POINTER(i32), i32, i32, i32, vp],
     "bz_yee2d_run": [vp, i32, vp, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp],
     "bz_yee2d_run2": [vp, i32, vp, i32, i32, vp, i32, i32, vp, vp, i32, vp, vp, i32, vp, vp, vp],
-    "bz_yee2d_runk": [vp, i32, vp, i32, i32, i32, vp, i32, i32, vp, vp, vp, i32, vp, vp, i32, vp, vp, vp],
+    "bz_yee2d_runk": [vp, i32, vp, i32, i32, i32, i32, vp, i32, i32, vp, vp, vp, i32, vp, vp, i32, vp, vp, vp],
     "bz_adj2d_run": [vp, i32, vp, i32, i32, i32, i32, vp],
     "bz_adj2d_run2": [vp, i32, vp, i32, i32, vp, i32, i32, i32, i32, i32, vp],
     "bz_nn_conv3x3": [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp],

@@ -29,6 +29,8 @@
 // so the whole run can be captured once in a hipGraph and replayed.
 #include "common.h"
 
+#include <type_traits>
+
 // No multiply-add contraction: the one-step and two-step kernels then round identically, so a run gives
 // the same bits whichever mix of them it uses.
 #pragma clang fp contract(off)
@@ -430,9 +432,21 @@ int run(const bz_sim2d* sims, const int* tiles, int ntiles, int* step, int nstep
 // read only in-grid neighbours for every updated in-grid entry. Monitor samples of each step are copied
 // from the owning tile to stage[k][point] and accumulated into the FP64 DFT by k_gather2d_k in step order.
 // Reads buffer rd, writes rd ^ 1.
-constexpr int KW = 4;
+// Row neighbours by DPP wave shifts: lane l receives lane l+1 (shl) or l-1 (shr); the end lanes read 0,
+// which reaches only cells within K of the region's edge.
+__device__ __forceinline__ float from_right(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_left(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+}
+// Keeps a value computed on every lane, so that a per-lane choice below is a select, not a branch.
+__device__ __forceinline__ float computed(float x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
 
-template <int B>
+template <int B, int KW>
 __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restrict__ sims,
                                                         const int* __restrict__ tiles, int ntiles, int K,
                                                         const int* __restrict__ step, int q, int rd,
@@ -465,6 +479,18 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
     const float rdx = S.rdx, rdy = S.rdy;
     const bool hxok = i < nx - 1;
     const bool xin = i >= 1 && i <= nx - 2;
+    // the source box, held in registers (the step loop stores to memory, so fields read through S would be
+    // reloaded on every row)
+    const int sx0 = S.src_x0, sy0 = S.src_y0, ssx = S.src_sx, ssy = S.src_sy;
+    const float* __restrict__ sadd = S.src_add;
+    const float* __restrict__ smul = S.src_mul;
+    const float* __restrict__ sgl = S.signal;
+    const bool src_col = ssx > 0 && i >= sx0 && i < sx0 + ssx;
+    // does the source box meet this wave's rows and this region's columns (uniform over the wave)
+    const int rx0 = tx * (LANES - 2 * K) - K;
+    const bool src_band = ssx > 0 && jb < sy0 + ssy && jb + B > sy0 && rx0 < sx0 + ssx && rx0 + LANES > sx0;
+    // every row of the wave is an interior row of the grid (the usual case): no per-row edge rules
+    const bool inner_rows = jb >= 1 && jb + B - 1 <= ny - 2;
 
     float ez[B], hx[B], hy[B], fa[B], fb[B], ca[B], cb[B];
     {
@@ -490,33 +516,57 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
     const int e0 = ent_start ? ent_start[t] : 0, e1 = ent_start ? ent_start[t + 1] : 0;
     const int tid = w * LANES + l;
 
+    float amp_next = sgl ? sgl[n0] : 0.0f;
     for (int k = 0; k < K; ++k) {
+        const float amp = amp_next;              // the next step's amplitude loads during this step
+        if (sgl && k + 1 < K) amp_next = sgl[n0 + k + 1];
         // H of step n0+k+1: Hy of the band's last row needs Ez of the next band's first row
         ez_top[w][l] = ez[0];
         __syncthreads();
         const float ezb = w < KW - 1 ? ez_top[w + 1][l] : ez[B - 1];
+        auto h_rows = [&](auto check) {
+            constexpr bool C = decltype(check)::value;
 #pragma unroll
-        for (int r = 0; r < B; ++r) {
-            const int j = jb + r;
-            const float ezR = __shfl_down(ez[r], 1, LANES);
-            const float ed = r < B - 1 ? ez[r + 1] : ezb;
-            hx[r] = hxok ? fa[r] * hx[r] - fb[r] * ((ezR - ez[r]) * rdy) : hx[r];
-            hy[r] = (j < ny - 1) ? fa[r] * hy[r] + fb[r] * ((ed - ez[r]) * rdx) : hy[r];
-        }
+            for (int r = 0; r < B; ++r) {
+                const int j = jb + r;
+                const float ezR = from_right(ez[r]);
+                const float ed = r < B - 1 ? ez[r + 1] : ezb;
+                const float hn = computed(fa[r] * hx[r] - fb[r] * ((ezR - ez[r]) * rdy));
+                hx[r] = hxok ? hn : hx[r];
+                if (!C || j < ny - 1) hy[r] = fa[r] * hy[r] + fb[r] * ((ed - ez[r]) * rdx);
+            }
+        };
+        if (inner_rows) h_rows(std::false_type{});
+        else h_rows(std::true_type{});
         // E: Ez of the band's first row needs the new Hy of the previous band's last row
         hy_bot[w][l] = hy[B - 1];
         __syncthreads();
         const float hya = w > 0 ? hy_bot[w - 1][l] : hy[0];
-        const float amp = S.signal ? S.signal[n0 + k] : 0.0f;
+        auto e_rows = [&](auto check) {
+            constexpr bool C = decltype(check)::value;
 #pragma unroll
-        for (int r = 0; r < B; ++r) {
-            const int j = jb + r;
-            const float hxl = __shfl_up(hx[r], 1, LANES);
-            const float hu = r > 0 ? hy[r - 1] : hya;
-            float e = (j >= 1 && j <= ny - 2 && xin) ? ca[r] * ez[r] + cb[r] * (-((hx[r] - hxl) * rdy) + (hy[r] - hu) * rdx)
-                                                     : ez[r];
-            if (S.src_sx && j >= S.src_y0 && j < S.src_y0 + S.src_sy) box_src(e, S, i, j, amp);
-            ez[r] = e;
+            for (int r = 0; r < B; ++r) {
+                const int j = jb + r;
+                const float hxl = from_left(hx[r]);
+                const float hu = r > 0 ? hy[r - 1] : hya;
+                if (!C || (j >= 1 && j <= ny - 2)) {
+                    const float en = computed(ca[r] * ez[r] + cb[r] * (-((hx[r] - hxl) * rdy) + (hy[r] - hu) * rdx));
+                    ez[r] = xin ? en : ez[r];
+                }
+            }
+        };
+        if (inner_rows) e_rows(std::false_type{});
+        else e_rows(std::true_type{});
+        // the source box, after E (it touches only the Ez of its own cells)
+        if (src_band) {
+#pragma unroll
+            for (int r = 0; r < B; ++r) {
+                const int j = jb + r;
+                if (j >= sy0 && j < sy0 + ssy && src_col) {
+                    const long rr = (long)(j - sy0) * ssx + (i - sx0);
+                    ez[r] = (smul ? smul[rr] : 1.0f) * ez[r] + sadd[rr] * amp;
+                }
+            }
         }
         // monitor samples of step n0+k owned by this tile: entries sorted by component
         if (e1 > e0) {
@@ -558,12 +608,8 @@ __global__ void k_gather2d_k(Gather2D G, const float* __restrict__ stage, const 
     const int f = (int)(g / G.npts);
     if (f >= max(G.nfreq, 1)) return;
     const int n0 = *step + K * q;
-    double re = 0.0, im = 0.0;
     double* a = f < G.nfreq ? G.dft + 2 * ((long)f * G.npts + p) : nullptr;
-    if (a) {
-        re = a[0];
-        im = a[1];
-    }
+    double re = a ? a[0] : 0.0, im = a ? a[1] : 0.0;
     bool any = false;
     for (int k = 0; k < K; ++k) {
         const int row = G.slot[n0 + k];
@@ -584,14 +630,14 @@ __global__ void k_gather2d_k(Gather2D G, const float* __restrict__ stage, const 
     }
 }
 
-template <int B>
+template <int B, int KW>
 int runk(const bz_sim2d* sims, const int* tiles, int ntiles, int K, int* step, int rd0, int nlaunch,
          const int* ent_start, const int* ents, float* stage, const Gather2D& G, hipStream_t st) {
     const dim3 b(LANES, KW);
     const long gather_threads = (long)G.npts * (G.nfreq > 0 ? G.nfreq : 1);
     const int gather_blocks = (int)((gather_threads + 255) / 256);
     for (int q = 0; q < nlaunch; ++q) {
-        hipLaunchKernelGGL(k_yee2d_tbk<B>, dim3(ntiles), b, 0, st, sims, tiles, ntiles, K, step, q, (rd0 + q) & 1,
+        hipLaunchKernelGGL((k_yee2d_tbk<B, KW>), dim3(ntiles), b, 0, st, sims, tiles, ntiles, K, step, q, (rd0 + q) & 1,
                            ent_start, ents, stage, G.npts);
         BZ_CHECK_LAUNCH("bz_yee2d_runk(steps)");
         if (G.npts > 0) {
@@ -670,15 +716,17 @@ extern "C" int bz_yee2d_run2(const bz_sim2d* sims, int nsims, const int* tiles, 
     return BZ_OK;
 }
 
-extern "C" int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int rows, int ksteps,
-                             int* step, int rd0, int nlaunch, const int* ent_start, const int* ents, float* stage,
-                             int npts, const int* slot, float* series, int nfreq, const double* phasor, double* dft,
-                             void* stream) {
+extern "C" int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int rows, int waves,
+                             int ksteps, int* step, int rd0, int nlaunch, const int* ent_start, const int* ents,
+                             float* stage, int npts, const int* slot, float* series, int nfreq, const double* phasor,
+                             double* dft, void* stream) {
     BZ_REQUIRE(sims && nsims > 0 && tiles && ntiles > 0 && step && nlaunch >= 0 && (rd0 == 0 || rd0 == 1),
                "bz_yee2d_runk: bad arguments");
-    BZ_REQUIRE(rows == 8 || rows == 16, "bz_yee2d_runk: rows must be 8 or 16");
-    BZ_REQUIRE(ksteps >= 1 && 2 * ksteps < LANES && 2 * ksteps < KW * rows,
-               "bz_yee2d_runk: ksteps must leave inner cells (2 * ksteps < 64 and < 4 * rows)");
+    const int shape = rows * 100 + waves;
+    BZ_REQUIRE(shape == 804 || shape == 808 || shape == 816 || shape == 408 || shape == 416 || shape == 1604,
+               "bz_yee2d_runk: (rows, waves) must be (8, 4), (8, 8), (8, 16), (4, 8), (4, 16) or (16, 4)");
+    BZ_REQUIRE(ksteps >= 1 && 2 * ksteps < LANES && 2 * ksteps < waves * rows,
+               "bz_yee2d_runk: ksteps must leave inner cells (2 * ksteps < 64 and < rows * waves)");
     BZ_REQUIRE(npts >= 0 && (npts == 0 || (ent_start && ents && stage && slot)),
                "bz_yee2d_runk: monitor points without tables");
     BZ_REQUIRE(nfreq >= 0 && (nfreq == 0 || (phasor && dft)), "bz_yee2d_runk: DFT requested without buffers");
@@ -695,6 +743,14 @@ extern "C" int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, 
     G.dft = dft;
     const hipStream_t st = (hipStream_t)stream;
     if (npts == 0) ent_start = ents = nullptr;
-    return rows == 8 ? runk<8>(sims, tiles, ntiles, ksteps, step, rd0, nlaunch, ent_start, ents, stage, G, st)
-                     : runk<16>(sims, tiles, ntiles, ksteps, step, rd0, nlaunch, ent_start, ents, stage, G, st);
+#define BZ_RUNK(R, W) runk<R, W>(sims, tiles, ntiles, ksteps, step, rd0, nlaunch, ent_start, ents, stage, G, st)
+    switch (shape) {
+        case 804: return BZ_RUNK(8, 4);
+        case 808: return BZ_RUNK(8, 8);
+        case 816: return BZ_RUNK(8, 16);
+        case 408: return BZ_RUNK(4, 8);
+        case 416: return BZ_RUNK(4, 16);
+        default: return BZ_RUNK(16, 4);
+    }
+#undef BZ_RUNK
 }

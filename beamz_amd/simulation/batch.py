@@ -25,7 +25,8 @@ from .engine import Fused2D
 class FDTDBatch:
     SEGS = Fused2D.SEGS
 
-    def __init__(self, sims, seg=None, graph=False, chunk=100, pairs=None, seg2=None, kstep=None, krows=16):
+    def __init__(self, sims, seg=None, graph=False, chunk=96, pairs=None, seg2=None, kstep=None, krows=None,
+                 kwaves=None):
         if not sims:
             raise ValueError("FDTDBatch needs at least one simulation")
         for s in sims:
@@ -38,7 +39,7 @@ class FDTDBatch:
         self.sims = list(sims)
         self.seg, self.graph, self.chunk = seg, bool(graph), int(chunk)
         self.pairs, self.seg2 = pairs, seg2      # pairs None: the engine decides by batch size
-        self.kstep, self.krows = kstep, krows    # K steps per pass (None or 0: off)
+        self.kstep, self.krows, self.kwaves = kstep, krows, kwaves    # K steps per pass (0: off, None: auto)
         self.num_steps = len(t0)
         self.current_step = 0
         self.engine = None
@@ -51,7 +52,8 @@ class FDTDBatch:
         for s in self.sims:
             s.initialize_simulation(save=False, live=False)
         self.engine = Fused2D([s.grid for s in self.sims], seg=self.seg, graph=self.graph, max_steps=self.num_steps,
-                              pairs=self.pairs, seg2=self.seg2, kstep=self.kstep, krows=self.krows)
+                              pairs=self.pairs, seg2=self.seg2, kstep=self.kstep, krows=self.krows,
+                              kwaves=self.kwaves)
         self.seg = self.engine.seg
         for s in self.sims:
             s.grid._f2d = False       # members advance through the batch engine only
@@ -97,11 +99,17 @@ class FDTDBatch:
         self.sims[0].synchronize()
 
     def autotune(self, steps=20, segs=None):
-        """Pick the tile height with the lowest measured step time (the two-step kernel's tiles when the
-        engine runs pairs, else the one-step kernel's); leaves the batch re-initialised."""
+        """Pick the kernel shape with the lowest measured step time: (K, rows, waves) when the engine runs K
+        steps per pass, else the tile height (the two-step kernel's tiles when it runs pairs, else the one-step
+        kernel's); leaves the batch re-initialised."""
         T = self.sims[0].backend.torch
         if self.engine is None:
             self.initialize()
+        if self.engine.kstep:
+            best, timings = self.engine.tune_kstep(steps=steps, candidates=segs)
+            self.kstep, self.krows, self.kwaves = best
+            self.initialize()
+            return best, timings
         pairs = self.engine.pairs
         steps += steps % 2
         best = None

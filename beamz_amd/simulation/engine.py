@@ -98,8 +98,9 @@ class Grid:
         self._f2d = None
         self.seg2d = None       # rows per tile of the fused 2D engine (None: by grid size)
         self.graph2d = False    # capture fused 2D runs as HIP graphs
-        self.kstep2d = 0        # fused 2D engine: K steps per pass (0: off)
-        self.krows2d = 16       # rows per wave band of the K-step kernel (8 or 16)
+        self.kstep2d = None     # fused 2D engine: K steps per pass (0: off; None: automatic)
+        self.krows2d = None     # K-step kernel: rows per wave (None: automatic)
+        self.kwaves2d = None    # K-step kernel: waves per workgroup (None: automatic)
         self.kseg = 16
         self.rows = 4
         self.tb2 = True         # 3D: two steps per launch where the step list allows it
@@ -124,8 +125,9 @@ class Grid:
             return f"bz_yee3d_step (fused z-march, {self.rows} rows/wave)" if self.kernel == "fused" else "bz_update_3d (two-pass)"
         if self.kernel != "fused":
             return "bz_update_h_2d + bz_update_e_2d"
-        if self.kstep2d:
-            k, r = int(self.kstep2d), 4 * int(self.krows2d)
+        f = self._f2d
+        if f and f.kstep:
+            k, r = f.kstep, f.krows * f.kwaves
             return f"bz_yee2d_runk ({k} steps per pass, 64x{r} regions, {64 - 2 * k}x{r - 2 * k} inner)"
         return "bz_yee2d_run (fused, batched)"
 
@@ -484,7 +486,7 @@ class Grid:
         if self._f2d is None:
             try:
                 self._f2d = Fused2D([self], seg=self.seg2d, graph=self.graph2d, kstep=self.kstep2d,
-                                    krows=self.krows2d)
+                                    krows=self.krows2d, kwaves=self.kwaves2d)
             except NotImplementedError:
                 self._f2d = False
         if self._f2d is False:
@@ -532,13 +534,14 @@ class Fused2D:
     XCD_ORDER = True        # tile lists in xcd_order (False: plain row-major order)
 
     def __init__(self, grids, seg=8, graph=False, max_steps=None, monitors=True, snapshots=None, pairs=None,
-                 seg2=None, xcd=None, kstep=None, krows=16):
+                 seg2=None, xcd=None, kstep=None, krows=None, kwaves=None):
         """monitors=False runs the steps without sampling any monitor (a recomputation);
         snapshots: per grid None or (x0, y0, sx, sy, tensor [steps][3][sy][sx]) for the snapshot box;
         pairs: advance two steps per launch (bz_yee2d_run2) where a run allows it (None: from PAIR_MIN_CELLS
         cells on; never with graphs), with tiles of 60 columns x seg2 rows (None: by size); xcd: order the tile
         lists by xcd_order (None: XCD_ORDER); kstep: advance K steps per pass (bz_yee2d_runk, regions of 64 x
-        4*krows cells) for runs of at least K steps, the rest by pairs or single steps (None or 0: off)."""
+        krows*kwaves cells) for runs of at least K steps, the rest by pairs or single steps (0: off; None: on
+        with auto_kstep's shape wherever it applies: no graphs, no grid power, no snapshots)."""
         if not grids:
             raise ValueError("Fused2D needs at least one grid")
         self.snapshots = list(snapshots) if snapshots is not None else [None] * len(grids)
@@ -611,10 +614,13 @@ class Fused2D:
                 self._setup_staging(pts)
             self.set_seg2(seg2)
         self._pts_host = pts if plans else None
-        self.kstep, self.krows = 0, 16
+        self.kstep, self.krows, self.kwaves = 0, 8, 8
         self.tilesk = self.ent_start = self.ents = self.stagek = None
+        if kstep is None and self.kstep_applies():
+            kstep, r, w = self.auto_kstep()
+            krows, kwaves = krows or r, kwaves or w
         if kstep:
-            self.set_kstep(kstep, krows)
+            self.set_kstep(kstep, krows or 8, kwaves or 8)
 
     def _setup_staging(self, pts):
         """Per simulation, a staging box around its monitor points for the first step of each pair."""
@@ -693,28 +699,65 @@ class Fused2D:
         self.tiles2 = T.from_numpy(tiles.ravel()).to(self.dev)
         self.ntiles2 = len(tiles)
 
-    KROWS = (8, 16)
+    KSHAPES = ((8, 4), (8, 8), (8, 16), (4, 8), (4, 16), (16, 4))     # (rows per wave, waves per workgroup)
+    # (K, rows, waves) candidates of tune_kstep; the automatic shapes were the fastest measured on 64 members
+    # of 256..1024 cells a side (30 M cells: 0.059 ms/step for (8, 8, 8)) and on one 800 x 400 grid
+    # (2.95 us/step for (8, 4, 8))
+    KCANDIDATES = ((8, 8, 8), (6, 8, 8), (8, 8, 4), (8, 4, 8), (12, 4, 16))
 
-    def set_kstep(self, K, rows=16):
+    def kstep_applies(self):
+        return (not self.use_graph and all(g.power is None for g in self.grids)
+                and all(sn is None for sn in self.snapshots))
+
+    def auto_kstep(self):
+        """(K, rows, waves): tall 64 x 64 regions from PAIR_MIN_CELLS cells on, else 64 x 32 regions, which
+        give a small grid more workgroups."""
+        big = sum(g.nx * g.ny for g in self.grids) >= self.PAIR_MIN_CELLS
+        return (8, 8, 8) if big else (8, 4, 8)
+
+    def tune_kstep(self, n0=0, steps=16, candidates=None):
+        """Time every (K, rows, waves) of `candidates` (default KCANDIDATES) on about `steps` steps from step
+        n0 and keep the fastest; returns (best, {shape: ms per step}). Advances the fields and the monitor sums
+        (callers reset them afterwards)."""
+        T = self.torch
+        if not self.kstep:
+            raise ValueError("tune_kstep needs an engine that runs K steps per pass")
+        timings = {}
+        for shape in candidates or self.KCANDIDATES:
+            K, rows, waves = shape
+            self.set_kstep(K, rows, waves)
+            n = max(1, steps // K) * K
+            self.run(n0, K)                                   # warm-up
+            e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
+            e0.record()
+            self.run(n0 + K, n)
+            e1.record()
+            T.cuda.synchronize(self.dev)
+            timings[shape] = e0.elapsed_time(e1) / n
+        best = min(timings, key=timings.get)
+        self.set_kstep(*best)
+        return best, timings
+
+    def set_kstep(self, K, rows=8, waves=8):
         """Tile list, monitor-owner tables and sample staging of the K-steps-per-pass kernel (K 0: off).
 
-        A tile's region is 64 columns x 4*rows rows; its inner (64-2K) x (4*rows-2K) cells tile the grid, and
-        each monitor point is sampled by the tile whose inner cells hold it."""
+        A tile's region is 64 columns x rows*waves rows; its inner (64-2K) x (rows*waves-2K) cells tile the
+        grid, and each monitor point is sampled by the tile whose inner cells hold it."""
         T = self.torch
         K = int(K or 0)
         if not K:
             self.kstep = 0
             return
-        rows = int(rows)
-        if rows not in self.KROWS or not 2 * K < min(64, 4 * rows):
-            raise ValueError(f"K steps per pass need rows in {self.KROWS} and 2K < min(64, 4 rows)")
+        rows, waves = int(rows), int(waves)
+        if (rows, waves) not in self.KSHAPES or not 2 * K < min(64, rows * waves):
+            raise ValueError(f"K steps per pass need (rows, waves) in {self.KSHAPES} and 2K < min(64, rows*waves)")
         if any(g.power is not None for g in self.grids):
             raise ValueError("K steps per pass do not accumulate grid power")
         if any(sn is not None for sn in self.snapshots):
             raise ValueError("K steps per pass do not record snapshots")
         if self.use_graph:
             raise ValueError("K steps per pass are not captured in graphs")
-        ix, iy = 64 - 2 * K, 4 * rows - 2 * K
+        ix, iy = 64 - 2 * K, rows * waves - 2 * K
         parts = []
         for s, g in enumerate(self.grids):
             gx, gy = -(-g.nx // ix), -(-g.ny // iy)
@@ -723,7 +766,7 @@ class Fused2D:
         tiles = np.concatenate(parts).astype(np.int32)
         if self.xcd:
             tiles = xcd_order(tiles, per_wg=1)
-        self.kstep, self.krows = K, rows
+        self.kstep, self.krows, self.kwaves = K, rows, waves
         self.tilesk = T.from_numpy(tiles.ravel()).to(self.dev)
         self.ntilesk = len(tiles)
         self.ent_start = self.ents = self.stagek = None
@@ -877,7 +920,7 @@ class Fused2D:
             nl = count // K
             ptr = lambda t: t.data_ptr() if t is not None else None
             N.call("bz_yee2d_runk", self.desc_dev.data_ptr(), len(self.grids), self.tilesk.data_ptr(), self.ntilesk,
-                   self.krows, K, self.step.data_ptr(), n0 & 1, nl, ptr(self.ent_start), ptr(self.ents),
+                   self.krows, self.kwaves, K, self.step.data_ptr(), n0 & 1, nl, ptr(self.ent_start), ptr(self.ents),
                    ptr(self.stagek), self.npts, ptr(self.slot), ptr(self.series), self.nfreq, ptr(self.phasor),
                    ptr(self.dft), g0.stream)
             for g in self.grids:

@@ -286,7 +286,8 @@ def build_runner(a, total):
         members = batch_2d(nsims=a.nsims, steps=max(total + 4, 16))
         sims = [bz.FDTD(m, time=t) for m, t, _ in members]
         batch = FDTDBatch(sims, seg=a.seg2d, graph=a.graph, chunk=a.chunk,
-                          pairs=None if a.pairs < 0 else bool(a.pairs), kstep=a.kstep, krows=a.krows)
+                          pairs=None if a.pairs < 0 else bool(a.pairs),
+                          kstep=None if a.kstep < 0 else a.kstep, krows=a.krows, kwaves=a.kwaves)
         batch.initialize()
         tuned = None
         if a.autotune:
@@ -297,7 +298,7 @@ def build_runner(a, total):
                 f"binary 2x2 um designs, 10-cell PML, 2 DFT ports x 8 wavelengths, "
                 f"{'two steps per launch' if batch.engine.pairs else 'one launch per step'}")
         if batch.engine.kstep:
-            K, R = batch.engine.kstep, 4 * batch.engine.krows
+            K, R = batch.engine.kstep, batch.engine.krows * batch.engine.kwaves
             desc = desc.rsplit(", ", 1)[0] + f", {K} steps per pass"
             kname = f"bz_yee2d_runk (fused, batched, {K} steps per pass, 64x{R} regions, {64 - 2 * K}x{R - 2 * K} inner)"
         elif batch.engine.pairs:
@@ -313,7 +314,7 @@ def build_runner(a, total):
         r.grids = [s.grid for s in sims]
         r.tuned = tuned
         if batch.engine.kstep:
-            r.seg_key = f"k{batch.engine.kstep}-rows{batch.engine.krows}"
+            r.seg_key = f"k{batch.engine.kstep}-{batch.engine.krows}x{batch.engine.kwaves}"
         else:
             r.seg_key = (f"pairs-seg{batch.engine.seg2}" if batch.engine.pairs else f"seg{batch.engine.seg}")
         r.seg_key += "-xcd" if batch.engine.xcd else ""
@@ -350,10 +351,11 @@ def build_runner(a, total):
     sim.grid.waves2 = a.waves2
     sim.grid.seg2d = a.seg2d
     sim.grid.graph2d = a.graph
-    sim.grid.kstep2d, sim.grid.krows2d = a.kstep, a.krows
+    sim.grid.kstep2d = None if a.kstep < 0 else a.kstep
+    sim.grid.krows2d, sim.grid.kwaves2d = a.krows, a.kwaves
     sim.initialize_simulation(save=False, live=False)
     r = Runner(sim._advance, sim.grid, cells, desc, sim.grid.kernel_name(), cpu)
-    r.steps_per_launch = 2 if (a.workload == "mmi3d" and a.tb2) else (a.kstep or 1)
+    r.steps_per_launch = 2 if (a.workload == "mmi3d" and a.tb2) else None      # 2D: set once the engine exists
     r.tuned = None
     return r
 
@@ -386,12 +388,15 @@ def main():
     ap.add_argument("--seg2d", type=int, default=None, help="rows per tile of the 2D engine (default: by size)")
     ap.add_argument("--xcd", type=int, default=1, help="2D engines: XCD-ordered tile lists (1) or row-major (0)")
     ap.add_argument("--pairs", type=int, default=-1, help="batch2d: two steps per launch (1), one (0), by size (-1)")
-    ap.add_argument("--kstep", type=int, default=0, help="2D engines: K steps per pass (0: off)")
-    ap.add_argument("--krows", type=int, default=16, help="2D K-step kernel: rows per wave band (8 or 16)")
+    ap.add_argument("--kstep", type=int, default=-1, help="2D engines: K steps per pass (0: off, -1: automatic)")
+    ap.add_argument("--krows", type=int, default=None, help="2D K-step kernel: rows per wave (default: automatic)")
+    ap.add_argument("--kwaves", type=int, default=None,
+                    help="2D K-step kernel: waves per workgroup (default: automatic)")
     ap.add_argument("--graph", action="store_true", help="capture 2D runs as HIP graphs")
     ap.add_argument("--autotune", action="store_true", help="batch2d: pick the tile height by timing")
-    ap.add_argument("--chunk", type=int, default=100, help="batch2d: steps per engine call")
-    ap.add_argument("--timing-steps", type=int, default=30)
+    ap.add_argument("--chunk", type=int, default=96, help="batch2d: steps per engine call")
+    ap.add_argument("--timing-steps", type=int, default=48,
+                    help="steps timed per launch with HIP events (a multiple of the steps per launch)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (3 launches) as a counter calibration kernel")
     a = ap.parse_args()
@@ -424,6 +429,10 @@ def main():
         del src, dst
     r.advance(a.warmup)
     torch.cuda.synchronize()
+    if a.workload == "wg2d":                  # the 2D engine is built on the first step
+        f = r.grid._f2d
+        r.kernel_name = r.grid.kernel_name()
+        r.steps_per_launch = f.kstep if f and f.kstep else 1
     stream = torch.cuda.current_stream()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()

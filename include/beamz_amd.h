@@ -158,14 +158,16 @@ int bz_yee2d_run2(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles,
 
 /* K steps per pass: nlaunch launches of ksteps steps of the bz_yee2d_run step each (same results, bit for
  * bit), launch q reading buffer (rd0 + q) & 1 and writing the other; the step counter advances by
- * ksteps * nlaunch. A tile is a region of 64 columns x 4*rows rows (rows 8 or 16) advanced ksteps steps in
- * registers; its inner (64 - 2 ksteps) x (4 rows - 2 ksteps) cells are stored. tiles: (sim, x-tile, y-tile)
+ * ksteps * nlaunch. A tile is a region of 64 columns x (waves * rows) rows, one workgroup of `waves` waves
+ * each holding `rows` rows in registers, (rows, waves) one of (8, 4), (8, 8), (8, 16), (4, 8), (4, 16),
+ * (16, 4); the region advances ksteps steps and its inner (64 - 2 ksteps) x (waves * rows - 2 ksteps)
+ * cells are stored. tiles: (sim, x-tile, y-tile)
  * in units of the inner size, one tile per workgroup (sim < 0: padding). Monitor point p is sampled by the
  * tile whose inner cells hold it: ent_start[t] .. ent_start[t+1]-1 index ents[2e] = p,
  * ents[2e+1] = (region row << 10) | (region column << 2) | comp, sorted by comp within a tile;
  * stage: device FP32 [ksteps][npts]. No grid power and no snapshots (use bz_yee2d_run / run2). */
-int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int rows, int ksteps, int* step,
-                  int rd0, int nlaunch, const int* ent_start, const int* ents, float* stage, int npts,
+int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int rows, int waves, int ksteps,
+                  int* step, int rd0, int nlaunch, const int* ent_start, const int* ents, float* stage, int npts,
                   const int* slot, float* series, int nfreq, const double* phasor, double* dft, void* stream);
 
 /* ---------------- reverse-time 2D adjoint ----------------

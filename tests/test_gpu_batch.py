@@ -41,6 +41,7 @@ def state(sim, mons):
 def test_batch_equals_single_runs_and_oracle(hip):
     single = []
     for sim, mons in members():
+        sim.grid.kstep2d = 0                 # alone: one step per launch; in the batch: K steps per pass
         sim.run(live=False, save=False)
         single.append(state(sim, mons))
     mem = members()
@@ -68,7 +69,7 @@ def test_batch_graph_replays_equal_eager(hip):
     torch = hip
     a = members(steps=320)
     b = members(steps=320)
-    eager = FDTDBatch([s for s, _ in a], seg=8, graph=False, chunk=40)
+    eager = FDTDBatch([s for s, _ in a], seg=8, graph=False, chunk=40, kstep=0)
     graph = FDTDBatch([s for s, _ in b], seg=8, graph=True, chunk=40)
     eager.initialize()
     graph.initialize()
@@ -88,13 +89,18 @@ def test_batch_graph_replays_equal_eager(hip):
             assert np.array_equal(A[k], B[k]), k
 
 
-def test_batch_autotune_keeps_results(hip):
+@pytest.mark.parametrize("kstep", [0, None])
+def test_batch_autotune_keeps_results(hip, kstep):
     ref_mem = members(steps=200)
-    FDTDBatch([s for s, _ in ref_mem], seg=8).run()
+    FDTDBatch([s for s, _ in ref_mem], seg=8, kstep=0).run()
     mem = members(steps=200)
-    batch = FDTDBatch([s for s, _ in mem], seg=8)
+    batch = FDTDBatch([s for s, _ in mem], seg=8, kstep=kstep)
     best, timings = batch.autotune(steps=10)
-    segs = batch.engine.SEGS2 if batch.engine.pairs else FDTDBatch.SEGS
+    if kstep is None:
+        segs = batch.engine.KCANDIDATES
+        assert (batch.engine.kstep, batch.engine.krows, batch.engine.kwaves) == best
+    else:
+        segs = batch.engine.SEGS2 if batch.engine.pairs else FDTDBatch.SEGS
     assert best in segs and set(timings) == set(segs)
     batch.advance(batch.num_steps)
     for (s, m), (r, rm) in zip(mem, ref_mem):
@@ -109,9 +115,9 @@ def test_two_step_pairs_equal_single_steps(hip, seg2):
     """Two steps per launch (staged intermediate monitor samples) must give the bits of one step per launch,
     with an odd step count (pairs plus a last single step) and any pair-tile height."""
     single = members(steps=301)
-    FDTDBatch([s for s, _ in single], pairs=False).run()
+    FDTDBatch([s for s, _ in single], pairs=False, kstep=0).run()
     pair = members(steps=301)
-    b = FDTDBatch([s for s, _ in pair], pairs=True, seg2=seg2, chunk=75)
+    b = FDTDBatch([s for s, _ in pair], pairs=True, seg2=seg2, chunk=75, kstep=0)
     b.run()
     assert b.engine.pairs
     for (s, m), (r, rm) in zip(pair, single):
@@ -149,19 +155,20 @@ def test_pair_snapshots_equal_one_step_snapshots(hip, count):
             assert np.array_equal(fa[k], fb[k]), k
 
 
-@pytest.mark.parametrize("K,rows,pairs", [(1, 8, False), (3, 16, False), (8, 16, True), (15, 8, False),
-                                          (31, 16, False)])
-def test_k_steps_per_pass_equal_single_steps(hip, K, rows, pairs):
+@pytest.mark.parametrize("K,rows,waves,pairs", [(1, 8, 4, False), (3, 16, 4, False), (8, 8, 8, True),
+                                                (15, 8, 4, False), (31, 16, 4, False), (6, 4, 8, False),
+                                                (12, 4, 16, False), (31, 8, 16, False)])
+def test_k_steps_per_pass_equal_single_steps(hip, K, rows, waves, pairs):
     """K steps per pass (register bands, LDS edge exchange, staged monitor samples) must give the bits of one
     step per launch: 301 steps in runs of 75 (K-step launches, then pairs or single steps for the rest), inner
     tiles from 62 x 62 down to 2 x 2 cells, uneven members, sources and DFT monitors."""
     single = members(steps=301)
-    FDTDBatch([s for s, _ in single], pairs=False).run()
+    FDTDBatch([s for s, _ in single], pairs=False, kstep=0).run()
     mem = members(steps=301)
-    b = FDTDBatch([s for s, _ in mem], pairs=pairs, kstep=K, krows=rows, chunk=75)
+    b = FDTDBatch([s for s, _ in mem], pairs=pairs, kstep=K, krows=rows, kwaves=waves, chunk=75)
     b.run()
     assert b.engine.kstep == K and b.engine.pairs == pairs
     for (s, m), (r, rm) in zip(mem, single):
         A, B = state(s, m), state(r, rm)
         for k in A:
-            assert np.array_equal(A[k], B[k]), (K, rows, s.nx, k)
+            assert np.array_equal(A[k], B[k]), (K, rows, waves, s.nx, k)

@@ -213,8 +213,8 @@ def _k_case(kind):
     return d, dx, t, mon
 
 
-@pytest.mark.parametrize("K,rows", [(5, 16), (12, 8)])
-def test_k_steps_per_pass_single_grid(hip, K, rows):
+@pytest.mark.parametrize("K,rows,waves", [(5, 16, 4), (12, 8, 4), (8, 8, 8)])
+def test_k_steps_per_pass_single_grid(hip, K, rows, waves):
     """One grid through the K-steps-per-pass kernel gives the bits of one step per launch: fields, the
     monitor's time series and its FP64 DFT, with a Gaussian point source and with a ModeSource line."""
     for kind in ("gauss", "mode"):
@@ -222,7 +222,7 @@ def test_k_steps_per_pass_single_grid(hip, K, rows):
         for k in (0, K):
             d, dx, t, mon = _k_case(kind)
             sim = bz.FDTD(d, time=t, resolution=dx)
-            sim.grid.kstep2d, sim.grid.krows2d = k, rows
+            sim.grid.kstep2d, sim.grid.krows2d, sim.grid.kwaves2d = k, rows, waves
             sim.run(live=False, save=False)
             r = {f: np.real(getattr(sim, f)).copy() for f in ("Ez", "Hx", "Hy")}
             for f in ("Ez", "Hx", "Hy"):
