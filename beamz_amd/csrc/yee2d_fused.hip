@@ -430,7 +430,8 @@ int run(const bz_sim2d* sims, const int* tiles, int ntiles, int* step, int nstep
 // Neighbours along a row come from wave shuffles, neighbours across band edges through LDS (two barriers
 // per step). Cells outside the grid compute garbage that never reaches a cell inside it: the update rules
 // read only in-grid neighbours for every updated in-grid entry. Monitor samples of each step are copied
-// from the owning tile to stage[k][point] and accumulated into the FP64 DFT by k_gather2d_k in step order.
+// from the owning tile to staging slot q & 1 of launch q, stage[q & 1][k][point], and accumulated into the
+// FP64 DFT in step order by extra workgroups of the next launch (by k_gather2d_k after the last one).
 // Reads buffer rd, writes rd ^ 1.
 // Row neighbours by DPP wave shifts: lane l receives lane l+1 (shl) or l-1 (shr); the end lanes read 0,
 // which reaches only cells within K of the region's edge.
@@ -446,19 +447,60 @@ __device__ __forceinline__ float computed(float x) {
     return x;
 }
 
+// Monitor samples of the K steps of launch q from stage[k][p]: thread g handles (point p, frequency f), and
+// each accumulator receives the K terms in step order, one rounded add each, as with one step per launch.
+__device__ __forceinline__ void gather_k(const Gather2D& G, const float* __restrict__ stage, int n0, int K, long g) {
+    const int p = (int)(g % G.npts);
+    const int f = (int)(g / G.npts);
+    if (f >= max(G.nfreq, 1)) return;
+    double* a = f < G.nfreq ? G.dft + 2 * ((long)f * G.npts + p) : nullptr;
+    double re = a ? a[0] : 0.0, im = a ? a[1] : 0.0;
+    bool any = false;
+    for (int k = 0; k < K; ++k) {
+        const int row = G.slot[n0 + k];
+        if (row < 0) continue;
+        any = true;
+        const float v = stage[(long)k * G.npts + p];
+        if (f == 0 && G.series) G.series[(long)row * G.npts + p] = v;
+        if (a) {
+            const double dv = v;
+            const double* ph = G.phasor + 2 * ((long)row * G.nfreq + f);
+            re += dv * ph[0];
+            im += dv * ph[1];
+        }
+    }
+    if (a && any) {
+        a[0] = re;
+        a[1] = im;
+    }
+}
+
+// The gather of the last launch of a call (the others run inside the next launch).
+__global__ void k_gather2d_k(Gather2D G, const float* __restrict__ stage, const int* __restrict__ step, int q, int K) {
+    gather_k(G, stage, *step + K * q, K, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 template <int B, int KW>
 __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restrict__ sims,
                                                         const int* __restrict__ tiles, int ntiles, int K,
                                                         const int* __restrict__ step, int q, int rd,
                                                         const int* __restrict__ ent_start,
-                                                        const int* __restrict__ ents, float* __restrict__ stage,
-                                                        int npts) {
+                                                        const int* __restrict__ ents, float* __restrict__ stage_all,
+                                                        Gather2D G) {
     constexpr int RH = KW * B;
     __shared__ float ez_top[KW][LANES];
     __shared__ float hy_bot[KW][LANES];
     __shared__ float smp[RH][LANES];
+    const int npts = G.npts;
     const int t = blockIdx.x;
-    if (t >= ntiles) return;
+    if (t >= ntiles) {
+        // monitor samples of the previous launch (q > 0): staging slot (q - 1) & 1, which this launch does not
+        // write
+        gather_k(G, stage_all + (long)((q - 1) & 1) * K * npts, *step + K * (q - 1), K,
+                 (long)(t - ntiles) * (LANES * KW) + threadIdx.y * LANES + threadIdx.x);
+        return;
+    }
+    float* __restrict__ stage = stage_all + (long)(q & 1) * K * npts;
     const int s = tiles[3 * t];
     if (s < 0) return;   // padding entry of a reordered tile list (uniform over the workgroup)
     const bz_sim2d& S = sims[s];
@@ -600,50 +642,24 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
     }
 }
 
-// Monitor samples of the K steps of launch q from stage[k][p]: thread g handles (point p, frequency f), and
-// each accumulator receives the K terms in step order, one rounded add each, as with one step per launch.
-__global__ void k_gather2d_k(Gather2D G, const float* __restrict__ stage, const int* __restrict__ step, int q, int K) {
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int p = (int)(g % G.npts);
-    const int f = (int)(g / G.npts);
-    if (f >= max(G.nfreq, 1)) return;
-    const int n0 = *step + K * q;
-    double* a = f < G.nfreq ? G.dft + 2 * ((long)f * G.npts + p) : nullptr;
-    double re = a ? a[0] : 0.0, im = a ? a[1] : 0.0;
-    bool any = false;
-    for (int k = 0; k < K; ++k) {
-        const int row = G.slot[n0 + k];
-        if (row < 0) continue;
-        any = true;
-        const float v = stage[(long)k * G.npts + p];
-        if (f == 0 && G.series) G.series[(long)row * G.npts + p] = v;
-        if (a) {
-            const double dv = v;
-            const double* ph = G.phasor + 2 * ((long)row * G.nfreq + f);
-            re += dv * ph[0];
-            im += dv * ph[1];
-        }
-    }
-    if (a && any) {
-        a[0] = re;
-        a[1] = im;
-    }
-}
-
 template <int B, int KW>
 int runk(const bz_sim2d* sims, const int* tiles, int ntiles, int K, int* step, int rd0, int nlaunch,
          const int* ent_start, const int* ents, float* stage, const Gather2D& G, hipStream_t st) {
     const dim3 b(LANES, KW);
     const long gather_threads = (long)G.npts * (G.nfreq > 0 ? G.nfreq : 1);
-    const int gather_blocks = (int)((gather_threads + 255) / 256);
+    const int gather_blocks = (int)((gather_threads + LANES * KW - 1) / (LANES * KW));
+    // the gather of launch q-1 runs as extra workgroups of launch q; the last launch's gets a launch of its own
     for (int q = 0; q < nlaunch; ++q) {
-        hipLaunchKernelGGL((k_yee2d_tbk<B, KW>), dim3(ntiles), b, 0, st, sims, tiles, ntiles, K, step, q, (rd0 + q) & 1,
-                           ent_start, ents, stage, G.npts);
+        const dim3 g(ntiles + (q > 0 && G.npts > 0 ? gather_blocks : 0));
+        hipLaunchKernelGGL((k_yee2d_tbk<B, KW>), g, b, 0, st, sims, tiles, ntiles, K, step, q, (rd0 + q) & 1,
+                           ent_start, ents, stage, G);
         BZ_CHECK_LAUNCH("bz_yee2d_runk(steps)");
-        if (G.npts > 0) {
-            hipLaunchKernelGGL(k_gather2d_k, dim3(gather_blocks), dim3(256), 0, st, G, stage, step, q, K);
-            BZ_CHECK_LAUNCH("bz_yee2d_runk(gather)");
-        }
+    }
+    if (G.npts > 0) {
+        const int q = nlaunch - 1;
+        hipLaunchKernelGGL(k_gather2d_k, dim3((int)((gather_threads + 255) / 256)), dim3(256), 0, st, G,
+                           stage + (long)(q & 1) * K * G.npts, step, q, K);
+        BZ_CHECK_LAUNCH("bz_yee2d_runk(gather)");
     }
     hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, step, K * nlaunch);
     BZ_CHECK_LAUNCH("bz_yee2d_runk(tick)");

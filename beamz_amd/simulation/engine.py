@@ -788,7 +788,7 @@ class Fused2D:
             start = np.searchsorted(t[order], np.arange(self.ntilesk + 1)).astype(np.int32)
             self.ents = T.from_numpy(ents.ravel()).to(self.dev)
             self.ent_start = T.from_numpy(start).to(self.dev)
-            self.stagek = T.zeros(K * self.npts, dtype=T.float32, device=self.dev)
+            self.stagek = T.zeros(2 * K * self.npts, dtype=T.float32, device=self.dev)   # two slots: q & 1
 
     def _describe(self, g, snap=None):
         T = self.torch
