@@ -570,6 +570,7 @@ class Fused2D:
         self.signal_len = 1 << 62
         self.descs = [self._describe(g, sn) for g, sn in zip(grids, self.snapshots)]
         self.desc_dev = T.zeros(len(grids) * C.sizeof(N.Sim2D), dtype=T.uint8, device=self.dev)
+        self._desc_cache = {}
         self.shift = None
         self.set_seg(seg)
         self.step = T.zeros(1, dtype=T.int32, device=self.dev)
@@ -645,6 +646,7 @@ class Fused2D:
             spts[sel[inside]] = ((j[inside] - y0) * sx + (i[inside] - x0)).astype(np.int32)
         self.spts = T.from_numpy(spts).to(self.dev)
         self.shift = None            # descriptors changed: upload again
+        self._desc_cache = {}
 
     SEGS2 = (4, 8, 16, 32, 64)
 
@@ -872,21 +874,35 @@ class Fused2D:
         self.graphs = {}
 
     def _sync_descs(self, n0, slot0=0):
-        """Upload the descriptors if the field buffers' parity or the first snapshot slot changed: step n0
-        must read buffer n0 & 1, and the next launch's step q writes snapshot slot slot0 + q."""
+        """Point the launches at descriptors for the field buffers' parity and the first snapshot slot: step n0
+        must read buffer n0 & 1, and the next launch's step q writes snapshot slot slot0 + q.
+
+        Each (parity, slot) layout is uploaded once, from pinned memory without blocking the host, into a
+        device array of its own that later runs reuse (a run never waits for the GPU to drain). Captured
+        graphs hold the address of the array, so with graphs one array is rewritten in stream order instead."""
+        T = self.torch
         shifts = [(g._cur - (n0 & 1)) % 2 for g in self.grids]
         key = (tuple(shifts), slot0)
         if key == self.shift:
             return
-        for d, g, sh, sn in zip(self.descs, self.grids, shifts, self.snapshots):
-            for p in (0, 1):
-                b = g._fb[(p + sh) % 2]
-                d.ez[p], d.hx[p], d.hy[p] = b["Ez"].data_ptr(), b["Hx"].data_ptr(), b["Hy"].data_ptr()
-            if sn is not None:
-                buf = sn[4]
-                d.snap = buf.data_ptr() + slot0 * buf[0].numel() * buf.element_size()
-        raw = b"".join(bytes(d) for d in self.descs)
-        self.desc_dev.copy_(self.torch.frombuffer(bytearray(raw), dtype=self.torch.uint8))
+        cached = None if self.use_graph else self._desc_cache.get(key)
+        if cached is None:
+            for d, g, sh, sn in zip(self.descs, self.grids, shifts, self.snapshots):
+                for p in (0, 1):
+                    b = g._fb[(p + sh) % 2]
+                    d.ez[p], d.hx[p], d.hy[p] = b["Ez"].data_ptr(), b["Hx"].data_ptr(), b["Hy"].data_ptr()
+                if sn is not None:
+                    buf = sn[4]
+                    d.snap = buf.data_ptr() + slot0 * buf[0].numel() * buf.element_size()
+            raw = T.frombuffer(bytearray(b"".join(bytes(d) for d in self.descs)), dtype=T.uint8).pin_memory()
+            if self.use_graph:
+                cached = self.desc_dev
+            else:
+                cached = T.empty(raw.numel(), dtype=T.uint8, device=self.dev)
+                if len(self._desc_cache) < 8:
+                    self._desc_cache[key] = cached
+            cached.copy_(raw, non_blocking=True)
+        self.desc_dev = cached
         self.shift = key
 
     def _launch(self, count):

@@ -59,6 +59,9 @@ MEASURED_TRAFFIC = {
     # the same in XCD order with the monitor gather of the previous pair in the same launch, the default
     # (same profile)
     ("batch2d", "pairs-seg8-xcd", 64): (2 * 474_825 + 400_315) * 1024,
+    # K = 8 steps per pass, 64 x 64 regions of 8 waves x 8 rows, XCD order, the previous launch's gather inside
+    # (profiles/yee2d_tbk_batch64_kernel_and_counters.md)
+    ("batch2d", "k8-8x8-xcd", 64): (2 * 306_109 + 359_078) * 1024,
 }
 
 
@@ -475,6 +478,11 @@ def main():
     if working_set < INFINITY_CACHE:
         out["roofline"]["note"] = ("the working set fits in the 256 MB Infinity Cache, so the 8 TB/s HBM peak "
                                    "is not this kernel's bound; frac is the algorithmic rate over the HBM peak")
+    if a.workload in ("batch2d", "wg2d") and out["roofline"]["steps_per_launch"] > 2:
+        out["roofline"]["note_kstep"] = (
+            "K steps per pass: a launch streams the fields and materials through HBM once for K steps, so the "
+            "one-step algorithmic rate exceeds the HBM peak (frac > 1) and traffic is per launch; the kernel is "
+            "bound by vector-ALU issue (profiles/yee2d_tbk_batch64_kernel_and_counters.md)")
     if r.tuned is not None:
         out["config"]["autotune_ms_per_step"] = {str(k): v for k, v in r.tuned[1].items()}
     if not a.no_cpu:
