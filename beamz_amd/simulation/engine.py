@@ -670,15 +670,19 @@ class Fused2D:
             raise ValueError("tune_seg2 needs an engine that runs pairs")
         steps += steps % 2
         timings = {}
-        for seg in self.SEGS2:
-            self.set_seg2(seg)
-            self.run(n0, 2)                                   # warm-up
-            e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
-            e0.record()
-            self.run(n0 + 2, steps)
-            e1.record()
-            T.cuda.synchronize(self.dev)
-            timings[seg] = e0.elapsed_time(e1) / steps
+        kstep, self.kstep = self.kstep, 0                    # time the pair kernel alone
+        try:
+            for seg in self.SEGS2:
+                self.set_seg2(seg)
+                self.run(n0, 2)                               # warm-up
+                e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
+                e0.record()
+                self.run(n0 + 2, steps)
+                e1.record()
+                T.cuda.synchronize(self.dev)
+                timings[seg] = e0.elapsed_time(e1) / steps
+        finally:
+            self.kstep = kstep
         best = min(timings, key=timings.get)
         self.set_seg2(best)
         return best, timings

@@ -279,7 +279,8 @@ def build_runner(a, total):
             return s.nx * s.ny * n / el, (f"{n} reference-order complex128 steps of one {s.nx}x{s.ny} design "
                                          f"({el:.1f} s); the reference's loop needs two such runs per gradient")
         r = Runner(iteration, sims[0].grid, cells, desc,
-                   "bz_yee2d_run2 (forward and recompute sweeps, pairs) + bz_adj2d_run (reverse sweep)", cpu)
+                   "forward sweep: bz_yee2d_runk (K steps per pass); recompute sweep with snapshots: bz_yee2d_run2 "
+                   "(pairs); reverse sweep: bz_adj2d_run2 / bz_adj2d_run", cpu)
         r.grids = [s.grid for s in sims]
         r.tuned = None
         r.per_step_cells = True
