@@ -42,7 +42,7 @@ class Sim2D(C.Structure):
                 ("rdx", f32), ("rdy", f32), ("src_x0", i32), ("src_y0", i32), ("src_sx", i32), ("src_sy", i32),
                 ("src_add", vp), ("src_mul", vp), ("signal", vp), ("power", vp),
                 ("snap_x0", i32), ("snap_y0", i32), ("snap_sx", i32), ("snap_sy", i32), ("snap", vp),
-                ("stg", vp), ("stg_x0", i32), ("stg_y0", i32), ("stg_sx", i32), ("stg_sy", i32)]
+                ("stg", vp), ("stg_x0", i32), ("stg_y0", i32), ("stg_sx", i32), ("stg_sy", i32), ("coef", vp)]
 
 
 class Adj2D(C.Structure):
@@ -67,6 +67,8 @@ SIGNATURES = {
                        C.POINTER(f32), i32, vp, C.POINTER(i32), i32, i32, i32, vp],
     "bz_yee2d_run": [vp, i32, vp, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp],
     "bz_yee2d_run2": [vp, i32, vp, i32, i32, vp, i32, i32, vp, vp, i32, vp, vp, i32, vp, vp, vp],
+    "bz_yee2d_coef": [vp, i32, i32, vp],
+    "bz_abi_sizes": [C.POINTER(i32), i32],
     "bz_yee2d_runk": [vp, i32, vp, i32, i32, i32, i32, vp, i32, i32, vp, vp, vp, i32, vp, vp, i32, vp, vp, vp],
     "bz_adj2d_run": [vp, i32, vp, i32, i32, i32, i32, vp],
     "bz_adj2d_run2": [vp, i32, vp, i32, i32, vp, i32, i32, i32, i32, i32, vp],

@@ -15,9 +15,14 @@ const char* last_error() { return g_error.c_str(); }
 
 extern "C" const char* bz_last_error(void) { return bz::last_error(); }
 
-
-
 extern "C" int bz_version(void) { return 100; }
+
+extern "C" int bz_abi_sizes(int* out, int n) {
+    BZ_REQUIRE(out && n >= 0, "bz_abi_sizes: null pointer");
+    const int sizes[3] = {(int)sizeof(bz_sim2d), (int)sizeof(bz_adj2d), (int)sizeof(bz_box_source)};
+    for (int q = 0; q < n && q < 3; ++q) out[q] = sizes[q];
+    return BZ_OK;
+}
 
 extern "C" int bz_device_count(int* count) {
     BZ_REQUIRE(count, "bz_device_count: null pointer");

@@ -29,6 +29,7 @@
 // so the whole run can be captured once in a hipGraph and replayed.
 #include "common.h"
 
+#include <algorithm>
 #include <type_traits>
 
 // No multiply-add contraction: the one-step and two-step kernels then round identically, so a run gives
@@ -433,6 +434,28 @@ int run(const bz_sim2d* sims, const int* tiles, int ntiles, int* step, int nstep
 // from the owning tile to staging slot q & 1 of launch q, stage[q & 1][k][point], and accumulated into the
 // FP64 DFT in step order by extra workgroups of the next launch (by k_gather2d_k after the last one).
 // Reads buffer rd, writes rd ^ 1.
+// Update coefficients of every cell: blockIdx.y is the simulation, the x blocks stride over its cells.
+__global__ __launch_bounds__(256) void k_coef2d(const bz_sim2d* __restrict__ sims) {
+    const bz_sim2d& S = sims[blockIdx.y];
+    float* __restrict__ cf = S.coef;
+    if (!cf) return;
+    Coef c;
+    c.kh = S.kh;
+    c.kbh = S.kbh;
+    c.ke = S.ke;
+    c.kbe = S.kbe;
+    const long plane = (long)S.ny * S.ld;
+    for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < plane; o += (long)gridDim.x * blockDim.x) {
+        float fa, fb, ca, cb;
+        h_coef(S.sig[o], c, fa, fb);
+        e_coef(S.sig[o], S.eps[o], c, ca, cb);
+        cf[o] = fa;
+        cf[plane + o] = fb;
+        cf[2 * plane + o] = ca;
+        cf[3 * plane + o] = cb;
+    }
+}
+
 // Row neighbours by DPP wave shifts: lane l receives lane l+1 (shl) or l-1 (shr); the end lanes read 0,
 // which reaches only cells within K of the region's edge.
 __device__ __forceinline__ float from_right(float x) {
@@ -531,28 +554,41 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
     // does the source box meet this wave's rows and this region's columns (uniform over the wave)
     const int rx0 = tx * (LANES - 2 * K) - K;
     const bool src_band = ssx > 0 && jb < sy0 + ssy && jb + B > sy0 && rx0 < sx0 + ssx && rx0 + LANES > sx0;
-    // every row of the wave is an interior row of the grid (the usual case): no per-row edge rules
-    const bool inner_rows = jb >= 1 && jb + B - 1 <= ny - 2;
+    // every row of the wave and every column of the region is interior to the grid (the usual case): no edge
+    // rules per row or per lane
+    const bool inner = jb >= 1 && jb + B - 1 <= ny - 2 && rx0 >= 1 && rx0 + LANES - 1 <= nx - 2;
 
     float ez[B], hx[B], hy[B], fa[B], fb[B], ca[B], cb[B];
     {
         const float* __restrict__ gez = S.ez[rd];
         const float* __restrict__ ghx = S.hx[rd];
         const float* __restrict__ ghy = S.hy[rd];
-        float sg[B], ee[B];
+        const float* __restrict__ cf = S.coef;
+        uint32_t o[B];
 #pragma unroll
         for (int r = 0; r < B; ++r) {
-            const uint32_t o = ((uint32_t)min(max(jb + r, 0), ny - 1) * (uint32_t)ld + ic) * 4u;
-            ez[r] = at(gez, o);
-            hx[r] = at(ghx, o);
-            hy[r] = at(ghy, o);
-            sg[r] = at(S.sig, o);
-            ee[r] = at(S.eps, o);
+            o[r] = ((uint32_t)min(max(jb + r, 0), ny - 1) * (uint32_t)ld + ic) * 4u;
+            ez[r] = at(gez, o[r]);
+            hx[r] = at(ghx, o[r]);
+            hy[r] = at(ghy, o[r]);
         }
+        if (cf) {
+            // precomputed by k_coef2d (the same roundings)
+            const uint32_t pb = (uint32_t)ny * (uint32_t)ld * 4u;
 #pragma unroll
-        for (int r = 0; r < B; ++r) {
-            h_coef(sg[r], c, fa[r], fb[r]);
-            e_coef(sg[r], ee[r], c, ca[r], cb[r]);
+            for (int r = 0; r < B; ++r) {
+                fa[r] = at(cf, o[r]);
+                fb[r] = at(cf, pb + o[r]);
+                ca[r] = at(cf, 2u * pb + o[r]);
+                cb[r] = at(cf, 3u * pb + o[r]);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < B; ++r) {
+                const float sg = at(S.sig, o[r]);
+                h_coef(sg, c, fa[r], fb[r]);
+                e_coef(sg, at(S.eps, o[r]), c, ca[r], cb[r]);
+            }
         }
     }
     const int e0 = ent_start ? ent_start[t] : 0, e1 = ent_start ? ent_start[t + 1] : 0;
@@ -573,12 +609,16 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
                 const int j = jb + r;
                 const float ezR = from_right(ez[r]);
                 const float ed = r < B - 1 ? ez[r + 1] : ezb;
-                const float hn = computed(fa[r] * hx[r] - fb[r] * ((ezR - ez[r]) * rdy));
-                hx[r] = hxok ? hn : hx[r];
+                if (C) {
+                    const float hn = computed(fa[r] * hx[r] - fb[r] * ((ezR - ez[r]) * rdy));
+                    hx[r] = hxok ? hn : hx[r];
+                } else {
+                    hx[r] = fa[r] * hx[r] - fb[r] * ((ezR - ez[r]) * rdy);
+                }
                 if (!C || j < ny - 1) hy[r] = fa[r] * hy[r] + fb[r] * ((ed - ez[r]) * rdx);
             }
         };
-        if (inner_rows) h_rows(std::false_type{});
+        if (inner) h_rows(std::false_type{});
         else h_rows(std::true_type{});
         // E: Ez of the band's first row needs the new Hy of the previous band's last row
         hy_bot[w][l] = hy[B - 1];
@@ -591,13 +631,15 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
                 const int j = jb + r;
                 const float hxl = from_left(hx[r]);
                 const float hu = r > 0 ? hy[r - 1] : hya;
-                if (!C || (j >= 1 && j <= ny - 2)) {
+                if (!C) {
+                    ez[r] = ca[r] * ez[r] + cb[r] * (-((hx[r] - hxl) * rdy) + (hy[r] - hu) * rdx);
+                } else if (j >= 1 && j <= ny - 2) {
                     const float en = computed(ca[r] * ez[r] + cb[r] * (-((hx[r] - hxl) * rdy) + (hy[r] - hu) * rdx));
                     ez[r] = xin ? en : ez[r];
                 }
             }
         };
-        if (inner_rows) e_rows(std::false_type{});
+        if (inner) e_rows(std::false_type{});
         else e_rows(std::true_type{});
         // the source box, after E (it touches only the Ez of its own cells)
         if (src_band) {
@@ -769,4 +811,12 @@ extern "C" int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, 
         default: return BZ_RUNK(16, 4);
     }
 #undef BZ_RUNK
+}
+
+extern "C" int bz_yee2d_coef(const bz_sim2d* sims, int nsims, int max_cells, void* stream) {
+    BZ_REQUIRE(sims && nsims > 0 && nsims <= 65535 && max_cells >= 0, "bz_yee2d_coef: bad arguments");
+    const int bx = (int)std::min<long>(64, ((long)max_cells + 255) / 256 + 1);
+    hipLaunchKernelGGL(k_coef2d, dim3(bx, nsims), dim3(256), 0, (hipStream_t)stream, sims);
+    BZ_CHECK_LAUNCH("bz_yee2d_coef");
+    return BZ_OK;
 }

@@ -617,6 +617,7 @@ class Fused2D:
         self._pts_host = pts if plans else None
         self.kstep, self.krows, self.kwaves = 0, 8, 8
         self.tilesk = self.ent_start = self.ents = self.stagek = None
+        self.coefs = None
         if kstep is None and self.kstep_applies():
             kstep, r, w = self.auto_kstep()
             krows, kwaves = krows or r, kwaves or w
@@ -773,6 +774,14 @@ class Fused2D:
         if self.xcd:
             tiles = xcd_order(tiles, per_wg=1)
         self.kstep, self.krows, self.kwaves = K, rows, waves
+        if self.coefs is None:
+            # per-cell update coefficients, computed at the start of every run (bz_yee2d_coef): the K-step kernel
+            # then loads four numbers per cell instead of evaluating three divisions
+            self.coefs = [T.empty((4, g.ny, g.ld), dtype=T.float32, device=self.dev) for g in self.grids]
+            for d, c in zip(self.descs, self.coefs):
+                d.coef = c.data_ptr()
+            self.shift = None
+            self._desc_cache = {}
         self.tilesk = T.from_numpy(tiles.ravel()).to(self.dev)
         self.ntilesk = len(tiles)
         self.ent_start = self.ents = self.stagek = None
@@ -939,6 +948,9 @@ class Fused2D:
             K = self.kstep
             nl = count // K
             ptr = lambda t: t.data_ptr() if t is not None else None
+            # coefficients of the current materials (a caller may have written eps or sig since the last run)
+            N.call("bz_yee2d_coef", self.desc_dev.data_ptr(), len(self.grids),
+                   max(g.ny * g.ld for g in self.grids), g0.stream)
             N.call("bz_yee2d_runk", self.desc_dev.data_ptr(), len(self.grids), self.tilesk.data_ptr(), self.ntilesk,
                    self.krows, self.kwaves, K, self.step.data_ptr(), n0 & 1, nl, ptr(self.ent_start), ptr(self.ents),
                    ptr(self.stagek), self.npts, ptr(self.slot), ptr(self.series), self.nfreq, ptr(self.phasor),

@@ -140,6 +140,8 @@ typedef struct bz_sim2d {
     float* snap;                          /* [nsteps][3][snap_sy][snap_sx] */
     float* stg;                           /* bz_yee2d_run2: [2][3][stg_sy][stg_sx] staging of the first step's */
     int stg_x0, stg_y0, stg_sx, stg_sy;   /* Ez, Hx, Hy for monitor samples, slot q & 1 for pair q (NULL: none) */
+    float* coef;                          /* NULL, or [4][ny][ld] FP32 update coefficients fa, fb, ca, cb of every
+                                           * cell as bz_yee2d_coef writes them (read by bz_yee2d_runk) */
 } bz_sim2d;
 
 int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step, int nsteps,
@@ -155,6 +157,15 @@ int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, 
 int bz_yee2d_run2(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step, int rd0,
                   int npairs, const int* pts, const int* spts, int npts, const int* slot, float* series,
                   int nfreq, const double* phasor, double* dft, void* stream);
+
+/* The update coefficients of every cell of every simulation whose coef is not NULL, from its eps and sig:
+ * s = sig*kh, fa = (1-s)/(1+s), fb = kbh/(1+s) (H); s = sig*ke/eps, ca = (1-s)/(1+s), cb = kbe/eps/(1+s)
+ * (E), rounded exactly as the step kernels round them. Must run again after eps or sig change. */
+int bz_yee2d_coef(const bz_sim2d* sims, int nsims, int max_cells, void* stream);
+
+/* Sizes of the descriptor structs as this library was compiled: out[0] = sizeof(bz_sim2d),
+ * out[1] = sizeof(bz_adj2d), out[2] = sizeof(bz_box_source) (bindings check their own layouts). */
+int bz_abi_sizes(int* out, int n);
 
 /* K steps per pass: nlaunch launches of ksteps steps of the bz_yee2d_run step each (same results, bit for
  * bit), launch q reading buffer (rd0 + q) & 1 and writing the other; the step counter advances by

@@ -57,3 +57,12 @@ def test_xcd_order_is_a_padded_permutation():
                     per_xcd.setdefault(w % 8, []).append(int(e[2]))
         for ids in per_xcd.values():
             assert ids == list(range(ids[0], ids[0] + len(ids)))      # contiguous, in order
+
+
+def test_descriptor_struct_layouts_match_the_library():
+    """The ctypes descriptors must have the sizes the C structs were compiled with (a field added on one side
+    only would shift every later field the kernels read)."""
+    import ctypes as C
+    out = (C.c_int * 3)()
+    N.call("bz_abi_sizes", out, 3)
+    assert list(out) == [C.sizeof(N.Sim2D), C.sizeof(N.Adj2D), C.sizeof(N.BoxSource)]
