@@ -80,10 +80,12 @@ class FDTDBatch:
         if self.engine is None:
             self.initialize()
         end = min(self.num_steps, self.current_step + int(count))
+        first = True
         while self.current_step < end:
             n = self.current_step
             c = min(self.chunk, end - n)
-            self.engine.run(n, c)
+            self.engine.run(n, c, coef=first)      # materials can only change between calls
+            first = False
             self.current_step += c
         for s in self.sims:
             s.current_step = self.current_step

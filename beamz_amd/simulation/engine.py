@@ -618,6 +618,7 @@ class Fused2D:
         self.kstep, self.krows, self.kwaves = 0, 8, 8
         self.tilesk = self.ent_start = self.ents = self.stagek = None
         self.coefs = None
+        self._coef_done = False
         if kstep is None and self.kstep_applies():
             kstep, r, w = self.auto_kstep()
             krows, kwaves = krows or r, kwaves or w
@@ -737,7 +738,7 @@ class Fused2D:
             self.run(n0, K)                                   # warm-up
             e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
             e0.record()
-            self.run(n0 + K, n)
+            self.run(n0 + K, n, coef=False)
             e1.record()
             T.cuda.synchronize(self.dev)
             timings[shape] = e0.elapsed_time(e1) / n
@@ -925,8 +926,10 @@ class Fused2D:
                self.seg, self.step.data_ptr(), count, ptr(self.pts), self.npts, ptr(self.slot), ptr(self.series),
                self.nfreq, ptr(self.phasor), ptr(self.dft), g0.stream)
 
-    def run(self, n0, count):
-        """Advance every grid by ``count`` steps starting at step index n0."""
+    def run(self, n0, count, coef=True):
+        """Advance every grid by ``count`` steps starting at step index n0. coef=False skips recomputing the
+        K-step kernel's update coefficients, for a caller that knows eps and sig are unchanged since its
+        previous run (an earlier chunk of the same run)."""
         if count <= 0:
             return
         T = self.torch
@@ -949,8 +952,10 @@ class Fused2D:
             nl = count // K
             ptr = lambda t: t.data_ptr() if t is not None else None
             # coefficients of the current materials (a caller may have written eps or sig since the last run)
-            N.call("bz_yee2d_coef", self.desc_dev.data_ptr(), len(self.grids),
-                   max(g.ny * g.ld for g in self.grids), g0.stream)
+            if coef or not self._coef_done:
+                N.call("bz_yee2d_coef", self.desc_dev.data_ptr(), len(self.grids),
+                       max(g.ny * g.ld for g in self.grids), g0.stream)
+                self._coef_done = True
             N.call("bz_yee2d_runk", self.desc_dev.data_ptr(), len(self.grids), self.tilesk.data_ptr(), self.ntilesk,
                    self.krows, self.kwaves, K, self.step.data_ptr(), n0 & 1, nl, ptr(self.ent_start), ptr(self.ents),
                    ptr(self.stagek), self.npts, ptr(self.slot), ptr(self.series), self.nfreq, ptr(self.phasor),

@@ -172,3 +172,36 @@ def test_k_steps_per_pass_equal_single_steps(hip, K, rows, waves, pairs):
         A, B = state(s, m), state(r, rm)
         for k in A:
             assert np.array_equal(A[k], B[k]), (K, rows, waves, s.nx, k)
+
+
+def test_new_materials_after_reset_take_effect(hip):
+    """The K-step kernel's per-cell coefficients follow the materials: writing a member's permittivity between
+    runs (as a design loop does) gives the bits of a batch built with that permittivity from the start."""
+    torch = hip
+    mem = members(steps=200)
+    b = FDTDBatch([s for s, _ in mem], chunk=48)
+    b.run()
+    assert b.engine.kstep
+    s0 = mem[0][0]
+    eps_new = s0.grid.eps.clone()
+    eps_new[20:40, 20:40] = 3.476 ** 2
+    b.reset()
+    s0.grid.eps.copy_(eps_new)
+    b.advance(b.num_steps)
+    for s, _ in mem:
+        s.finalize_simulation()
+    fresh = members(steps=200)
+    f = FDTDBatch([s for s, _ in fresh], chunk=48)
+    f.initialize()
+    fresh[0][0].grid.eps.copy_(eps_new)
+    f.advance(f.num_steps)
+    for s, _ in fresh:
+        s.finalize_simulation()
+    for (s, m), (r, rm) in zip(mem, fresh):
+        A, B = state(s, m), state(r, rm)
+        for k in A:
+            assert np.array_equal(A[k], B[k]), (s.nx, k)
+    # and the change mattered: the member differs from a run on the original permittivity
+    orig = members(steps=200)
+    FDTDBatch([s for s, _ in orig], chunk=48).run()
+    assert not np.array_equal(np.real(orig[0][0].Ez), np.real(mem[0][0].Ez))
