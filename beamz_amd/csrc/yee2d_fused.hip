@@ -557,6 +557,10 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
     // every row of the wave and every column of the region is interior to the grid (the usual case): no edge
     // rules per row or per lane
     const bool inner = jb >= 1 && jb + B - 1 <= ny - 2 && rx0 >= 1 && rx0 + LANES - 1 <= nx - 2;
+    // does the snapshot box meet this wave's rows and this region's inner columns (uniform over the wave)
+    const bool snap_band = S.snap && jb < S.snap_y0 + S.snap_sy && jb + B > S.snap_y0 &&
+                           rx0 + K < S.snap_x0 + S.snap_sx && rx0 + LANES - K > S.snap_x0;
+    const bool lane_inner = l >= K && l < LANES - K;
 
     float ez[B], hx[B], hy[B], fa[B], fb[B], ca[B], cb[B];
     {
@@ -620,6 +624,14 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
         };
         if (inner) h_rows(std::false_type{});
         else h_rows(std::true_type{});
+        // snapshot of call-relative step q*K+k at the inner cells: the Ez this step reads, the Hx, Hy it wrote
+        if (snap_band) {
+#pragma unroll
+            for (int r = 0; r < B; ++r) {
+                const int rr = w * B + r;
+                if (rr >= K && rr < RH - K && lane_inner) snap_put(S, q * K + k, i, jb + r, ez[r], hx[r], hy[r]);
+            }
+        }
         // E: Ez of the band's first row needs the new Hy of the previous band's last row
         hy_bot[w][l] = hy[B - 1];
         __syncthreads();

@@ -541,7 +541,7 @@ class Fused2D:
         cells on; never with graphs), with tiles of 60 columns x seg2 rows (None: by size); xcd: order the tile
         lists by xcd_order (None: XCD_ORDER); kstep: advance K steps per pass (bz_yee2d_runk, regions of 64 x
         krows*kwaves cells) for runs of at least K steps, the rest by pairs or single steps (0: off; None: on
-        with auto_kstep's shape wherever it applies: no graphs, no grid power, no snapshots)."""
+        with auto_kstep's shape wherever it applies: no graphs, no grid power)."""
         if not grids:
             raise ValueError("Fused2D needs at least one grid")
         self.snapshots = list(snapshots) if snapshots is not None else [None] * len(grids)
@@ -714,8 +714,7 @@ class Fused2D:
     KCANDIDATES = ((8, 8, 8), (6, 8, 8), (8, 8, 4), (8, 4, 8), (12, 4, 16))
 
     def kstep_applies(self):
-        return (not self.use_graph and all(g.power is None for g in self.grids)
-                and all(sn is None for sn in self.snapshots))
+        return not self.use_graph and all(g.power is None for g in self.grids)
 
     def auto_kstep(self):
         """(K, rows, waves): tall 64 x 64 regions from PAIR_MIN_CELLS cells on, else 64 x 32 regions, which
@@ -761,8 +760,6 @@ class Fused2D:
             raise ValueError(f"K steps per pass need (rows, waves) in {self.KSHAPES} and 2K < min(64, rows*waves)")
         if any(g.power is not None for g in self.grids):
             raise ValueError("K steps per pass do not accumulate grid power")
-        if any(sn is not None for sn in self.snapshots):
-            raise ValueError("K steps per pass do not record snapshots")
         if self.use_graph:
             raise ValueError("K steps per pass are not captured in graphs")
         ix, iy = 64 - 2 * K, rows * waves - 2 * K
@@ -964,7 +961,8 @@ class Fused2D:
                 g._cur = (g._cur + nl) % 2
             n0, count = n0 + nl * K, count - nl * K
             if count:
-                self._sync_descs(n0)                     # the step counter is already there
+                # the step counter is already there; later steps write snapshot slots from nl*K on
+                self._sync_descs(n0, total - count)
         rem = count
         if self.pairs and count >= 2:
             # pairs of steps; pair q reads the buffer holding step n0 + 2q and writes the other one
@@ -978,7 +976,7 @@ class Fused2D:
                 g._cur = (g._cur + npairs) % 2
             rem = count - 2 * npairs
             if rem:
-                self._sync_descs(n0 + 2 * npairs, 2 * npairs)      # step counter is already there
+                self._sync_descs(n0 + 2 * npairs, total - rem)     # step counter is already there
         if rem and self.use_graph and rem > 1:
             # replays of one captured GRAPH_CHUNK-step graph, then one graph for the rest: the step base lives
             # in device memory, so each replay continues where the previous one stopped

@@ -176,8 +176,8 @@ int bz_abi_sizes(int* out, int n);
  * in units of the inner size, one tile per workgroup (sim < 0: padding). Monitor point p is sampled by the
  * tile whose inner cells hold it: ent_start[t] .. ent_start[t+1]-1 index ents[2e] = p,
  * ents[2e+1] = (region row << 10) | (region column << 2) | comp, sorted by comp within a tile;
- * stage: device FP32 [2][ksteps][npts] (launch q writes slot q & 1). No grid power and no snapshots (use
- * bz_yee2d_run / run2). */
+ * stage: device FP32 [2][ksteps][npts] (launch q writes slot q & 1). Snapshot boxes as for bz_yee2d_run:
+ * step k of launch q writes slot q*ksteps + k. No grid power (use bz_yee2d_run / run2). */
 int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int rows, int waves, int ksteps,
                   int* step, int rd0, int nlaunch, const int* ent_start, const int* ents, float* stage, int npts,
                   const int* slot, float* series, int nfreq, const double* phasor, double* dft, void* stream);

@@ -129,30 +129,33 @@ def test_two_step_pairs_equal_single_steps(hip, seg2):
 @pytest.mark.parametrize("count", [7, 10])
 def test_pair_snapshots_equal_one_step_snapshots(hip, count):
     """Snapshot boxes written by two-step launches (slots 2q, 2q+1, and a last single step after the pairs)
-    equal those of one step per launch, bit for bit, as do the fields."""
+    and by K-step launches (slots qK+k, then pairs and a single step for the rest of the run) equal those of
+    one step per launch, bit for bit, as do the fields."""
     torch = hip
     from beamz_amd.simulation.engine import Fused2D
     got = {}
-    for pairs in (False, True):
+    for key, kstep, pairs in (("one", 0, False), ("pairs", 0, True), ("k3", 3, True), ("k4", 4, True)):
         sims = [s for s, _ in members(steps=64)]
         for s in sims:
             s.initialize_simulation(save=False, live=False)
         bufs = [torch.full((count, 3, 30, 40), -7.0, dtype=torch.float32, device="cuda") for _ in sims]
         snaps = [(10, 12, 40, 30, b) for b in bufs]
-        eng = Fused2D([s.grid for s in sims], seg=8, monitors=False, snapshots=snaps, pairs=pairs, seg2=7)
+        eng = Fused2D([s.grid for s in sims], seg=8, monitors=False, snapshots=snaps, pairs=pairs, seg2=7,
+                      kstep=kstep, krows=8, kwaves=4)
         for s in sims:
             s.grid._f2d = False
         eng.run(0, 5)
         eng.run(5, count)
-        assert eng.pairs == pairs
-        got[pairs] = ([b.cpu().numpy() for b in bufs], [{k: s.field(k).cpu().numpy() for k in ("Ez", "Hx", "Hy")}
-                                                          for s in sims])
-    for a, b in zip(got[False][0], got[True][0]):
-        assert not np.any(a == -7.0)
-        assert np.array_equal(a, b)
-    for fa, fb in zip(got[False][1], got[True][1]):
-        for k in fa:
-            assert np.array_equal(fa[k], fb[k]), k
+        assert eng.pairs == pairs and eng.kstep == kstep
+        got[key] = ([b.cpu().numpy() for b in bufs], [{k: s.field(k).cpu().numpy() for k in ("Ez", "Hx", "Hy")}
+                                                       for s in sims])
+    for key in ("pairs", "k3", "k4"):
+        for a, b in zip(got["one"][0], got[key][0]):
+            assert not np.any(a == -7.0)
+            assert np.array_equal(a, b), key
+        for fa, fb in zip(got["one"][1], got[key][1]):
+            for k in fa:
+                assert np.array_equal(fa[k], fb[k]), (key, k)
 
 
 @pytest.mark.parametrize("K,rows,waves,pairs", [(1, 8, 4, False), (3, 16, 4, False), (8, 8, 8, True),
