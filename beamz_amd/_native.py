@@ -51,7 +51,8 @@ class Adj2D(C.Structure):
                 ("rdx", f32), ("rdy", f32), ("src_x0", i32), ("src_y0", i32), ("src_sx", i32), ("src_sy", i32),
                 ("src_mul", vp), ("box_x0", i32), ("box_y0", i32), ("box_sx", i32), ("box_sy", i32),
                 ("snap", vp), ("snap_step0", i32), ("grad", vp),
-                ("inj_x0", i32), ("inj_y0", i32), ("inj_sx", i32), ("inj_sy", i32), ("inj_w", vp), ("phasor", vp), ("nf", i32)]
+                ("inj_x0", i32), ("inj_y0", i32), ("inj_sx", i32), ("inj_sy", i32), ("inj_w", vp), ("phasor", vp), ("nf", i32),
+                ("coef", vp)]
 
 # name -> argtypes (restype is int unless listed in _RESTYPE)
 SIGNATURES = {
@@ -72,6 +73,7 @@ SIGNATURES = {
     "bz_yee2d_runk": [vp, i32, vp, i32, i32, i32, i32, vp, i32, i32, vp, vp, vp, i32, vp, vp, i32, vp, vp, vp],
     "bz_adj2d_run": [vp, i32, vp, i32, i32, i32, i32, vp],
     "bz_adj2d_run2": [vp, i32, vp, i32, i32, vp, i32, i32, i32, i32, i32, vp],
+    "bz_adj2d_runk": [vp, i32, vp, i32, i32, i32, i32, vp, i32, i32, i32, i32, i32, vp],
     "bz_nn_conv3x3": [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp],
     "bz_nn_conv3x3_dgrad": [vp, vp, vp, i32, i32, i32, i32, i32, vp],
     "bz_nn_wgrad_work": [i32, i32, i32, i32, i32, i32],

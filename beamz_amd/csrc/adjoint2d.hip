@@ -317,6 +317,163 @@ __global__ __launch_bounds__(LANES* WPG) void k_adj2d_tb2(const bz_adj2d* __rest
     }
 }
 
+
+// ---------------- K reverse steps per pass ----------------
+// The forward K-step scheme (yee2d_fused.hip) mirrored: a workgroup of KW waves holds a region of 64 columns
+// x KW*B rows, wave w rows w*B .. w*B+B-1 in registers (lane l = column), and runs reverse steps n_top ..
+// n_top-K+1 on the whole region; the cells within K of its edge go stale one ring per step, the inner ones
+// are stored. w = cb lE'' plays Ez (muY needs w of the next row, muX w of the next column) and fb mu plays H
+// (lE needs fb muY of the previous row, fb muX of the previous column): right and left neighbours by DPP
+// wave shifts, band edges through LDS. Each inner cell is owned by one tile, which injects the objective
+// and accumulates its gradient, step by step in the one-step order. Reads buffer rb, writes rb ^ 1.
+__device__ __forceinline__ float from_right(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_left(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+}
+
+template <int B, int KW>
+__global__ __launch_bounds__(LANES* KW) void k_adj2d_tbk(const bz_adj2d* __restrict__ adjs,
+                                                        const int* __restrict__ tiles, int ntiles, int K, int n_top,
+                                                        int rb) {
+    constexpr int RH = KW * B;
+    __shared__ float w_top[KW][LANES];
+    __shared__ float fmy_bot[KW][LANES];
+    const int t = blockIdx.x;
+    if (t >= ntiles) return;
+    const int s = tiles[3 * t];
+    if (s < 0) return;   // padding entry of a reordered tile list (uniform over the workgroup)
+    const bz_adj2d& A = adjs[s];
+    const int tx = tiles[3 * t + 1], ty = tiles[3 * t + 2];
+    const int nx = A.nx, ny = A.ny, ld = A.ld;
+    const int l = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int rx0 = tx * (LANES - 2 * K) - K;
+    const int i = rx0 + l;
+    const int jb = ty * (RH - 2 * K) - K + w * B;
+    const uint32_t ic = (uint32_t)min(max(i, 0), nx - 1);
+    Coef c;
+    c.kh = A.kh;
+    c.kbh = A.kbh;
+    c.ke = A.ke;
+    c.kbe = A.kbe;
+    const float rdx = A.rdx, rdy = A.rdy;
+    const bool xin = i >= 1 && i <= nx - 2;
+    const bool hxok = i <= nx - 2;
+    const bool lane_inner = l >= K && l < LANES - K;
+    // uniform over the wave: does it meet the source box, the injection box, the design box (inner cells)
+    const bool src_band = A.src_mul && jb < A.src_y0 + A.src_sy && jb + B > A.src_y0 && rx0 < A.src_x0 + A.src_sx &&
+                          rx0 + LANES > A.src_x0;
+    const bool inj_band = A.inj_w && jb < A.inj_y0 + A.inj_sy && jb + B > A.inj_y0 &&
+                          rx0 < A.inj_x0 + A.inj_sx && rx0 + LANES > A.inj_x0;
+    const bool grad_band = A.grad && jb < A.box_y0 + A.box_sy && jb + B > A.box_y0 &&
+                           rx0 + K < A.box_x0 + A.box_sx && rx0 + LANES - K > A.box_x0;
+    const bool inner = jb >= 1 && jb + B - 1 <= ny - 2 && rx0 >= 1 && rx0 + LANES - 1 <= nx - 2;
+
+    float le[B], lx[B], ly[B], fa[B], fb[B], ca[B], cb[B];
+    {
+        const float* __restrict__ gle = A.le[rb];
+        const float* __restrict__ glx = A.lhx[rb];
+        const float* __restrict__ gly = A.lhy[rb];
+        const float* __restrict__ cf = A.coef;
+        uint32_t o[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            o[r] = ((uint32_t)min(max(jb + r, 0), ny - 1) * (uint32_t)ld + ic) * 4u;
+            le[r] = at(gle, o[r]);
+            lx[r] = at(glx, o[r]);
+            ly[r] = at(gly, o[r]);
+        }
+        if (cf) {
+            const uint32_t pb = (uint32_t)ny * (uint32_t)ld * 4u;
+#pragma unroll
+            for (int r = 0; r < B; ++r) {
+                fa[r] = at(cf, o[r]);
+                fb[r] = at(cf, pb + o[r]);
+                ca[r] = at(cf, 2u * pb + o[r]);
+                cb[r] = at(cf, 3u * pb + o[r]);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < B; ++r) {
+                const float sg = at(A.sig, o[r]);
+                h_coef(sg, c, fa[r], fb[r]);
+                e_coef(sg, at(A.eps, o[r]), c, ca[r], cb[r]);
+            }
+        }
+    }
+
+    for (int k = 0; k < K; ++k) {
+        const int n = n_top - k;
+        // lE'' = M lE (kept in le) and w = cb lE'' on the interior
+        if (src_band) {
+#pragma unroll
+            for (int r = 0; r < B; ++r) le[r] = src_mul_at(A, i, jb + r) * le[r];
+        }
+        float wv[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const int j = jb + r;
+            wv[r] = (inner || (xin && j >= 1 && j <= ny - 2)) ? cb[r] * le[r] : 0.0f;
+        }
+        // mu: muY of the band's last row needs w of the next band's first row
+        w_top[w][l] = wv[0];
+        __syncthreads();
+        const float wb = w < KW - 1 ? w_top[w + 1][l] : wv[B - 1];
+        float fmx[B], fmy[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const int j = jb + r;
+            const float wR = from_right(wv[r]);
+            const float wd = r < B - 1 ? wv[r + 1] : wb;
+            const float mx = (inner || hxok) ? lx[r] + (wR - wv[r]) * rdy : 0.0f;
+            const float my = (inner || (j >= 0 && j <= ny - 2)) ? ly[r] + (wv[r] - wd) * rdx : 0.0f;
+            fmx[r] = fb[r] * mx;
+            fmy[r] = fb[r] * my;
+            lx[r] = fa[r] * mx;
+            ly[r] = fa[r] * my;
+        }
+        // lE: the band's first row needs fb muY of the previous band's last row
+        fmy_bot[w][l] = fmy[B - 1];
+        __syncthreads();
+        const float fu = w > 0 ? fmy_bot[w - 1][l] : fmy[0];
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const int j = jb + r;
+            const float left = from_left(fmx[r]);
+            const float fmxl = (inner || i >= 1) ? left : 0.0f;
+            const float fmyu = r > 0 ? fmy[r - 1] : fu;
+            const bool in = inner || (xin && j >= 1 && j <= ny - 2);
+            const float l2 = le[r];
+            float lnew = (in ? ca[r] * l2 : l2) + (fmx[r] - fmxl) * rdy + (fmyu - fmy[r]) * rdx;
+            if (inj_band) inject_at(A, lnew, i, j, n);
+            const int rr = w * B + r;
+            if (grad_band && in && lane_inner && rr >= K && rr < RH - K) {
+                const uint32_t o = ((uint32_t)min(max(j, 0), ny - 1) * (uint32_t)ld + ic) * 4u;
+                grad_at(A, c, i, j, n, l2, at(A.eps, o), at(A.sig, o), cb[r], rdx, rdy);
+            }
+            le[r] = lnew;
+        }
+    }
+
+    // the inner cells hold lam_{n_top-K+1}
+    if (!lane_inner || i >= nx) return;
+    float* __restrict__ ole = A.le[rb ^ 1];
+    float* __restrict__ olx = A.lhx[rb ^ 1];
+    float* __restrict__ oly = A.lhy[rb ^ 1];
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        const int rr = w * B + r, j = jb + r;
+        if (rr >= K && rr < RH - K && j < ny) {
+            const uint32_t o = ((uint32_t)j * (uint32_t)ld + (uint32_t)i) * 4u;
+            put(ole, o, le[r]);
+            put(olx, o, lx[r]);
+            put(oly, o, ly[r]);
+        }
+    }
+}
+
 // lE_n += Re(w_p phasor[n-1]) on the injection boxes: the terminal condition lam_N = dJ/du_N.
 __global__ void k_adj_inject(const bz_adj2d* __restrict__ adjs, int n) {
     const bz_adj2d& A = adjs[blockIdx.y];
@@ -381,5 +538,42 @@ extern "C" int bz_adj2d_inject(const bz_adj2d* adjs, int nadj, int max_inj, int 
     if (max_inj == 0 || n < 1) return BZ_OK;
     hipLaunchKernelGGL(k_adj_inject, dim3((max_inj + 255) / 256, nadj), dim3(256), 0, (hipStream_t)stream, adjs, n);
     BZ_CHECK_LAUNCH("bz_adj2d_inject");
+    return BZ_OK;
+}
+
+extern "C" int bz_adj2d_runk(const bz_adj2d* adjs, int nadj, const int* tilesk, int ntilesk, int rows, int waves,
+                             int ksteps, const int* tiles, int ntiles, int seg, int n_hi, int nsteps, int flip,
+                             void* stream) {
+    BZ_REQUIRE(adjs && nadj > 0 && tilesk && ntilesk > 0 && nsteps >= 0 && n_hi - nsteps >= 0 &&
+                   (flip == 0 || flip == 1),
+               "bz_adj2d_runk: bad arguments");
+    const int shape = rows * 100 + waves;
+    BZ_REQUIRE(shape == 804 || shape == 808 || shape == 816 || shape == 408 || shape == 416 || shape == 1604,
+               "bz_adj2d_runk: (rows, waves) must be (8, 4), (8, 8), (8, 16), (4, 8), (4, 16) or (16, 4)");
+    BZ_REQUIRE(ksteps >= 1 && 2 * ksteps < LANES && 2 * ksteps < waves * rows,
+               "bz_adj2d_runk: ksteps must leave inner cells (2 * ksteps < 64 and < rows * waves)");
+    BZ_REQUIRE(nsteps % ksteps == 0 || (tiles && ntiles > 0),
+               "bz_adj2d_runk: a step count that is not a multiple of ksteps needs one-step tiles");
+    const hipStream_t st = (hipStream_t)stream;
+    const int nl = nsteps / ksteps;
+    for (int q = 0; q < nl; ++q) {
+        const int n_top = n_hi - 1 - q * ksteps;
+        const int rb = (n_top + 1 + flip) & 1;
+        const dim3 b(LANES, waves);
+#define BZ_ADJK(R, W) hipLaunchKernelGGL((k_adj2d_tbk<R, W>), dim3(ntilesk), b, 0, st, adjs, tilesk, ntilesk, ksteps, \
+                                         n_top, rb)
+        switch (shape) {
+            case 804: BZ_ADJK(8, 4); break;
+            case 808: BZ_ADJK(8, 8); break;
+            case 816: BZ_ADJK(8, 16); break;
+            case 408: BZ_ADJK(4, 8); break;
+            case 416: BZ_ADJK(4, 16); break;
+            default: BZ_ADJK(16, 4); break;
+        }
+#undef BZ_ADJK
+        BZ_CHECK_LAUNCH("bz_adj2d_runk");
+        flip = (flip + ksteps - 1) & 1;
+    }
+    if (nsteps % ksteps) return run_seg(adjs, tiles, ntiles, seg, n_hi - nl * ksteps, nsteps % ksteps, flip, st);
     return BZ_OK;
 }

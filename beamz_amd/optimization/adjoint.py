@@ -64,7 +64,8 @@ class AdjointProblem2D:
 
     REVERSE_SEGS2 = (4, 8, 16)      # pair-tile heights tried for the recompute and reverse sweeps
 
-    def __init__(self, sims, monitors, weights, box, checkpoint_every=None, seg=None, fscale=None, pairs=None):
+    def __init__(self, sims, monitors, weights, box, checkpoint_every=None, seg=None, fscale=None, pairs=None,
+                 kstep=None):
         self.sims = list(sims)
         self.monitors = list(monitors)
         if len(self.monitors) != len(self.sims):
@@ -83,9 +84,12 @@ class AdjointProblem2D:
             if not (x0 >= 1 and y0 >= 1 and x0 + sx <= s.nx - 1 and y0 + sy <= s.ny - 1):
                 raise ValueError("the design box must lie inside the grid interior")
         self.N = self.sims[0].num_steps
-        self.K = int(checkpoint_every or max(1, math.ceil(math.sqrt(self.N))))
+        # checkpoint interval: ceil(sqrt N), rounded up to a multiple of 8 so that the K-step launches of
+        # 8 steps (the engines' automatic choice) tile every segment
+        self.K = int(checkpoint_every or min(self.N, 8 * math.ceil(math.ceil(math.sqrt(self.N)) / 8)))
         self.seg = seg
         self.pairs = pairs            # two steps per launch in all three sweeps (None: by size)
+        self.kstep = kstep            # K steps per launch in all three sweeps (None: automatic, 0: off)
         self._rev_tuned = False
         self.torch = self.sims[0].backend.torch
         self.dev = self.sims[0].backend.dev
@@ -101,8 +105,8 @@ class AdjointProblem2D:
             return
         for s in self.sims:
             s.initialize_simulation(save=False, live=False)
-        self.fwd = Fused2D(grids, seg=self.seg, max_steps=self.N, pairs=self.pairs)
-        if self.fwd.pairs and self.N >= 24:
+        self.fwd = Fused2D(grids, seg=self.seg, max_steps=self.N, pairs=self.pairs, kstep=self.kstep)
+        if self.fwd.pairs and not self.fwd.kstep and self.N >= 24:
             self.fwd.tune_seg2(steps=10)
             self._reset()
         x0, y0, sx, sy = self.box
@@ -113,7 +117,8 @@ class AdjointProblem2D:
         # 8-row kernel holds 144 VGPRs and runs 3 waves per SIMD)
         seg = min(self.seg, 8) if self.seg is not None else 2
         self.rec = Fused2D(grids, seg=seg, monitors=False, snapshots=snaps, pairs=self.fwd.pairs,
-                           seg2=self.fwd.seg2 if self.fwd.pairs else None)
+                           seg2=self.fwd.seg2 if self.fwd.pairs else None, kstep=self.fwd.kstep,
+                           krows=self.fwd.krows, kwaves=self.fwd.kwaves)
         for g in grids:
             g._f2d = False
         self._ready = True
@@ -139,10 +144,17 @@ class AdjointProblem2D:
         self._setup()          # first call: builds the engines; later calls: resets fields and monitor sums
         self.set_eps(eps_box)
         grids = [s.grid for s in self.sims]
-        self.ckpt = []
+        nseg = -(-self.N // self.K)
+        if getattr(self, "_ckpt_buf", None) is None or len(self._ckpt_buf[0]) != nseg:
+            # one buffer per grid, reused by every evaluation: (segment, component, ny, ld)
+            self._ckpt_buf = [T.empty((nseg, 3) + tuple(g.f["Ez"].shape), dtype=T.float32, device=self.dev)
+                              for g in grids]
+        self.ckpt = [[buf[k] for buf in self._ckpt_buf] for k in range(nseg)]
         n = 0
-        while n < self.N:
-            self.ckpt.append([T.stack([g.f["Ez"], g.f["Hx"], g.f["Hy"]]) for g in grids])
+        for k in range(nseg):
+            # the segment's initial fields, all grids in one multi-tensor copy
+            T._foreach_copy_([t for ck in self.ckpt[k] for t in ck.unbind(0)],
+                             [g.f[c] for g in grids for c in ("Ez", "Hx", "Hy")])
             cnt = min(self.K, self.N - n)
             self.fwd.run(n, cnt)
             n += cnt
@@ -188,6 +200,8 @@ class AdjointProblem2D:
             d.inj_w = w.data_ptr()
             d.phasor = self._phasor[b].data_ptr()
             d.nf = self.nf
+            # the recompute engine's coefficient arrays (it refreshes them at the start of every segment)
+            d.coef = self.rec.coefs[b].data_ptr() if self.rec.kstep else None
             descs.append(d)
         return descs
 
@@ -234,23 +248,29 @@ class AdjointProblem2D:
             """Recompute segment k from its checkpoint (snapshots on), then walk it backwards; returns flip."""
             n0 = k * self.K
             cnt = min(self.K, self.N - n0)
-            for g, ck in zip(grids, self.ckpt[k]):
-                g.f["Ez"].copy_(ck[0]); g.f["Hx"].copy_(ck[1]); g.f["Hy"].copy_(ck[2])
+            T._foreach_copy_([g.f[c] for g in grids for c in ("Ez", "Hx", "Hy")],
+                             [t for ck in self.ckpt[k] for t in ck.unbind(0)])
             self.rec.run(n0, cnt)
             for d in descs:
                 d.snap_step0 = n0
-            raw = b"".join(bytes(d) for d in descs)
-            ddev.copy_(T.frombuffer(bytearray(raw), dtype=T.uint8))
-            if self.rec.pairs:
-                N.call("bz_adj2d_run2", ddev.data_ptr(), B, self.rec.tiles2.data_ptr(), self.rec.ntiles2,
-                       self.rec.seg2, self.rec.tiles.data_ptr(), self.rec.ntiles, self.rec.seg, n0 + cnt, cnt, flip, st)
+            # stream-ordered upload from pinned memory: the host does not wait for the GPU to drain
+            raw = T.frombuffer(bytearray(b"".join(bytes(d) for d in descs)), dtype=T.uint8).pin_memory()
+            ddev.copy_(raw, non_blocking=True)
+            rec = self.rec
+            if rec.kstep:
+                N.call("bz_adj2d_runk", ddev.data_ptr(), B, rec.tilesk.data_ptr(), rec.ntilesk, rec.krows,
+                       rec.kwaves, rec.kstep, rec.tiles.data_ptr(), rec.ntiles, rec.seg, n0 + cnt, cnt, flip, st)
+                return (flip + (cnt // rec.kstep) * (rec.kstep - 1)) & 1
+            if rec.pairs:
+                N.call("bz_adj2d_run2", ddev.data_ptr(), B, rec.tiles2.data_ptr(), rec.ntiles2,
+                       rec.seg2, rec.tiles.data_ptr(), rec.ntiles, rec.seg, n0 + cnt, cnt, flip, st)
                 return flip ^ ((cnt // 2) & 1)
             N.call("bz_adj2d_run", ddev.data_ptr(), B, self.rec.tiles.data_ptr(), self.rec.ntiles, self.rec.seg,
                    n0 + cnt, cnt, st)
             return flip
 
         nseg = len(self.ckpt)
-        if self.rec.pairs and not self._rev_tuned and nseg > 1:
+        if self.rec.pairs and not self.rec.kstep and not self._rev_tuned and nseg > 1:
             # the recompute and reverse sweeps share the pair tiles: time one segment of both per tile height
             # and keep the fastest (the adjoint state and gradient are cleared again below)
             ms = {}
@@ -270,7 +290,7 @@ class AdjointProblem2D:
                     t.zero_()
             grad.zero_()
         N.call("bz_adj2d_inject", ddev.data_ptr(), B, max_inj, self.N, st)
-        flip = 0                    # pairs of reverse steps move the adjoint state across its two buffers
+        flip = 0                    # multi-step launches move the adjoint state across its two buffers
         for k in range(nseg - 1, -1, -1):
             flip = segment(k, flip)
         self._keep = (lam, inj, ddev)

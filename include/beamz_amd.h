@@ -35,9 +35,13 @@
  *                           helper.py:231-256 (monitor samples + DFT), helper.py:120-187 (power)
  *   bz_yee3d_step2 ........ two consecutive bz_yee3d_step steps in one pass (same bits)
  *   bz_yee2d_run2 ......... two consecutive bz_yee2d_run steps per pass (same bits, no grid power)
+ *   bz_yee2d_runk ......... K consecutive bz_yee2d_run steps per pass (same bits, no grid power), with
+ *                           bz_yee2d_coef filling the per-cell update coefficients it reads
  *   bz_adj2d_run / _inject  reverse-time adjoint of the 2D step for the topology gradient of
  *                           examples/4_topo.py:255-321 (the reference differentiates with a second
- *                           forward run, 4_topo.py:282-285; this is the exact discrete adjoint)
+ *                           forward run, 4_topo.py:282-285; this is the exact discrete adjoint);
+ *                           bz_adj2d_run2 / _runk: two / K reverse steps per pass (same bits)
+ *   bz_abi_sizes .......... descriptor struct sizes for binding checks; no reference counterpart
  *   bz_nn_* ............... CNN surrogate kernels; no reference counterpart
  */
 #ifndef BEAMZ_AMD_H
@@ -210,6 +214,8 @@ typedef struct bz_adj2d {
     const double* inj_w;                  /* [nf][inj_sy][inj_sx] complex, or NULL */
     const double* phasor;                 /* [num_steps][nf] complex phasors of the sample after step n */
     int nf;                               /* number of objective frequencies */
+    const float* coef;                    /* NULL, or [4][ny][ld] update coefficients as bz_yee2d_coef writes
+                                           * them for the same eps and sig (read by bz_adj2d_runk) */
 } bz_adj2d;
 
 /* Reverse steps n = n_hi-1 down to n_hi-nsteps of every adjoint described in `adjs` (device
@@ -225,6 +231,14 @@ int bz_adj2d_run2(const bz_adj2d* adjs, int nadj, const int* tiles2, int ntiles2
                   int ntiles, int seg, int n_hi, int nsteps, int flip, void* stream);
 /* lE_n += sum_f Re(inj_w[f] * phasor[n-1][f]) over each injection box (max_inj >= the largest box's cell count):
  * the terminal condition lam_N = dJ/du_N before the first reverse step. */
+/* K reverse steps per pass: nsteps / ksteps launches, each doing reverse steps n .. n-ksteps+1 over regions
+ * of 64 columns x (rows * waves) rows ((rows, waves) as for bz_yee2d_runk) whose inner cells are stored,
+ * then the nsteps % ksteps last steps with the one-step kernel (tiles, ntiles, seg as for bz_adj2d_run).
+ * Same results, bit for bit, as bz_adj2d_run. The adjoint state of lam_m sits in buffer (m + flip) & 1;
+ * a K-step launch moves it so that flip becomes (flip + ksteps - 1) & 1, which the caller tracks. */
+int bz_adj2d_runk(const bz_adj2d* adjs, int nadj, const int* tilesk, int ntilesk, int rows, int waves, int ksteps,
+                  const int* tiles, int ntiles, int seg, int n_hi, int nsteps, int flip, void* stream);
+
 int bz_adj2d_inject(const bz_adj2d* adjs, int nadj, int max_inj, int n, void* stream);
 
 /* ---------------- CNN surrogate (BF16 MFMA, FP32 accumulate) ----------------

@@ -214,29 +214,32 @@ def test_c4_gradient_vs_central_differences_on_32_pixels(hip):
 def test_pair_sweeps_give_the_same_gradient_and_engines_are_reused(hip):
     """Forward, recompute and reverse sweeps in two-step launches (checkpoint interval 23, so every segment ends
     with a single step: its snapshot slot follows the pairs', and the reverse state has moved across the two
-    adjoint buffers an odd number of times) give the one-step gradient bit for bit; later
-    evaluations on the same problem (engines reused, fields and monitor sums reset in place) repeat it, also
-    after a different design was evaluated in between."""
+    adjoint buffers an odd number of times) and in K-step launches (K = 5 and 8, whole launches then pairs or
+    single steps) give the one-step gradient bit for bit; later evaluations on the same problem (engines
+    reused, fields and monitor sums reset in place) repeat it, also after a different design was evaluated in
+    between."""
     torch = hip
     eps, sig, _ = materials(seed=2)
     x0, y0, sx, sy = BOX
     e_box = torch.from_numpy(eps[y0:y0 + sy, x0:x0 + sx]).float().cuda()[None]
     out = {}
-    for pairs in (False, True):
+    for key, pairs, kstep in (("one", False, 0), ("pairs", True, 0), ("k5", True, 5), ("k8", False, 8)):
         sim, mon, src, t = build(eps, sig, 300)
         npts = len(mon.get_grid_points_2d(DX, DX))
         c = np.random.default_rng(7).standard_normal(npts) + 1j * np.random.default_rng(8).standard_normal(npts)
-        prob = AdjointProblem2D([sim], [mon], c, BOX, checkpoint_every=23, pairs=pairs)
+        prob = AdjointProblem2D([sim], [mon], c, BOX, checkpoint_every=23, pairs=pairs, kstep=kstep)
         J = prob.forward(e_box).cpu().numpy()
         g = prob.gradient().cpu().numpy()
-        out[pairs] = (J, g)
+        out[key] = (J, g)
         assert prob.fwd.pairs == pairs and prob.rec.pairs == pairs
+        assert prob.fwd.kstep == kstep and prob.rec.kstep == kstep
         fwd, rec = prob.fwd, prob.rec
         prob.forward(e_box * 1.01)
         prob.gradient()
         J2 = prob.forward(e_box).cpu().numpy()
         g2 = prob.gradient().cpu().numpy()
         assert prob.fwd is fwd and prob.rec is rec
-        assert np.array_equal(J2, J) and np.array_equal(g2, g), pairs
-    assert np.array_equal(out[True][0], out[False][0])
-    assert np.array_equal(out[True][1], out[False][1])
+        assert np.array_equal(J2, J) and np.array_equal(g2, g), key
+    for key in ("pairs", "k5", "k8"):
+        assert np.array_equal(out[key][0], out["one"][0]), key
+        assert np.array_equal(out[key][1], out["one"][1]), key
