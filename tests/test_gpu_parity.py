@@ -200,6 +200,9 @@ def _k_case(kind):
     if kind == "gauss":
         d, dx, t, mon = waveguide_2d(nx=150, ny=90, steps=203, npml=10, src_x=20, mon_x=120, n_freq=4)
         return d, dx, t, mon
+    if kind == "tiny":       # smaller than one region's inner cells: the region overhangs every edge
+        d, dx, t, mon = waveguide_2d(nx=30, ny=14, steps=77, npml=3, src_x=6, mon_x=25, n_freq=2)
+        return d, dx, t, mon
     lam = 1.55 * um
     d, dx = waveguide_design_2d(nx=140, ny=70, npml=10)
     dt = 0.99 * dx / (bz.LIGHT_SPEED * np.sqrt(2))
@@ -217,7 +220,7 @@ def _k_case(kind):
 def test_k_steps_per_pass_single_grid(hip, K, rows, waves):
     """One grid through the K-steps-per-pass kernel gives the bits of one step per launch: fields, the
     monitor's time series and its FP64 DFT, with a Gaussian point source and with a ModeSource line."""
-    for kind in ("gauss", "mode"):
+    for kind in ("gauss", "mode", "tiny"):
         out = []
         for k in (0, K):
             d, dx, t, mon = _k_case(kind)
