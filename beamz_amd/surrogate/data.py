@@ -74,9 +74,11 @@ class TransmissionProblem:
         if not 0 < chunk.shape[0] <= len(self.sims):
             raise ValueError(f"a chunk holds 1..{len(self.sims)} density maps")
         self.batch.reset()
-        for k, s in enumerate(self.sims):
-            e = self.eps_of(chunk[k] if k < chunk.shape[0] else chunk[0])
-            s.grid.eps[self.y0:self.y0 + self.px, self.x0:self.x0 + self.px] = e
+        # permittivities of the whole chunk in one op, written into the members' design boxes in one
+        # multi-tensor copy (members past the chunk repeat its first map)
+        e = self.eps_of(chunk)
+        src = [e[k] if k < chunk.shape[0] else e[0] for k in range(len(self.sims))]
+        T._foreach_copy_([s.grid.eps[self.y0:self.y0 + self.px, self.x0:self.x0 + self.px] for s in self.sims], src)
         self.batch.advance(self.batch.num_steps)
 
     def collect(self, m):
