@@ -201,7 +201,8 @@ class AdjointProblem2D:
             d.phasor = self._phasor[b].data_ptr()
             d.nf = self.nf
             # the recompute engine's coefficient arrays (it refreshes them at the start of every segment)
-            d.coef = self.rec.coefs[b].data_ptr() if self.rec.kstep else None
+            cf = self.rec.coefs[b] if self.rec.kstep else None
+            d.coef = cf.data_ptr() if cf is not None else None
             descs.append(d)
         return descs
 
