@@ -106,7 +106,15 @@ class AdjointProblem2D:
         for s in self.sims:
             s.initialize_simulation(save=False, live=False)
         self.fwd = Fused2D(grids, seg=self.seg, max_steps=self.N, pairs=self.pairs, kstep=self.kstep)
-        if self.fwd.pairs and not self.fwd.kstep and self.N >= 24:
+        if self.fwd.kstep and self.N >= 64:
+            # region shape of the K-step sweeps, timed once, among the shapes whose K divides the checkpoint
+            # interval (every segment then runs as whole K-step launches); the recompute and reverse sweeps
+            # use the same one
+            shapes = [c for c in self.fwd.KCANDIDATES if self.K % c[0] == 0]
+            if len(shapes) > 1:
+                self.fwd.tune_kstep(steps=16, candidates=shapes)
+                self._reset()
+        elif self.fwd.pairs and self.N >= 24:
             self.fwd.tune_seg2(steps=10)
             self._reset()
         x0, y0, sx, sy = self.box
