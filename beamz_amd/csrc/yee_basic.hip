@@ -136,6 +136,41 @@ __global__ void k_gather(CompBase base, const int* __restrict__ comp, const int6
     }
 }
 
+// The samples of two consecutive steps (n from base0/off0, n+1 from base1/off1) in one launch, thread per
+// (point, frequency), p fastest; each accumulator receives step n's term and then step n+1's, the two
+// rounded adds of two k_gather launches in that order.
+__global__ void k_gather2(CompBase base0, const int64_t* __restrict__ off0, CompBase base1,
+                          const int* __restrict__ comp, const int64_t* __restrict__ off1, int n, int act0, int act1,
+                          float* __restrict__ series0, float* __restrict__ series1, int nfreq,
+                          const double* __restrict__ phasor0, const double* __restrict__ phasor1,
+                          double* __restrict__ dft) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = (int)(g % n), k = (int)(g / n);
+    if (k >= max(nfreq, 1)) return;
+    const int c = comp[p];
+    const float v0 = act0 ? base0.p[c][off0[p]] : 0.0f;
+    const float v1 = act1 ? base1.p[c][off1[p]] : 0.0f;
+    if (k == 0) {
+        if (act0 && series0) series0[p] = v0;
+        if (act1 && series1) series1[p] = v1;
+    }
+    if (k >= nfreq) return;
+    double* a = dft + 2 * ((long)k * n + p);
+    double re = a[0], im = a[1];
+    if (act0) {
+        const double dv = (double)v0;
+        re += dv * phasor0[2 * k];
+        im += dv * phasor0[2 * k + 1];
+    }
+    if (act1) {
+        const double dv = (double)v1;
+        re += dv * phasor1[2 * k];
+        im += dv * phasor1[2 * k + 1];
+    }
+    a[0] = re;
+    a[1] = im;
+}
+
 // helper.py:152-187 (real fields)
 __global__ __launch_bounds__(TX* TY) void k_pow2d(double* __restrict__ acc, const float* __restrict__ ez,
                                                    const float* __restrict__ hx, const float* __restrict__ hy,
@@ -282,5 +317,26 @@ extern "C" int bz_nonfinite_count(const float* p, long n, unsigned int* count, v
     const long blocks = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
     hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, p, n, count);
     BZ_CHECK_LAUNCH("bz_nonfinite_count");
+    return BZ_OK;
+}
+
+extern "C" int bz_gather2(const float* const base0[6], const int64_t* off0, const float* const base1[6],
+                          const int* comp, const int64_t* off1, int n, int act0, int act1, float* series0,
+                          float* series1, int nfreq, const double* phasor0, const double* phasor1, double* dft,
+                          void* stream) {
+    if (n == 0 || (!act0 && !act1)) return BZ_OK;
+    BZ_REQUIRE(base0 && off0 && base1 && comp && off1 && n > 0 && nfreq >= 0, "bz_gather2: bad arguments");
+    BZ_REQUIRE(nfreq == 0 || (dft && (!act0 || phasor0) && (!act1 || phasor1)),
+               "bz_gather2: DFT requested without buffers");
+    CompBase c0, c1;
+    for (int q = 0; q < 6; ++q) {
+        c0.p[q] = base0[q];
+        c1.p[q] = base1[q];
+    }
+    const long threads = (long)n * (nfreq > 0 ? nfreq : 1);
+    BZ_REQUIRE(threads / 256 < (1L << 31) - 1, "bz_gather2: too many points x frequencies");
+    hipLaunchKernelGGL(k_gather2, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, (hipStream_t)stream, c0, off0,
+                       c1, comp, off1, n, act0, act1, series0, series1, nfreq, phasor0, phasor1, dft);
+    BZ_CHECK_LAUNCH("bz_gather2");
     return BZ_OK;
 }

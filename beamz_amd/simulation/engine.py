@@ -471,10 +471,17 @@ class Grid:
             e1.record()
             self._events.append((e0, e1, 2))
         self._cur = 1 - self._cur
-        if need:
-            for t in stg["plans"]:
-                self._gather(stg["base"], t, n, off=t.stage_off)
-        self._gathers(n + 1)
+        # the samples of both steps, one launch per table: step n from the staging box, step n+1 from the fields
+        base1 = N.ptrs6([self.f[k].data_ptr() for k in self.names] + [0] * (6 - len(self.names)))
+        for t in stg["plans"]:
+            q0, q1 = t.slot.get(n), t.slot.get(n + 1)
+            if q0 is None and q1 is None:
+                continue
+            ser = lambda q: t.series[q].data_ptr() if (t.series is not None and q is not None) else None
+            ph = lambda q: t.phasor[q].data_ptr() if (t.nfreq and q is not None) else None
+            N.call("bz_gather2", stg["base"], t.stage_off.data_ptr(), base1, t.comp.data_ptr(), t.off.data_ptr(),
+                   t.n, int(q0 is not None), int(q1 is not None), ser(q0), ser(q1), t.nfreq, ph(q0), ph(q1),
+                   t.dft.data_ptr() if t.nfreq else None, self.stream)
 
     # ---------------- fused, batched 2D path ----------------
     def invalidate(self):

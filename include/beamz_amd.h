@@ -284,6 +284,13 @@ int bz_affine_points(float* field, const int64_t* off, const float* mul, const f
                      void* stream);
 
 /* ---------------- monitors ---------------- */
+/* Two consecutive steps' samples in one launch: step n from comp_base0/off0 (if act0), step n+1 from
+ * comp_base1/off1 (if act1), same comp table; series rows series0/series1 (either may be NULL), phasor rows
+ * phasor0/phasor1; each accumulator gets step n's term and then step n+1's (the bits of two bz_gather calls). */
+int bz_gather2(const float* const comp_base0[6], const int64_t* off0, const float* const comp_base1[6],
+               const int* comp, const int64_t* off1, int n, int act0, int act1, float* series0, float* series1,
+               int nfreq, const double* phasor0, const double* phasor1, double* dft, void* stream);
+
 /* For p < n: v = comp_base[comp[p]][off[p]].
  *   series (if not NULL): series[p] = v            (caller offsets it to the step's row)
  *   dft (if nfreq > 0):   dft[k*n + p] += v * phasor[k]   for k < nfreq, FP64 complex
