@@ -502,6 +502,32 @@ class Grid:
         return True
 
 
+def kstep_owner_tables(tiles, pts, dims, K, rh):
+    """Monitor-point owner tables of the K-step kernel (bz_yee2d_runk).
+
+    tiles: (ntiles, 3) = (sim, x-tile, y-tile) in units of the inner size (64 - 2K) x (rh - 2K), sim < 0 for
+    padding; pts: (npts, 3) = (sim, comp, offset into [ny][ld]); dims: (nsims, 3) = (nx, ny, ld). Each point
+    belongs to the tile whose inner cells hold it. Returns (start, ents): start[t] .. start[t+1]-1 index the
+    rows of ents that tile t owns, sorted by component, ents[e] = (point index, (region row << 10) |
+    (region column << 2) | comp)."""
+    tiles = np.asarray(tiles, np.int64).reshape(-1, 3)
+    pts = np.asarray(pts, np.int64).reshape(-1, 3)
+    ix, iy = 64 - 2 * K, rh - 2 * K
+    sim, comp = pts[:, 0], pts[:, 1]
+    nxs, nys, lds = (dims[sim, c] for c in range(3))
+    j, i = np.divmod(pts[:, 2], lds)
+    if np.any(i >= nxs) or np.any(j >= nys):
+        raise ValueError("a monitor point lies outside its grid")
+    tx, ty = i // ix, j // iy
+    index = {(int(a), int(b), int(c)): q for q, (a, b, c) in enumerate(tiles) if a >= 0}
+    t = np.array([index[(int(a), int(b), int(c))] for a, b, c in zip(sim, tx, ty)], np.int64)
+    code = ((j - (ty * iy - K)) << 10) | ((i - (tx * ix - K)) << 2) | comp
+    order = np.lexsort((comp, t))
+    ents = np.stack([order, code[order]], axis=1).astype(np.int32)
+    start = np.searchsorted(t[order], np.arange(len(tiles) + 1)).astype(np.int32)
+    return start, ents
+
+
 def xcd_order(tiles, per_wg=4, nxcd=8):
     """Reorder a row-major tile list (rows of (sim, x-tile, y-tile)) for the XCDs of the GPU.
 
@@ -793,21 +819,8 @@ class Fused2D:
         self.ntilesk = len(tiles)
         self.ent_start = self.ents = self.stagek = None
         if self.npts:
-            pts = self._pts_host
-            sim, comp = pts[:, 0].astype(np.int64), pts[:, 1].astype(np.int64)
-            lds = np.array([g.ld for g in self.grids], np.int64)[sim]
-            j, i = np.divmod(pts[:, 2].astype(np.int64), lds)
-            nxs = np.array([g.nx for g in self.grids], np.int64)[sim]
-            nys = np.array([g.ny for g in self.grids], np.int64)[sim]
-            if np.any(i >= nxs) or np.any(j >= nys):
-                raise ValueError("a monitor point lies outside its grid")
-            tx, ty = i // ix, j // iy
-            index = {(int(a), int(b), int(c)): q for q, (a, b, c) in enumerate(tiles) if a >= 0}
-            t = np.array([index[(int(a), int(b), int(c))] for a, b, c in zip(sim, tx, ty)], np.int64)
-            code = ((j - (ty * iy - K)) << 10) | ((i - (tx * ix - K)) << 2) | comp
-            order = np.lexsort((comp, t))
-            ents = np.stack([order, code[order]], axis=1).astype(np.int32)
-            start = np.searchsorted(t[order], np.arange(self.ntilesk + 1)).astype(np.int32)
+            dims = np.array([(g.nx, g.ny, g.ld) for g in self.grids], np.int64)
+            start, ents = kstep_owner_tables(tiles, self._pts_host, dims, K, rows * waves)
             self.ents = T.from_numpy(ents.ravel()).to(self.dev)
             self.ent_start = T.from_numpy(start).to(self.dev)
             self.stagek = T.zeros(2 * K * self.npts, dtype=T.float32, device=self.dev)   # two slots: q & 1

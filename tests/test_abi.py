@@ -66,3 +66,37 @@ def test_descriptor_struct_layouts_match_the_library():
     out = (C.c_int * 3)()
     N.call("bz_abi_sizes", out, 3)
     assert list(out) == [C.sizeof(N.Sim2D), C.sizeof(N.Adj2D), C.sizeof(N.BoxSource)]
+
+
+def test_kstep_owner_tables_give_each_point_one_owner():
+    """Host tables of the K-step kernel: every monitor point is listed once, by the tile whose inner cells
+    hold it, and its packed (region row, region column, component) decodes back to its grid cell."""
+    import numpy as np
+    from beamz_amd.simulation.engine import kstep_owner_tables, xcd_order
+    rng = np.random.default_rng(5)
+    dims = np.array([(150, 90, 192), (37, 260, 64), (301, 33, 320)], np.int64)   # (nx, ny, ld)
+    for K, rh in ((8, 64), (1, 32), (15, 32), (31, 64)):
+        ix, iy = 64 - 2 * K, rh - 2 * K
+        rows = []
+        for s, (nx, ny, _) in enumerate(dims):
+            gx, gy = -(-nx // ix), -(-ny // iy)
+            t = np.stack(np.meshgrid(np.arange(gx), np.arange(gy), indexing="xy"), -1).reshape(-1, 2)
+            rows.append(np.concatenate([np.full((len(t), 1), s), t], axis=1))
+        tiles = xcd_order(np.concatenate(rows), per_wg=1)
+        sim = rng.integers(0, 3, 400)
+        jj = rng.integers(0, dims[sim, 1])
+        ii = rng.integers(0, dims[sim, 0])
+        comp = rng.integers(0, 3, 400)
+        pts = np.stack([sim, comp, jj * dims[sim, 2] + ii], axis=1)
+        start, ents = kstep_owner_tables(tiles, pts, dims, K, rh)
+        assert len(start) == len(tiles) + 1 and start[-1] == len(pts)
+        assert sorted(ents[:, 0].tolist()) == list(range(len(pts)))
+        for t in range(len(tiles)):
+            e = ents[start[t]:start[t + 1]]
+            assert np.all(np.diff(e[:, 1] & 3) >= 0)              # sorted by component within a tile
+            for p, code in e:
+                s, tx, ty = tiles[t]
+                row, col, c = code >> 10, (code >> 2) & 255, code & 3
+                assert K <= row < rh - K and K <= col < 64 - K     # an inner cell of the region
+                assert (sim[p], comp[p]) == (s, c)
+                assert (ty * iy - K + row, tx * ix - K + col) == (jj[p], ii[p])
