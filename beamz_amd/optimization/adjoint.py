@@ -164,7 +164,7 @@ class AdjointProblem2D:
             T._foreach_copy_([t for ck in self.ckpt[k] for t in ck.unbind(0)],
                              [g.f[c] for g in grids for c in ("Ez", "Hx", "Hy")])
             cnt = min(self.K, self.N - n)
-            self.fwd.run(n, cnt)
+            self.fwd.run(n, cnt, coef=(k == 0))       # the materials are fixed for the whole sweep
             n += cnt
         for s in self.sims:
             s.current_step = self.N
@@ -253,13 +253,16 @@ class AdjointProblem2D:
         max_inj = max(i[2] * i[3] for i in inj)
         st = grids[0].stream
 
+        first = [True]
+
         def segment(k, flip):
             """Recompute segment k from its checkpoint (snapshots on), then walk it backwards; returns flip."""
             n0 = k * self.K
             cnt = min(self.K, self.N - n0)
             T._foreach_copy_([g.f[c] for g in grids for c in ("Ez", "Hx", "Hy")],
                              [t for ck in self.ckpt[k] for t in ck.unbind(0)])
-            self.rec.run(n0, cnt)
+            self.rec.run(n0, cnt, coef=first[0])      # coefficients once per gradient: the materials are fixed
+            first[0] = False
             for d in descs:
                 d.snap_step0 = n0
             # stream-ordered upload from pinned memory: the host does not wait for the GPU to drain
