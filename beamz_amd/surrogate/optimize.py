@@ -9,7 +9,7 @@
   random pick, which is what the surrogate gains over screening without it.
 * ``SurrogateLoop``: rounds of propose (mutations of the best verified designs
   climbed along the surrogate's input gradient) -> rank -> verify the top-k
-  with FDTD -> fine-tune on the verified samples; FDTD stays the judge of
+  with FDTD -> fine-tune on the verified samples; FDTD stays the arbiter of
   every accepted design.
 """
 from __future__ import annotations

@@ -409,7 +409,7 @@ def main():
     if a.warmup is None:
         a.warmup = 1 if a.workload == "adjoint2d" else 20
     if a.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        print(json.dumps({"error": "bench.py measures one GPU (the engine has no multi-GPU path); run with --gpus 1"}))
         sys.exit(2)
     import torch
 
