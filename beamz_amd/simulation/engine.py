@@ -574,7 +574,7 @@ class Fused2D:
         cells on; never with graphs), with tiles of 60 columns x seg2 rows (None: by size); xcd: order the tile
         lists by xcd_order (None: XCD_ORDER); kstep: advance K steps per pass (bz_yee2d_runk, regions of 64 x
         krows*kwaves cells) for runs of at least K steps, the rest by pairs or single steps (0: off; None: on
-        with auto_kstep's shape wherever it applies: no graphs, no grid power)."""
+        with auto_kstep's shape wherever it applies: no grid power)."""
         if not grids:
             raise ValueError("Fused2D needs at least one grid")
         self.snapshots = list(snapshots) if snapshots is not None else [None] * len(grids)
@@ -747,7 +747,7 @@ class Fused2D:
     KCANDIDATES = ((8, 8, 8), (6, 8, 8), (8, 8, 4), (8, 4, 8), (12, 4, 16))
 
     def kstep_applies(self):
-        return not self.use_graph and all(g.power is None for g in self.grids)
+        return all(g.power is None for g in self.grids)
 
     def auto_kstep(self):
         """(K, rows, waves): tall 64 x 64 regions from PAIR_MIN_CELLS cells on, else 64 x 32 regions, which
@@ -793,8 +793,6 @@ class Fused2D:
             raise ValueError(f"K steps per pass need (rows, waves) in {self.KSHAPES} and 2K < min(64, rows*waves)")
         if any(g.power is not None for g in self.grids):
             raise ValueError("K steps per pass do not accumulate grid power")
-        if self.use_graph:
-            raise ValueError("K steps per pass are not captured in graphs")
         ix, iy = 64 - 2 * K, rows * waves - 2 * K
         parts = []
         for s, g in enumerate(self.grids):
@@ -805,6 +803,7 @@ class Fused2D:
         if self.xcd:
             tiles = xcd_order(tiles, per_wg=1)
         self.kstep, self.krows, self.kwaves = K, rows, waves
+        self.graphs = {}                     # captured launches hold the tables' addresses
         if self.coefs is None:
             # per-cell update coefficients, computed at the start of every run (bz_yee2d_coef): the K-step kernel
             # then loads four numbers per cell instead of evaluating three divisions. The kernels address them
@@ -938,6 +937,13 @@ class Fused2D:
         self.desc_dev = cached
         self.shift = key
 
+    def _launch_k(self, nl, rd0):
+        ptr = lambda t: t.data_ptr() if t is not None else None
+        N.call("bz_yee2d_runk", self.desc_dev.data_ptr(), len(self.grids), self.tilesk.data_ptr(), self.ntilesk,
+               self.krows, self.kwaves, self.kstep, self.step.data_ptr(), rd0, nl, ptr(self.ent_start),
+               ptr(self.ents), ptr(self.stagek), self.npts, ptr(self.slot), ptr(self.series), self.nfreq,
+               ptr(self.phasor), ptr(self.dft), self.grids[0].stream)
+
     def _launch(self, count):
         ptr = lambda t: t.data_ptr() if t is not None else None
         g0 = self.grids[0]
@@ -975,10 +981,19 @@ class Fused2D:
                 N.call("bz_yee2d_coef", self.desc_dev.data_ptr(), len(self.grids),
                        max(g.ny * g.ld for g in self.grids), g0.stream)
                 self._coef_done = True
-            N.call("bz_yee2d_runk", self.desc_dev.data_ptr(), len(self.grids), self.tilesk.data_ptr(), self.ntilesk,
-                   self.krows, self.kwaves, K, self.step.data_ptr(), n0 & 1, nl, ptr(self.ent_start), ptr(self.ents),
-                   ptr(self.stagek), self.npts, ptr(self.slot), ptr(self.series), self.nfreq, ptr(self.phasor),
-                   ptr(self.dft), g0.stream)
+            if self.use_graph:
+                # one captured graph per (launch count, first buffer); the step base lives in device memory
+                key = ("k", nl, n0 & 1)
+                gr = self.graphs.get(key)
+                if gr is None:
+                    gr = T.cuda.CUDAGraph()
+                    T.cuda.synchronize(self.dev)
+                    with T.cuda.graph(gr):
+                        self._launch_k(nl, n0 & 1)
+                    self.graphs[key] = gr
+                gr.replay()
+            else:
+                self._launch_k(nl, n0 & 1)
             for g in self.grids:
                 g._cur = (g._cur + nl) % 2
             n0, count = n0 + nl * K, count - nl * K

@@ -70,9 +70,10 @@ def test_batch_graph_replays_equal_eager(hip):
     a = members(steps=320)
     b = members(steps=320)
     eager = FDTDBatch([s for s, _ in a], seg=8, graph=False, chunk=40, kstep=0)
-    graph = FDTDBatch([s for s, _ in b], seg=8, graph=True, chunk=40)
+    graph = FDTDBatch([s for s, _ in b], seg=8, graph=True, chunk=40)     # K-step launches, captured
     eager.initialize()
     graph.initialize()
+    assert graph.engine.use_graph and graph.engine.kstep
     for r in range(8):                      # the 40-step graph is captured once and replayed 8 times
         eager.advance(40)
         graph.advance(40)
