@@ -100,12 +100,12 @@ template <int R>
 struct Carry2 {
     float ex0[R], ey0[R];              // E(step n) x, y at plane k
     float h1x[R], h1y[R], h1z[R];      // H(step n+1) at plane k-1
-    float sg1[R], ee1[R];              // sigma, eps at plane k-1
+    float fa1[R], fb1[R], ca1[R], cb1[R];   // update coefficients at plane k-1
     float e1x[R], e1y[R], e1z[R];      // E(step n+1) at plane k-1
     float f1x[R], f1y[R], f1z[R];      // E(step n+1) at plane k-2
     float h2x[R], h2y[R], h2z[R];      // H(step n+2) at plane k-2
     float g2x[R], g2y[R];              // H(step n+2) x, y at plane k-3
-    float sg2[R], ee2[R];              // sigma, eps at plane k-2
+    float ca2[R], cb2[R];              // E coefficients at plane k-2
 };
 
 template <int R, int W>
@@ -151,8 +151,11 @@ __global__ __launch_bounds__(LANES* W) void k_yee3d_tb2(Fields6 S, Fields6 D, co
         C.f1x[r] = C.f1y[r] = C.f1z[r] = 0.0f;
         C.h2x[r] = C.h2y[r] = C.h2z[r] = 0.0f;
         C.g2x[r] = C.g2y[r] = 0.0f;
-        C.sg1[r] = C.sg2[r] = 0.0f;
-        C.ee1[r] = C.ee2[r] = 1.0f;
+        // the coefficients of sigma 0, eps 1 (planes before the march has loaded any)
+        hcoef(0.0f, c, C.fa1[r], C.fb1[r]);
+        ecoef(0.0f, 1.0f, c, C.ca1[r], C.cb1[r]);
+        C.ca2[r] = C.ca1[r];
+        C.cb2[r] = C.cb1[r];
     }
     const int kstart = k0 > 0 ? k0 - 2 : 0;
     {
@@ -180,6 +183,13 @@ __global__ __launch_bounds__(LANES* W) void k_yee3d_tb2(Fields6 S, Fields6 D, co
             sg[r] = at(sig, kp + ro[r]);
             ee[r] = at(eps, kp + ro[r]);
         }
+        // the update coefficients of plane k, once: step n+1 uses them here, step n+2 one and two planes later
+        float fa0[R], fb0[R], ca0[R], cb0[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            hcoef(sg[r], c, fa0[r], fb0[r]);
+            ecoef(sg[r], ee[r], c, ca0[r], cb0[r]);
+        }
         // ---- exchange A: first rows go up a wave, last rows go down ----
         xa[w][0][l] = ez[0];
         xa[w][1][l] = C.ex0[0];
@@ -202,8 +212,7 @@ __global__ __launch_bounds__(LANES* W) void k_yee3d_tb2(Fields6 S, Fields6 D, co
                 const float exN = r + 1 < R ? C.ex0[r + 1] : dEx0;
                 const float ezR = __shfl_down(ez[r], 1, LANES);
                 const float eyR = __shfl_down(C.ey0[r], 1, LANES);
-                float fa, fb;
-                hcoef(sg[r], c, fa, fb);
+                const float fa = fa0[r], fb = fb0[r];
                 const bool jh = j < ny - 1;
                 h1x[r] = (kh && jh) ? fa * hx[r] - fb * ((ezN - ez[r]) * rdy - (ey1[r] - C.ey0[r]) * rdz) : hx[r];
                 h1y[r] = (kh && hyok) ? fa * hy[r] - fb * ((ex1[r] - C.ex0[r]) * rdz - (ezR - ez[r]) * rdx) : hy[r];
@@ -229,8 +238,7 @@ __global__ __launch_bounds__(LANES* W) void k_yee3d_tb2(Fields6 S, Fields6 D, co
                 const float hzL = __shfl_up(h1z[r], 1, LANES);
                 const float hyL = __shfl_up(h1y[r], 1, LANES);
                 const bool yin = j >= 1 && j <= ny - 2;
-                float ca, cb;
-                ecoef(sg[r], ee[r], c, ca, cb);
+                const float ca = ca0[r], cb = cb0[r];
                 float ex = (kin && yin && xe) ? ca * C.ex0[r] + cb * ((h1z[r] - hzU) * rdy - (h1y[r] - C.h1y[r]) * rdz)
                                               : C.ex0[r];
                 float ey = (kin && j <= ny - 2 && xin)
@@ -275,8 +283,7 @@ __global__ __launch_bounds__(LANES* W) void k_yee3d_tb2(Fields6 S, Fields6 D, co
                 const float exN = r + 1 < R ? C.e1x[r + 1] : dE1x;
                 const float ezR = __shfl_down(C.e1z[r], 1, LANES);
                 const float eyR = __shfl_down(C.e1y[r], 1, LANES);
-                float fa, fb;
-                hcoef(C.sg1[r], c, fa, fb);
+                const float fa = C.fa1[r], fb = C.fb1[r];
                 const bool jh = j < ny - 1;
                 h2x[r] = (kh && jh) ? fa * C.h1x[r] - fb * ((ezN - C.e1z[r]) * rdy - (e1y[r] - C.e1y[r]) * rdz)
                                     : C.h1x[r];
@@ -301,8 +308,7 @@ __global__ __launch_bounds__(LANES* W) void k_yee3d_tb2(Fields6 S, Fields6 D, co
                 const float hzL = __shfl_up(C.h2z[r], 1, LANES);
                 const float hyL = __shfl_up(C.h2y[r], 1, LANES);
                 const bool yin = j >= 1 && j <= ny - 2;
-                float ca, cb;
-                ecoef(C.sg2[r], C.ee2[r], c, ca, cb);
+                const float ca = C.ca2[r], cb = C.cb2[r];
                 float ex = (kin && yin && xe)
                                ? ca * C.f1x[r] + cb * ((C.h2z[r] - hzU) * rdy - (C.h2y[r] - C.g2y[r]) * rdz)
                                : C.f1x[r];
@@ -340,10 +346,12 @@ __global__ __launch_bounds__(LANES* W) void k_yee3d_tb2(Fields6 S, Fields6 D, co
             C.h1x[r] = h1x[r];
             C.h1y[r] = h1y[r];
             C.h1z[r] = h1z[r];
-            C.sg2[r] = C.sg1[r];
-            C.ee2[r] = C.ee1[r];
-            C.sg1[r] = sg[r];
-            C.ee1[r] = ee[r];
+            C.ca2[r] = C.ca1[r];
+            C.cb2[r] = C.cb1[r];
+            C.fa1[r] = fa0[r];
+            C.fb1[r] = fb0[r];
+            C.ca1[r] = ca0[r];
+            C.cb1[r] = cb0[r];
             C.ex0[r] = ex1[r];
             C.ey0[r] = ey1[r];
         }
