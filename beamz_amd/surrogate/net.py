@@ -29,12 +29,22 @@ class Surrogate:
                 P[f"b{lvl}{s}"] = np.zeros(o)
         P["fw"] = rng.standard_normal((self.nout, LEVELS[-1][1])) * np.sqrt(1.0 / LEVELS[-1][1])
         P["fb"] = np.zeros(self.nout)
-        self.params = {k: torch.tensor(v, dtype=torch.float32, device=self.dev).contiguous() for k, v in P.items()}
+        # parameters, gradients and Adam moments each live in one flat buffer (every tensor a 256-byte aligned
+        # view), so that zeroing the gradients and the Adam step are one launch each
+        offs, o = {}, 0
+        for k, v in P.items():
+            offs[k] = o
+            o += -(-v.size // 64) * 64
+        self._flat = {n: torch.zeros(o, dtype=torch.float32, device=self.dev) for n in ("p", "g", "m", "v")}
+
+        def views(buf):
+            return {k: buf[offs[k]:offs[k] + v.size].view(v.shape) for k, v in P.items()}
+        self.params, self.grads = views(self._flat["p"]), views(self._flat["g"])
+        self.m, self.v = views(self._flat["m"]), views(self._flat["v"])
+        for k, v in P.items():
+            self.params[k].copy_(torch.from_numpy(np.asarray(v, dtype=np.float32)))
         # outputs are head outputs times out_scale (set from the training targets, so the head works at O(1))
         self.out_scale = torch.ones(self.nout, dtype=torch.float32, device=self.dev)
-        self.m = {k: torch.zeros_like(v) for k, v in self.params.items()}
-        self.v = {k: torch.zeros_like(v) for k, v in self.params.items()}
-        self.grads = {k: torch.zeros_like(v) for k, v in self.params.items()}
         self.step_count = 0
         self.work, self.work_for = None, None      # workspace of the ordered gradient reductions
         self.w16, self.wt16 = {}, {}
@@ -126,8 +136,7 @@ class Surrogate:
         T = self.torch
         cache = self.cache
         st = self.stream
-        for g in self.grads.values():
-            g.zero_()
+        self._flat["g"].zero_()
         B, H, W = cache["shape"]
         work = self._workspace(B, H, W)
         wp, wn = work.data_ptr(), work.numel()
@@ -179,10 +188,9 @@ class Surrogate:
     # ---------------- training / inference ----------------
     def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
         self.step_count += 1
-        st = self.stream
-        for k, p in self.params.items():
-            N.call("bz_nn_adam", p.data_ptr(), self.grads[k].data_ptr(), self.m[k].data_ptr(), self.v[k].data_ptr(),
-                   p.numel(), lr, b1, b2, eps, self.step_count, st)
+        f = self._flat          # one elementwise update over every parameter (the padding stays zero)
+        N.call("bz_nn_adam", f["p"].data_ptr(), f["g"].data_ptr(), f["m"].data_ptr(), f["v"].data_ptr(),
+               f["p"].numel(), lr, b1, b2, eps, self.step_count, self.stream)
         self.sync_weights()
 
     def train_step(self, x, target, lr=1e-3):
