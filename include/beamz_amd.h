@@ -20,9 +20,9 @@
  *   bz_update_3d .......... FDTD._update_3d_fields, beamz/simulation/fdtd.py:200-310
  *   bz_add_points / bz_affine_points
  *                           source injection, beamz/simulation/helper.py:5-118
- *   bz_gather ............. monitor sampling, beamz/simulation/helper.py:231-256 with
- *                           beamz/devices/monitors.py:279-397, plus the on-the-fly DFT
- *                           (no reference counterpart)
+ *   bz_gather / bz_gather2  monitor sampling of one step / of both steps of a two-step pass,
+ *                           beamz/simulation/helper.py:231-256 with beamz/devices/monitors.py:279-397,
+ *                           plus the on-the-fly DFT (no reference counterpart)
  *   bz_grid_power_2d/3d ... grid power accumulation, beamz/simulation/helper.py:120-187
  *   bz_raster_2d/3d ....... RegularGrid.__rasterize__ / RegularGrid3D.__rasterize_3d__ +
  *                           _process_3d_pml, beamz/simulation/meshing.py:92-536, 616-781,
