@@ -386,13 +386,13 @@ __global__ __launch_bounds__(LANES* KW) void k_adj2d_tbk(const bz_adj2d* __restr
             ly[r] = at(gly, o[r]);
         }
         if (cf) {
-            const uint32_t pb = (uint32_t)ny * (uint32_t)ld * 4u;
 #pragma unroll
             for (int r = 0; r < B; ++r) {
-                fa[r] = at(cf, o[r]);
-                fb[r] = at(cf, pb + o[r]);
-                ca[r] = at(cf, 2u * pb + o[r]);
-                cb[r] = at(cf, 3u * pb + o[r]);
+                const float4 q = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(cf) + 4u * o[r]);
+                fa[r] = q.x;
+                fb[r] = q.y;
+                ca[r] = q.z;
+                cb[r] = q.w;
             }
         } else {
 #pragma unroll

@@ -449,10 +449,7 @@ __global__ __launch_bounds__(256) void k_coef2d(const bz_sim2d* __restrict__ sim
         float fa, fb, ca, cb;
         h_coef(S.sig[o], c, fa, fb);
         e_coef(S.sig[o], S.eps[o], c, ca, cb);
-        cf[o] = fa;
-        cf[plane + o] = fb;
-        cf[2 * plane + o] = ca;
-        cf[3 * plane + o] = cb;
+        reinterpret_cast<float4*>(cf)[o] = make_float4(fa, fb, ca, cb);
     }
 }
 
@@ -577,14 +574,14 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
             hy[r] = at(ghy, o[r]);
         }
         if (cf) {
-            // precomputed by k_coef2d (the same roundings)
-            const uint32_t pb = (uint32_t)ny * (uint32_t)ld * 4u;
+            // precomputed by k_coef2d (the same roundings), one 16-byte (fa, fb, ca, cb) per cell
 #pragma unroll
             for (int r = 0; r < B; ++r) {
-                fa[r] = at(cf, o[r]);
-                fb[r] = at(cf, pb + o[r]);
-                ca[r] = at(cf, 2u * pb + o[r]);
-                cb[r] = at(cf, 3u * pb + o[r]);
+                const float4 q = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(cf) + 4u * o[r]);
+                fa[r] = q.x;
+                fb[r] = q.y;
+                ca[r] = q.z;
+                cb[r] = q.w;
             }
         } else {
 #pragma unroll

@@ -808,7 +808,7 @@ class Fused2D:
             # per-cell update coefficients, computed at the start of every run (bz_yee2d_coef): the K-step kernel
             # then loads four numbers per cell instead of evaluating three divisions. The kernels address them
             # with 32-bit byte offsets; a grid too large for that keeps coef NULL and the kernels divide.
-            self.coefs = [T.empty((4, g.ny, g.ld), dtype=T.float32, device=self.dev)
+            self.coefs = [T.empty((g.ny, g.ld, 4), dtype=T.float32, device=self.dev)
                           if 16 * g.ny * g.ld < 2 ** 32 else None for g in self.grids]
             for d, c in zip(self.descs, self.coefs):
                 d.coef = c.data_ptr() if c is not None else None

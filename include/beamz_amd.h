@@ -144,7 +144,7 @@ typedef struct bz_sim2d {
     float* snap;                          /* [nsteps][3][snap_sy][snap_sx] */
     float* stg;                           /* bz_yee2d_run2: [2][3][stg_sy][stg_sx] staging of the first step's */
     int stg_x0, stg_y0, stg_sx, stg_sy;   /* Ez, Hx, Hy for monitor samples, slot q & 1 for pair q (NULL: none) */
-    float* coef;                          /* NULL, or [4][ny][ld] FP32 update coefficients fa, fb, ca, cb of every
+    float* coef;                          /* NULL, or [ny][ld][4] FP32 update coefficients (fa, fb, ca, cb) of every
                                            * cell as bz_yee2d_coef writes them (read by bz_yee2d_runk) */
 } bz_sim2d;
 
@@ -214,7 +214,7 @@ typedef struct bz_adj2d {
     const double* inj_w;                  /* [nf][inj_sy][inj_sx] complex, or NULL */
     const double* phasor;                 /* [num_steps][nf] complex phasors of the sample after step n */
     int nf;                               /* number of objective frequencies */
-    const float* coef;                    /* NULL, or [4][ny][ld] update coefficients as bz_yee2d_coef writes
+    const float* coef;                    /* NULL, or [ny][ld][4] update coefficients as bz_yee2d_coef writes
                                            * them for the same eps and sig (read by bz_adj2d_runk) */
 } bz_adj2d;
 
