@@ -60,8 +60,8 @@ MEASURED_TRAFFIC = {
     # (same profile)
     ("batch2d", "pairs-seg8-xcd", 64): (2 * 474_825 + 400_315) * 1024,
     # K = 8 steps per pass, 64 x 64 regions of 8 waves x 8 rows, XCD order, the previous launch's gather inside,
-    # per-cell coefficient arrays (profiles/yee2d_tbk_batch64_kernel_and_counters.md)
-    ("batch2d", "k8-8x8-xcd", 64): (2 * 427_494 + 358_977) * 1024,
+    # interleaved per-cell coefficients (profiles/yee2d_tbk_batch64_kernel_and_counters.md)
+    ("batch2d", "k8-8x8-xcd", 64): (2 * 423_209 + 358_906) * 1024,
 }
 
 
