@@ -100,3 +100,18 @@ def test_kstep_owner_tables_give_each_point_one_owner():
                 assert K <= row < rh - K and K <= col < 64 - K     # an inner cell of the region
                 assert (sim[p], comp[p]) == (s, c)
                 assert (ty * iy - K + row, tx * ix - K + col) == (jj[p], ii[p])
+
+
+def test_kstep_shapes_are_valid():
+    """Every automatic and tunable K-step shape satisfies the kernel's limits (a region of 64 x rows*waves
+    cells with inner cells left after K steps on each side), and one-tile-per-workgroup ordering is a
+    padded permutation."""
+    import numpy as np
+    from beamz_amd.simulation.engine import Fused2D, xcd_order
+    for K, rows, waves in Fused2D.KCANDIDATES + ((8, 8, 8), (8, 4, 8)):
+        assert (rows, waves) in Fused2D.KSHAPES and 2 * K < min(64, rows * waves)
+        assert waves * 64 <= 1024
+    tiles = np.stack([np.zeros(37, np.int32), np.arange(37), np.arange(37) % 5], axis=1)
+    out = xcd_order(tiles, per_wg=1)
+    real = out[out[:, 0] >= 0]
+    assert len(out) % 8 == 0 and sorted(map(tuple, real)) == sorted(map(tuple, tiles))
