@@ -3,17 +3,21 @@
 New API (the reference runs one ``FDTD`` at a time, fdtd.py:411-429 per step):
 ``FDTDBatch`` takes ordinary 2D ``FDTD`` objects (each built from its own
 design, time axis, source and monitors exactly as for a single run) and
-advances all of them with one launch per step of the fused 2D engine
-(``bz_yee2d_run``): every simulation is cut into tiles of 62 columns x ``seg``
-rows, the tiles of the whole batch form one work list, and the monitor samples
-and FP64 DFT sums of every member are gathered into one table. Each member's
-step is bit-for-bit the step it takes when run alone.
+advances all of them through one work list of the fused 2D engine. By default
+(``kstep=None``: on unless a member records field power; the region shape
+follows the batch's cell count) one launch of ``bz_yee2d_runk`` advances every
+member K steps: each simulation is cut into regions of 64 columns x
+(rows x waves) rows whose inner cells are stored after K steps held in
+registers. ``kstep=0`` falls back to one or two steps per launch
+(``bz_yee2d_run`` / the pair kernel) on tiles of 62 columns x ``seg`` rows. The
+monitor samples and FP64 DFT sums of every member are gathered into one table.
+Each member's step is bit-for-bit the step it takes when run alone.
 
 The members must share their time axis, their monitor record schedule and
 their DFT frequencies (one phasor table serves the batch), and each may have at
-most one source. ``autotune`` picks the tile height by timing a few steps of
-each candidate on the batch itself (HIP events), then restores the initial
-state.
+most one source. ``autotune`` picks the kernel shape (K, rows, waves; or the
+tile height without K steps) by timing a few steps of each candidate on the
+batch itself (HIP events), then restores the initial state.
 """
 from __future__ import annotations
 
