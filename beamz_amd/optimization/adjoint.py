@@ -10,15 +10,16 @@ the discrete forward map itself:
   frequency of the Ez samples a line monitor records after every step,
   projected on weights c_p (normally the conjugated, normalised mode profile
   of the output guide, ``mode_weights``);
-* forward: the fused 2D engine over the whole batch of designs (two steps
-  per launch from 1 M cells on), with the full field state checkpointed
-  every K ~ sqrt(N) steps;
+* forward: the fused 2D engine over the whole batch of designs, with the
+  full field state checkpointed every K ~ sqrt(N) steps (a multiple of 8);
+  by default it advances 8 steps per launch (``bz_yee2d_runk``);
 * reverse: for each segment, newest first, the segment is recomputed from its
   checkpoint by the same engine while it snapshots Ez, Hx, Hy inside the
-  design box, then the reverse-time kernel (``bz_adj2d_run``, or two steps
-  per launch with ``bz_adj2d_run2``) walks the segment backwards,
+  design box, then the reverse-time kernel walks the segment backwards,
   transposing each step and accumulating dJ/d eps_r in FP64 inside the
-  design box. Pairs and single steps give the same bits.
+  design box: 8 steps per launch (``bz_adj2d_runk``) by default, else two
+  (``bz_adj2d_run2``) or one (``bz_adj2d_run``). Every launch shape gives
+  the same bits.
 
 Memory: ceil(N/K) checkpoints of three field arrays plus K design-box
 snapshots per design. ``FDTDObjective`` wraps this as a torch.autograd

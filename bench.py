@@ -278,9 +278,8 @@ def build_runner(a, total):
             el = time.perf_counter() - t0
             return s.nx * s.ny * n / el, (f"{n} reference-order complex128 steps of one {s.nx}x{s.ny} design "
                                          f"({el:.1f} s); the reference's loop needs two such runs per gradient")
-        r = Runner(iteration, sims[0].grid, cells, desc,
-                   "forward sweep: bz_yee2d_runk (K steps per pass); recompute sweep with snapshots: bz_yee2d_run2 "
-                   "(pairs); reverse sweep: bz_adj2d_run2 / bz_adj2d_run", cpu)
+        r = Runner(iteration, sims[0].grid, cells, desc, None, cpu)    # kernel names: the engines exist after warmup
+        r.prob = prob
         r.grids = [s.grid for s in sims]
         r.tuned = None
         r.per_step_cells = True
@@ -466,6 +465,17 @@ def main():
     traffic = MEASURED_TRAFFIC.get(key)
     # bytes of field and material arrays the timed kernels stream (every member's, both ping-pong sets)
     working_set = sum(g._flat.numel() * g._flat.element_size() for g in getattr(r, "grids", [r.grid]))
+    if a.workload == "adjoint2d":             # the sweeps' engines exist once a gradient has run
+        f = r.prob.fwd
+        if f.kstep:
+            shape = f"{f.kstep} steps per pass, 64x{f.krows * f.kwaves} regions"
+            r.kernel_name = (f"forward sweep and recompute sweep (with design-box snapshots): bz_yee2d_runk; "
+                             f"reverse sweep: bz_adj2d_runk; {shape}")
+        else:
+            r.kernel_name = ("forward sweep and recompute sweep (with design-box snapshots): "
+                             + ("bz_yee2d_run2; reverse sweep: bz_adj2d_run2" if f.pairs
+                                else "bz_yee2d_run; reverse sweep: bz_adj2d_run"))
+        r.steps_per_launch = f.kstep or (2 if f.pairs else 1)
     out = {
         "metric": "Yee cell-updates/s", "value": value, "unit": "cell-updates/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
