@@ -396,7 +396,8 @@ def main():
     ap.add_argument("--kwaves", type=int, default=None,
                     help="2D K-step kernel: waves per workgroup (default: automatic)")
     ap.add_argument("--graph", action="store_true", help="capture 2D runs as HIP graphs")
-    ap.add_argument("--autotune", action="store_true", help="batch2d: pick the tile height by timing")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batch2d: pick the launch shape (K steps, rows, waves; or the tile height) by timing")
     ap.add_argument("--chunk", type=int, default=96, help="batch2d: steps per engine call")
     ap.add_argument("--timing-steps", type=int, default=48,
                     help="steps timed per launch with HIP events (a multiple of the steps per launch)")
