@@ -655,6 +655,8 @@ class Fused2D:
         if kstep is None and self.kstep_applies():
             kstep, r, w = self.auto_kstep()
             krows, kwaves = krows or r, kwaves or w
+        if kstep and not self.kstep_applies():
+            kstep = 0           # grid power needs every step: pairs or single steps instead
         if kstep:
             self.set_kstep(kstep, krows or 8, kwaves or 8)
 
@@ -964,6 +966,10 @@ class Fused2D:
             raise ValueError(f"a run of {count} steps overflows the snapshot buffers ({self.snap_steps} slots)")
         if self.npts and n0 + count > self.nslot:
             self._make_slot(n0 + count)
+        if coef:
+            # the materials may have changed: the next K-step launch recomputes the coefficients, also when
+            # this run is too short for one and a later run of the same materials passes coef=False
+            self._coef_done = False
         self._sync_descs(n0)
         self.step.fill_(n0)
         g0 = self.grids[0]
@@ -977,7 +983,7 @@ class Fused2D:
             nl = count // K
             ptr = lambda t: t.data_ptr() if t is not None else None
             # coefficients of the current materials (a caller may have written eps or sig since the last run)
-            if coef or not self._coef_done:
+            if not self._coef_done:
                 N.call("bz_yee2d_coef", self.desc_dev.data_ptr(), len(self.grids),
                        max(g.ny * g.ld for g in self.grids), g0.stream)
                 self._coef_done = True

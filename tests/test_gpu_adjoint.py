@@ -217,7 +217,7 @@ def test_pair_sweeps_give_the_same_gradient_and_engines_are_reused(hip):
     adjoint buffers an odd number of times) and in K-step launches (K = 5 and 8, whole launches then pairs or
     single steps) give the one-step gradient bit for bit; later evaluations on the same problem (engines
     reused, fields and monitor sums reset in place) repeat it, also after a different design was evaluated in
-    between."""
+    between, and that in-between design's J and gradient equal those of a freshly built problem."""
     torch = hip
     eps, sig, _ = materials(seed=2)
     x0, y0, sx, sy = BOX
@@ -234,12 +234,22 @@ def test_pair_sweeps_give_the_same_gradient_and_engines_are_reused(hip):
         assert prob.fwd.pairs == pairs and prob.rec.pairs == pairs
         assert prob.fwd.kstep == kstep and prob.rec.kstep == kstep
         fwd, rec = prob.fwd, prob.rec
-        prob.forward(e_box * 1.01)
-        prob.gradient()
+        # the in-between design on the reused engines (300 steps = 13 x 23 + 1: the newest segment is one step,
+        # shorter than a K-step launch, and the coefficients must still follow the new design) against a
+        # freshly built problem evaluated at that design
+        Jm = prob.forward(e_box * 1.01).cpu().numpy()
+        gm = prob.gradient().cpu().numpy()
+        sim_f, mon_f, _, _ = build(eps, sig, 300)
+        fresh = AdjointProblem2D([sim_f], [mon_f], c, BOX, checkpoint_every=23, pairs=pairs, kstep=kstep)
+        Jf = fresh.forward(e_box * 1.01).cpu().numpy()
+        gf = fresh.gradient().cpu().numpy()
+        assert np.array_equal(Jm, Jf) and np.array_equal(gm, gf), key
+        assert not np.array_equal(gm, g), key
         J2 = prob.forward(e_box).cpu().numpy()
         g2 = prob.gradient().cpu().numpy()
         assert prob.fwd is fwd and prob.rec is rec
         assert np.array_equal(J2, J) and np.array_equal(g2, g), key
+        out[key] += (Jm, gm)
     for key in ("pairs", "k5", "k8"):
-        assert np.array_equal(out[key][0], out["one"][0]), key
-        assert np.array_equal(out[key][1], out["one"][1]), key
+        for q in range(4):
+            assert np.array_equal(out[key][q], out["one"][q]), (key, q)
