@@ -192,6 +192,108 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee2d_fused(const bz_sim2d* __re
     }
 }
 
+// ---------------- the (Hz, Ex, Ey) polarisation ----------------
+// The z-invariant restriction of the reference's 3D step (fdtd.py:200-310 with every d/dz = 0) to the
+// components Hz (ny-1, nx-1), Ex (ny, nx-1), Ey (ny-1, nx); the descriptor's ez / hx / hy slots hold Hz / Ex / Ey.
+//   Hz[j,i] <- fa Hz - fb ((Ey[j,i+1] - Ey[j,i]) / dx - (Ex[j+1,i] - Ex[j,i]) / dy),  fa, fb of sigma[j,i]
+//   Ex[j,i] <- ca Ex + cb (Hz[j,i] - Hz[j-1,i]) / dy           (1 <= j <= ny-2; every i <= nx-2)
+//   Ey[j,i] <- ca Ey - cb (Hz[j,i] - Hz[j,i-1]) / dx           (1 <= i <= nx-2; every j <= ny-2)
+// with ca, cb of eps_r[j,i], sigma[j,i]; the source box is added to Hz after the E update (the reference's
+// order: H, E, sources), and grid power accumulates Sx^2 + Sy^2 with S = E x H = (Ey Hz, -Ex Hz).
+// Same tiling as k_yee2d_fused: lane l owns column i0-1+l, lane 63 supplies Ey at i+1 and lane 0 Hz at i-1.
+template <int SEG>
+__global__ __launch_bounds__(LANES* WPG) void k_yee2d_hz(const bz_sim2d* __restrict__ sims,
+                                                        const int* __restrict__ tiles, int ntiles,
+                                                        const int* __restrict__ step, int q, Gather2D G) {
+    const int n = *step + q;
+    const int tile_blocks = (ntiles + WPG - 1) / WPG;
+    if ((int)blockIdx.x >= tile_blocks) {
+        gather_point(sims, G, ((int)blockIdx.x - tile_blocks) * GPB + threadIdx.y * LANES + threadIdx.x, n - 1);
+        return;
+    }
+    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * WPG + threadIdx.y);
+    if (t >= ntiles) return;
+    const int rd = n & 1, wr = rd ^ 1;
+    const int s = tiles[3 * t], tx = tiles[3 * t + 1], ty = tiles[3 * t + 2];
+    if (s < 0) return;
+    const bz_sim2d& S = sims[s];
+    const int nx = S.nx, ny = S.ny, ld = S.ld;
+    const int l = threadIdx.x;
+    const int i = tx * OX - 1 + l;
+    const int j0 = ty * SEG;
+    const float* __restrict__ ghz = S.ez[rd];
+    const float* __restrict__ gex = S.hx[rd];
+    const float* __restrict__ gey = S.hy[rd];
+    Coef c;
+    c.kh = S.kh;
+    c.kbh = S.kbh;
+    c.ke = S.ke;
+    c.kbe = S.kbe;
+    const float rdx = S.rdx, rdy = S.rdy;
+    const uint32_t ic = (uint32_t)min(max(i, 0), nx - 1);
+    // register row r = row j0-1+r, clamped into the grid
+    uint32_t ro[SEG + 2];
+#pragma unroll
+    for (int r = 0; r < SEG + 2; ++r) {
+        const int j = min(max(j0 - 1 + r, 0), ny - 1);
+        ro[r] = ((uint32_t)j * (uint32_t)ld + ic) * 4u;
+    }
+    float ex[SEG + 2], hz[SEG + 1], ey[SEG + 1], sg[SEG + 1], ee[SEG + 1];
+#pragma unroll
+    for (int r = 0; r < SEG + 2; ++r) {
+        ex[r] = at(gex, ro[r]);
+        if (r < SEG + 1) {
+            hz[r] = at(ghz, ro[r]);
+            ey[r] = at(gey, ro[r]);
+            sg[r] = at(S.sig, ro[r]);
+            ee[r] = r > 0 ? at(S.eps, ro[r]) : 1.0f;
+        }
+    }
+    // Hz of rows j0-1 .. j0+SEG-1 (row j0-1 feeds Ex of row j0)
+    float hzn[SEG + 1], hzl[SEG + 1];
+    const bool hzx = i < nx - 1;
+#pragma unroll
+    for (int r = 0; r < SEG + 1; ++r) {
+        const int j = j0 - 1 + r;
+        float fa, fb;
+        h_coef(sg[r], c, fa, fb);
+        const float eyR = __shfl_down(ey[r], 1, LANES);
+        hzn[r] = (hzx && j < ny - 1) ? fa * hz[r] - fb * ((eyR - ey[r]) * rdx - (ex[r + 1] - ex[r]) * rdy) : hz[r];
+        hzl[r] = __shfl_up(hzn[r], 1, LANES);
+    }
+    // Ex, Ey of rows j0 .. j0+SEG-1, the source on Hz, stores
+    const bool out = l > 0 && l < LANES - 1 && i < nx;
+    const float amp = S.signal ? S.signal[n] : 0.0f;
+    float* __restrict__ ohz = S.ez[wr];
+    float* __restrict__ oex = S.hx[wr];
+    float* __restrict__ oey = S.hy[wr];
+#pragma unroll
+    for (int r = 1; r < SEG + 1; ++r) {
+        const int j = j0 - 1 + r;
+        if (out && j < ny) {
+            float ca, cb;
+            e_coef(sg[r], ee[r], c, ca, cb);
+            const float exn = (j >= 1 && j <= ny - 2 && hzx) ? ca * ex[r] + cb * ((hzn[r] - hzn[r - 1]) * rdy) : ex[r];
+            const float eyn = (j <= ny - 2 && i >= 1 && i <= nx - 2) ? ca * ey[r] + cb * (-((hzn[r] - hzl[r]) * rdx))
+                                                                     : ey[r];
+            float hzo = hzn[r];
+            const int a = i - S.src_x0, b = j - S.src_y0;
+            if (a >= 0 && a < S.src_sx && b >= 0 && b < S.src_sy) {
+                const long rr = (long)b * S.src_sx + a;
+                hzo = (S.src_mul ? S.src_mul[rr] : 1.0f) * hzo + S.src_add[rr] * amp;
+            }
+            put(ohz, ro[r], hzo);
+            put(oex, ro[r], exn);
+            put(oey, ro[r], eyn);
+            if (S.power) {
+                const double h = hzo;
+                const double sx = (double)eyn * h, sy = -(double)exn * h;
+                S.power[(long)j * ld + i] += sx * sx + sy * sy;
+            }
+        }
+    }
+}
+
 __global__ void k_gather2d(const bz_sim2d* __restrict__ sims, Gather2D G, const int* __restrict__ step, int q) {
     gather_point(sims, G, blockIdx.x * blockDim.x + threadIdx.x, *step + q);
 }
@@ -400,13 +502,13 @@ __global__ void k_gather2d_pair(const bz_sim2d* __restrict__ sims, Gather2D G, c
     gather_pair(sims, G, spts, step, q, wr, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-template <int SEG>
+template <int SEG, bool HZ = false>
 int run(const bz_sim2d* sims, const int* tiles, int ntiles, int* step, int nsteps, const Gather2D& G,
         hipStream_t st) {
     const int tile_blocks = (ntiles + WPG - 1) / WPG;
     const int gather_blocks = (G.npts + GPB - 1) / GPB;
     const dim3 b(LANES, WPG);
-    auto kern = k_yee2d_fused<SEG>;
+    auto kern = HZ ? k_yee2d_hz<SEG> : k_yee2d_fused<SEG>;
     for (int q = 0; q < nsteps; ++q) {
         const dim3 g(tile_blocks + (q > 0 ? gather_blocks : 0));
         hipLaunchKernelGGL(kern, g, b, 0, st, sims, tiles, ntiles, step, q, G);
@@ -741,6 +843,31 @@ extern "C" int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, i
         case 8: return run<8>(sims, tiles, ntiles, step, nsteps, G, st);
         case 16: return run<16>(sims, tiles, ntiles, step, nsteps, G, st);
         default: bz::set_error("bz_yee2d_run: seg must be 2, 4, 8 or 16"); return BZ_EINVAL;
+    }
+}
+
+extern "C" int bz_yee2d_hz_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step,
+                               int nsteps, const int* pts, int npts, const int* slot, float* series, int nfreq,
+                               const double* phasor, double* dft, void* stream) {
+    BZ_REQUIRE(sims && nsims > 0 && tiles && ntiles > 0 && step && nsteps >= 0, "bz_yee2d_hz_run: bad arguments");
+    BZ_REQUIRE(npts >= 0 && (npts == 0 || (pts && slot)), "bz_yee2d_hz_run: monitor points without a slot table");
+    BZ_REQUIRE(nfreq >= 0 && (nfreq == 0 || (phasor && dft)), "bz_yee2d_hz_run: DFT requested without buffers");
+    if (nsteps == 0) return BZ_OK;
+    Gather2D G;
+    G.pts = pts;
+    G.npts = npts;
+    G.slot = slot;
+    G.series = series;
+    G.nfreq = nfreq;
+    G.phasor = phasor;
+    G.dft = dft;
+    const hipStream_t st = (hipStream_t)stream;
+    switch (seg) {
+        case 2: return run<2, true>(sims, tiles, ntiles, step, nsteps, G, st);
+        case 4: return run<4, true>(sims, tiles, ntiles, step, nsteps, G, st);
+        case 8: return run<8, true>(sims, tiles, ntiles, step, nsteps, G, st);
+        case 16: return run<16, true>(sims, tiles, ntiles, step, nsteps, G, st);
+        default: bz::set_error("bz_yee2d_hz_run: seg must be 2, 4, 8 or 16"); return BZ_EINVAL;
     }
 }
 

@@ -184,11 +184,20 @@ class Monitor:
 
         Sums Re(S . n) over the monitor points from the DFT fields, with the
         reference's own Poynting conventions: in 2D (monitors.py:407-421)
-        Sx = -Ez Hy and Sy = Ez Hx, here with conj(H) on the DFT amplitudes;
-        a vertical line is x-normal, a horizontal one y-normal. In 3D, the
-        plane's normal component of E x conj(H). Units are field-product sums
-        (no cell area factor): ratios between monitors of one grid are the
-        port transmissions.
+        Sx = -Ez Hy and Sy = Ez Hx, here with conj(H) on the DFT amplitudes
+        (the (Hz, Ex, Ey) family: Sx = Ey Hz, Sy = -Ex Hz); a vertical line is
+        x-normal, a horizontal one y-normal. In 3D, the plane's normal
+        component of E x conj(H). Units are field-product sums (no cell area
+        factor): ratios between monitors of one grid are the port
+        transmissions.
+
+        The products pair the E and H samples of the same array index, as the
+        reference's own power (monitors.py:426-434) does, without moving them
+        to a common point of the Yee cell: the components sit up to half a cell
+        apart in space, and H half a step apart in time. For a port plane
+        across a guided mode this changes the flux by a factor close to 1 that
+        is the same at every port of one grid, so port ratios keep it only to
+        that order.
         """
         self._materialize()
         if not self.dft:
@@ -196,7 +205,10 @@ class Monitor:
         D = self.dft
         if not self.is_3d:
             vertical = abs(self.end[0] - self.start[0]) <= abs(self.end[1] - self.start[1])
-            s = -D["Ez"] * np.conj(D["Hy"]) if vertical else D["Ez"] * np.conj(D["Hx"])
+            if "Hz" in D:       # the (Hz, Ex, Ey) family: S = E x conj(H) = (Ey conj(Hz), -Ex conj(Hz))
+                s = D["Ey"] * np.conj(D["Hz"]) if vertical else -D["Ex"] * np.conj(D["Hz"])
+            else:
+                s = -D["Ez"] * np.conj(D["Hy"]) if vertical else D["Ez"] * np.conj(D["Hx"])
             return np.real(s).reshape(s.shape[0], -1).sum(axis=1)
         n = self.plane_normal if self.end is None else "z"
         a, b = {"x": ("y", "z"), "y": ("z", "x"), "z": ("x", "y")}[n]

@@ -55,7 +55,9 @@ def calc_optimal_fdtd_params(wavelength, n_max, dims=2, safety_factor=0.999, poi
 
 
 def display_status(message, status="info"):
-    log.info("[%s] %s", status, message)
+    """Log a status line; "warning" and "error" go out at those levels, everything else at INFO."""
+    level = {"warning": logging.WARNING, "error": logging.ERROR}.get(status, logging.INFO)
+    log.log(level, "[%s] %s", status, message)
 
 
 def display_header(title, subtitle=None):

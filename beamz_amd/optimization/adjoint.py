@@ -2,8 +2,9 @@
 
 New API; the reference's topology loop (examples/4_topo.py:255-321) estimates
 the gradient by the time-reversed overlap of a forward and a separate
-"adjoint source" run (4_topo.py:282-285, kept here as
-``overlap_gradient_reference`` for comparison). Here the gradient is that of
+"adjoint source" run (4_topo.py:279-286; on the GPU here as
+``overlap_gradient_reference``, with the reference's adjoint source as
+``reference_adjoint_source``, for comparison). Here the gradient is that of
 the discrete forward map itself:
 
 * objective: J = |a|^2 with a = sum_p c_p Ez_hat(p, w), the DFT at one
@@ -80,8 +81,8 @@ class AdjointProblem2D:
         self.box = tuple(int(v) for v in box)
         x0, y0, sx, sy = self.box
         for s in self.sims:
-            if s.is_3d:
-                raise ValueError("2D simulations only")
+            if s.is_3d or s.grid.pol != "Ez":
+                raise ValueError("2D simulations of the (Ez, Hx, Hy) polarisation only")
             if not (x0 >= 1 and y0 >= 1 and x0 + sx <= s.nx - 1 and y0 + sy <= s.ny - 1):
                 raise ValueError("the design box must lie inside the grid interior")
         self.N = self.sims[0].num_steps
@@ -308,6 +309,67 @@ class AdjointProblem2D:
             flip = segment(k, flip)
         self._keep = (lam, inj, ddev)
         return grad
+
+
+def reference_adjoint_source(design, monitor, dx, dy, target, signal, wavelength, transmission_weight=0.5,
+                             mode_weight=0.5):
+    """The adjoint source of the reference's topology example (examples/4_topo.py:214-233): a ModeSource line
+    through the monitor's centre, as long as the monitor's line, injecting against the output direction
+    (-y for a horizontal monitor, as there; -x for a vertical one) without direction enforcement, whose sample
+    amplitudes are transmission_weight + mode_weight * target(s), the target mode samples interpolated along
+    the line from the monitor's grid points (0 outside them). The mode solve only places the samples (its
+    amplitudes are replaced), so the analytical slab solver is used."""
+    from ..devices.sources import ModeSource
+    pts = monitor.get_grid_points_2d(dx, dy)
+    horizontal = abs(monitor.end[0] - monitor.start[0]) >= abs(monitor.end[1] - monitor.start[1])
+    axis = 0 if horizontal else 1
+    pos = np.array([p[axis] * (dx if horizontal else dy) for p in pts], dtype=float)
+    length = float(np.hypot(monitor.end[0] - monitor.start[0], monitor.end[1] - monitor.start[1]))
+    center = ((monitor.start[0] + monitor.end[0]) / 2, (monitor.start[1] + monitor.end[1]) / 2)
+    src = ModeSource(design=design, position=center, width=length, wavelength=wavelength, signal=signal,
+                     direction="-y" if horizontal else "-x", num_modes=1, mode_solver="analytical")
+    src.enforce_direction = False
+    tgt = np.asarray(target, dtype=complex)
+    prof = src.mode_profiles[0]
+    at = np.array([p[1 + axis] for p in prof], dtype=float)
+    xpos = pos
+    re = np.interp(at, xpos, tgt.real, left=0.0, right=0.0)
+    im = np.interp(at, xpos, tgt.imag, left=0.0, right=0.0)
+    for q, p in enumerate(prof):
+        p[0] = transmission_weight + mode_weight * (re[q] + 1j * im[q])
+    return src
+
+
+def overlap_gradient_reference(fwd_sim, adj_sim, box, kstep=None):
+    """The reference's gradient estimate (examples/4_topo.py:279-286) on the GPU:
+    grad = sum_k Re(Ez_adj(k) conj(Ez_fwd(N-1-k))) over the design box, Ez(k) the field after step k of a
+    forward run and of a separate run driven by the adjoint source (same time axis); with real fields the
+    product needs no conjugate. No factor of dt, eps0 or the material contrast, as in the reference.
+
+    fwd_sim, adj_sim: 2D ``FDTD`` objects of the (Ez, Hx, Hy) polarisation; box: (x0, y0, sx, sy). Each run
+    records Ez inside the box after every step (the engine's snapshot box, in HBM: 12 * N * sx * sy bytes),
+    and leaves its simulation's fields at step N. Returns the (sy, sx) FP64 torch tensor."""
+    x0, y0, sx, sy = (int(v) for v in box)
+    N = fwd_sim.num_steps
+    if adj_sim.num_steps != N or not np.array_equal(np.asarray(adj_sim.time), np.asarray(fwd_sim.time)):
+        raise ValueError("the forward and adjoint runs must share their time axis")
+    series = []
+    for s in (fwd_sim, adj_sim):
+        if s.is_3d or s.grid.pol != "Ez":
+            raise ValueError("2D simulations of the (Ez, Hx, Hy) polarisation only")
+        T = s.backend.torch
+        s.initialize_simulation(save=False, live=False)
+        buf = T.zeros((N, 3, sy, sx), dtype=T.float32, device=s.backend.dev)
+        eng = Fused2D([s.grid], monitors=False, snapshots=[(x0, y0, sx, sy, buf)], kstep=kstep)
+        eng.run(0, N)
+        s.grid._f2d = None
+        s.current_step = N
+        # snapshot slot q holds the Ez that step q read, i.e. the field after step q - 1
+        last = s.grid.view("Ez")[y0:y0 + sy, x0:x0 + sx]
+        series.append(T.cat([buf[1:, 0], last[None]]).double())
+        del buf, eng
+    fwd, adj = series
+    return (adj * fwd.flip(0)).sum(dim=0)
 
 
 class FDTDObjective:

@@ -23,15 +23,19 @@ from ..const import EPS_0, MU_0
 
 COMP3 = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 COMP2 = ("Ez", "Hx", "Hy")
+COMP2H = ("Hz", "Ex", "Ey")     # the (Hz, Ex, Ey) polarisation, in the slots of Ez, Hx, Hy
 
 
 def align(n, a=64):
     return (n + a - 1) // a * a
 
 
-def ref_shapes(nx, ny, nz=None):
-    """Reference field shapes (fdtd.py:151-186)."""
+def ref_shapes(nx, ny, nz=None, pol="Ez"):
+    """Reference field shapes (fdtd.py:151-186); the 2D (Hz, Ex, Ey) family takes those components' shapes of
+    the 3D layout."""
     if nz is None:
+        if pol == "Hz":
+            return {"Hz": (ny - 1, nx - 1), "Ex": (ny, nx - 1), "Ey": (ny - 1, nx)}
         return {"Ez": (ny, nx), "Hx": (ny, nx - 1), "Hy": (ny - 1, nx)}
     return {"Ex": (nz, ny, nx - 1), "Ey": (nz, ny - 1, nx), "Ez": (nz - 1, ny, nx),
             "Hx": (nz - 1, ny - 1, nx), "Hy": (nz - 1, ny, nx - 1), "Hz": (nz, ny - 1, nx - 1)}
@@ -52,12 +56,16 @@ class Grid:
 
     STAGGER = 1088      # floats between consecutive arrays beyond their size (4 KiB + 256 B)
 
-    def __init__(self, backend, eps, sigma, dx, dy, dt, dz=None):
+    def __init__(self, backend, eps, sigma, dx, dy, dt, dz=None, pol="Ez"):
+        """pol: "Ez" (the reference's 2D family, and 3D) or "Hz" (2D (Hz, Ex, Ey) family; fused engine only)."""
         self.backend = backend
         torch = backend.torch
         self.torch = torch
         self.dev = backend.dev
         self.is_3d = eps.ndim == 3
+        if pol not in ("Ez", "Hz") or (self.is_3d and pol != "Ez"):
+            raise ValueError("polarization must be 'Ez', or 'Hz' for a 2D grid")
+        self.pol = pol
         self.dx, self.dy, self.dz, self.dt = float(dx), float(dy), (None if dz is None else float(dz)), float(dt)
         if self.is_3d:
             self.nz, self.ny, self.nx = eps.shape
@@ -71,7 +79,7 @@ class Grid:
         self.ld = align(self.nx)
         box = (self.nz, self.ny, self.ld) if self.is_3d else (self.ny, self.ld)
         self.box = box
-        self.names = COMP3 if self.is_3d else COMP2
+        self.names = COMP3 if self.is_3d else (COMP2H if pol == "Hz" else COMP2)
         # All arrays of one simulation (two field sets for ping-pong, permittivity, conductivity) share
         # one allocation, their bases staggered by 4 KiB + 256 B: the update kernels read every array at
         # the same element offset at once, and equally aligned bases would put those concurrent streams
@@ -88,7 +96,7 @@ class Grid:
         self.eps.fill_(1.0)
         self.sig = views[2 * nc + 1]
         self.set_materials(eps, sigma)
-        self.shapes = ref_shapes(self.nx, self.ny, self.nz)
+        self.shapes = ref_shapes(self.nx, self.ny, self.nz, pol)
         self.sources = []
         self.monitors = []
         self.power = None
@@ -123,6 +131,8 @@ class Grid:
                 return (f"bz_yee3d_step2 (two steps per pass, {self.waves2} waves x {self.rows2} rows, "
                         f"{self.kseg2_eff()}-plane segments)")
             return f"bz_yee3d_step (fused z-march, {self.rows} rows/wave)" if self.kernel == "fused" else "bz_update_3d (two-pass)"
+        if self.pol == "Hz":
+            return "bz_yee2d_hz_run ((Hz, Ex, Ey), fused, batched)"
         if self.kernel != "fused":
             return "bz_update_h_2d + bz_update_e_2d"
         f = self._f2d
@@ -252,6 +262,8 @@ class Grid:
 
     # ---------------- one step ----------------
     def _update(self):
+        if self.pol == "Hz":
+            raise NotImplementedError("the (Hz, Ex, Ey) polarisation runs on the fused 2D engine only")
         st = self.stream
         if self.is_3d:
             ptr = N.ptrs6([self.f[k].data_ptr() for k in COMP3])
@@ -494,7 +506,12 @@ class Grid:
             try:
                 self._f2d = Fused2D([self], seg=self.seg2d, graph=self.graph2d, kstep=self.kstep2d,
                                     krows=self.krows2d, kwaves=self.kwaves2d)
-            except NotImplementedError:
+            except NotImplementedError as e:
+                if self.pol == "Hz":
+                    raise
+                from ..helpers import display_status
+                display_status(f"2D run leaves the fused engine ({e}): one kernel per update, "
+                               "source and monitor gather instead", "warning")
                 self._f2d = False
         if self._f2d is False:
             return False
@@ -582,11 +599,14 @@ class Fused2D:
         T = g0.torch
         self.torch, self.dev, self.grids = T, g0.dev, list(grids)
         self.xcd = self.XCD_ORDER if xcd is None else bool(xcd)
+        self.pol = g0.pol
         for g in grids:
             if g.is_3d:
                 raise ValueError("Fused2D runs 2D grids only")
             if g.dev != g0.dev:
                 raise ValueError("all grids of a batch must be on one device")
+            if g.pol != self.pol:
+                raise ValueError("all grids of a batch must share one polarisation")
             if len(g.sources) > 1:
                 raise NotImplementedError("the fused 2D engine takes at most one source per simulation")
             if 4 * g.ny * g.ld >= 2 ** 32:
@@ -637,6 +657,11 @@ class Fused2D:
             self.npts, self.nfreq = ntot, p0.nfreq
         self.use_graph = bool(graph)
         self.graphs = {}
+        if self.pol == "Hz":
+            # the (Hz, Ex, Ey) family has the one-step kernel only
+            if any(sn is not None for sn in self.snapshots):
+                raise ValueError("snapshot boxes need the (Ez, Hx, Hy) polarisation")
+            pairs, kstep = False, 0
         if pairs is None:
             # small batches are launch-latency bound, and a pair's serial row march is slower there
             # (800 x 400: 13.4 us/step in pairs against 10.2 with one step per launch)
@@ -949,7 +974,7 @@ class Fused2D:
     def _launch(self, count):
         ptr = lambda t: t.data_ptr() if t is not None else None
         g0 = self.grids[0]
-        N.call("bz_yee2d_run", self.desc_dev.data_ptr(), len(self.grids), self.tiles.data_ptr(), self.ntiles,
+        N.call("bz_yee2d_hz_run" if self.pol == "Hz" else "bz_yee2d_run", self.desc_dev.data_ptr(), len(self.grids), self.tiles.data_ptr(), self.ntiles,
                self.seg, self.step.data_ptr(), count, ptr(self.pts), self.npts, ptr(self.slot), ptr(self.series),
                self.nfreq, ptr(self.phasor), ptr(self.dft), g0.stream)
 

@@ -15,6 +15,13 @@ deliberate:
 * ``live`` plotting is not performed;
 * a ``GaussianSource`` passed through ``devices=`` is accepted (the reference
   raises AttributeError there because the source has no ``design``).
+
+Extension (no reference counterpart): ``polarization="Hz"`` runs a 2D design
+in the (Hz, Ex, Ey) family instead of the reference's (Ez, Hx, Hy) one. Its
+update is the reference's 3D step (fdtd.py:200-310) with every d/dz = 0, kept
+to Hz (ny-1, nx-1), Ex (ny, nx-1), Ey (ny-1, nx); sources add their patches or
+profiles to Hz (clipped to Hz's cells), monitors sample Hz, Ex, Ey, and the
+Poynting vector is S = E x H = (Ey Hz, -Ex Hz).
 """
 from __future__ import annotations
 
@@ -28,7 +35,7 @@ from ..devices.monitors import Monitor
 from ..devices.sources import GaussianSource, ModeSource
 from ..helpers import check_fdtd_stability, display_status
 from .backends import get_backend
-from .engine import COMP2, COMP3, Grid
+from .engine import COMP3, Grid
 from .meshing import BaseMeshGrid, RegularGrid, RegularGrid3D
 
 
@@ -117,7 +124,7 @@ def mode_source_cells(src, dx, dy, dz, nx, ny, nz, ez_nz, is_3d):
 
 class FDTD:
     def __init__(self, design, time, mesh: str = "regular", resolution: float = 0.02e-6, backend="hip",
-                 backend_options=None, devices=None):
+                 backend_options=None, devices=None, *, polarization="Ez"):
         provided = None
         if isinstance(design, BaseMeshGrid):
             provided = design
@@ -157,7 +164,11 @@ class FDTD:
         self.time = time
         self.dt = time[1] - time[0]
         self.num_steps = len(time)
-        self.grid = Grid(self.backend, self.epsilon_r, self.sigma, self.dx, self.dy, self.dt, self.dz)
+        if polarization not in ("Ez", "Hz") or (self.is_3d and polarization != "Ez"):
+            raise ValueError("polarization must be 'Ez' (the reference's 2D family, and 3D) or 'Hz' (2D only)")
+        self.polarization = polarization
+        self.grid = Grid(self.backend, self.epsilon_r, self.sigma, self.dx, self.dy, self.dt, self.dz,
+                         pol=polarization)
         self.sources = list(design.sources)
         self.monitors = list(design.monitors)
         for dev in devices or []:
@@ -173,7 +184,7 @@ class FDTD:
                 self.monitors.append(dev)
             else:
                 raise TypeError(f"Unsupported device type: {type(dev)}")
-        names = COMP3 if self.is_3d else COMP2
+        names = self.grid.names
         self.results = {k: [] for k in names}
         self.results["t"] = []
         self.monitor_data = {}
@@ -223,23 +234,28 @@ class FDTD:
         g.sources = []
         g.invalidate()
         ez_nz = self.nz - 1 if self.is_3d else None
+        target = g.names[0] if not self.is_3d else "Ez"
+        # the (Hz, Ex, Ey) family injects into Hz, whose cells are (ny-1, nx-1)
+        bx, by = (self.nx - 1, self.ny - 1) if target == "Hz" else (self.nx, self.ny)
         for src in self.sources:
             sig = np.asarray(src.signal)
             if sig.ndim == 0 or len(sig) < self.num_steps:
                 raise ValueError("source signal must hold at least one value per time step")
             if isinstance(src, ModeSource):
-                idx, mul, add = mode_source_cells(src, self.dx, self.dy, self.dz, self.nx, self.ny, self.nz, ez_nz,
+                idx, mul, add = mode_source_cells(src, self.dx, self.dy, self.dz, bx, by, self.nz, ez_nz,
                                                   self.is_3d)
                 if len(idx):
-                    g.add_point_source("Ez", idx, add, mul, sig)
+                    g.add_point_source(target, idx, add, mul, sig)
             else:
                 if self.is_3d:
                     idx, w = gaussian_weights_3d(src.position, src.width, self.dx, self.dy, self.dz, self.nx, self.ny,
                                                  self.nz, ez_nz)
                 else:
                     idx, w = gaussian_weights_2d(src.position, src.width, self.dx, self.dy, self.nx, self.ny)
+                    keep = (idx[:, 0] < by) & (idx[:, 1] < bx)
+                    idx, w = idx[keep], w[keep]
                 if len(idx):
-                    g.add_point_source("Ez", idx, w, None, sig)
+                    g.add_point_source(target, idx, w, None, sig)
 
     def _monitor_samples(self, mon):
         """(comp indices, offsets, layout) of one monitor, following monitors.py:204-273, 324-379."""
@@ -247,9 +263,10 @@ class FDTD:
         shp = g.shapes
         if not self.is_3d:
             pts = mon.get_grid_points_2d(self.dx, self.dy)
-            sentinel = (COMP2.index("Hx"), g.offset(0, 0, self.nx - 1))
+            # the second component (Hx, or Ex) is never written at column nx-1: a zero to read
+            sentinel = (1, g.offset(0, 0, self.nx - 1))
             comps, offs = [], []
-            for ci, name in enumerate(COMP2):
+            for ci, name in enumerate(g.names):
                 sy, sx = shp[name]
                 for x, y in pts:
                     if 0 <= y < sy and 0 <= x < sx:
@@ -307,6 +324,9 @@ class FDTD:
         g.invalidate()
         self._plans = []
         for mon in self.monitors:
+            if not self.is_3d and set(mon._fields) != set(g.names) | {"t"}:
+                mon._fields = {k: [] for k in g.names}
+                mon._fields["t"] = []
             comps, offs, layout = self._monitor_samples(mon)
             last = mon.last_record_step
             steps = []
@@ -333,7 +353,7 @@ class FDTD:
         done = [s for s in plan.record_steps if s < self.current_step]
         nrec = len(done)
         new = slice(plan.fetched, nrec)
-        names = COMP3 if self.is_3d else COMP2
+        names = self.grid.names
         if nrec > plan.fetched and plan.series is not None:
             ser = plan.series[new].detach().cpu().numpy().astype(np.float64)
             ts = [float(plan.t_all[s]) if s > 0 else 0 for s in done[new]]
@@ -355,6 +375,10 @@ class FDTD:
                     sx, sy, sz = ey * hz - ez * hy, ez * hx - ex * hz, ex * hy - ey * hx
                     pm = np.sqrt(sx ** 2 + sy ** 2 + sz ** 2)
                     tot = pm.reshape(pm.shape[0], -1).sum(axis=1)
+                elif self.polarization == "Hz":
+                    hz, ex, ey = comp["Hz"], comp["Ex"], comp["Ey"]
+                    pm = np.sqrt((ey * hz) ** 2 + (-ex * hz) ** 2)
+                    tot = pm.sum(axis=1)
                 else:
                     ez, hx, hy = comp["Ez"], comp["Hx"], comp["Hy"]
                     pm = np.sqrt((-ez * hy) ** 2 + (ez * hx) ** 2)
@@ -392,7 +416,7 @@ class FDTD:
                               decimate_save=1, accumulate_power=False, save_memory_mode=False,
                               fields_to_cache: Optional[Sequence[str]] = None):
         if save_fields is None:
-            save_fields = list(COMP3 if self.is_3d else COMP2)
+            save_fields = list(self.grid.names)
         self._cache_fields = list(fields_to_cache) if fields_to_cache else []
         self._cache_frequency = 1 if self._cache_fields else None
         self.start_time = datetime.datetime.now()
@@ -536,7 +560,7 @@ class FDTD:
     def estimate_memory_usage(self, time_steps=None, save_fields=None):
         """helper.py:258-302 (8 bytes per saved value, as the reference assumes)."""
         steps = self.num_steps if time_steps is None else time_steps
-        names = save_fields or list(COMP3 if self.is_3d else COMP2)
+        names = save_fields or list(self.grid.names)
         sizes = {k: int(np.prod(v)) * 8 for k, v in self.grid.shapes.items()}
         total = steps * 8 + sum(sizes.get(k, 0) * steps for k in names)
         mb = 1024 * 1024

@@ -14,9 +14,9 @@
  *     entries outside the reference's staggered shape stay zero.
  *
  * Reference interfaces replaced (paths relative to the beamz repository):
- *   bz_update_h_2d ........ Backend.update_h_fields, beamz/simulation/backends/base.py:123-126,
+ *   bz_update_h_2d ........ Backend.update_h_fields, beamz/simulation/backends/base.py:36-39,
  *                           implemented by numpy_backend.py:34-61
- *   bz_update_e_2d ........ Backend.update_e_field, base.py:128-131; numpy_backend.py:63-88
+ *   bz_update_e_2d ........ Backend.update_e_field, base.py:41-44; numpy_backend.py:63-88
  *   bz_update_3d .......... FDTD._update_3d_fields, beamz/simulation/fdtd.py:200-310
  *   bz_add_points / bz_affine_points
  *                           source injection, beamz/simulation/helper.py:5-118
@@ -33,6 +33,9 @@
  *   bz_yee2d_run .......... fused multi-step 2D engine for many independent simulations at once:
  *                           per step numpy_backend.py:34-88 (H, E), helper.py:96-118 (source),
  *                           helper.py:231-256 (monitor samples + DFT), helper.py:120-187 (power)
+ *   bz_yee2d_hz_run ....... the (Hz, Ex, Ey) polarisation of bz_yee2d_run: the reference's 3D update
+ *                           fdtd.py:200-310 restricted to d/dz = 0 (its 2D path numpy_backend.py:34-88 has
+ *                           only the (Ez, Hx, Hy) family)
  *   bz_yee3d_step2 ........ two consecutive bz_yee3d_step steps in one pass (same bits)
  *   bz_yee2d_run2 ......... two consecutive bz_yee2d_run steps per pass (same bits, no grid power)
  *   bz_yee2d_runk ......... K consecutive bz_yee2d_run steps per pass (same bits, no grid power), with
@@ -151,6 +154,17 @@ typedef struct bz_sim2d {
 int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step, int nsteps,
                  const int* pts, int npts, const int* slot, float* series, int nfreq, const double* phasor,
                  double* dft, void* stream);
+
+/* The (Hz, Ex, Ey) polarisation: nsteps steps of the z-invariant restriction of fdtd.py:200-310 (the
+ * reference's 3D update with every d/dz = 0, kept to the components Hz (ny-1, nx-1), Ex (ny, nx-1),
+ * Ey (ny-1, nx)); the reference's 2D path has only the (Ez, Hx, Hy) family (numpy_backend.py:34-88), and
+ * this is its E <-> H counterpart. Same arguments as bz_yee2d_run, with the descriptor's ez / hx / hy slots
+ * holding Hz / Ex / Ey and monitor comp 0 / 1 / 2 meaning Hz / Ex / Ey. The source box is added to Hz after
+ * the E update; grid power accumulates Sx^2 + Sy^2 with (Sx, Sy) = (Ey Hz, -Ex Hz) on the Hz cells' box.
+ * No snapshot boxes. */
+int bz_yee2d_hz_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step,
+                    int nsteps, const int* pts, int npts, const int* slot, float* series, int nfreq,
+                    const double* phasor, double* dft, void* stream);
 
 /* Two steps per pass: npairs pairs of the bz_yee2d_run step (same results, bit for bit), pair q reading
  * buffer (rd0 + q) & 1 and writing the other, so after the call the state sits in buffer

@@ -13,6 +13,9 @@ Follows, op for op and in the same order:
                             beamz/devices/monitors.py:204-273, 279-397
   * Poynting power ........ beamz/devices/monitors.py:407-439,
                             beamz/simulation/helper.py:120-187
+  * (Hz, Ex, Ey) 2D family  no reference 2D counterpart; fdtd.py:200-310 with
+                            d/dz = 0 (pinned against the 3D restatement's
+                            middle plane of a z-invariant run)
   * DFT accumulation ...... no reference counterpart; defined as the FP64 DFT
                             F_k = sum_n f_n exp(-i w_k t_n) of the very samples
                             the reference's monitor records (same t_n).
@@ -115,9 +118,39 @@ def update_fields_3d(F, eps_r, sigma, dx, dy, dz, dt, mu0=MU_0, eps0=EPS_0):
     Ez[:, 1:-1, 1:-1] = coef[2][0] * Ez[:, 1:-1, 1:-1] + coef[2][1] * chz
 
 
-def field_shapes(nx, ny, nz=None):
-    """fdtd.py:151-186: reference array shapes, [z,y,x] / [y,x] order."""
+# --------------------------------------------------------------------------
+# 2D (Hz, Ex, Ey) family: no reference 2D counterpart (the reference's 2D path
+# is the (Ez, Hx, Hy) family only). Defined as fdtd.py:200-310 with every
+# d/dz = 0, kept to Hz, Ex, Ey: the same expressions with the z-differences
+# (exactly 0 for z-invariant fields) left out, which changes no rounding
+# (x - 0 = x, 0 - x = -x).
+# --------------------------------------------------------------------------
+def update_fields_2d_hz(F, eps_r, sigma, dx, dy, dt, mu0=MU_0, eps0=EPS_0):
+    Ex, Ey, Hz = F["Ex"], F["Ey"], F["Hz"]
+    sm = sigma[:-1, :-1] * mu0 / eps0                          # fdtd.py: sigma[:, :-1, :-1]
+    cez = (Ey[:, 1:] - Ey[:, :-1]) / dx - (Ex[1:, :] - Ex[:-1, :]) / dy
+    den = 1.0 + sm * dt / (2.0 * mu0)
+    fa = (1.0 - sm * dt / (2.0 * mu0)) / den
+    fb = (dt / mu0) / den
+    Hz[:] = fa * Hz - fb * cez
+    e_x, s_x = eps_r[1:-1, :-1], sigma[1:-1, :-1]
+    e_y, s_y = eps_r[:-1, 1:-1], sigma[:-1, 1:-1]
+    chx = (Hz[1:, :] - Hz[:-1, :]) / dy                      # dHz/dy on Ex's interior rows
+    chy = -((Hz[:, 1:] - Hz[:, :-1]) / dx)                   # dHx/dz - dHz/dx with dHx/dz = 0
+    coef = []
+    for e, s in ((e_x, s_x), (e_y, s_y)):
+        den = 1.0 + s * dt / (2.0 * eps0 * e)
+        coef.append(((1.0 - s * dt / (2.0 * eps0 * e)) / den, (dt / (eps0 * e)) / den))
+    Ex[1:-1, :] = coef[0][0] * Ex[1:-1, :] + coef[0][1] * chx
+    Ey[:, 1:-1] = coef[1][0] * Ey[:, 1:-1] + coef[1][1] * chy
+
+
+def field_shapes(nx, ny, nz=None, polarization="Ez"):
+    """fdtd.py:151-186: reference array shapes, [z,y,x] / [y,x] order; the 2D (Hz, Ex, Ey) family takes the
+    shapes of those components in the 3D layout."""
     if nz is None:
+        if polarization == "Hz":
+            return {"Hz": (ny - 1, nx - 1), "Ex": (ny, nx - 1), "Ey": (ny - 1, nx)}
         return {"Ez": (ny, nx), "Hx": (ny, nx - 1), "Hy": (ny - 1, nx)}
     return {"Ex": (nz, ny, nx - 1), "Ey": (nz, ny - 1, nx), "Ez": (nz - 1, ny, nx),
             "Hx": (nz - 1, ny - 1, nx), "Hy": (nz - 1, ny, nx - 1), "Hz": (nz, ny - 1, nx - 1)}
@@ -252,6 +285,28 @@ def power_2d(Ez_v, Hx_v, Hy_v):
     return pm, np.sum(pm)
 
 
+def power_2d_hz(Hz_v, Ex_v, Ey_v):
+    """The monitor power of monitors.py:407-417 for the (Hz, Ex, Ey) family: S = E x H = (Ey Hz, -Ex Hz)."""
+    hz, ex, ey = np.array(Hz_v), np.array(Ex_v), np.array(Ey_v)
+    sx = ey * hz
+    sy = -ex * hz
+    pm = np.sqrt(sx ** 2 + sy ** 2)
+    return pm, np.sum(pm)
+
+
+def grid_power_2d_hz(Hz, Ex, Ey):
+    """helper.py:152-179's Sx^2 + Sy^2 for the (Hz, Ex, Ey) family (real fields), every component
+    zero-padded to the (ny, nx) box."""
+    ny, nx = Ex.shape[0], Ey.shape[1]
+    hz = np.zeros((ny, nx)); ex = np.zeros((ny, nx)); ey = np.zeros((ny, nx))
+    hz[:-1, :-1] = np.real(Hz)
+    ex[:, :-1] = np.real(Ex)
+    ey[:-1, :] = np.real(Ey)
+    sx = ey * hz
+    sy = -ex * hz
+    return sx ** 2 + sy ** 2
+
+
 def power_3d(Ex, Ey, Ez, Hx, Hy, Hz):
     """monitors.py:426-434."""
     sx = Ey * Hz - Ez * Hy
@@ -311,8 +366,11 @@ class RefSim:
     """
 
     def __init__(self, eps_r, sigma, dx, dy, dt, num_steps, dz=None, sources=(), monitors=(),
-                 dtype=np.float64, accumulate_power=False, design_dims=None):
+                 dtype=np.float64, accumulate_power=False, design_dims=None, polarization="Ez"):
         self.is_3d = eps_r.ndim == 3
+        if polarization not in ("Ez", "Hz") or (self.is_3d and polarization != "Ez"):
+            raise ValueError("polarization is 'Ez' or (2D only) 'Hz'")
+        self.pol = polarization
         self.eps_r = np.asarray(eps_r, dtype=np.float64)
         self.sigma = np.asarray(sigma, dtype=np.float64)
         self.dx, self.dy, self.dz, self.dt = dx, dy, dz, dt
@@ -330,14 +388,15 @@ class RefSim:
         else:
             self.ny, self.nx = eps_r.shape
             self.nz = None
-            shp = field_shapes(self.nx, self.ny)
-        # fdtd.py:178,185-186: 2D Ez complex, H real until first upcast
+            shp = field_shapes(self.nx, self.ny, polarization=polarization)
+        # fdtd.py:178,185-186: 2D Ez complex, H real until first upcast (the Hz family: all of dtype)
         self.F = {}
         for k, s in shp.items():
             dt_k = dtype
-            if (not self.is_3d) and k != "Ez" and dtype == np.complex128:
+            if (not self.is_3d) and self.pol == "Ez" and k != "Ez" and dtype == np.complex128:
                 dt_k = np.float64
             self.F[k] = np.zeros(s, dtype=dt_k)
+        self.names = tuple(shp)
         self.t = 0
         self.current_step = 0
         self.mon_series = [dict(t=[], Ez=[], Hx=[], Hy=[], Ex=[], Ey=[], Hz=[], power=[]) for _ in self.monitors]
@@ -347,6 +406,8 @@ class RefSim:
     def simulate_step(self):
         if self.is_3d:
             update_fields_3d(self.F, self.eps_r, self.sigma, self.dx, self.dy, self.dz, self.dt)
+        elif self.pol == "Hz":
+            update_fields_2d_hz(self.F, self.eps_r, self.sigma, self.dx, self.dy, self.dt)
         else:
             F = self.F
             F["Hx"], F["Hy"] = update_h_fields_2d(F["Hx"], F["Hy"], F["Ez"], self.sigma, self.dx, self.dy, self.dt)
@@ -354,6 +415,10 @@ class RefSim:
 
     # helper.py:5-118
     def apply_sources(self):
+        """helper.py:5-118 into Ez; the (Hz, Ex, Ey) family adds the same patches and profiles into Hz,
+        clipped to Hz's (ny-1, nx-1) cells."""
+        if self.pol == "Hz":
+            return self._apply_sources_hz()
         Ez = self.F["Ez"]
         n = self.current_step
         for si, src in enumerate(self.sources):
@@ -403,6 +468,37 @@ class RefSim:
                     y0, y1, x0, x1, w = self._src_cache[key]
                     Ez[y0:y1, x0:x1] += w * mod
 
+    def _apply_sources_hz(self):
+        Hz = self.F["Hz"]
+        hy, hx = Hz.shape
+        n = self.current_step
+        for si, src in enumerate(self.sources):
+            mod = src.signal[n]
+            if hasattr(src, "mode_profiles"):
+                key = ("m", si)
+                if key not in self._src_cache:
+                    self._src_cache[key] = mode_points(src, self.dx, self.dy, None, hx, hy, None, None, False)
+                enforce = getattr(src, "enforce_direction", True)
+                for _, y, x, amp in self._src_cache[key]:
+                    val = amp * mod
+                    if not np.iscomplexobj(Hz):
+                        val = np.real(val)
+                    Hz[y, x] += val
+                    if enforce:
+                        d = src.direction
+                        if d == "+x" and x > 0: Hz[y, x - 1] = 0
+                        elif d == "-x" and x < hx - 1: Hz[y, x + 1] = 0
+                        elif d == "+y" and y > 0: Hz[y - 1, x] = 0
+                        elif d == "-y" and y < hy - 1: Hz[y + 1, x] = 0
+            else:
+                key = ("g", si)
+                if key not in self._src_cache:
+                    y0, y1, x0, x1, w = gaussian_patch_2d(src.position, src.width, self.dx, self.dy, self.nx, self.ny)
+                    y1c, x1c = min(y1, hy), min(x1, hx)
+                    self._src_cache[key] = (y0, y1c, x0, x1c, w[: max(0, y1c - y0), : max(0, x1c - x0)])
+                y0, y1, x0, x1, w = self._src_cache[key]
+                Hz[y0:y1, x0:x1] += w * mod
+
     # helper.py:231-256 -> monitors.py:279-397
     def record_monitors(self):
         F = self.F
@@ -411,7 +507,7 @@ class RefSim:
             if not self.is_3d:
                 pts = monitor_points_2d(mon.start, mon.end, self.dx, self.dy)
                 vals = {}
-                for name in ("Ez", "Hx", "Hy"):
+                for name in self.names:
                     a = F[name]
                     row = []
                     for xi, yi in pts:
@@ -423,7 +519,10 @@ class RefSim:
                     vals[name] = row
                     rec[name].append(row)
                 rec["t"].append(self.t)
-                _, tot = power_2d(vals["Ez"], vals["Hx"], vals["Hy"])
+                if self.pol == "Hz":
+                    _, tot = power_2d_hz(vals["Hz"], vals["Ex"], vals["Ey"])
+                else:
+                    _, tot = power_2d(vals["Ez"], vals["Hx"], vals["Hy"])
                 rec["power"].append(tot)
             else:
                 W, H, D = self.design_dims
@@ -441,7 +540,12 @@ class RefSim:
         """helper.py:120-187."""
         if not self.accumulate_power:
             return
-        pm = grid_power_3d(self.F) if self.is_3d else grid_power_2d(self.F["Ez"], self.F["Hx"], self.F["Hy"])
+        if self.is_3d:
+            pm = grid_power_3d(self.F)
+        elif self.pol == "Hz":
+            pm = grid_power_2d_hz(self.F["Hz"], self.F["Ex"], self.F["Ey"])
+        else:
+            pm = grid_power_2d(self.F["Ez"], self.F["Hx"], self.F["Hy"])
         if self.power_accumulated is None:
             self.power_accumulated = pm.copy()
         else:

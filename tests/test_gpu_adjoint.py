@@ -9,7 +9,8 @@ import pytest
 
 import beamz_amd as bz
 from beamz_amd.devices.signals import gaussian_pulse
-from beamz_amd.optimization.adjoint import AdjointProblem2D, FDTDObjective, density_to_eps
+from beamz_amd.optimization.adjoint import (AdjointProblem2D, FDTDObjective, density_to_eps,
+                                            overlap_gradient_reference, reference_adjoint_source)
 from beamz_amd.simulation.meshing import RegularGrid
 from oracle import adjoint_ref as A
 from oracle import fdtd_ref as R
@@ -160,6 +161,39 @@ def test_multi_frequency_adjoint_matches_fp64_oracle(hip):
     assert np.linalg.norm(g - gr) / np.linalg.norm(gr) < 1e-3
 
 
+def _oracle_box_series(eps, sig, sim, sources, box):
+    x0, y0, sx, sy = box
+    ref = R.RefSim(eps, sig, sim.dx, sim.dy, sim.dt, sim.num_steps, sources=sources)
+    out = []
+    while ref.step():
+        out.append(ref.F["Ez"][y0:y0 + sy, x0:x0 + sx].copy())
+    return out
+
+
+def test_reference_overlap_gradient_matches_oracle(hip):
+    """overlap_gradient_reference: the reference's time-reversed overlap (examples/4_topo.py:279-286) of a
+    forward run and a run driven by its adjoint source (4_topo.py:214-233, reference_adjoint_source), against
+    the FP64 oracle's runs of the same two sources and oracle.topo_ref.overlap_gradient; 1e-4 rel-L2."""
+    eps, sig, _ = materials(seed=5)
+    nsteps = 260
+    sim, mon, src, t = build(eps, sig, nsteps)
+    sim_a, _, _, _ = build(eps, sig, nsteps)
+    npts = len(mon.get_grid_points_2d(DX, DX))
+    rng = np.random.default_rng(9)
+    target = rng.standard_normal(npts) + 1j * rng.standard_normal(npts)
+    target /= np.linalg.norm(target)
+    adj = reference_adjoint_source(sim_a.design, mon, DX, DX, target, src.signal, 1.55e-6)
+    sim_a.sources = [adj]
+    g = overlap_gradient_reference(sim, sim_a, BOX).cpu().numpy()
+    e32, s32 = eps.astype(np.float32).astype(np.float64), sig.astype(np.float32).astype(np.float64)
+    fwd = _oracle_box_series(e32, s32, sim, [src], BOX)
+    aser = _oracle_box_series(e32, s32, sim, [adj], BOX)
+    g_ref = TR.overlap_gradient(aser, fwd)
+    err = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    print(f"reference overlap gradient vs oracle: rel-L2 {err:.2e}")
+    assert err < 1e-4, err
+
+
 def test_c4_gradient_vs_central_differences_on_32_pixels(hip):
     """The full-size topology workload (600 x 600 cells, 120 x 120 design pixels, 3000 steps, cone filter r=3,
     tanh projection beta=8): d FOM / d rho from the adjoint + autograd chain against central differences
@@ -209,6 +243,34 @@ def test_c4_gradient_vs_central_differences_on_32_pixels(hip):
     err = float(np.linalg.norm(fd - ga) / np.linalg.norm(fd))
     print(f"C4 FD check: cosine {cos:.6f}, rel-L2 {err:.3e}, J {float(J):.6e}")
     assert cos > 0.99 and err < 2e-2, (cos, err)
+    # the reference's own gradient estimate (examples/4_topo.py:279-286, its adjoint source of :214-233 with the
+    # target mode conj(c)) at the same design, chained through the same filter and projection: compared by
+    # cosine and sign, since it is a heuristic and not the derivative of the FOM
+    e0 = density_to_eps(torch.tensor(rho0[None], dtype=torch.float32, device="cuda"), ec, eh, 3, 8.0)[0]
+    sims2 = []
+    for _ in range(2):
+        m2, t2, _, _, _ = topo_bend_2d()
+        s2 = bz.FDTD(m2, time=t2)
+        s2.grid.eps[y0:y0 + sy, x0:x0 + sx] = e0
+        sims2.append(s2)
+    sim_f, sim_a = sims2
+    sim_a.sources = [reference_adjoint_source(sim_a.design, mon, sim_f.dx, sim_f.dy, np.conj(c),
+                                              sim_f.sources[0].signal, 1.55e-6)]
+    g_ov = overlap_gradient_reference(sim_f, sim_a, box)
+    rho_r = torch.tensor(rho0[None], dtype=torch.float32, device="cuda", requires_grad=True)
+    (density_to_eps(rho_r, ec, eh, 3, 8.0)[0].double() * g_ov).sum().backward()
+    g_ref = rho_r.grad.cpu().numpy()[0]
+    gr = np.array([g_ref[j, i] for (j, i) in pix])
+    cos_ref_fd = float(fd @ gr / (np.linalg.norm(fd) * np.linalg.norm(gr)))
+    cos_ref_exact = float((g_ref * g).sum() / (np.linalg.norm(g_ref) * np.linalg.norm(g)))
+    top = np.abs(g) >= np.quantile(np.abs(g), 0.95)
+    sign_top = float(np.mean(np.sign(g_ref[top]) == np.sign(g[top])))
+    sign_pix = float(np.mean(np.sign(gr) == np.sign(fd)))
+    print(f"C4 reference overlap gradient: cosine vs FD {cos_ref_fd:+.4f} (sign agreement {sign_pix:.2f} on the "
+          f"32 pixels), cosine vs the exact gradient {cos_ref_exact:+.4f} over the design box (sign agreement "
+          f"{sign_top:.2f} on its 5 % largest entries)")
+    assert np.isfinite(g_ref).all() and np.abs(g_ref).max() > 0
+    assert cos > abs(cos_ref_fd)            # the exact adjoint predicts the FOM change better than the estimate
 
 
 def test_pair_sweeps_give_the_same_gradient_and_engines_are_reused(hip):
