@@ -1,54 +1,73 @@
-"""Stateful optimizer wrapper (beamz/optimization/optimizers.py:23-107).
+"""Gradient-step rules for design updates.
 
-Adam with beta1=0.9, beta2=0.999, eps=1e-8 by default; ``step`` returns the
-update to add to the parameters (the reference's sign convention).
+Public interface of beamz/optimization/optimizers.py:66-107: ``Optimizer(method,
+learning_rate, options)``, whose ``step(gradient)`` returns the increment to add
+to the design variables (a descent step: the minus sign is inside) and whose
+``reset()`` forgets the accumulated state.
+
+The one rule is Adam (Kingma and Ba, 2015): exponential moving averages of the
+gradient and of its square, each divided by its bias correction 1 - beta^t,
+increment = -lr * mean_hat / (sqrt(square_hat) + epsilon), with the published
+defaults beta1 = 0.9, beta2 = 0.999, epsilon = 1e-8 (the ones the reference
+uses). ``options`` overrides them by name.
 """
 from __future__ import annotations
-
-from dataclasses import dataclass, field
-from typing import Dict, Mapping, MutableMapping
 
 import numpy as np
 
 
-def _adam_update(grad, learning_rate, state, *, beta1=0.9, beta2=0.999, epsilon=1e-8):
-    m = state.setdefault("m", np.zeros_like(grad))
-    v = state.setdefault("v", np.zeros_like(grad))
-    t = state["t"] = state.get("t", 0) + 1
-    m *= beta1
-    m += (1.0 - beta1) * grad
-    v *= beta2
-    v += (1.0 - beta2) * (grad ** 2)
-    m_hat = m / (1.0 - beta1 ** t)
-    v_hat = v / (1.0 - beta2 ** t)
-    return -learning_rate * m_hat / (np.sqrt(v_hat) + epsilon)
+class AdamRule:
+    """Adam's state for one array of design variables."""
+
+    name = "adam"
+
+    def __init__(self, beta1=0.9, beta2=0.999, epsilon=1e-8):
+        self.beta1, self.beta2, self.epsilon = float(beta1), float(beta2), float(epsilon)
+        self.reset()
+
+    def reset(self):
+        self.mean = None        # moving average of the gradient
+        self.square = None      # moving average of its elementwise square
+        self.count = 0
+
+    def increment(self, g, lr):
+        if self.mean is None or self.mean.shape != g.shape:
+            self.mean = np.zeros_like(g)
+            self.square = np.zeros_like(g)
+            self.count = 0
+        self.count += 1
+        b1, b2 = self.beta1, self.beta2
+        self.mean = b1 * self.mean + (1.0 - b1) * g
+        self.square = b2 * self.square + (1.0 - b2) * np.square(g)
+        mean_hat = self.mean / (1.0 - b1 ** self.count)
+        square_hat = self.square / (1.0 - b2 ** self.count)
+        return -lr * mean_hat / (np.sqrt(square_hat) + self.epsilon)
 
 
-_OPTIMIZER_REGISTRY: Dict[str, object] = {"adam": _adam_update}
+RULES = {rule.name: rule for rule in (AdamRule,)}
 
 
-@dataclass
 class Optimizer:
-    method: str = "adam"
-    learning_rate: float = 1e-2
-    options: Mapping[str, float] = field(default_factory=dict)
-    _state: MutableMapping[str, np.ndarray] = field(default_factory=dict, init=False)
+    """Keeps one update rule's state across iterations; ``step`` maps a gradient to the increment."""
 
-    def __post_init__(self):
-        key = self.method.lower()
-        if key not in _OPTIMIZER_REGISTRY:
-            raise ValueError(f"Unknown optimizer method '{self.method}'. Available: {', '.join(sorted(_OPTIMIZER_REGISTRY))}.")
-        self._update_fn = _OPTIMIZER_REGISTRY[key]
+    def __init__(self, method="adam", learning_rate=1e-2, options=None):
+        rule = RULES.get(str(method).lower())
+        if rule is None:
+            raise ValueError(f"optimizer {method!r} is not available; choose one of {sorted(RULES)}")
+        self.method = method
+        self.learning_rate = learning_rate
+        self.options = dict(options or {})
+        self.rule = rule(**self.options)
 
     def step(self, gradient):
         if not isinstance(gradient, np.ndarray):
-            raise TypeError("Gradient must be a numpy.ndarray")
+            raise TypeError(f"the gradient must be a numpy array, not {type(gradient).__name__}")
         if gradient.size == 0:
             return np.zeros_like(gradient)
-        return self._update_fn(gradient, self.learning_rate, self._state, **self.options)
+        return self.rule.increment(gradient, self.learning_rate)
 
     def reset(self):
-        self._state.clear()
+        self.rule.reset()
 
 
-__all__ = ["Optimizer"]
+__all__ = ["Optimizer", "AdamRule"]
