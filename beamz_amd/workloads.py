@@ -141,11 +141,12 @@ def topo_bend_2d(n=600, design_px=120, steps=3000, npml=20, dx=20e-9, lam0=1.55 
     return mesh, t, mon, (x0, y0, design_px, design_px), c
 
 
-def mmi_rectangles(W, H, rng, jitter=0.2):
-    """Input guide, 2.5 x 6 um body and two output guides, each size scaled by 1 + U(-jitter, jitter)."""
+def mmi_rectangles(W, H, rng, jitter=0.2, scale=1.0):
+    """Input guide, 2.5 x 6 um body and two output guides (in-plane sizes times ``scale``), each size scaled by
+    1 + U(-jitter, jitter)."""
     j = lambda: 1.0 + rng.uniform(-jitter, jitter)
-    gw_in, gw_o1, gw_o2 = 0.5 * µm * j(), 0.5 * µm * j(), 0.5 * µm * j()
-    body_w, body_l = 2.5 * µm * j(), 6.0 * µm * j()
+    gw_in, gw_o1, gw_o2 = 0.5 * µm * scale * j(), 0.5 * µm * scale * j(), 0.5 * µm * scale * j()
+    body_w, body_l = 2.5 * µm * scale * j(), 6.0 * µm * scale * j()
     x0 = (W - body_l) / 2
     yc = H / 2
     sep = body_w / 4 * j()
@@ -156,13 +157,16 @@ def mmi_rectangles(W, H, rng, jitter=0.2):
     return rects, (yc + sep, yc - sep), x0 + body_l
 
 
-def soi_mmi_3d(nx=480, ny=240, nz=160, steps=3000, npml=12, dx=25e-9, seed=0, port_freqs=8, z_pml=True):
-    """Returns (mesh, design3d, time array, monitors)."""
+def soi_mmi_3d(nx=480, ny=240, nz=160, steps=3000, npml=12, dx=25e-9, seed=0, port_freqs=8, z_pml=True, scale=1.0,
+               record_ports=False):
+    """Returns (mesh, design3d, time array, monitors). ``scale`` shrinks the in-plane MMI sizes and the port
+    planes (not the 220 nm Si layer), for parity runs on grids small enough for the FP64 oracle;
+    record_ports=True keeps the port planes' time series too (the DFT sums are always kept)."""
     rng = np.random.default_rng(seed)
     W, H, D = nx * dx, ny * dx, nz * dx
     z_si0 = D / 2 - 0.11 * µm
     z_si1 = z_si0 + 0.22 * µm
-    rects, port_y, body_end = mmi_rectangles(W, H, rng)
+    rects, port_y, body_end = mmi_rectangles(W, H, rng, scale=scale)
 
     def layer(bg, with_mmi):
         d2 = Design(W, H, material=Material(bg), auto_pml=False)
@@ -184,9 +188,10 @@ def soi_mmi_3d(nx=480, ny=240, nz=160, steps=3000, npml=12, dx=25e-9, seed=0, po
     mons = []
     xp = W - (npml + 16) * dx
     for q, yp in enumerate(port_y):
-        m = Monitor(design=d3, plane_normal="x", plane_position=xp, start=(xp, yp - 0.6 * µm, zc - 0.4 * µm),
-                    size=(1.2 * µm, 0.8 * µm), dft_frequencies=freqs, record_fields=False, accumulate_power=False,
-                    name=f"port{q + 1}")
+        hy, hz = 0.6 * µm * scale, 0.4 * µm * min(1.0, 2 * scale)
+        m = Monitor(design=d3, plane_normal="x", plane_position=xp, start=(xp, yp - hy, zc - hz),
+                    size=(2 * hy, 2 * hz), dft_frequencies=freqs, record_fields=record_ports,
+                    accumulate_power=False, name=f"port{q + 1}")
         d3 += m
         mons.append(m)
     return mesh, d3, t, mons

@@ -145,6 +145,54 @@ def update_fields_2d_hz(F, eps_r, sigma, dx, dy, dt, mu0=MU_0, eps0=EPS_0):
     Ey[:, 1:-1] = coef[1][0] * Ey[:, 1:-1] + coef[1][1] * chy
 
 
+# --------------------------------------------------------------------------
+# The same 2D/3D updates in C (oracle/yee_ref.c, built by oracle/build.py), for
+# long runs: the same expressions in the same order, so the same bits
+# (tests/test_oracle_c.py). Real FP64 fields only.
+# --------------------------------------------------------------------------
+_CLIB = None
+
+
+def _clib():
+    global _CLIB
+    if _CLIB is None:
+        import ctypes as C
+        import os
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liboref.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run python -m oracle.build")
+        lib = C.CDLL(path)
+        dp, i32, f64 = C.c_void_p, C.c_int, C.c_double
+        lib.oref_step2d.argtypes = [dp] * 5 + [i32] * 2 + [f64] * 5
+        lib.oref_step2d.restype = None
+        lib.oref_step3d.argtypes = [dp] * 8 + [i32] * 3 + [f64] * 6
+        lib.oref_step3d.restype = None
+        _CLIB = lib
+    return _CLIB
+
+
+def _ptr(a, shape=None):
+    if a.dtype != np.float64 or not a.flags.c_contiguous or (shape is not None and a.shape != tuple(shape)):
+        raise ValueError("the C oracle takes C-contiguous float64 arrays of the reference shapes")
+    return a.ctypes.data
+
+
+def step_2d_c(F, eps_r, sigma, dx, dy, dt, mu_0=MU_0, eps_0=EPS_0):
+    """update_h_fields_2d then update_e_field_2d, in place on F (real fields)."""
+    ny, nx = eps_r.shape
+    shp = field_shapes(nx, ny)
+    _clib().oref_step2d(*(_ptr(F[k], shp[k]) for k in ("Ez", "Hx", "Hy")), _ptr(sigma, (ny, nx)),
+                        _ptr(eps_r, (ny, nx)), nx, ny, dx, dy, dt, mu_0, eps_0)
+
+
+def step_3d_c(F, eps_r, sigma, dx, dy, dz, dt, mu0=MU_0, eps0=EPS_0):
+    """update_fields_3d, in place on F (real fields)."""
+    nz, ny, nx = eps_r.shape
+    shp = field_shapes(nx, ny, nz)
+    _clib().oref_step3d(*(_ptr(F[k], shp[k]) for k in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")),
+                        _ptr(eps_r, (nz, ny, nx)), _ptr(sigma, (nz, ny, nx)), nx, ny, nz, dx, dy, dz, dt, mu0, eps0)
+
+
 def field_shapes(nx, ny, nz=None, polarization="Ez"):
     """fdtd.py:151-186: reference array shapes, [z,y,x] / [y,x] order; the 2D (Hz, Ex, Ey) family takes the
     shapes of those components in the 3D layout."""
@@ -276,6 +324,36 @@ def slice_field_3d(arr, sl, plane_normal):
     return arr.copy()
 
 
+def true_plane_3d(arr, sl, plane_normal):
+    """x- or y-normal plane of one component as a plane (the reference's fallback copies the whole volume,
+    monitors.py:356-358; this is the documented fix used for flux ports). sl is get_grid_slice_3d's
+    (y_slice, x_index, z_slice) or (y_index, x_slice, z_slice); the fixed index and the ranges are clamped
+    to the component's own shape as the z-plane branch does (monitors.py:326-337). Returns [z][y] (x-normal)
+    or [z][x] (y-normal)."""
+    a0, a1, a2 = arr.shape
+    if plane_normal == "x":
+        fixed = min(max(0, sl[1]), a2 - 1)
+        u_rng, n_u = sl[0], a1
+    else:
+        fixed = min(max(0, sl[0]), a1 - 1)
+        u_rng, n_u = sl[1], a2
+    us = u_rng.start or 0; ue = u_rng.stop or n_u
+    zs = sl[2].start or 0; ze = sl[2].stop or a0
+    us = max(0, min(us, n_u - 1)); ue = max(us, min(ue, n_u))
+    zs = max(0, min(zs, a0 - 1)); ze = max(zs, min(ze, a0))
+    if plane_normal == "x":
+        return arr[zs:ze, us:ue, fixed].copy()
+    return arr[zs:ze, fixed, us:ue].copy()
+
+
+def plane_flux(dft, plane_normal):
+    """Net spectral flux through a plane from its DFT fields (no reference counterpart):
+    sum over the plane of Re(E x conj(H)) . n, per frequency; dft[c] has shape (nfreq, ...)."""
+    a, b = {"x": ("y", "z"), "y": ("z", "x"), "z": ("x", "y")}[plane_normal]
+    s = dft["E" + a] * np.conj(dft["H" + b]) - dft["E" + b] * np.conj(dft["H" + a])
+    return np.real(s).reshape(s.shape[0], -1).sum(axis=1)
+
+
 def power_2d(Ez_v, Hx_v, Hy_v):
     """monitors.py:407-417: per-point |S| and the step total."""
     ez, hx, hy = np.array(Ez_v), np.array(Hx_v), np.array(Hy_v)
@@ -366,13 +444,21 @@ class RefSim:
     """
 
     def __init__(self, eps_r, sigma, dx, dy, dt, num_steps, dz=None, sources=(), monitors=(),
-                 dtype=np.float64, accumulate_power=False, design_dims=None, polarization="Ez"):
+                 dtype=np.float64, accumulate_power=False, design_dims=None, polarization="Ez", native=False,
+                 true_planes=False):
+        """native=True advances the (Ez, Hx, Hy) and 3D updates with the C restatement (same bits, real
+        fields only). true_planes=True samples x- and y-normal 3D monitors as planes (true_plane_3d) instead
+        of the reference's whole-volume copy."""
+        self.native = bool(native)
+        self.true_planes = bool(true_planes)
+        if self.native and dtype != np.float64:
+            raise ValueError("the C updates take real FP64 fields")
         self.is_3d = eps_r.ndim == 3
         if polarization not in ("Ez", "Hz") or (self.is_3d and polarization != "Ez"):
             raise ValueError("polarization is 'Ez' or (2D only) 'Hz'")
         self.pol = polarization
-        self.eps_r = np.asarray(eps_r, dtype=np.float64)
-        self.sigma = np.asarray(sigma, dtype=np.float64)
+        self.eps_r = np.ascontiguousarray(eps_r, dtype=np.float64)
+        self.sigma = np.ascontiguousarray(sigma, dtype=np.float64)
         self.dx, self.dy, self.dz, self.dt = dx, dy, dz, dt
         self.num_steps = num_steps
         self.sources = list(sources)
@@ -404,8 +490,12 @@ class RefSim:
 
     # fdtd.py:188-198 / 200-310
     def simulate_step(self):
-        if self.is_3d:
+        if self.is_3d and self.native:
+            step_3d_c(self.F, self.eps_r, self.sigma, self.dx, self.dy, self.dz, self.dt)
+        elif self.is_3d:
             update_fields_3d(self.F, self.eps_r, self.sigma, self.dx, self.dy, self.dz, self.dt)
+        elif self.native and self.pol == "Ez":
+            step_2d_c(self.F, self.eps_r, self.sigma, self.dx, self.dy, self.dt)
         elif self.pol == "Hz":
             update_fields_2d_hz(self.F, self.eps_r, self.sigma, self.dx, self.dy, self.dt)
         else:
@@ -527,7 +617,8 @@ class RefSim:
             else:
                 W, H, D = self.design_dims
                 sl = monitor_slice_3d(mon, W, H, D, self.dx, self.dy, self.dz)
-                sls = {k: slice_field_3d(F[k], sl, mon.plane_normal) for k in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")}
+                cut = true_plane_3d if (self.true_planes and mon.plane_normal in ("x", "y")) else slice_field_3d
+                sls = {k: cut(F[k], sl, mon.plane_normal) for k in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")}
                 my = min(s.shape[0] for s in sls.values()); mx = min(s.shape[1] for s in sls.values())
                 for k in sls:
                     sls[k] = sls[k][:my, :mx]

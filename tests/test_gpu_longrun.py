@@ -1,0 +1,143 @@
+"""GPU parity over whole runs, against the FP64 oracle advanced by its C restatement
+(oracle/yee_ref.c, bit-identical to the NumPy restatement, tests/test_oracle_c.py).
+
+- C1 in full: 800 x 400 cells, all 4000 steps, the 16-wavelength DFT line at x = 700.
+- A 3D SOI MMI at dx = 25 nm for the full 3000 steps of the 3D workload, on a 120 x 72 x 40 cross-section
+  (in-plane sizes scaled by 0.3 so the FP64 oracle finishes), with both x-normal port planes: fields, the
+  port DFTs of all six components and their flux.
+- The full 3D grid, 480 x 240 x 160 (three 54-plane segments of the two-step kernel), for 12 steps.
+- 3D grid power, and saved / cached results (save=True, fields_to_cache).
+
+Tolerances (relative L2, complex-aware): fields, series and spectra 1e-4; power 2e-4. The measured errors
+are printed."""
+import numpy as np
+import pytest
+
+import beamz_amd as bz
+from beamz_amd.workloads import pulse_signal, soi_mmi_3d, waveguide_2d
+from oracle import fdtd_ref as R
+
+pytestmark = pytest.mark.gpu
+
+COMP3 = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.complex128); b = np.asarray(b, dtype=np.complex128)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _oracle_built():
+    R._clib()
+
+
+def test_c1_full_run_matches_fp64_oracle(hip):
+    d, dx, t, mon = waveguide_2d()                  # 800 x 400, 4000 steps, line at x = 700, 16 wavelengths
+    sim = bz.FDTD(d, time=t, resolution=dx)
+    sim.run(live=False, save=False)
+    assert sim.grid._f2d.kstep and len(t) == 4000
+    ref = R.RefSim(sim.epsilon_r, sim.sigma, dx, dx, float(t[1] - t[0]), len(t), sources=d.sources,
+                   monitors=[mon], native=True).run()
+    errs = {}
+    for k in ("Ez", "Hx", "Hy"):
+        errs[k] = rel_l2(sim.field(k).cpu().numpy(), ref.F[k])
+        ser = np.array(ref.mon_series[0][k])
+        errs["series " + k] = rel_l2(np.array(mon.fields[k]), ser)
+        F = R.dft_of_series(ser, ref.mon_series[0]["t"], 2 * np.pi * mon.dft_frequencies)
+        errs["dft " + k] = rel_l2(mon.dft[k], F)
+    errs["power"] = rel_l2(mon.power_history, ref.mon_series[0]["power"])
+    print("C1 4000 steps, rel-L2:", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert np.abs(np.array(ref.mon_series[0]["Ez"])).max() > 1e-3          # the pulse reached the line
+    for k, v in errs.items():
+        assert v < (2e-4 if k == "power" else 1e-4), (k, v)
+
+
+def test_3d_mmi_3000_steps_fields_ports_and_flux_match_fp64_oracle(hip):
+    mesh, d3, t, mons = soi_mmi_3d(120, 72, 40, steps=3000, scale=0.3, record_ports=True)
+    sim = bz.FDTD(mesh, time=t)
+    sim.run(live=False, save=False)
+    assert sim.grid.tb2
+    ref = R.RefSim(sim.epsilon_r, sim.sigma, sim.dx, sim.dy, sim.dt, len(t), dz=sim.dz, sources=d3.sources,
+                   monitors=mons, design_dims=(d3.width, d3.height, d3.depth), native=True, true_planes=True).run()
+    errs = {}
+    for k in COMP3:
+        errs[k] = rel_l2(sim.field(k).cpu().numpy(), ref.F[k])
+    for q, m in enumerate(mons):
+        ts = ref.mon_series[q]["t"]
+        D = {}
+        for k in COMP3:
+            ser = np.array(ref.mon_series[q][k])                 # (steps, z, y)
+            assert np.array(m.fields[k]).shape == ser.shape, k
+            D[k] = R.dft_of_series(ser, ts, 2 * np.pi * m.dft_frequencies)
+            errs[f"port{q + 1} dft {k}"] = rel_l2(m.dft[k], D[k])
+        errs[f"port{q + 1} flux"] = rel_l2(m.dft_flux(), R.plane_flux(D, "x"))
+        assert np.all(np.abs(R.plane_flux(D, "x")) > 0)
+    print("3D MMI 120x72x40, 3000 steps, rel-L2:", {k: f"{v:.2e}" for k, v in errs.items()})
+    for k, v in errs.items():
+        assert v < 1e-4, (k, v)
+
+
+def test_full_c2_grid_matches_fp64_oracle(hip):
+    torch = hip
+    mesh, d3, t, mons = soi_mmi_3d(480, 240, 160, steps=12)
+    sim = bz.FDTD(mesh, time=t)
+    sim.run(live=False, save=False)
+    assert sim.grid.tb2 and sim.grid.kseg2_eff() == 54
+    got = {k: sim.field(k).cpu().numpy() for k in COMP3}
+    del sim
+    torch.cuda.empty_cache()
+    ref = R.RefSim(mesh.permittivity, mesh.conductivity, mesh.dx, mesh.dy, float(t[1] - t[0]), len(t), dz=mesh.dz,
+                   sources=d3.sources, native=True).run()
+    errs = {k: rel_l2(got[k], ref.F[k]) for k in COMP3}
+    print("C2 480x240x160, 12 steps, rel-L2:", {k: f"{v:.2e}" for k, v in errs.items()})
+    for k in COMP3:
+        assert np.abs(ref.F[k]).max() > 0, k
+        assert errs[k] < 1e-4, (k, errs[k])
+
+
+def test_3d_grid_power_matches_oracle(hip):
+    dx = 50e-9
+    nx, ny, nz = 40, 32, 24
+    d = bz.Design(nx * dx, ny * dx, depth=nz * dx, material=bz.Material(1.444 ** 2), pml_size=6 * dx)
+    d += bz.Rectangle(position=(0, ny * dx / 2 - 0.25e-6, nz * dx / 2), width=nx * dx, height=0.5e-6,
+                      depth=0.22e-6, material=bz.Material(3.476 ** 2))
+    dt = 0.99 * dx / (bz.LIGHT_SPEED * np.sqrt(3))
+    t = np.arange(200) * dt
+    d += bz.GaussianSource(position=(10 * dx, ny * dx / 2, nz * dx / 2), width=dx, signal=pulse_signal(t))
+    sim = bz.FDTD(d, time=t, resolution=dx)
+    sim.run(live=False, save=False, accumulate_power=True)
+    ref = R.RefSim(sim.epsilon_r, sim.sigma, dx, dx, dt, len(t), dz=dx, sources=d.sources, accumulate_power=True,
+                   design_dims=(d.width, d.height, d.depth)).run()
+    assert sim.power_accumulation_count == ref.power_count == len(t)
+    want = ref.power_accumulated / ref.power_count
+    assert sim.power_accumulated.shape == want.shape
+    err = rel_l2(sim.power_accumulated, want)
+    print(f"3D grid power rel-L2 {err:.2e}")
+    assert err < 2e-4
+
+
+def test_saved_and_cached_results_match_oracle(hip):
+    """save=True with decimate_save=3 and fields_to_cache=["Ez"] (helper.py:189-229): Ez kept complex every
+    step, Hx and Hy stored as |.| every third step and at the last, t of every save."""
+    d, dx, t, mon = waveguide_2d(nx=96, ny=48, steps=40, npml=8, src_x=20, mon_x=70, n_freq=2)
+    sim = bz.FDTD(d, time=t, resolution=dx)
+    res = sim.run(live=False, save=True, decimate_save=3, fields_to_cache=["Ez"])
+    ref = R.RefSim(sim.epsilon_r, sim.sigma, dx, dx, float(t[1] - t[0]), len(t), sources=d.sources)
+    want = {"t": [], "Ez": [], "Hx": [], "Hy": []}
+    for n in range(len(t)):
+        tn = ref.t
+        ref.step()
+        full = n % 3 == 0 or n == len(t) - 1
+        want["t"].append(tn)
+        want["Ez"].append(ref.F["Ez"].copy())
+        if full:
+            want["Hx"].append(np.abs(ref.F["Hx"]))
+            want["Hy"].append(np.abs(ref.F["Hy"]))
+    assert len(res["t"]) == len(want["t"]) == len(t)
+    assert np.allclose(res["t"], want["t"], rtol=1e-12)
+    assert len(res["Ez"]) == len(t) and np.iscomplexobj(res["Ez"][0])
+    assert len(res["Hx"]) == len(want["Hx"]) == 14 and not np.iscomplexobj(res["Hx"][0])
+    for k in ("Ez", "Hx", "Hy"):
+        err = rel_l2(np.array(res[k]), np.array(want[k]))
+        assert err < 1e-4, (k, err)

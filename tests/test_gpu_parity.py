@@ -13,7 +13,7 @@ um = 1e-6
 
 
 def rel_l2(a, b):
-    a = np.asarray(a, dtype=np.float64); b = np.asarray(b, dtype=np.float64)
+    a = np.asarray(a, dtype=np.complex128); b = np.asarray(b, dtype=np.complex128)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 

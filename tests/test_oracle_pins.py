@@ -258,3 +258,39 @@ def test_mode_injection_direction_zeroing_3d(direction, zeroed):
     want[2, 2, 3] += 1.0
     want[zeroed] = 0.0
     assert np.array_equal(sim.F["Ez"], want)
+
+
+# ----------------------------------------------------------------------------- x/y-normal planes and flux
+def test_true_plane_3d_brute_force_codes_and_clamping():
+    nz, ny, nx = 6, 7, 8
+    code = lambda s: np.fromfunction(lambda k, j, i: 10000 * k + 100 * j + i, s)
+    Hy = code((nz - 1, ny, nx - 1))          # 5 planes, 7 rows, 7 columns
+    Ey = code((nz, ny - 1, nx))
+    # x-normal: (y slice, x index, z slice) -> [z][y]; z stops at Hy's 5 planes
+    got = R.true_plane_3d(Hy, (slice(1, 5), 3, slice(2, 9)), "x")
+    assert np.array_equal(got, [[10000 * k + 100 * j + 3 for j in range(1, 5)] for k in range(2, 5)])
+    # the fixed x index clamps to the component's last column (nx - 2 for Hy)
+    got = R.true_plane_3d(Hy, (slice(0, 2), nx - 1, slice(0, 1)), "x")
+    assert np.array_equal(got, [[100 * j + nx - 2 for j in range(0, 2)]])
+    # y-normal: (y index, x slice, z slice) -> [z][x]; y clamps to Ey's last row (ny - 2)
+    got = R.true_plane_3d(Ey, (ny - 1, slice(2, 5), slice(1, 3)), "y")
+    assert np.array_equal(got, [[10000 * k + 100 * (ny - 2) + i for i in range(2, 5)] for k in range(1, 3)])
+    got[0, 0] = -1
+    assert Ey[1, ny - 2, 2] != -1
+
+
+def test_plane_flux_plane_wave_sign_and_conjugate():
+    eta = 376.7
+    phi = np.linspace(0.3, 2.0, 5)[:, None] * np.ones((1, 12))        # 5 frequencies, 12 plane points
+    one = np.exp(1j * phi)
+    zero = np.zeros_like(one)
+    # +x wave: Ey = e^{i phi}, Hz = e^{i phi} / eta -> Re(Ey conj Hz) = 1/eta per point (no conj: cos(2 phi))
+    D = {"Ex": zero, "Ey": one, "Ez": zero, "Hx": zero, "Hy": zero, "Hz": one / eta}
+    np.testing.assert_allclose(R.plane_flux(D, "x"), 12 / eta, rtol=1e-14)
+    # the other polarisation: Ez = 1, Hy = -1/eta travels along +x too
+    D = {"Ex": zero, "Ey": zero, "Ez": one, "Hx": zero, "Hy": -one / eta, "Hz": zero}
+    np.testing.assert_allclose(R.plane_flux(D, "x"), 12 / eta, rtol=1e-14)
+    # through a y-normal plane it carries nothing; a z-normal plane sees Ex, Hy
+    assert np.allclose(R.plane_flux(D, "y"), 0.0)
+    D = {"Ex": one, "Ey": zero, "Ez": zero, "Hx": zero, "Hy": one / eta, "Hz": zero}
+    np.testing.assert_allclose(R.plane_flux(D, "z"), 12 / eta, rtol=1e-14)
