@@ -949,7 +949,9 @@ class Fused2D:
             for d, g, sh, sn in zip(self.descs, self.grids, shifts, self.snapshots):
                 for p in (0, 1):
                     b = g._fb[(p + sh) % 2]
-                    d.ez[p], d.hx[p], d.hy[p] = b["Ez"].data_ptr(), b["Hx"].data_ptr(), b["Hy"].data_ptr()
+                    # slots (ez, hx, hy) hold the grid's three components in order: (Ez, Hx, Hy) or (Hz, Ex, Ey)
+                    c0, c1, c2 = g.names
+                    d.ez[p], d.hx[p], d.hy[p] = b[c0].data_ptr(), b[c1].data_ptr(), b[c2].data_ptr()
                 if sn is not None:
                     buf = sn[4]
                     d.snap = buf.data_ptr() + slot0 * buf[0].numel() * buf.element_size()

@@ -167,13 +167,15 @@ def _clib():
         lib.oref_step2d.restype = None
         lib.oref_step3d.argtypes = [dp] * 8 + [i32] * 3 + [f64] * 6
         lib.oref_step3d.restype = None
+        lib.oref_step3d_f32store.argtypes = [dp] * 8 + [i32] * 3 + [f64] * 6
+        lib.oref_step3d_f32store.restype = None
         _CLIB = lib
     return _CLIB
 
 
-def _ptr(a, shape=None):
-    if a.dtype != np.float64 or not a.flags.c_contiguous or (shape is not None and a.shape != tuple(shape)):
-        raise ValueError("the C oracle takes C-contiguous float64 arrays of the reference shapes")
+def _ptr(a, shape=None, dtype=np.float64):
+    if a.dtype != dtype or not a.flags.c_contiguous or (shape is not None and a.shape != tuple(shape)):
+        raise ValueError(f"the C oracle takes C-contiguous {np.dtype(dtype).name} arrays of the reference shapes")
     return a.ctypes.data
 
 
@@ -191,6 +193,17 @@ def step_3d_c(F, eps_r, sigma, dx, dy, dz, dt, mu0=MU_0, eps0=EPS_0):
     shp = field_shapes(nx, ny, nz)
     _clib().oref_step3d(*(_ptr(F[k], shp[k]) for k in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")),
                         _ptr(eps_r, (nz, ny, nx)), _ptr(sigma, (nz, ny, nx)), nx, ny, nz, dx, dy, dz, dt, mu0, eps0)
+
+
+def step_3d_c_f32store(F, eps_r, sigma, dx, dy, dz, dt, mu0=MU_0, eps0=EPS_0):
+    """update_fields_3d's FP64 expressions on fields stored in FP32 (float32 arrays in F, each result rounded
+    once to float when stored). Not a reference path: it measures the accuracy floor that FP32 storage alone
+    sets on a long run, whatever the arithmetic."""
+    nz, ny, nx = eps_r.shape
+    shp = field_shapes(nx, ny, nz)
+    _clib().oref_step3d_f32store(*(_ptr(F[k], shp[k], np.float32) for k in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")),
+                                 _ptr(eps_r, (nz, ny, nx)), _ptr(sigma, (nz, ny, nx)), nx, ny, nz, dx, dy, dz, dt,
+                                 mu0, eps0)
 
 
 def field_shapes(nx, ny, nz=None, polarization="Ez"):
@@ -445,14 +458,18 @@ class RefSim:
 
     def __init__(self, eps_r, sigma, dx, dy, dt, num_steps, dz=None, sources=(), monitors=(),
                  dtype=np.float64, accumulate_power=False, design_dims=None, polarization="Ez", native=False,
-                 true_planes=False):
+                 true_planes=False, f32_storage=False):
         """native=True advances the (Ez, Hx, Hy) and 3D updates with the C restatement (same bits, real
         fields only). true_planes=True samples x- and y-normal 3D monitors as planes (true_plane_3d) instead
-        of the reference's whole-volume copy."""
+        of the reference's whole-volume copy. f32_storage=True (3D, native) keeps the fields in float32 and
+        advances them with the FP64 expressions (step_3d_c_f32store): the FP32-storage accuracy floor."""
         self.native = bool(native)
         self.true_planes = bool(true_planes)
+        self.f32_storage = bool(f32_storage)
         if self.native and dtype != np.float64:
             raise ValueError("the C updates take real FP64 fields")
+        if self.f32_storage and not (self.native and eps_r.ndim == 3):
+            raise ValueError("f32_storage needs native=True and a 3D grid")
         self.is_3d = eps_r.ndim == 3
         if polarization not in ("Ez", "Hz") or (self.is_3d and polarization != "Ez"):
             raise ValueError("polarization is 'Ez' or (2D only) 'Hz'")
@@ -481,7 +498,7 @@ class RefSim:
             dt_k = dtype
             if (not self.is_3d) and self.pol == "Ez" and k != "Ez" and dtype == np.complex128:
                 dt_k = np.float64
-            self.F[k] = np.zeros(s, dtype=dt_k)
+            self.F[k] = np.zeros(s, dtype=np.float32 if self.f32_storage else dt_k)
         self.names = tuple(shp)
         self.t = 0
         self.current_step = 0
@@ -490,7 +507,9 @@ class RefSim:
 
     # fdtd.py:188-198 / 200-310
     def simulate_step(self):
-        if self.is_3d and self.native:
+        if self.is_3d and self.f32_storage:
+            step_3d_c_f32store(self.F, self.eps_r, self.sigma, self.dx, self.dy, self.dz, self.dt)
+        elif self.is_3d and self.native:
             step_3d_c(self.F, self.eps_r, self.sigma, self.dx, self.dy, self.dz, self.dt)
         elif self.is_3d:
             update_fields_3d(self.F, self.eps_r, self.sigma, self.dx, self.dy, self.dz, self.dt)

@@ -65,3 +65,29 @@ def test_refsim_native_equals_numpy_with_sources_and_monitors():
             assert np.array_equal(a.F[k], b.F[k]), (dims, k)
         for k in ("Ez", "power"):
             assert np.array_equal(np.array(a.mon_series[0][k]), np.array(b.mon_series[0][k])), (dims, k)
+
+
+def test_f32_storage_step_rounds_the_fp64_step_once():
+    """step_3d_c_f32store: the FP64 expressions on float32-stored fields. From float32 fields, one step's H
+    is float32(the FP64 step's H) exactly; E, which reads the stored (rounded) H, differs from the FP64 step's
+    E by that rounding carried through the curl, under 1e-6 of the largest |E| (a wrong term or index would
+    differ by order one); over 40 steps the two runs stay close but not equal."""
+    rng = np.random.default_rng(24)
+    nz, ny, nx = 9, 11, 13
+    dx, dy, dz, dt = 1e-7, 1.2e-7, 0.9e-7, 1.5e-16
+    eps = 1 + 10 * rng.random((nz, ny, nx)); sig = rng.random((nz, ny, nx)) * 5e4
+    F32 = {k: rng.standard_normal(s).astype(np.float32) for k, s in R.field_shapes(nx, ny, nz).items()}
+    F64 = {k: v.astype(np.float64) for k, v in F32.items()}
+    R.step_3d_c_f32store(F32, eps, sig, dx, dy, dz, dt)
+    R.step_3d_c(F64, eps, sig, dx, dy, dz, dt)
+    for k in ("Hx", "Hy", "Hz"):
+        assert F32[k].dtype == np.float32 and np.array_equal(F32[k], F64[k].astype(np.float32)), k
+    for k in ("Ex", "Ey", "Ez"):
+        d = np.abs(F32[k].astype(np.float64) - F64[k]).max() / np.abs(F64[k]).max()
+        assert 0 < d < 1e-6, (k, d)
+    for _ in range(39):
+        R.step_3d_c_f32store(F32, eps, sig, dx, dy, dz, dt)
+        R.step_3d_c(F64, eps, sig, dx, dy, dz, dt)
+    for k in F64:
+        e = np.linalg.norm(F32[k] - F64[k]) / np.linalg.norm(F64[k])
+        assert 0 < e < 1e-5, (k, e)
