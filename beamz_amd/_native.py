@@ -66,6 +66,9 @@ SIGNATURES = {
                       i32, i32, vp],
     "bz_yee3d_step2": [P6, P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, C.POINTER(BoxSource),
                        C.POINTER(f32), i32, vp, C.POINTER(i32), i32, i32, i32, vp],
+    "bz_yee3d_step2_noalloc": [P6, P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64,
+                               C.POINTER(BoxSource), C.POINTER(f32), i32, C.POINTER(i32), vp, vp, vp, i32, i32, i32,
+                               vp, vp, i32, vp, vp, vp, i32, i32, i32, vp],
     "bz_yee2d_run": [vp, i32, vp, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp],
     "bz_yee2d_hz_run": [vp, i32, vp, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp],
     "bz_yee2d_run2": [vp, i32, vp, i32, i32, vp, i32, i32, vp, vp, i32, vp, vp, i32, vp, vp, vp],

@@ -27,6 +27,9 @@
 // the halo columns, rows and planes.
 #include "common.h"
 
+#include <mutex>
+#include <string>
+
 // No multiply-add contraction in this file or in yee3d_fused.hip: both kernels then evaluate every
 // update with the same rounded operations, which is what makes them bit-identical.
 #pragma clang fp contract(off)
@@ -432,4 +435,75 @@ extern "C" int bz_yee3d_step2(float* const src[6], float* const dst[6], const fl
         case 4: return launch<4, 8>(S, D, eps_r, sigma, nx, ny, nz, ld, kseg, c, rx, ry, rz, bs, stg, st);
         default: bz::set_error("bz_yee3d_step2: rows must be 2, 3 or 4"); return BZ_EINVAL;
     }
+}
+
+// ---------------- allocation-free variant: staging box in a library-owned device array ----------------
+// bz_yee3d_step2 with a staging box, then bz_gather2 of both steps' samples, where the box lives in
+// `scratch`, a file-scope device array (one copy per GPU, held until the process ends). Its address on a
+// device is looked up once, with that device current, and kept per device: the array's name is a host-side
+// stand-in that no kernel may be given.
+namespace {
+static __device__ __align__(256) float scratch[BZ_NOALLOC_STAGE_FLOATS];
+
+constexpr int MAX_DEVICES = 64;
+float* g_scratch[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+int scratch_ptr(float** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return bz::hip_fail("bz_yee3d_step2_noalloc: hipGetDevice", e);
+    BZ_REQUIRE(dev >= 0 && dev < MAX_DEVICES, "bz_yee3d_step2_noalloc: device index beyond 64");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return bz::hip_fail("bz_yee3d_step2_noalloc: hipGetSymbolAddress", e);
+        BZ_REQUIRE(p, "bz_yee3d_step2_noalloc: no device address for the staging array");
+        g_scratch[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch[dev];
+    return BZ_OK;
+}
+}  // namespace
+
+extern "C" int bz_yee3d_step2_noalloc(float* const src[6], float* const dst[6], const float* eps_r,
+                                      const float* sigma, int nx, int ny, int nz, int ld, double dx, double dy,
+                                      double dz, double dt, double mu0, double eps0, const bz_box_source* boxes,
+                                      const float* amp1, int nbox, const int* stage_box, const int* comp,
+                                      const int64_t* stage_off, const int64_t* field_off, int npts, int act0,
+                                      int act1, float* series0, float* series1, int nfreq, const double* phasor0,
+                                      const double* phasor1, double* dft, int kseg, int rows, int waves,
+                                      void* stream) {
+    BZ_REQUIRE(npts >= 0, "bz_yee3d_step2_noalloc: negative point count");
+    const bool sample = npts > 0 && (act0 || act1);
+    float* stage = nullptr;
+    if (sample) {
+        BZ_REQUIRE(stage_box && comp && stage_off && field_off, "bz_yee3d_step2_noalloc: monitor table incomplete");
+        BZ_REQUIRE(stage_box[3] > 0 && stage_box[4] > 0 && stage_box[5] > 0,
+                   "bz_yee3d_step2_noalloc: empty staging box");
+        const double need = 6.0 * stage_box[3] * stage_box[4] * stage_box[5];
+        if (need > (double)BZ_NOALLOC_STAGE_FLOATS) {
+            bz::set_error("bz_yee3d_step2_noalloc: staging box of " + std::to_string((long long)need) +
+                          " floats exceeds the library's " + std::to_string(BZ_NOALLOC_STAGE_FLOATS) +
+                          "-float array");
+            return BZ_EINVAL;
+        }
+        const int rc = scratch_ptr(&stage);
+        if (rc != BZ_OK) return rc;
+    }
+    int rc = bz_yee3d_step2(src, dst, eps_r, sigma, nx, ny, nz, ld, dx, dy, dz, dt, mu0, eps0, boxes, amp1, nbox,
+                            stage, sample ? stage_box : nullptr, kseg, rows, waves, stream);
+    if (rc != BZ_OK || !sample) return rc;
+    // the staged step n+1 is every cell of the box, written by the launch above before this gather reads it
+    // (same stream), so the array needs no clearing
+    const long plane = (long)stage_box[3] * stage_box[4] * stage_box[5];
+    const float* base0[6];
+    const float* base1[6];
+    for (int c = 0; c < 6; ++c) {
+        base0[c] = stage + c * plane;
+        base1[c] = dst[c];
+    }
+    return bz_gather2(base0, stage_off, base1, comp, field_off, npts, act0, act1, series0, series1, nfreq, phasor0,
+                      phasor1, dft, stream);
 }

@@ -112,6 +112,26 @@ int bz_yee3d_step2(float* const src[6], float* const dst[6], const float* eps_r,
                    double mu0, double eps0, const bz_box_source* boxes, const float* amp1, int nbox,
                    float* stage, const int* stage_box, int kseg, int rows, int waves, void* stream);
 
+/* ---------------- allocation-free two-step 3D pass with monitor sampling ----------------
+ * No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): bz_yee3d_step2 with a staging box
+ * followed by bz_gather2 of both steps, where the staging box lives in a device array the library owns
+ * (BZ_NOALLOC_STAGE_FLOATS floats per GPU, held until the process ends) instead of a caller buffer.
+ *   stage_box = {x0, y0, z0, sx, sy, sz} (host), 6*sx*sy*sz <= BZ_NOALLOC_STAGE_FLOATS, else BZ_EINVAL;
+ *   comp[npts], stage_off[npts] (offset into a component's [sz][sy][sx] block), field_off[npts] (offset
+ *   into a field array) and the series / phasor / dft arguments are those of bz_gather2 with base0 = the
+ *   staged step n+1 and base1 = dst (step n+2). npts == 0 or act0 == act1 == 0 samples nothing.
+ * Allocates nothing (no hipMalloc, no library allocation). Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (two streams, two threads) race on it. */
+#define BZ_NOALLOC_STAGE_FLOATS (1 << 20)   /* 4 MiB per GPU */
+int bz_yee3d_step2_noalloc(float* const src[6], float* const dst[6], const float* eps_r, const float* sigma,
+                           int nx, int ny, int nz, int ld, double dx, double dy, double dz, double dt,
+                           double mu0, double eps0, const bz_box_source* boxes, const float* amp1, int nbox,
+                           const int* stage_box, const int* comp, const int64_t* stage_off, const int64_t* field_off,
+                           int npts, int act0, int act1, float* series0, float* series1, int nfreq,
+                           const double* phasor0, const double* phasor1, double* dft, int kseg, int rows, int waves,
+                           void* stream);
+
 /* ---------------- fused, batched 2D engine ----------------
  * Runs nsteps full steps (numpy_backend.py:34-88, then the source box, then optional grid power,
  * then monitor sampling) of every simulation described in `sims` (device array), starting at the
