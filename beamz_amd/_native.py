@@ -20,7 +20,8 @@ class Shape(C.Structure):
     _fields_ = [("kind", i32), ("perm", f64), ("permb", f64), ("cond", f64), ("bbox", f64 * 6),
                 ("px", f64), ("py", f64), ("w", f64), ("h", f64), ("cx", f64), ("cy", f64), ("r", f64),
                 ("r_in", f64), ("r_out", f64), ("nv", i32), ("v", C.POINTER(f64)), ("nholes", i32),
-                ("hole_nv", C.POINTER(i32)), ("hv", C.POINTER(f64))]
+                ("hole_nv", C.POINTER(i32)), ("hv", C.POINTER(f64)), ("cell", C.POINTER(f64)),
+                ("cell_i0", i32), ("cell_j0", i32), ("cell_ni", i32), ("cell_nj", i32)]
 
 
 class Pml(C.Structure):
@@ -101,6 +102,8 @@ SIGNATURES = {
     "bz_gather": [P6, vp, vp, i32, vp, i32, vp, vp, vp],
     "bz_grid_power_2d": [vp, vp, vp, vp, i32, i32, i32, vp],
     "bz_grid_power_3d": [vp, P6, i32, i32, i32, i32, vp],
+    "bz_grid_power_2d_cplx": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, vp],
+    "bz_grid_power_3d_cplx": [vp, P6, P6, i32, i32, i32, i32, vp],
     "bz_raster_2d": [C.POINTER(Shape), i32, C.POINTER(f64), C.POINTER(Pml), i32, f64, f64, f64, i32, i32,
                      vp, vp, vp],
     "bz_raster_3d": [C.POINTER(Shape), i32, C.POINTER(f64), C.POINTER(Pml), i32, f64, f64, f64, f64, f64,

@@ -121,10 +121,29 @@ struct Maps {
         M[idx] = s.permb;
         C[idx] = s.cond;
     }
+    // the same with the material of cell (i, j) of a per-cell table (a spatially varying material)
+    inline void blend_cell(long idx, int cnt, int ns, const bz_shape& s, long i, long j) {
+        const double bf = (double)cnt / (double)ns;
+        const long n = (long)s.cell_ni * s.cell_nj, c = (i - s.cell_i0) * s.cell_nj + (j - s.cell_j0);
+        P[idx] = P[idx] * (1 - bf) + s.cell[c] * bf;
+        M[idx] = M[idx] * (1 - bf) + s.cell[n + c] * bf;
+        C[idx] = C[idx] * (1 - bf) + s.cell[2 * n + c] * bf;
+    }
+    inline void fill_cell(long idx, const bz_shape& s, long i, long j) {
+        const long n = (long)s.cell_ni * s.cell_nj, c = (i - s.cell_i0) * s.cell_nj + (j - s.cell_j0);
+        P[idx] = s.cell[c];
+        M[idx] = s.cell[n + c];
+        C[idx] = s.cell[2 * n + c];
+    }
 };
+
+inline bool in_cells(const bz_shape& s, long i0, long i1, long j0, long j1) {
+    return i0 >= s.cell_i0 && i1 <= (long)s.cell_i0 + s.cell_ni && j0 >= s.cell_j0 && j1 <= (long)s.cell_j0 + s.cell_nj;
+}
 
 bool check_shape(const bz_shape& s) {
     if (s.kind < 0 || s.kind > 3) return false;
+    if (s.cell && (s.kind != 0 || s.cell_ni <= 0 || s.cell_nj <= 0)) return false;
     if (s.kind == 3 && (s.nv < 0 || (s.nv > 0 && !s.v))) return false;
     if (s.nholes > 0 && (!s.hole_nv || !s.hv)) return false;
     return true;
@@ -176,9 +195,17 @@ extern "C" int bz_raster_2d(const bz_shape* shapes, int nshapes, const double* b
             long n0i = ipart((py + 0.25 * cs) / cs); if (n0i < 0) n0i = 0;
             long n1j = ipart(std::floor((px + s.w - 0.25 * cs) / cs)); if (n1j > gw) n1j = gw;
             long n1i = ipart(std::floor((py + s.h - 0.25 * cs) / cs)); if (n1i > gh) n1i = gh;
+            if (s.cell && ((n1i > n0i && n1j > n0j && !in_cells(s, n0i, n1i, n0j, n1j)) ||
+                           (r1i > r0i && r1j > r0j && !in_cells(s, r0i, r1i, r0j, r1j)))) {
+                bz::set_error("bz_raster_2d: a rectangle's per-cell material table does not cover its cells");
+                return BZ_EINVAL;
+            }
             if (n1i > n0i && n1j > n0j)
                 for (long i = n0i; i < n1i; ++i)
-                    for (long j = n0j; j < n1j; ++j) mp.fill(i * gw + j, s);
+                    for (long j = n0j; j < n1j; ++j) {
+                        if (s.cell) mp.fill_cell(i * gw + j, s, i, j);
+                        else mp.fill(i * gw + j, s);
+                    }
             const long hi = r1i - r0i, wj = r1j - r0j;
             if (hi <= 0 || wj <= 0) continue;
             for (long a = 0; a < hi; ++a)
@@ -207,7 +234,10 @@ extern "C" int bz_raster_2d(const bz_shape* shapes, int nshapes, const double* b
                             const double xs = xc[j] + off[kx], ys = yc[i] + off[ky];
                             if (px <= xs && xs < px + s.w && py <= ys && ys < py + s.h) ++cnt;
                         }
-                    if (cnt > 0) mp.blend(i * gw + j, cnt, ns, s);
+                    if (cnt > 0) {
+                        if (s.cell) mp.blend_cell(i * gw + j, cnt, ns, s, i, j);
+                        else mp.blend(i * gw + j, cnt, ns, s);
+                    }
                 }
         } else if (s.kind == 1 || s.kind == 2) {
             // meshing.py:259-361 circle / ring paths: full cells first, then boundary blends
@@ -314,7 +344,7 @@ extern "C" int bz_raster_3d(const bz_shape* shapes, int nshapes, const double* b
         return BZ_EINVAL;
     }
     for (int q = 0; q < nshapes; ++q)
-        if (!check_shape(shapes[q])) {
+        if (!check_shape(shapes[q]) || shapes[q].cell) {
             bz::set_error("bz_raster_3d: malformed shape descriptor");
             return BZ_EINVAL;
         }

@@ -186,6 +186,24 @@ __global__ __launch_bounds__(TX* TY) void k_pow2d(double* __restrict__ acc, cons
     acc[o] += sx * sx + sy * sy;
 }
 
+// helper.py:152-179 with complex fields held as a real and an imaginary set:
+// Sx = -Ez_re Hy_re - Ez_im Hy_im, Sy = Ez_re Hx_re + Ez_im Hx_im, acc += Sx^2 + Sy^2
+__global__ __launch_bounds__(TX* TY) void k_pow2d_cplx(double* __restrict__ acc, const float* __restrict__ ez,
+                                                        const float* __restrict__ hx, const float* __restrict__ hy,
+                                                        const float* __restrict__ ezi, const float* __restrict__ hxi,
+                                                        const float* __restrict__ hyi, int nx, int ny, int ld) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    if (i >= nx || j >= ny) return;
+    const long o = (long)j * ld + i;
+    const double er = ez[o], ei = ezi[o];
+    const bool xin = i < nx - 1, yin = j < ny - 1;
+    const double hxr = xin ? (double)hx[o] : 0.0, hxm = xin ? (double)hxi[o] : 0.0;
+    const double hyr = yin ? (double)hy[o] : 0.0, hym = yin ? (double)hyi[o] : 0.0;
+    const double sx = -er * hyr - ei * hym, sy = er * hxr + ei * hxm;
+    acc[o] += sx * sx + sy * sy;
+}
+
 // helper.py:124-151 (real fields) on the corner-aligned (nz-1, ny-1, nx-1) box
 __global__ __launch_bounds__(TX* TY) void k_pow3d(double* __restrict__ acc, const float* __restrict__ Ex,
                                                    const float* __restrict__ Ey, const float* __restrict__ Ez,
@@ -198,6 +216,29 @@ __global__ __launch_bounds__(TX* TY) void k_pow3d(double* __restrict__ acc, cons
     const long o = (long)k * ny * ld + (long)j * ld + i;
     const double ex = Ex[o], ey = Ey[o], ez = Ez[o], hx = Hx[o], hy = Hy[o], hz = Hz[o];
     const double sx = ey * hz - ez * hy, sy = ez * hx - ex * hz, sz = ex * hy - ey * hx;
+    acc[o] += sqrt(sx * sx + sy * sy + sz * sz);
+}
+
+// helper.py:124-143 with complex fields held as a real (f) and an imaginary (g) set:
+// S = Re(E x conj(H)), e.g. Sx = (Ey_re Hz_re + Ey_im Hz_im) - (Ez_re Hy_re + Ez_im Hy_im); acc += |S|
+struct Six {
+    const float* p[6];
+};
+__global__ __launch_bounds__(TX* TY) void k_pow3d_cplx(double* __restrict__ acc, Six f, Six g, int nx, int ny,
+                                                        int nz, int ld) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx - 1 || j >= ny - 1 || k >= nz - 1) return;
+    const long o = (long)k * ny * ld + (long)j * ld + i;
+    double r[6], m[6];
+    for (int c = 0; c < 6; ++c) {
+        r[c] = f.p[c][o];
+        m[c] = g.p[c][o];
+    }
+    // re(a conj b) = a_re b_re + a_im b_im; components 0..5 = Ex, Ey, Ez, Hx, Hy, Hz
+    auto rc = [&](int a, int b) { return r[a] * r[b] + m[a] * m[b]; };
+    const double sx = rc(1, 5) - rc(2, 4), sy = rc(2, 3) - rc(0, 5), sz = rc(0, 4) - rc(1, 3);
     acc[o] += sqrt(sx * sx + sy * sy + sz * sz);
 }
 
@@ -284,6 +325,32 @@ extern "C" int bz_grid_power_2d(double* acc, const float* Ez, const float* Hx, c
     const dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY), b(TX, TY);
     hipLaunchKernelGGL(k_pow2d, g, b, 0, (hipStream_t)stream, acc, Ez, Hx, Hy, nx, ny, ld);
     BZ_CHECK_LAUNCH("bz_grid_power_2d");
+    return BZ_OK;
+}
+
+extern "C" int bz_grid_power_2d_cplx(double* acc, const float* Ez, const float* Hx, const float* Hy,
+                                     const float* Ez_im, const float* Hx_im, const float* Hy_im, int nx, int ny,
+                                     int ld, void* stream) {
+    BZ_REQUIRE(acc && Ez && Hx && Hy && Ez_im && Hx_im && Hy_im && nx >= 2 && ny >= 2 && ld >= nx,
+               "bz_grid_power_2d_cplx: bad arguments");
+    const dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY), b(TX, TY);
+    hipLaunchKernelGGL(k_pow2d_cplx, g, b, 0, (hipStream_t)stream, acc, Ez, Hx, Hy, Ez_im, Hx_im, Hy_im, nx, ny, ld);
+    BZ_CHECK_LAUNCH("bz_grid_power_2d_cplx");
+    return BZ_OK;
+}
+
+extern "C" int bz_grid_power_3d_cplx(double* acc, float* const f[6], float* const f_im[6], int nx, int ny, int nz,
+                                     int ld, void* stream) {
+    BZ_REQUIRE(acc && f && f_im && nx >= 2 && ny >= 2 && nz >= 2 && ld >= nx, "bz_grid_power_3d_cplx: bad arguments");
+    Six a, c;
+    for (int q = 0; q < 6; ++q) {
+        BZ_REQUIRE(f[q] && f_im[q], "bz_grid_power_3d_cplx: null field");
+        a.p[q] = f[q];
+        c.p[q] = f_im[q];
+    }
+    const dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY, nz), b(TX, TY);
+    hipLaunchKernelGGL(k_pow3d_cplx, g, b, 0, (hipStream_t)stream, acc, a, c, nx, ny, nz, ld);
+    BZ_CHECK_LAUNCH("bz_grid_power_3d_cplx");
     return BZ_OK;
 }
 

@@ -32,17 +32,73 @@ def _is_rectangle(s):
     return any(c.__name__ == "Rectangle" for c in type(s).__mro__)
 
 
-def _material(m):
+def material_sample(m, x=0.0, y=0.0, z=0.0):
+    """(permittivity, permeability, conductivity) of a material at a point, as the reference rasterizer reads
+    it (meshing.py:21-51 of its _get_material_properties_safe): a spatially varying material is evaluated at
+    (x, y, z) with z = 0 in 2D, one-element arrays become scalars, and any failure of the evaluation gives
+    the defaults (1, 1, 0) with a warning."""
+    if m is None:
+        return 1.0, 1.0, 0.0
     if hasattr(m, "get_permittivity"):
-        raise NotImplementedError("CustomMaterial structures cannot be rasterized by beamz_amd; "
-                                  "write the permittivity into grid.permittivity instead")
+        try:
+            vals = [m.get_permittivity(x, y, z), m.get_permeability(x, y, z), m.get_conductivity(x, y, z)]
+            return tuple(float(v.item() if hasattr(v, "item") else v) for v in vals)
+        except Exception as e:          # the reference catches every evaluation error the same way
+            display_status(f"CustomMaterial evaluation failed: {e}, using defaults", "warning")
+            return (float(getattr(m, "default_permittivity", 1.0)), float(getattr(m, "default_permeability", 1.0)),
+                    float(getattr(m, "default_conductivity", 0.0)))
     return float(m.permittivity), float(m.permeability), float(m.conductivity)
+
+
+def _material(m):
+    return material_sample(m)
+
+
+def _is_varying(m):
+    return hasattr(m, "get_permittivity")
+
+
+def _cell_table(m, s, design, resolution):
+    """Per-cell (perm, permb, cond) of a spatially varying material over the cells an axis-aligned rectangle
+    writes: its interior fill (evaluated at each interior cell centre, meshing.py:209-217) and its edge cells
+    (blended with the material at the cell centre, :243-247). Returns (table [3][ni][nj], i0, j0)."""
+    cs = resolution
+    gw, gh = int(design.width / cs), int(design.height / cs)
+    xc = np.linspace(0.5 * cs, design.width - 0.5 * cs, gw)
+    yc = np.linspace(0.5 * cs, design.height - 0.5 * cs, gh)
+    px, py, w, h = s.position[0], s.position[1], s.width, s.height
+    r0j, r0i = max(0, int(px / cs)), max(0, int(py / cs))
+    r1j, r1i = min(gw, int(np.ceil((px + w) / cs))), min(gh, int(np.ceil((py + h) / cs)))
+    n0j, n0i = max(0, int((px + 0.25 * cs) / cs)), max(0, int((py + 0.25 * cs) / cs))
+    n1j = min(gw, int(np.floor((px + w - 0.25 * cs) / cs)))
+    n1i = min(gh, int(np.floor((py + h - 0.25 * cs) / cs)))
+    i0, i1 = min(r0i, n0i), max(r1i, n1i, min(r0i, n0i) + 1)
+    j0, j1 = min(r0j, n0j), max(r1j, n1j, min(r0j, n0j) + 1)
+    i0, j0 = max(0, min(i0, gh - 1)), max(0, min(j0, gw - 1))
+    i1, j1 = max(i0 + 1, min(i1, gh)), max(j0 + 1, min(j1, gw))
+    ni, nj = i1 - i0, j1 - j0
+    tab = np.empty((3, ni, nj))
+    grid_only = type(m).__name__ == "CustomMaterial" and all(
+        getattr(m, k + "_func", None) is None for k in ("permittivity", "permeability", "conductivity")) \
+        and hasattr(m, "_value")
+    if grid_only:
+        # interpolators (or defaults) only: each point is evaluated independently, so one vectorised call gives
+        # the values of per-point calls
+        X, Y = np.meshgrid(xc[j0:j1], yc[i0:i1])
+        for q, k in enumerate(("permittivity", "permeability", "conductivity")):
+            v = m._value(k, X.ravel(), Y.ravel(), 0.0)
+            tab[q] = np.broadcast_to(np.asarray(v, dtype=np.float64), (X.size,)).reshape(ni, nj)
+    else:
+        for a in range(ni):
+            for b in range(nj):
+                tab[:, a, b] = material_sample(m, xc[j0 + b], yc[i0 + a])
+    return np.ascontiguousarray(tab), i0, j0
 
 
 class _Descriptors:
     """Keeps the ctypes arrays alive while the native call runs."""
 
-    def __init__(self, design, three_d):
+    def __init__(self, design, three_d, resolution=None):
         self.keep = []
         shapes = []
         for s in design.structures[1:]:
@@ -52,6 +108,7 @@ class _Descriptors:
                 continue
             if getattr(s, "material", None) is None:
                 continue
+            # a spatially varying material counts once, at (0, 0, 0), in 3D and for what only reads one value
             perm, permb, cond = _material(s.material)
             bbox = s.get_bounding_box()
             if bbox is None:
@@ -69,6 +126,17 @@ class _Descriptors:
                     and verts[0][1] - s.position[1] == 0:
                 d.kind = 0
                 d.px, d.py, d.w, d.h = float(s.position[0]), float(s.position[1]), float(s.width), float(s.height)
+                if _is_varying(s.material):
+                    tab, i0, j0 = _cell_table(s.material, s, design, resolution)
+                    self.keep.append(tab)
+                    d.cell = tab.ctypes.data_as(C.POINTER(C.c_double))
+                    d.cell_i0, d.cell_j0, d.cell_ni, d.cell_nj = i0, j0, tab.shape[1], tab.shape[2]
+            elif (not three_d) and _is_varying(s.material):
+                # the reference assigns None (NaN) to fully covered cells of such shapes and stops at the first
+                # edge blend with a TypeError (meshing.py:297-300, 408-411, 530-531); refused here instead
+                raise NotImplementedError(f"a spatially varying material on a {type(s).__name__} is rasterized "
+                                          "only as an axis-aligned Rectangle (the reference's other paths "
+                                          "write NaN and stop part-way)")
             elif (not three_d) and hasattr(s, "radius"):
                 d.kind = 1
                 d.cx, d.cy, d.r = float(s.position[0]), float(s.position[1]), float(s.radius)
@@ -139,7 +207,7 @@ class RegularGrid(BaseMeshGrid):
             eps, mu, sig = _arrays
         else:
             eps, mu, sig = (np.empty((gh, gw)) for _ in range(3))
-            D = _Descriptors(design, three_d=False)
+            D = _Descriptors(design, three_d=False, resolution=resolution)
             N.call("bz_raster_2d", D.shapes, D.nshapes, D.bg, D.pmls, D.npml, float(design.width),
                    float(design.height), float(resolution), gw, gh, eps.ctypes.data, mu.ctypes.data, sig.ctypes.data)
         self.permittivity, self.permeability, self.conductivity = eps, mu, sig

@@ -24,6 +24,7 @@
  *                           beamz/simulation/helper.py:231-256 with beamz/devices/monitors.py:279-397,
  *                           plus the on-the-fly DFT (no reference counterpart)
  *   bz_grid_power_2d/3d ... grid power accumulation, beamz/simulation/helper.py:120-187
+ *   bz_grid_power_2d/3d_cplx  the same for complex fields (a real and an imaginary run)
  *   bz_raster_2d/3d ....... RegularGrid.__rasterize__ / RegularGrid3D.__rasterize_3d__ +
  *                           _process_3d_pml, beamz/simulation/meshing.py:92-536, 616-781,
  *                           with PML.get_conductivity, beamz/design/pml.py:21-62
@@ -337,6 +338,13 @@ int bz_grid_power_2d(double* acc, const float* Ez, const float* Hx, const float*
                      void* stream);
 /* Grid power 3D: acc (on the [nz-1][ny-1][nx-1] corner box, pitch ld) += |E x H|. */
 int bz_grid_power_3d(double* acc, float* const f[6], int nx, int ny, int nz, int ld, void* stream);
+/* The same with complex fields held as a real and an imaginary set of arrays (the reference's complex128
+ * fields, helper.py:124-143 and :152-179): 2D Sx = -(Ez_re Hy_re + Ez_im Hy_im), Sy = Ez_re Hx_re + Ez_im Hx_im,
+ * acc += Sx^2 + Sy^2; 3D acc += |Re(E x conj H)|. */
+int bz_grid_power_2d_cplx(double* acc, const float* Ez, const float* Hx, const float* Hy, const float* Ez_im,
+                          const float* Hx_im, const float* Hy_im, int nx, int ny, int ld, void* stream);
+int bz_grid_power_3d_cplx(double* acc, float* const f[6], float* const f_im[6], int nx, int ny, int nz, int ld,
+                          void* stream);
 
 /* ---------------- numerical watchdog ----------------
  * count (device, caller-zeroed) += number of NaN or infinite entries of p[0 .. n). No reference
@@ -355,6 +363,13 @@ typedef struct bz_shape {
     int nholes;             /* holes: hole_nv[h] vertices each, packed in hv */
     const int* hole_nv;
     const double* hv;
+    /* kind 0 in 2D with a spatially varying material (the reference's CustomMaterial, meshing.py:162-169,
+     * 209-217): cell = {perm, permb, cond} sampled at the cell centres of rows cell_i0 .. cell_i0+cell_ni-1
+     * and columns cell_j0 .. cell_j0+cell_nj-1, as [3][cell_ni][cell_nj]; the interior fill and the edge
+     * blends take the cell's own values. NULL: the constant perm, permb, cond. Every cell the rectangle
+     * writes must lie in the box, else BZ_EINVAL. */
+    const double* cell;
+    int cell_i0, cell_j0, cell_ni, cell_nj;
 } bz_shape;
 
 typedef struct bz_pml {

@@ -115,8 +115,32 @@ def pml_conductivity(b, x, y, dx=None, dt=None, eps_avg=None):
     return c
 
 
-def _mat(m):
+def _props(m, x=0, y=0, z=0):
+    """meshing.py:21-51 (_get_material_properties_safe): a material with getters is evaluated at (x, y, z),
+    .item() taken where present; any failure gives the defaults."""
+    if m is None:
+        return 1.0, 1.0, 0.0
+    if hasattr(m, "get_permittivity"):
+        try:
+            p = m.get_permittivity(x, y, z)
+            b = m.get_permeability(x, y, z)
+            c = m.get_conductivity(x, y, z)
+            if hasattr(p, "item"):
+                p = p.item()
+            if hasattr(b, "item"):
+                b = b.item()
+            if hasattr(c, "item"):
+                c = c.item()
+            return p, b, c
+        except Exception:
+            return (getattr(m, "default_permittivity", 1.0), getattr(m, "default_permeability", 1.0),
+                    getattr(m, "default_conductivity", 0.0))
     return m.permittivity, m.permeability, m.conductivity
+
+
+def _mat(m):
+    """Background (meshing.py:131-136) and 3D (:674) reads: _props at (0, 0, 0)."""
+    return _props(m)
 
 
 def _is_rectangle(s):
@@ -160,9 +184,8 @@ def rasterize_2d(design, resolution):
             continue
         if getattr(s, "material", None) is None:
             continue
-        if hasattr(s.material, "get_permittivity"):
-            raise NotImplementedError("CustomMaterial rasterization is not restated")
-        mp, mb, mc = _mat(s.material)
+        custom = hasattr(s.material, "get_permittivity")      # meshing.py:162-169
+        mp, mb, mc = (None, None, None) if custom else _mat(s.material)
         bbox = s.get_bounding_box()
         if len(bbox) == 6:
             minx, miny, _, maxx, maxy, _ = bbox
@@ -182,7 +205,12 @@ def rasterize_2d(design, resolution):
             n1j = min(gw, int(np.floor((px + s.width - 0.25 * cs) / cs)))
             n1i = min(gh, int(np.floor((py + s.height - 0.25 * cs) / cs)))
             if n1i > n0i and n1j > n0j:
-                P[n0i:n1i, n0j:n1j] = mp; M[n0i:n1i, n0j:n1j] = mb; C[n0i:n1i, n0j:n1j] = mc
+                if custom:                                       # meshing.py:209-217
+                    for i in range(n0i, n1i):
+                        for j in range(n0j, n1j):
+                            P[i, j], M[i, j], C[i, j] = _props(s.material, xc[j], yc[i])
+                else:
+                    P[n0i:n1i, n0j:n1j] = mp; M[n0i:n1i, n0j:n1j] = mb; C[n0i:n1i, n0j:n1j] = mc
             bm = np.zeros((r1i - r0i, r1j - r0j), dtype=bool)
             if r0i < n0i: bm[:n0i - r0i, :] = True
             if n1i < r1i: bm[n1i - r0i:, :] = True
@@ -198,7 +226,12 @@ def rasterize_2d(design, resolution):
                     if px <= xs < px + s.width and py <= ys < py + s.height:
                         cnt += 1
                 if cnt > 0:
+                    if custom:                                   # meshing.py:243-247
+                        mp, mb, mc = _props(s.material, xc[j], yc[i])
                     blend(i, j, cnt, mp, mb, mc)
+        elif custom:
+            # the reference writes None (NaN) into covered cells here and stops at its first blend
+            raise NotImplementedError("a spatially varying material is restated on axis-aligned rectangles only")
         elif hasattr(s, "radius"):
             cx0, cy0 = s.position[0], s.position[1]
             R = s.radius
