@@ -107,6 +107,9 @@ class AdjointProblem2D:
             return
         for s in self.sims:
             s.initialize_simulation(save=False, live=False)
+            if s.grid_im is not None:
+                raise NotImplementedError("the adjoint runs real forward fields: build the simulations with "
+                                          "complex_fields=False or real sources")
         self.fwd = Fused2D(grids, seg=self.seg, max_steps=self.N, pairs=self.pairs, kstep=self.kstep)
         if self.fwd.kstep and self.N >= 64:
             # region shape of the K-step sweeps, timed once, among the shapes whose K divides the checkpoint
@@ -348,7 +351,9 @@ def overlap_gradient_reference(fwd_sim, adj_sim, box, kstep=None):
 
     fwd_sim, adj_sim: 2D ``FDTD`` objects of the (Ez, Hx, Hy) polarisation; box: (x0, y0, sx, sy). Each run
     records Ez inside the box after every step (the engine's snapshot box, in HBM: 12 * N * sx * sy bytes),
-    and leaves its simulation's fields at step N. Returns the (sy, sx) FP64 torch tensor."""
+    and leaves its simulation's fields at step N. With complex fields (a complex source: the reference's
+    complex128 path) each run also records its imaginary grid, and the product is
+    Re(adj conj(fwd)) = adj_re fwd_re + adj_im fwd_im. Returns the (sy, sx) FP64 torch tensor."""
     x0, y0, sx, sy = (int(v) for v in box)
     N = fwd_sim.num_steps
     if adj_sim.num_steps != N or not np.array_equal(np.asarray(adj_sim.time), np.asarray(fwd_sim.time)):
@@ -359,17 +364,23 @@ def overlap_gradient_reference(fwd_sim, adj_sim, box, kstep=None):
             raise ValueError("2D simulations of the (Ez, Hx, Hy) polarisation only")
         T = s.backend.torch
         s.initialize_simulation(save=False, live=False)
-        buf = T.zeros((N, 3, sy, sx), dtype=T.float32, device=s.backend.dev)
-        eng = Fused2D([s.grid], monitors=False, snapshots=[(x0, y0, sx, sy, buf)], kstep=kstep)
-        eng.run(0, N)
-        s.grid._f2d = None
+        parts = []
+        for g in s._grids():
+            buf = T.zeros((N, 3, sy, sx), dtype=T.float32, device=s.backend.dev)
+            eng = Fused2D([g], monitors=False, snapshots=[(x0, y0, sx, sy, buf)], kstep=kstep)
+            eng.run(0, N)
+            g._f2d = None
+            # snapshot slot q holds the Ez that step q read, i.e. the field after step q - 1
+            last = g.view("Ez")[y0:y0 + sy, x0:x0 + sx]
+            parts.append(T.cat([buf[1:, 0], last[None]]).double())
+            del buf, eng
         s.current_step = N
-        # snapshot slot q holds the Ez that step q read, i.e. the field after step q - 1
-        last = s.grid.view("Ez")[y0:y0 + sy, x0:x0 + sx]
-        series.append(T.cat([buf[1:, 0], last[None]]).double())
-        del buf, eng
+        series.append(parts)
     fwd, adj = series
-    return (adj * fwd.flip(0)).sum(dim=0)
+    out = (adj[0] * fwd[0].flip(0)).sum(dim=0)
+    if len(adj) > 1 and len(fwd) > 1:
+        out += (adj[1] * fwd[1].flip(0)).sum(dim=0)
+    return out
 
 
 class FDTDObjective:

@@ -55,6 +55,9 @@ class FDTDBatch:
     def initialize(self):
         for s in self.sims:
             s.initialize_simulation(save=False, live=False)
+            if s.grid_im is not None:
+                raise NotImplementedError("batch members with complex sources (an imaginary grid): run them "
+                                          "alone, or build them with complex_fields=False")
         self.engine = Fused2D([s.grid for s in self.sims], seg=self.seg, graph=self.graph, max_steps=self.num_steps,
                               pairs=self.pairs, seg2=self.seg2, kstep=self.kstep, krows=self.krows,
                               kwaves=self.kwaves)

@@ -6,9 +6,16 @@ Same constructor, ``initialize_simulation`` / ``step`` / ``finalize_simulation``
 ``current_step``) and result semantics as the reference. Differences, each
 deliberate:
 
-* fields are real FP32 on the GPU; the field attributes return host
-  complex128 copies in the reference's shapes (equal to the real part of the
-  reference's complex fields; the imaginary part is zero for real signals);
+* fields are FP32 on the GPU; the field attributes return host complex128
+  copies in the reference's shapes. The reference's NumPy path keeps complex
+  fields (fdtd.py:151-186) and adds complex source values unreduced
+  (helper.py:29-35); every update has real coefficients, so the real and the
+  imaginary parts evolve independently. When a source carries a complex
+  amplitude or signal, the engine runs a second real grid for the imaginary
+  part (``complex_fields``, automatic by default) and the attributes, monitor
+  series, spectra and power combine the two; otherwise the imaginary part is
+  zero and one real grid runs. ``complex_fields=False`` keeps the real part
+  only, as the reference's real-backend branch does (helper.py:29-33);
 * every time step runs on the device: update, sources, monitor sampling and
   power accumulation never copy fields to the host; saved results do;
 * ``run(steps=k)`` stops after k steps (the reference ignores ``steps``);
@@ -73,13 +80,15 @@ def gaussian_weights_3d(position, width, dx, dy, dz, nx, ny, nz, ez_nz):
 
 
 def mode_source_cells(src, dx, dy, dz, nx, ny, nz, ez_nz, is_3d):
-    """Affine per-cell map (mul, add) equivalent to helper.py:8-54's sequential point loop."""
+    """Affine per-cell map (mul, add) equivalent to helper.py:8-54's sequential point loop: after the loop a
+    cell holds mul * (its value before) + add * signal[n]. add is complex (the profile's amplitudes); mul is 0
+    where a later point's direction enforcement zeroed the cell (which also drops what earlier points added)."""
     acc = {}
     order = []
 
     def cell(key):
         if key not in acc:
-            acc[key] = [1.0, 0.0]
+            acc[key] = [1.0, 0j]
             order.append(key)
         return acc[key]
 
@@ -102,7 +111,7 @@ def mode_source_cells(src, dx, dy, dz, nx, ny, nz, ez_nz, is_3d):
                 continue
             zt = 0
             key = (y, x)
-        cell(key)[1] += float(np.real(amp))
+        cell(key)[1] += complex(amp)
         if enforce:
             d = src.direction
             up = None
@@ -115,16 +124,16 @@ def mode_source_cells(src, dx, dy, dz, nx, ny, nz, ez_nz, is_3d):
             if up is not None:
                 if is_3d and len(up) == 2:
                     up = (zt, *up)
-                cell(up)[:] = [0.0, 0.0]
+                cell(up)[:] = [0.0, 0j]
     idx = np.array(order, dtype=np.int64).reshape(len(order), 3 if is_3d else 2)
     mul = np.array([acc[k][0] for k in order])
-    add = np.array([acc[k][1] for k in order])
+    add = np.array([acc[k][1] for k in order], dtype=np.complex128)
     return idx, mul, add
 
 
 class FDTD:
     def __init__(self, design, time, mesh: str = "regular", resolution: float = 0.02e-6, backend="hip",
-                 backend_options=None, devices=None, *, polarization="Ez"):
+                 backend_options=None, devices=None, *, polarization="Ez", complex_fields=None):
         provided = None
         if isinstance(design, BaseMeshGrid):
             provided = design
@@ -160,7 +169,6 @@ class FDTD:
         self.mu_r = mesh_obj.permeability
         self.sigma = mesh_obj.conductivity
         self.backend = get_backend(name=backend, **(backend_options or {}))
-        self.is_complex_backend = False
         self.time = time
         self.dt = time[1] - time[0]
         self.num_steps = len(time)
@@ -169,6 +177,9 @@ class FDTD:
         self.polarization = polarization
         self.grid = Grid(self.backend, self.epsilon_r, self.sigma, self.dx, self.dy, self.dt, self.dz,
                          pol=polarization)
+        self.complex_fields = complex_fields      # None: a second, imaginary grid when a source is complex
+        self.grid_im = None
+        self._power_c = None
         self.sources = list(design.sources)
         self.monitors = list(design.monitors)
         for dev in devices or []:
@@ -200,14 +211,22 @@ class FDTD:
 
     # ---------------- field attributes ----------------
     def _host(self, name):
-        return self.grid.view(name).detach().cpu().numpy().astype(np.complex128)
+        a = self.grid.view(name).detach().cpu().numpy().astype(np.complex128)
+        if self.grid_im is not None:
+            a += 1j * self.grid_im.view(name).detach().cpu().numpy()
+        return a
 
     def _set_field(self, name, value):
-        v = np.real(np.asarray(value)).astype(np.float32)
-        tgt = self.grid.view(name)
-        if tuple(v.shape) != tuple(tgt.shape):
-            raise ValueError(f"{name} must have shape {tuple(tgt.shape)}")
-        tgt.copy_(self.backend.torch.from_numpy(np.ascontiguousarray(v)).to(tgt.device))
+        value = np.asarray(value)
+        parts = [(self.grid, np.real(value))]
+        if self.grid_im is not None:
+            parts.append((self.grid_im, np.imag(value)))
+        for g, part in parts:
+            v = part.astype(np.float32)
+            tgt = g.view(name)
+            if tuple(v.shape) != tuple(tgt.shape):
+                raise ValueError(f"{name} must have shape {tuple(tgt.shape)}")
+            tgt.copy_(self.backend.torch.from_numpy(np.ascontiguousarray(v)).to(tgt.device))
 
     Ez = property(lambda s: s._host("Ez"), lambda s, v: s._set_field("Ez", v))
     Hx = property(lambda s: s._host("Hx"), lambda s, v: s._set_field("Hx", v))
@@ -217,8 +236,38 @@ class FDTD:
     Hz = property(lambda s: s._host("Hz"), lambda s, v: s._set_field("Hz", v))
 
     def field(self, name):
-        """Device (torch) view of a field component in the reference's shape; no copy."""
+        """Device (torch) view of a field component in the reference's shape; no copy. With complex fields
+        this is the real part (``field_imag``: the imaginary part)."""
         return self.grid.view(name)
+
+    def field_imag(self, name):
+        """Device view of the imaginary part of a component (None when the fields are real)."""
+        return None if self.grid_im is None else self.grid_im.view(name)
+
+    @property
+    def is_complex_backend(self):
+        """True when the fields carry an imaginary part (the reference's complex128 fields)."""
+        return self.grid_im is not None
+
+    @is_complex_backend.setter
+    def is_complex_backend(self, value):
+        pass                            # set by the reference's constructor; here it follows the sources
+
+    def _wants_complex(self):
+        if self.complex_fields is not None:
+            return bool(self.complex_fields)
+        for src in self.sources:
+            sig = np.asarray(src.signal)
+            if np.iscomplexobj(sig) and np.any(np.imag(sig[: self.num_steps]) != 0):
+                return True
+            if isinstance(src, ModeSource):
+                amps = np.asarray([p[0] for p in src.mode_profiles[0]])
+                if np.iscomplexobj(amps) and np.any(np.imag(amps) != 0):
+                    return True
+        return False
+
+    def _grids(self):
+        return [self.grid] if self.grid_im is None else [self.grid, self.grid_im]
 
     # ---------------- setup ----------------
     def _time_table(self):
@@ -230,11 +279,15 @@ class FDTD:
         return t
 
     def _plan_sources(self):
-        g = self.grid
-        g.sources = []
-        g.invalidate()
+        """Point sources of every design source on the real grid and, with complex fields, on the imaginary
+        one. A source adds add * signal[n] (complex add and signal) after mul * field: the real grid gets
+        Re(add) Re(s) - Im(add) Im(s), the imaginary grid Re(add) Im(s) + Im(add) Re(s), each product term a
+        point source of its own with the direction-enforcing mul on the first."""
+        for g in self._grids():
+            g.sources = []
+            g.invalidate()
         ez_nz = self.nz - 1 if self.is_3d else None
-        target = g.names[0] if not self.is_3d else "Ez"
+        target = self.grid.names[0] if not self.is_3d else "Ez"
         # the (Hz, Ex, Ey) family injects into Hz, whose cells are (ny-1, nx-1)
         bx, by = (self.nx - 1, self.ny - 1) if target == "Hz" else (self.nx, self.ny)
         for src in self.sources:
@@ -244,8 +297,6 @@ class FDTD:
             if isinstance(src, ModeSource):
                 idx, mul, add = mode_source_cells(src, self.dx, self.dy, self.dz, bx, by, self.nz, ez_nz,
                                                   self.is_3d)
-                if len(idx):
-                    g.add_point_source(target, idx, add, mul, sig)
             else:
                 if self.is_3d:
                     idx, w = gaussian_weights_3d(src.position, src.width, self.dx, self.dy, self.dz, self.nx, self.ny,
@@ -254,8 +305,24 @@ class FDTD:
                     idx, w = gaussian_weights_2d(src.position, src.width, self.dx, self.dy, self.nx, self.ny)
                     keep = (idx[:, 0] < by) & (idx[:, 1] < bx)
                     idx, w = idx[keep], w[keep]
-                if len(idx):
-                    g.add_point_source(target, idx, w, None, sig)
+                mul, add = None, np.asarray(w, dtype=np.complex128)
+            if not len(idx):
+                continue
+            s_re, s_im = np.real(sig).astype(np.float64), np.imag(sig).astype(np.float64)
+            a_re, a_im = np.real(add), np.imag(add)
+            terms = {id(self.grid): [(a_re, s_re), (-a_im, s_im)]}
+            if self.grid_im is not None:
+                terms[id(self.grid_im)] = [(a_im, s_re), (a_re, s_im)]
+            for g in self._grids():
+                first = True
+                for a, sg in terms[id(g)]:
+                    # a term that adds nothing is kept when it carries the zeroing mul
+                    if not first and (not np.any(a) or not np.any(sg)):
+                        continue
+                    if first and mul is None and (not np.any(a) or not np.any(sg)):
+                        continue
+                    g.add_point_source(target, idx, a, mul if first else None, sg)
+                    first = False
 
     def _monitor_samples(self, mon):
         """(comp indices, offsets, layout) of one monitor, following monitors.py:204-273, 324-379."""
@@ -320,8 +387,9 @@ class FDTD:
 
     def _plan_monitors(self, t_all):
         g = self.grid
-        g.monitors = []
-        g.invalidate()
+        for gg in self._grids():
+            gg.monitors = []
+            gg.invalidate()
         self._plans = []
         for mon in self.monitors:
             if not self.is_3d and set(mon._fields) != set(g.names) | {"t"}:
@@ -337,6 +405,9 @@ class FDTD:
             omegas = None if mon.dft_frequencies is None else 2 * np.pi * mon.dft_frequencies
             keep = bool(mon.should_record_fields or mon.accumulate_power)
             plan = g.add_gather(comps, offs, steps, t_all, omegas, keep_series=keep)
+            # the imaginary grid samples the same points; its series and spectra are the imaginary parts
+            plan.im = None if self.grid_im is None else self.grid_im.add_gather(comps, offs, steps, t_all, omegas,
+                                                                                keep_series=keep)
             plan.monitor, plan.layout, plan.t_all = mon, layout, t_all
             plan.fetched = 0
             mon._pull = self._make_pull(plan)
@@ -356,6 +427,8 @@ class FDTD:
         names = self.grid.names
         if nrec > plan.fetched and plan.series is not None:
             ser = plan.series[new].detach().cpu().numpy().astype(np.float64)
+            if plan.im is not None:
+                ser = ser + 1j * plan.im.series[new].detach().cpu().numpy()
             ts = [float(plan.t_all[s]) if s > 0 else 0 for s in done[new]]
             if self.is_3d:
                 m0, m1 = plan.layout[1]
@@ -402,6 +475,9 @@ class FDTD:
         if plan.nfreq:
             d = plan.dft.detach().cpu().numpy()
             z = d[..., 0] + 1j * d[..., 1]
+            if plan.im is not None:
+                d = plan.im.dft.detach().cpu().numpy()
+                z = z + 1j * (d[..., 0] + 1j * d[..., 1])      # DFT(re) + i DFT(im)
             if self.is_3d:
                 m0, m1 = plan.layout[1]
                 z = z.reshape(plan.nfreq, 6, m0, m1)
@@ -439,14 +515,30 @@ class FDTD:
         self._effective_save_freq = 1 * decimate_save
         if self._cache_frequency is None:
             self._cache_frequency = self._effective_save_freq
-        self.grid.reset_fields()
+        if self._wants_complex():
+            if self.grid_im is None:
+                self.grid_im = Grid(self.backend, self.epsilon_r, self.sigma, self.dx, self.dy, self.dt, self.dz,
+                                    pol=self.polarization)
+                self._copy_settings(self.grid, self.grid_im)
+            # the imaginary grid shares the real grid's materials, also when they were edited on the device
+            self.grid_im.eps.copy_(self.grid.eps)
+            self.grid_im.sig.copy_(self.grid.sig)
+        else:
+            self.grid_im = None
+        for g in self._grids():
+            g.reset_fields()
+            g.power = None
+            g.invalidate()
         self.power_accumulated = None
         self.power_accumulation_count = 0
+        self._power_c = None
         if accumulate_power:
-            self.grid.enable_power()
-        else:
-            self.grid.power = None
-        self.grid.invalidate()
+            if self.grid_im is None:
+                self.grid.enable_power()
+            else:
+                # Re(E x conj H) needs both parts of each step: accumulated here after both grids stepped
+                self._power_c = self.backend.torch.zeros(self.grid.box, dtype=self.backend.torch.float64,
+                                                         device=self.backend.dev)
         self._t_all = self._time_table()
         self._plan_sources()
         self._plan_monitors(self._t_all)
@@ -481,7 +573,7 @@ class FDTD:
         if self.current_step >= self.num_steps:
             return False
         n = self.current_step
-        self.grid.step(n)
+        self._run_grids(n, 1)
         self._save(n)
         self.t += self.dt
         self.current_step += 1
@@ -496,15 +588,56 @@ class FDTD:
             if not self._needs_host(n):
                 while stop < end and not self._needs_host(stop):
                     stop += 1
-                self.grid.run(n, stop - n)
+                self._run_grids(n, stop - n)
                 for _ in range(stop - n):
                     self.t += self.dt
                 self.current_step = stop
             else:
                 self.step()
 
+    @staticmethod
+    def _copy_settings(src, dst):
+        for k in ("kernel", "seg2d", "graph2d", "kstep2d", "krows2d", "kwaves2d", "kseg", "rows", "tb2", "kseg2",
+                  "rows2", "waves2"):
+            setattr(dst, k, getattr(src, k))
+
+    def _run_grids(self, n, count):
+        """Steps n .. n+count-1 on the real grid and, with complex fields, on the imaginary one (with grid
+        power: step by step, accumulating |S| of the combined fields after each)."""
+        if self.grid_im is None:
+            self.grid.run(n, count)
+            return
+        self._copy_settings(self.grid, self.grid_im)
+        if self._power_c is None:
+            self.grid.run(n, count)
+            self.grid_im.run(n, count)
+            return
+        from .. import _native as N
+        g, h = self.grid, self.grid_im
+        for q in range(n, n + count):
+            g.run(q, 1)
+            h.run(q, 1)
+            if self.is_3d:
+                N.call("bz_grid_power_3d_cplx", self._power_c.data_ptr(), N.ptrs6([g.f[k].data_ptr() for k in COMP3]),
+                       N.ptrs6([h.f[k].data_ptr() for k in COMP3]), self.nx, self.ny, self.nz, g.ld, g.stream)
+            else:
+                if self.polarization != "Ez":
+                    raise NotImplementedError("grid power of complex (Hz, Ex, Ey) fields")
+                N.call("bz_grid_power_2d_cplx", self._power_c.data_ptr(), g.f["Ez"].data_ptr(), g.f["Hx"].data_ptr(),
+                       g.f["Hy"].data_ptr(), h.f["Ez"].data_ptr(), h.f["Hx"].data_ptr(), h.f["Hy"].data_ptr(),
+                       self.nx, self.ny, g.ld, g.stream)
+            self.power_accumulation_count += 1
+
     def finalize_simulation(self):
-        if self.grid.power is not None:
+        if self._power_c is not None:
+            p = self._power_c.detach().cpu().numpy()
+            if self.is_3d:
+                p = p[: self.nz - 1, : self.ny - 1, : self.nx - 1]
+            else:
+                p = p[:, : self.nx]
+            c = self.power_accumulation_count
+            self.power_accumulated = p / c if c else p
+        elif self.grid.power is not None:
             self.power_accumulation_count = self.grid.power_count
             p = self.grid.power.detach().cpu().numpy()
             if self.is_3d:
@@ -548,9 +681,10 @@ class FDTD:
         T = self.backend.torch
         names = list(self.grid.names)
         count = T.zeros(len(names), dtype=T.int32, device=self.backend.dev)
-        for q, k in enumerate(names):
-            t = self.grid.f[k]
-            N.call("bz_nonfinite_count", t.data_ptr(), t.numel(), count.data_ptr() + 4 * q, self.grid.stream)
+        for g in self._grids():
+            for q, k in enumerate(names):
+                t = g.f[k]
+                N.call("bz_nonfinite_count", t.data_ptr(), t.numel(), count.data_ptr() + 4 * q, g.stream)
         c = count.cpu().numpy()
         return {k: int(v) for k, v in zip(names, c) if v}
 

@@ -141,6 +141,7 @@ class SurrogateLoop:
         self.rng = np.random.default_rng(seed)
         self.pool_rho, self.pool_fom, self.pool_T = None, None, None
         self.history = []
+        self.rounds = []          # per round: the picks' verified and predicted spectra, compared
 
     def propose(self):
         torch = self.net.torch
@@ -158,10 +159,16 @@ class SurrogateLoop:
 
     def step(self, finetune_epochs=3, lr=5e-4):
         cand = self.propose()
-        score = predict_T(self.net, cand, self.log_space).mean(axis=1)
-        pick = cand[np.argsort(-score)[:self.top]]
+        pred = predict_T(self.net, cand, self.log_space)
+        score = pred.mean(axis=1)
+        order = np.argsort(-score)[:self.top]
+        pick = cand[order]
         T = self.verifier.transmission(pick).astype(np.float32)
         fom = T.mean(axis=1)
+        # the surrogate's error against FDTD on the designs it picked (before it is fine-tuned on them)
+        self.rounds.append({"verified_fom_mean": float(fom.mean()), "verified_fom_max": float(fom.max()),
+                            "predicted_fom_mean": float(score[order].mean()),
+                            "rel_error": float(np.linalg.norm(pred[order] - T) / max(np.linalg.norm(T), 1e-30))})
         self.pool_rho = np.concatenate([self.pool_rho, pick])
         self.pool_T = np.concatenate([self.pool_T, T])
         self.pool_fom = np.concatenate([self.pool_fom, fom])
