@@ -127,6 +127,114 @@ __global__ __launch_bounds__(256) void k_conv_gemm(const uint16_t* __restrict__ 
     }
 }
 
+// Forward (sign = +1) or backward-data (sign = -1) 3x3 convolution by a march down image rows (W = 64 or 128):
+//   out[p][n] = act(bias[n] + sum_{ky, kx, c} src[p + sign (ky - 1, kx - 1)][c] * wk[n][ky][kx][c]).
+// A workgroup owns CB output channels (blockIdx.y) and a band of output rows (blockIdx.z, rows flattened as
+// b * H + y). It keeps the weights of its channels (all taps, all CIN input channels) in LDS for the whole band,
+// and a ring of four input rows, pixel-major with a zero pixel on each side: the rows r - 1 .. r + 1 that
+// output row r reads and the row being stored. The rows a step needs two steps later are loaded into registers
+// at its start and stored after the next step (one barrier per row), so each input row is read from memory
+// once per band and channel block. Wave w computes the 32-pixel block w % (W / 32) and the 32-channel block
+// w / (W / 32) of every row: MFMAs with M = pixels, N = output channels, K = input channels, nine taps of
+// CIN / 16 steps. A tap shifts whole pixels, so every fragment is one aligned 16-byte LDS read; pixel strides
+// of CIN + 8 and weight rows of 9 CIN + 8 elements put the 16 rows of a read on distinct banks.
+template <int W, int CIN, int CB>
+__global__ __launch_bounds__(64 * (W / 32) * (CB / 32)) void k_conv_march(const uint16_t* __restrict__ src,
+                                                                          const uint16_t* __restrict__ wk,
+                                                                          const float* __restrict__ bias,
+                                                                          uint16_t* __restrict__ out, int H,
+                                                                          int rows_total, int Cout, int sign,
+                                                                          int relu, int band) {
+    constexpr int NT = 64 * (W / 32) * (CB / 32);
+    constexpr int WS = 9 * CIN + 8, XC = CIN + 8, XSLOT = (W + 2) * XC;
+    constexpr int CHUNKS = W * CIN / 8, NCH = (CHUNKS + NT - 1) / NT;   // 16-byte chunks of one input row
+    __shared__ __attribute__((aligned(16))) uint16_t lds[CB * WS + 4 * XSLOT];
+    uint16_t* wl = lds;
+    uint16_t* ring = lds + CB * WS;
+    const int t = threadIdx.x, l = t & 63, wave = t >> 6;
+    const int n0 = blockIdx.y * CB;
+    const int r0 = blockIdx.z * band, r1 = min(rows_total, r0 + band);
+    for (int q = t; q < 4 * XSLOT / 8; q += NT) reinterpret_cast<uint4*>(ring)[q] = make_uint4(0, 0, 0, 0);
+    for (int q = t; q < CB * (9 * CIN / 8); q += NT) {
+        const int n = q / (9 * CIN / 8), k8 = q - n * (9 * CIN / 8);
+        *reinterpret_cast<uint4*>(wl + n * WS + 8 * k8) =
+            *reinterpret_cast<const uint4*>(wk + (long)(n0 + n) * 9 * CIN + 8 * k8);
+    }
+    uint4 va[NCH], vb[NCH];
+    auto load = [&](uint4 (&v)[NCH], int q) {      // input row q (contiguous W * CIN elements), zeros outside
+        const bool ok = q >= 0 && q < rows_total;
+#pragma unroll
+        for (int i = 0; i < NCH; ++i) {
+            const int ch = t + i * NT;
+            v[i] = make_uint4(0, 0, 0, 0);
+            if (ok && ch < CHUNKS) v[i] = *reinterpret_cast<const uint4*>(src + (long)q * W * CIN + 8 * ch);
+        }
+    };
+    auto store = [&](const uint4 (&v)[NCH], int q) {
+        uint16_t* slot = ring + ((q + 4) & 3) * XSLOT;
+#pragma unroll
+        for (int i = 0; i < NCH; ++i) {
+            const int ch = t + i * NT;
+            if (ch < CHUNKS) {
+                const int p = ch / (CIN / 8), c8 = ch - p * (CIN / 8);
+                *reinterpret_cast<uint4*>(slot + (p + 1) * XC + 8 * c8) = v[i];
+            }
+        }
+    };
+    __syncthreads();
+    for (int q = r0 - 1; q <= r0 + 1; ++q) {
+        load(va, q);
+        store(va, q);
+    }
+    if (r0 + 1 < r1) load(va, r0 + 2);
+    __syncthreads();
+    const int mb = wave % (W / 32), nb = wave / (W / 32);
+    const int m = l & 31, g = l >> 5;
+    const int n = n0 + nb * 32 + m;
+    const float bn = bias ? bias[n] : 0.0f;
+    const uint16_t* wrow = wl + (nb * 32 + m) * WS + 8 * g;
+    auto compute = [&](int r) {
+        const int y = r % H;
+        f32x16 acc;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky) {
+            const int dy = sign * (ky - 1);
+            if (y + dy < 0 || y + dy >= H) continue;                   // zero padding above / below
+            const uint16_t* xrow = ring + ((r + dy + 4) & 3) * XSLOT + 8 * g;
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+                const uint16_t* a = xrow + (mb * 32 + m + sign * (kx - 1) + 1) * XC;
+                const uint16_t* b = wrow + (3 * ky + kx) * CIN;
+#pragma unroll
+                for (int k = 0; k < CIN; k += 16)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                        __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(a + k)),
+                        __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(b + k)), acc, 0, 0, 0);
+            }
+        }
+        uint16_t* o = out + ((long)r * W + mb * 32) * Cout + n;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            float v = acc[e] + bn;
+            if (relu) v = fmaxf(v, 0.0f);
+            o[(long)(8 * (e >> 2) + 4 * g + (e & 3)) * Cout] = f2b(v);
+        }
+    };
+    for (int r = r0; r < r1; r += 2) {
+        if (r + 2 < r1) load(vb, r + 3);
+        compute(r);
+        if (r + 1 < r1) store(va, r + 2);
+        __syncthreads();
+        if (r + 1 >= r1) break;
+        if (r + 3 < r1) load(va, r + 4);
+        compute(r + 1);
+        if (r + 2 < r1) store(vb, r + 3);
+        __syncthreads();
+    }
+}
+
 // Forward / backward-data 3x3 convolution with the input halo staged once per channel chunk.
 // A workgroup computes 64 consecutive pixels (one image; a 64-pixel row segment if W >= 64, else 64 / W
 // whole rows) x 64 output channels. For each chunk of CC input channels it loads the (rows + 2) x (cols + 2)
@@ -299,6 +407,170 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t* __restrict__
                 atomicAdd(gw + (long)m * N + n, acc[r]);
         }
     }
+}
+
+// Weight and bias gradients by a march down image rows (W = 16, 32, 64 or 128 pixels per row):
+//   dW[co][ky][kx][ci] = sum_p gz[p][co] * x[p + (ky - 1, kx - 1)][ci],   db[co] = sum_p gz[p][co]
+// over the pixels of one band of rows (rows flattened as b * H + y), as MFMA GEMMs with K = pixels.
+// A workgroup owns one block of CB = 32 NCO output and IB = 32 NCI input channels and computes all nine taps
+// and the bias for it, so each activation and each gradient entry of its band is read from memory once. A
+// step covers R rows (R W >= 64 pixels); LDS holds a ring of 2R + 2 input rows (the R + 2 rows a step reads and
+// the R rows stored for the next step) and 2R gradient rows, all transposed to channel-major rows of pixels.
+// The rows a step needs two steps later are loaded into registers at its start and stored into LDS after the
+// next step (one barrier per step).
+// The workgroup has 3 NCO NCI waves: wave w takes the 32 x 32 channel sub-block w / 3 and the kernel row
+// ky = w % 3 (three taps, 3 x 16 accumulators per lane) over every pixel; the ky = 1 waves of the ci-block 0
+// also accumulate the bias (a product with a ones operand). The +-1 pixel shifts of kx = 0 and 2 are formed
+// from the aligned 16-byte read of kx = 1 and one neighbouring dword by byte alignment (the LDS rows carry
+// zero pads of 8 and 16 elements). Image edges: a wave skips the row y - 1 < 0 or y + 1 >= H (zero padding);
+// pixel edges read the pads. Partial sums go to part[band][Co][9 Ci] and bpart[band][Co] (blockIdx.y == 0).
+// LDS rows are padded to strides of 12 mod 64 (inputs) and 4 mod 64 (gradients) dwords, so the 16-byte
+// fragment reads of 16 consecutive rows touch distinct banks.
+template <int NCO, int NCI, int W, int R>
+__global__ __launch_bounds__(192 * NCO * NCI) void k_wgrad_march(const uint16_t* __restrict__ x,
+                                                                 const uint16_t* __restrict__ gz,
+                                                                 float* __restrict__ part, float* __restrict__ bpart,
+                                                                 int H, int rows_total, int Ci, int Co, int band) {
+    constexpr int CB = 32 * NCO, IB = 32 * NCI, NT = 192 * NCO * NCI;
+    constexpr int XS = W + 24, GS = W + 8;                  // row strides (elements) of the x and gz tiles
+    constexpr int XSLOT = IB * XS, GSLOT = CB * GS, NXS = 2 * R + 2, NGS = 2 * R;
+    constexpr int NBX = (W / 8) * (IB / 8), NBG = (W / 8) * (CB / 8);
+    static_assert(R * (NBX + NBG) <= NT, "one staging block per thread and set");
+    __shared__ __attribute__((aligned(16))) uint16_t lds[NXS * XSLOT + NGS * GSLOT];
+    uint16_t* xs = lds;
+    uint16_t* gs = lds + NXS * XSLOT;
+    const int t = threadIdx.x, l = t & 63, wave = t >> 6;
+    const int co_base = blockIdx.x * CB, ci_base = blockIdx.y * IB;
+    const int r0 = blockIdx.z * band, r1 = min(rows_total, r0 + band);
+    // zero both tiles once: the pads stay zero, the staged rows overwrite the rest
+    for (int q = t; q < (NXS * XSLOT + NGS * GSLOT) / 8; q += NT)
+        reinterpret_cast<uint4*>(lds)[q] = make_uint4(0, 0, 0, 0);
+    // staging role: one 8-pixel x 8-channel block of one of R input rows (t < R NBX) or gradient rows
+    const bool isx = t < R * NBX, isg = !isx && t < R * (NBX + NBG);
+    const int u = isx ? t : t - R * NBX;
+    const int nb = isx ? NBX : NBG, nch = isx ? IB / 8 : CB / 8;
+    const int rr = u / nb, cc = (u % nb) % nch, pg = (u % nb) / nch;    // row of the set, channel chunk, pixels
+    // two register sets in flight: rows loaded at the start of a step are stored after the next step
+    uint4 va[8], vb[8];
+    auto load = [&](uint4 (&v)[8], int q) {   // row q of x (isx) or gz (isg), zeros outside 0 .. rows_total-1
+        const bool ok = (isx || isg) && q >= 0 && q < rows_total;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            v[j] = make_uint4(0, 0, 0, 0);
+            if (ok) {
+                const long pix = (long)q * W + pg * 8 + j;
+                v[j] = isx ? *reinterpret_cast<const uint4*>(x + pix * Ci + ci_base + cc * 8)
+                           : *reinterpret_cast<const uint4*>(gz + pix * Co + co_base + cc * 8);
+            }
+        }
+    };
+    auto store = [&](const uint4 (&v)[8], int q) {   // transpose the 8 x 8 block to channel rows, store in q's slot
+        if (!(isx || isg)) return;
+        uint16_t* dst = isx ? xs + ((q + NXS) % NXS) * XSLOT + (cc * 8) * XS + 8 + pg * 8
+                            : gs + (q % NGS) * GSLOT + (cc * 8) * GS + pg * 8;
+        const int stride = isx ? XS : GS;
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(v);   // w[4j + d]: pixel j, channels 2d, 2d+1
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const uint32_t sel = (c & 1) ? 0x07060302u : 0x05040100u;   // high or low halves of two dwords
+            const int d = c >> 1;
+            uint4 o;
+            o.x = __builtin_amdgcn_perm(w[4 * 1 + d], w[4 * 0 + d], sel);
+            o.y = __builtin_amdgcn_perm(w[4 * 3 + d], w[4 * 2 + d], sel);
+            o.z = __builtin_amdgcn_perm(w[4 * 5 + d], w[4 * 4 + d], sel);
+            o.w = __builtin_amdgcn_perm(w[4 * 7 + d], w[4 * 6 + d], sel);
+            *reinterpret_cast<uint4*>(dst + c * stride) = o;
+        }
+    };
+    // this thread's row of the set a step stores: input rows rs + R + 1 + rr, gradient rows rs + R + rr
+    auto next_row = [&](int rs) { return isx ? rs + R + 1 + rr : rs + R + rr; };
+    __syncthreads();
+    // prologue: input rows r0 - 1 .. r0 + R, gradient rows r0 .. r0 + R - 1, then the set for the second step
+    if (isx)
+        for (int q = r0 - 1 + rr; q <= r0 + R; q += R) {
+            load(va, q);
+            store(va, q);
+        }
+    if (isg) {
+        load(va, r0 + rr);
+        store(va, r0 + rr);
+    }
+    if (r0 + R < r1) load(va, next_row(r0));
+    __syncthreads();
+    const int s = wave / 3, ky = wave % 3;
+    const int co32 = s % NCO, ci32 = s / NCO;
+    const int row = l & 31, g = l >> 5;
+    const bool with_bias = blockIdx.y == 0 && ci32 == 0 && ky == 1;
+    f32x16 acc[3], accb;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[k][e] = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) accb[e] = 0.0f;
+    const uint32_t one2 = 0x3F803F80u;                        // two bf16 ones
+    const bf16x8 ones = __builtin_bit_cast(bf16x8, make_uint4(one2, one2, one2, one2));
+    auto compute = [&](int rs) {
+        for (int r = rs; r < min(rs + R, r1); ++r) {
+            const int y = r % H;
+            const uint16_t* grow = gs + (r % NGS) * GSLOT + (co32 * 32 + row) * GS + 8 * g;
+            // this wave's input row r + ky - 1 exists (else zero padding: nothing to add)
+            if (!((ky == 0 && y == 0) || (ky == 2 && y == H - 1))) {
+                const uint16_t* xrow = xs + ((r + ky - 1 + NXS) % NXS) * XSLOT + (ci32 * 32 + row) * XS + 8 + 8 * g;
+#pragma unroll
+                for (int k = 0; k < W / 16; ++k) {
+                    const bf16x8 a = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(grow + 16 * k));
+                    if (with_bias) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, ones, accb, 0, 0, 0);
+                    const uint16_t* xr = xrow + 16 * k;
+                    const uint4 cur = *reinterpret_cast<const uint4*>(xr);
+                    const uint32_t prev = *reinterpret_cast<const uint32_t*>(xr - 2);
+                    const uint32_t next = *reinterpret_cast<const uint32_t*>(xr + 8);
+                    const uint32_t m1 = __builtin_amdgcn_alignbyte(cur.y, cur.x, 2);   // pixels 1, 2
+                    const uint32_t m2 = __builtin_amdgcn_alignbyte(cur.z, cur.y, 2);   // 3, 4
+                    const uint32_t m3 = __builtin_amdgcn_alignbyte(cur.w, cur.z, 2);   // 5, 6
+                    const uint4 left = make_uint4(__builtin_amdgcn_alignbyte(cur.x, prev, 2), m1, m2, m3);   // p - 1
+                    const uint4 right = make_uint4(m1, m2, m3, __builtin_amdgcn_alignbyte(next, cur.w, 2));  // p + 1
+                    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, left), acc[0], 0,
+                                                                     0, 0);
+                    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, cur), acc[1], 0,
+                                                                     0, 0);
+                    acc[2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, right), acc[2], 0,
+                                                                     0, 0);
+                }
+            } else if (with_bias) {
+#pragma unroll
+                for (int k = 0; k < W / 16; ++k)
+                    accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                        __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(grow + 16 * k)), ones, accb, 0, 0,
+                        0);
+            }
+        }
+    };
+    // step rs: the set for step rs + 2R is loaded while the other set (loaded during the previous step, for
+    // step rs + R) is stored after the compute, into ring slots no read of step rs touches
+    for (int rs = r0; rs < r1; rs += 2 * R) {
+        if (rs + 2 * R < r1) load(vb, next_row(rs + R));
+        compute(rs);
+        if (rs + R < r1) store(va, next_row(rs));
+        __syncthreads();
+        if (rs + R >= r1) break;
+        if (rs + 3 * R < r1) load(va, next_row(rs + 2 * R));
+        compute(rs + R);
+        if (rs + 2 * R < r1) store(vb, next_row(rs + R));
+        __syncthreads();
+    }
+    const int N = 9 * Ci;
+    const int co = co_base + co32 * 32, ci = ci_base + ci32 * 32 + row;
+#pragma unroll
+    for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int m = co + 8 * (e >> 2) + 4 * g + (e & 3);
+            part[((long)blockIdx.z * Co + m) * N + (3 * ky + kx) * Ci + ci] = acc[kx][e];
+        }
+    if (with_bias && row == 0)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) bpart[(long)blockIdx.z * Co + co + 8 * (e >> 2) + 4 * g + (e & 3)] = accb[e];
 }
 
 // Deterministic second pass of the split reductions: Z partial rows of `count` floats are added in a fixed
@@ -708,6 +980,38 @@ inline dim3 grid1(long n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)
 
 }  // namespace
 
+// Row-march convolution for rows of 64 or 128 pixels: input channels 32 (or 64 for 64-pixel rows), output
+// channels in blocks of 32 (64 where 32 input channels meet 64-pixel rows); bands of at least 16 rows, about
+// 1024 workgroups. Returns false where no shape applies (the caller takes the tiled GEMM).
+static bool conv_march(const uint16_t* src, const uint16_t* wk, const float* bias, uint16_t* out, int B, int H, int W,
+                       int Cin, int Cout, int sign, int relu, hipStream_t st) {
+    int cb;
+    if (W == 128 && Cin == 32 && Cout % 32 == 0) cb = 32;
+    else if (W == 64 && Cin == 32 && Cout % 64 == 0) cb = 64;
+    else if (W == 64 && (Cin == 32 || Cin == 64) && Cout % 32 == 0) cb = 32;
+    else return false;
+    const long rows = (long)B * H;
+    const long nblk = Cout / cb;
+    long nz = std::max(1L, std::min(rows / 16, 1024 / nblk));
+    const int band = (int)((rows + nz - 1) / nz);
+    nz = (rows + band - 1) / band;
+    const dim3 grid(1, (unsigned)nblk, (unsigned)nz);
+    const int rt = (int)rows;
+    if (W == 128)
+        hipLaunchKernelGGL((k_conv_march<128, 32, 32>), grid, dim3(256), 0, st, src, wk, bias, out, H, rt, Cout, sign,
+                           relu, band);
+    else if (Cin == 32 && cb == 64)
+        hipLaunchKernelGGL((k_conv_march<64, 32, 64>), grid, dim3(256), 0, st, src, wk, bias, out, H, rt, Cout, sign,
+                           relu, band);
+    else if (Cin == 32)
+        hipLaunchKernelGGL((k_conv_march<64, 32, 32>), grid, dim3(128), 0, st, src, wk, bias, out, H, rt, Cout, sign,
+                           relu, band);
+    else
+        hipLaunchKernelGGL((k_conv_march<64, 64, 32>), grid, dim3(128), 0, st, src, wk, bias, out, H, rt, Cout, sign,
+                           relu, band);
+    return true;
+}
+
 extern "C" int bz_nn_conv3x3(const uint16_t* x, const uint16_t* w16, const float* bias, uint16_t* y, int B, int H,
                              int W, int Ci, int Co, int relu, void* stream) {
     BZ_REQUIRE(x && w16 && y && B > 0 && H > 0 && W > 0, "bz_nn_conv3x3: bad arguments");
@@ -715,7 +1019,9 @@ extern "C" int bz_nn_conv3x3(const uint16_t* x, const uint16_t* w16, const float
     const long P = (long)B * H * W;
     BZ_REQUIRE(P * (long)(Ci > Co ? Ci : Co) < (1L << 31), "bz_nn_conv3x3: tensor too large");
     Geo g{B, H, W};
-    if (halo_ok(g) && Ci == 32)
+    if (conv_march(x, w16, bias, y, B, H, W, Ci, Co, 1, relu, (hipStream_t)stream))
+        ;
+    else if (halo_ok(g) && Ci == 32)
         hipLaunchKernelGGL(k_conv_halo<32>, dim3((unsigned)(P / 64), (Co + TN - 1) / TN), dim3(256), 0,
                            (hipStream_t)stream, x, w16, bias, y, g, Ci, Co, 1, relu);
     else
@@ -732,7 +1038,9 @@ extern "C" int bz_nn_conv3x3_dgrad(const uint16_t* gz, const uint16_t* wt16, uin
     BZ_REQUIRE(Co % 32 == 0 && Ci % 32 == 0, "bz_nn_conv3x3_dgrad: Co must be a multiple of 32 and Ci of 32");
     const long P = (long)B * H * W;
     Geo g{B, H, W};
-    if (halo_ok(g) && Co == 32)
+    if (conv_march(gz, wt16, nullptr, gx, B, H, W, Co, Ci, -1, 0, (hipStream_t)stream))
+        ;
+    else if (halo_ok(g) && Co == 32)
         hipLaunchKernelGGL(k_conv_halo<32>, dim3((unsigned)(P / 64), (Ci + TN - 1) / TN), dim3(256), 0,
                            (hipStream_t)stream, gz, wt16, (const float*)nullptr, gx, g, Co, Ci, -1, 0);
     else
@@ -781,11 +1089,55 @@ static void tree_sum(float* part, long Z, long stride, long count, float* scratc
     hipLaunchKernelGGL(k_sum_parts, grid1(count), dim3(256), 0, st, cur, (int)Z, sc, count, out);
 }
 
+// The row march (k_wgrad_march) serves rows of 16, 32, 64 or 128 pixels with channel counts in multiples of 32:
+// 32-wide channel blocks for 128-pixel rows, 64-wide where the counts allow otherwise (16- and 32-pixel rows
+// need 64-wide output blocks, 16-pixel rows 64-wide input blocks too), R = max(1, 64 / W) rows per step. The
+// rows are cut into nz bands (multiples of R rows): about 2048 workgroups, at least 16 rows per band, at most
+// 16 M partial floats.
+struct MarchPlan {
+    bool ok;
+    int nco, nci, rows_per_step, nz, band;
+};
+inline MarchPlan march_plan(int B, int H, int W, int Ci, int Co) {
+    MarchPlan m{false, 1, 1, 1, 0, 0};
+    if (Ci % 32 || Co % 32 || (W != 16 && W != 32 && W != 64 && W != 128)) return m;
+    if (W <= 32 && Co % 64) return m;
+    if (W == 16 && Ci % 64) return m;
+    const int nco = (W <= 64 && Co % 64 == 0) ? 2 : 1;
+    const int nci = (W <= 64 && Ci % 64 == 0) ? 2 : 1;
+    const int R = W >= 64 ? 1 : 64 / W;
+    const long rows = (long)B * H;
+    const long nblk = (long)(Co / (32 * nco)) * (Ci / (32 * nci));
+    const long nw = (long)Co * 9 * Ci;
+    long nz = std::max(1L, 2048 / nblk);
+    nz = std::min(nz, std::max(1L, rows / 16));
+    nz = std::min(nz, std::max(1L, (16L << 20) / nw));
+    long band = (rows + nz - 1) / nz;
+    band = (band + R - 1) / R * R;
+    m.ok = true;
+    m.nco = nco;
+    m.nci = nci;
+    m.rows_per_step = R;
+    m.band = (int)band;
+    m.nz = (int)((rows + band - 1) / band);
+    return m;
+}
+
 extern "C" long bz_nn_wgrad_work(int B, int H, int W, int Ci, int Co, int splits) {
     if (B <= 0 || H <= 0 || W <= 0 || Ci <= 0 || Co <= 0 || splits <= 0) return -1;
-    const WgradPlan q = wgrad_plan((long)B * H * W, Co, splits);
     const long nw = (long)Co * 9 * Ci;
+    const MarchPlan mp = march_plan(B, H, W, Ci, Co);
+    if (mp.ok) return mp.nz * nw + mp.nz * Co + tree_rows(mp.nz) * nw;
+    const WgradPlan q = wgrad_plan((long)B * H * W, Co, splits);
     return q.nz * nw + q.nblk * Co + std::max(tree_rows(q.nz) * nw, tree_rows(q.nblk) * Co);
+}
+
+template <int NCO, int NCI, int W>
+static void launch_march(const MarchPlan& mp, const uint16_t* x, const uint16_t* gz, float* part, float* bpart,
+                         int B, int H, int Ci, int Co, hipStream_t st) {
+    constexpr int R = W >= 64 ? 1 : 64 / W;
+    hipLaunchKernelGGL((k_wgrad_march<NCO, NCI, W, R>), dim3(Co / (32 * NCO), Ci / (32 * NCI), mp.nz),
+                       dim3(192 * NCO * NCI), 0, st, x, gz, part, bpart, H, B * H, Ci, Co, mp.band);
 }
 
 extern "C" int bz_nn_conv3x3_wgrad(const uint16_t* x, const uint16_t* gz, float* gw, float* gb, int B, int H, int W,
@@ -797,10 +1149,35 @@ extern "C" int bz_nn_conv3x3_wgrad(const uint16_t* x, const uint16_t* gz, float*
     const WgradPlan q = wgrad_plan(P, Co, splits);
     const long nw = (long)Co * 9 * Ci;
     const long scratch_rows = std::max(tree_rows(q.nz) * nw, tree_rows(q.nblk) * Co);
-    BZ_REQUIRE(!work || nwork >= q.nz * nw + q.nblk * Co + scratch_rows,
-               "bz_nn_conv3x3_wgrad: workspace smaller than bz_nn_wgrad_work");
     Geo g{B, H, W};
     const hipStream_t st = (hipStream_t)stream;
+    const MarchPlan mp = march_plan(B, H, W, Ci, Co);
+    if (work && mp.ok) {
+        BZ_REQUIRE(nwork >= mp.nz * nw + mp.nz * Co + tree_rows(mp.nz) * nw,
+                   "bz_nn_conv3x3_wgrad: workspace smaller than bz_nn_wgrad_work");
+        float* wpart = work;
+        float* bpart = work + (long)mp.nz * nw;
+        const int key = W * 4 + (mp.nco - 1) * 2 + (mp.nci - 1);
+        switch (key) {
+            case 128 * 4: launch_march<1, 1, 128>(mp, x, gz, wpart, bpart, B, H, Ci, Co, st); break;
+            case 64 * 4: launch_march<1, 1, 64>(mp, x, gz, wpart, bpart, B, H, Ci, Co, st); break;
+            case 64 * 4 + 2: launch_march<2, 1, 64>(mp, x, gz, wpart, bpart, B, H, Ci, Co, st); break;
+            case 64 * 4 + 1: launch_march<1, 2, 64>(mp, x, gz, wpart, bpart, B, H, Ci, Co, st); break;
+            case 64 * 4 + 3: launch_march<2, 2, 64>(mp, x, gz, wpart, bpart, B, H, Ci, Co, st); break;
+            case 32 * 4 + 2: launch_march<2, 1, 32>(mp, x, gz, wpart, bpart, B, H, Ci, Co, st); break;
+            case 32 * 4 + 3: launch_march<2, 2, 32>(mp, x, gz, wpart, bpart, B, H, Ci, Co, st); break;
+            case 16 * 4 + 3: launch_march<2, 2, 16>(mp, x, gz, wpart, bpart, B, H, Ci, Co, st); break;
+            default: bz::set_error("bz_nn_conv3x3_wgrad: no row-march shape"); return BZ_EINVAL;
+        }
+        BZ_CHECK_LAUNCH("bz_nn_conv3x3_wgrad(march)");
+        float* scratch = bpart + (long)mp.nz * Co;
+        tree_sum(wpart, mp.nz, nw, nw, scratch, gw, st);
+        tree_sum(bpart, mp.nz, Co, Co, scratch, gb, st);
+        BZ_CHECK_LAUNCH("bz_nn_conv3x3_wgrad(sums)");
+        return BZ_OK;
+    }
+    BZ_REQUIRE(!work || nwork >= q.nz * nw + q.nblk * Co + scratch_rows,
+               "bz_nn_conv3x3_wgrad: workspace smaller than bz_nn_wgrad_work");
     float* wpart = work;
     float* bpart = work ? work + q.nz * nw : nullptr;
     hipLaunchKernelGGL(k_conv_wgrad, dim3((Co + TM - 1) / TM, (9 * Ci + TN - 1) / TN, q.nz), dim3(256), 0, st, x, gz,

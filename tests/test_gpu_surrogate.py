@@ -78,11 +78,15 @@ def test_conv_layers_match_oracle(hip):
 
 
 @pytest.mark.parametrize("B,H,W,Ci,Co", [(2, 8, 16, 32, 96), (1, 2, 128, 32, 64), (3, 4, 32, 32, 32),
-                                         (2, 8, 16, 64, 32)])
+                                         (2, 8, 16, 64, 32), (2, 5, 128, 32, 32), (3, 4, 64, 32, 64),
+                                         (2, 3, 64, 64, 64), (1, 9, 64, 32, 32), (3, 1, 128, 32, 32),
+                                         (1, 17, 64, 64, 32)])
 def test_halo_convolutions_match_oracle(hip, B, H, W, Ci, Co):
-    """Forward and backward-data convolutions on shapes the halo-staged kernel takes (32-channel inputs on
-    64-pixel tiles of whole rows or row segments, an output count that is not a multiple of 64; the last
-    case takes it for backward-data only), within one bfloat16 step of the rounded FP64 oracle."""
+    """Forward and backward-data convolutions on the halo-staged kernel's shapes (32-channel inputs on 64-pixel
+    tiles of whole rows or row segments, an output count that is not a multiple of 64; the fourth case takes
+    it for backward-data only) and on the row march's (rows of 64 or 128 pixels: every channel-block shape,
+    bands spanning images, single-row images, one image over two bands), within one bfloat16 step of the
+    rounded FP64 oracle."""
     torch = hip
     rng = np.random.default_rng(B * 1000 + W)
     x = S.bf16(rng.standard_normal((B, H, W, Ci)))
@@ -273,3 +277,34 @@ def test_surrogate_loop_rounds_verify_above_random_and_report_the_error(hip):
     bests = [b for _, b in loop.history]
     assert all(b1 >= b0 for b0, b1 in zip(bests, bests[1:]))
     assert all(r["verified_fom_mean"] > rand.mean() for r in loop.rounds)
+
+
+@pytest.mark.parametrize("B,H,W,Ci,Co", [(2, 5, 128, 32, 32), (4, 10, 64, 32, 64), (2, 3, 64, 64, 64),
+                                         (1, 4, 64, 64, 32), (3, 1, 64, 32, 32), (1, 9, 128, 64, 32),
+                                         (2, 6, 32, 64, 128), (3, 5, 32, 32, 64), (1, 7, 32, 64, 128),
+                                         (4, 9, 16, 64, 64), (1, 16, 16, 128, 64), (5, 1, 16, 64, 64)])
+def test_row_march_weight_gradients_match_oracle(hip, B, H, W, Ci, Co):
+    """Weight and bias gradients of rows of 16 to 128 pixels (the row-march kernel, with the ordered
+    reduction): every channel-block shape and rows per step, bands that span images, a band whose last step
+    is short, single-row images (both outer kernel rows skipped) and one image over several bands; within
+    1e-4 of FP64, and the same bits on a second call."""
+    torch = hip
+    rng = np.random.default_rng(B * 1000 + H * 10 + Ci + Co)
+    x = S.bf16(rng.standard_normal((B, H, W, Ci)))
+    gz = S.bf16(rng.standard_normal((B, H, W, Co)))
+    w = S.bf16(rng.standard_normal((Co, 3, 3, Ci)) * 0.05)
+    _, dw, db = S.conv3x3_backward(x, w, gz)
+    st = torch.cuda.current_stream().cuda_stream
+    xd, gzd = to16(torch, x), to16(torch, gz)
+    need = N.lib().bz_nn_wgrad_work(B, H, W, Ci, Co, 3)
+    work = torch.empty(need, dtype=torch.float32).cuda()
+    runs = []
+    for _ in range(2):
+        gw = torch.zeros((Co, 3, 3, Ci), dtype=torch.float32).cuda()
+        gb = torch.full((Co,), 0.5, dtype=torch.float32).cuda()          # the sums add to what is there
+        N.call("bz_nn_conv3x3_wgrad", xd.data_ptr(), gzd.data_ptr(), gw.data_ptr(), gb.data_ptr(), B, H, W, Ci,
+               Co, 3, work.data_ptr(), need, st)
+        runs.append((gw.cpu().numpy(), gb.cpu().numpy() - 0.5))
+    assert rel_l2(runs[0][0], dw) < 1e-4, rel_l2(runs[0][0], dw)
+    assert rel_l2(runs[0][1], db) < 1e-5, rel_l2(runs[0][1], db)
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
