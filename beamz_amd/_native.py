@@ -82,6 +82,7 @@ SIGNATURES = {
     "bz_adj2d_runk": [vp, i32, vp, i32, i32, i32, i32, vp, i32, i32, i32, i32, i32, vp],
     "bz_nn_conv3x3": [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp],
     "bz_nn_conv3x3_dgrad": [vp, vp, vp, i32, i32, i32, i32, i32, vp],
+    "bz_nn_conv3x3_dgrad_relu": [vp, vp, vp, vp, i32, i32, i32, i32, i32, vp],
     "bz_nn_wgrad_work": [i32, i32, i32, i32, i32, i32],
     "bz_nn_conv3x3_wgrad": [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp, i64, vp],
     "bz_nn_first_work": [i32, i32, i32, i32],
@@ -91,6 +92,7 @@ SIGNATURES = {
     "bz_nonfinite_count": [vp, i64, vp, vp],
     "bz_nn_avgpool2": [vp, vp, i32, i32, i32, i32, vp],
     "bz_nn_avgpool2_bwd": [vp, vp, i32, i32, i32, i32, vp],
+    "bz_nn_avgpool2_bwd_relu": [vp, vp, vp, i32, i32, i32, i32, vp],
     "bz_nn_head": [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp],
     "bz_nn_head_bwd": [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp],
     "bz_nn_weights": [vp, vp, vp, i32, i32, vp],

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <string>
 
 namespace {
 
@@ -59,8 +60,9 @@ __device__ __forceinline__ void pixel(int p, const Geo& g, int& b, int& y, int& 
 // are zero). The next slice's loads are issued before the current slice's MFMAs.
 template <int FK>
 __global__ __launch_bounds__(256) void k_conv_gemm(const uint16_t* __restrict__ src, const uint16_t* __restrict__ wk,
-                                                   const float* __restrict__ bias, uint16_t* __restrict__ out, Geo g,
-                                                   int Cs, int N, int sign, int relu) {
+                                                   const float* __restrict__ bias, uint16_t* __restrict__ out,
+                                                   const uint16_t* __restrict__ mask, Geo g, int Cs, int N, int sign,
+                                                   int relu) {
     constexpr int FLD = FK + 8, NH = FK / 32;
     __shared__ __attribute__((aligned(16))) uint16_t As[TM][FLD];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][FLD];
@@ -122,7 +124,8 @@ __global__ __launch_bounds__(256) void k_conv_gemm(const uint16_t* __restrict__ 
         if (m < P) {
             float v = acc[r] + bn;
             if (relu) v = fmaxf(v, 0.0f);
-            out[(long)m * N + n] = f2b(v);
+            const long o = (long)m * N + n;
+            out[o] = (mask && !(b2f(mask[o]) > 0.0f)) ? (uint16_t)0 : f2b(v);
         }
     }
 }
@@ -136,13 +139,15 @@ __global__ __launch_bounds__(256) void k_conv_gemm(const uint16_t* __restrict__ 
 // at its start and stored after the next step (one barrier per row), so each input row is read from memory
 // once per band and channel block. Wave w computes the 32-pixel block w % (W / 32) and the 32-channel block
 // w / (W / 32) of every row: MFMAs with M = pixels, N = output channels, K = input channels, nine taps of
-// CIN / 16 steps. A tap shifts whole pixels, so every fragment is one aligned 16-byte LDS read; pixel strides
+// CIN / 16 steps. With a mask (the forward input of a backward-data step, an activation after ReLU), outputs
+// where the mask is not positive are stored as 0: the ReLU derivative of the layer below, fused. A tap shifts whole pixels, so every fragment is one aligned 16-byte LDS read; pixel strides
 // of CIN + 8 and weight rows of 9 CIN + 8 elements put the 16 rows of a read on distinct banks.
-template <int W, int CIN, int CB>
+template <int W, int CIN, int CB, bool MASK>
 __global__ __launch_bounds__(64 * (W / 32) * (CB / 32)) void k_conv_march(const uint16_t* __restrict__ src,
                                                                           const uint16_t* __restrict__ wk,
                                                                           const float* __restrict__ bias,
-                                                                          uint16_t* __restrict__ out, int H,
+                                                                          uint16_t* __restrict__ out,
+                                                                          const uint16_t* __restrict__ mask, int H,
                                                                           int rows_total, int Cout, int sign,
                                                                           int relu, int band) {
     constexpr int NT = 64 * (W / 32) * (CB / 32);
@@ -214,12 +219,14 @@ __global__ __launch_bounds__(64 * (W / 32) * (CB / 32)) void k_conv_march(const 
                         __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(b + k)), acc, 0, 0, 0);
             }
         }
-        uint16_t* o = out + ((long)r * W + mb * 32) * Cout + n;
+        const long o0 = ((long)r * W + mb * 32) * Cout + n;
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             float v = acc[e] + bn;
             if (relu) v = fmaxf(v, 0.0f);
-            o[(long)(8 * (e >> 2) + 4 * g + (e & 3)) * Cout] = f2b(v);
+            const long o = o0 + (long)(8 * (e >> 2) + 4 * g + (e & 3)) * Cout;
+            if (MASK) out[o] = b2f(mask[o]) > 0.0f ? f2b(v) : (uint16_t)0;
+            else out[o] = f2b(v);
         }
     };
     for (int r = r0; r < r1; r += 2) {
@@ -248,7 +255,7 @@ __global__ __launch_bounds__(64 * (W / 32) * (CB / 32)) void k_conv_march(const 
 // replace (92 -> 125 us), so those layers keep k_conv_gemm; staging the weights in LDS is the next step.
 template <int CC>
 __global__ __launch_bounds__(256) void k_conv_halo(const uint16_t* __restrict__ src, const uint16_t* __restrict__ wk,
-                                                   const float* __restrict__ bias, uint16_t* __restrict__ out, Geo g,
+                                                   const float* __restrict__ bias, uint16_t* __restrict__ out, const uint16_t* __restrict__ mask, Geo g,
                                                    int Cs, int N, int sign, int relu) {
     constexpr int SLD = CC + 8;              // halfwords per halo slot (16-byte pad: conflict-free reads)
     constexpr int KS = CC / 16;              // MFMA k-steps per tap and chunk
@@ -311,7 +318,8 @@ __global__ __launch_bounds__(256) void k_conv_halo(const uint16_t* __restrict__ 
         const int mm = p0 + wm * 32 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
         float v = acc[r] + bn;
         if (relu) v = fmaxf(v, 0.0f);
-        out[(long)mm * N + n] = f2b(v);
+        const long o = (long)mm * N + n;
+        out[o] = (mask && !(b2f(mask[o]) > 0.0f)) ? (uint16_t)0 : f2b(v);
     }
 }
 
@@ -872,6 +880,28 @@ __global__ void k_avgpool2_bwd8(const uint16_t* __restrict__ gy, uint16_t* __res
     *reinterpret_cast<uint4*>(gx + q * C + c) = o;
 }
 
+// k_avgpool2_bwd8 with the ReLU derivative of the layer below fused: 0 where y (that layer's output) is not
+// positive; the same bits as the two passes.
+__global__ void k_avgpool2_bwd8_relu(const uint16_t* __restrict__ gy, const uint16_t* __restrict__ y,
+                                     uint16_t* __restrict__ gx, Geo g, int C) {
+    const int C8 = C / 8;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)g.B * g.H * g.W * C8) return;
+    const int c = (int)(idx % C8) * 8;
+    const long q = idx / C8;
+    const int xx = (int)(q % g.W), yy = (int)((q / g.W) % g.H), b = (int)(q / ((long)g.W * g.H));
+    const int Ho = g.H / 2, Wo = g.W / 2;
+    const uint4 v = *reinterpret_cast<const uint4*>(gy + (((long)b * Ho + yy / 2) * Wo + xx / 2) * C + c);
+    const uint4 m = *reinterpret_cast<const uint4*>(y + q * C + c);
+    const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+    const uint16_t* mh = reinterpret_cast<const uint16_t*>(&m);
+    uint4 o;
+    uint16_t* oh = reinterpret_cast<uint16_t*>(&o);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) oh[e] = b2f(mh[e]) > 0.0f ? f2b(0.25f * b2f(h[e])) : (uint16_t)0;
+    *reinterpret_cast<uint4*>(gx + q * C + c) = o;
+}
+
 __global__ void k_avgpool2(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, Geo g, int C) {
     const int Ho = g.H / 2, Wo = g.W / 2;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -983,8 +1013,8 @@ inline dim3 grid1(long n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)
 // Row-march convolution for rows of 64 or 128 pixels: input channels 32 (or 64 for 64-pixel rows), output
 // channels in blocks of 32 (64 where 32 input channels meet 64-pixel rows); bands of at least 16 rows, about
 // 1024 workgroups. Returns false where no shape applies (the caller takes the tiled GEMM).
-static bool conv_march(const uint16_t* src, const uint16_t* wk, const float* bias, uint16_t* out, int B, int H, int W,
-                       int Cin, int Cout, int sign, int relu, hipStream_t st) {
+static bool conv_march(const uint16_t* src, const uint16_t* wk, const float* bias, uint16_t* out, const uint16_t* mask,
+                       int B, int H, int W, int Cin, int Cout, int sign, int relu, hipStream_t st) {
     int cb;
     if (W == 128 && Cin == 32 && Cout % 32 == 0) cb = 32;
     else if (W == 64 && Cin == 32 && Cout % 64 == 0) cb = 64;
@@ -997,17 +1027,32 @@ static bool conv_march(const uint16_t* src, const uint16_t* wk, const float* bia
     nz = (rows + band - 1) / band;
     const dim3 grid(1, (unsigned)nblk, (unsigned)nz);
     const int rt = (int)rows;
+    if (mask) {
+        if (W == 128)
+            hipLaunchKernelGGL((k_conv_march<128, 32, 32, true>), grid, dim3(256), 0, st, src, wk, bias, out, mask, H, rt,
+                               Cout, sign, relu, band);
+        else if (Cin == 32 && cb == 64)
+            hipLaunchKernelGGL((k_conv_march<64, 32, 64, true>), grid, dim3(256), 0, st, src, wk, bias, out, mask, H, rt,
+                               Cout, sign, relu, band);
+        else if (Cin == 32)
+            hipLaunchKernelGGL((k_conv_march<64, 32, 32, true>), grid, dim3(128), 0, st, src, wk, bias, out, mask, H, rt,
+                               Cout, sign, relu, band);
+        else
+            hipLaunchKernelGGL((k_conv_march<64, 64, 32, true>), grid, dim3(128), 0, st, src, wk, bias, out, mask, H, rt,
+                               Cout, sign, relu, band);
+        return true;
+    }
     if (W == 128)
-        hipLaunchKernelGGL((k_conv_march<128, 32, 32>), grid, dim3(256), 0, st, src, wk, bias, out, H, rt, Cout, sign,
+        hipLaunchKernelGGL((k_conv_march<128, 32, 32, false>), grid, dim3(256), 0, st, src, wk, bias, out, mask, H, rt, Cout, sign,
                            relu, band);
     else if (Cin == 32 && cb == 64)
-        hipLaunchKernelGGL((k_conv_march<64, 32, 64>), grid, dim3(256), 0, st, src, wk, bias, out, H, rt, Cout, sign,
+        hipLaunchKernelGGL((k_conv_march<64, 32, 64, false>), grid, dim3(256), 0, st, src, wk, bias, out, mask, H, rt, Cout, sign,
                            relu, band);
     else if (Cin == 32)
-        hipLaunchKernelGGL((k_conv_march<64, 32, 32>), grid, dim3(128), 0, st, src, wk, bias, out, H, rt, Cout, sign,
+        hipLaunchKernelGGL((k_conv_march<64, 32, 32, false>), grid, dim3(128), 0, st, src, wk, bias, out, mask, H, rt, Cout, sign,
                            relu, band);
     else
-        hipLaunchKernelGGL((k_conv_march<64, 64, 32>), grid, dim3(128), 0, st, src, wk, bias, out, H, rt, Cout, sign,
+        hipLaunchKernelGGL((k_conv_march<64, 64, 32, false>), grid, dim3(128), 0, st, src, wk, bias, out, mask, H, rt, Cout, sign,
                            relu, band);
     return true;
 }
@@ -1019,36 +1064,56 @@ extern "C" int bz_nn_conv3x3(const uint16_t* x, const uint16_t* w16, const float
     const long P = (long)B * H * W;
     BZ_REQUIRE(P * (long)(Ci > Co ? Ci : Co) < (1L << 31), "bz_nn_conv3x3: tensor too large");
     Geo g{B, H, W};
-    if (conv_march(x, w16, bias, y, B, H, W, Ci, Co, 1, relu, (hipStream_t)stream))
+    if (conv_march(x, w16, bias, y, nullptr, B, H, W, Ci, Co, 1, relu, (hipStream_t)stream))
         ;
     else if (halo_ok(g) && Ci == 32)
         hipLaunchKernelGGL(k_conv_halo<32>, dim3((unsigned)(P / 64), (Co + TN - 1) / TN), dim3(256), 0,
-                           (hipStream_t)stream, x, w16, bias, y, g, Ci, Co, 1, relu);
+                           (hipStream_t)stream, x, w16, bias, y, (const uint16_t*)nullptr, g, Ci, Co, 1, relu);
     else
         hipLaunchKernelGGL((9 * Ci) % 64 == 0 ? k_conv_gemm<64> : k_conv_gemm<32>,
                            dim3((unsigned)((P + TM - 1) / TM), (Co + TN - 1) / TN), dim3(256), 0, (hipStream_t)stream,
-                           x, w16, bias, y, g, Ci, Co, 1, relu);
+                           x, w16, bias, y, (const uint16_t*)nullptr, g, Ci, Co, 1, relu);
     BZ_CHECK_LAUNCH("bz_nn_conv3x3");
+    return BZ_OK;
+}
+
+static int conv_dgrad(const uint16_t* gz, const uint16_t* wt16, const uint16_t* mask, uint16_t* gx, int B, int H,
+                      int W, int Ci, int Co, void* stream, const char* name) {
+    if (!(gz && wt16 && gx && B > 0 && H > 0 && W > 0)) {
+        bz::set_error(std::string(name) + ": bad arguments");
+        return BZ_EINVAL;
+    }
+    if (Co % 32 || Ci % 32) {
+        bz::set_error(std::string(name) + ": Co must be a multiple of 32 and Ci of 32");
+        return BZ_EINVAL;
+    }
+    const long P = (long)B * H * W;
+    Geo g{B, H, W};
+    if (conv_march(gz, wt16, nullptr, gx, mask, B, H, W, Co, Ci, -1, 0, (hipStream_t)stream))
+        ;
+    else if (halo_ok(g) && Co == 32)
+        hipLaunchKernelGGL(k_conv_halo<32>, dim3((unsigned)(P / 64), (Ci + TN - 1) / TN), dim3(256), 0,
+                           (hipStream_t)stream, gz, wt16, (const float*)nullptr, gx, mask, g, Co, Ci, -1, 0);
+    else
+        hipLaunchKernelGGL((9 * Co) % 64 == 0 ? k_conv_gemm<64> : k_conv_gemm<32>,
+                           dim3((unsigned)((P + TM - 1) / TM), (Ci + TN - 1) / TN), dim3(256), 0, (hipStream_t)stream,
+                           gz, wt16, (const float*)nullptr, gx, mask, g, Co, Ci, -1, 0);
+    BZ_CHECK_LAUNCH(name);
     return BZ_OK;
 }
 
 extern "C" int bz_nn_conv3x3_dgrad(const uint16_t* gz, const uint16_t* wt16, uint16_t* gx, int B, int H, int W, int Ci,
                                    int Co, void* stream) {
-    BZ_REQUIRE(gz && wt16 && gx && B > 0 && H > 0 && W > 0, "bz_nn_conv3x3_dgrad: bad arguments");
-    BZ_REQUIRE(Co % 32 == 0 && Ci % 32 == 0, "bz_nn_conv3x3_dgrad: Co must be a multiple of 32 and Ci of 32");
-    const long P = (long)B * H * W;
-    Geo g{B, H, W};
-    if (conv_march(gz, wt16, nullptr, gx, B, H, W, Co, Ci, -1, 0, (hipStream_t)stream))
-        ;
-    else if (halo_ok(g) && Co == 32)
-        hipLaunchKernelGGL(k_conv_halo<32>, dim3((unsigned)(P / 64), (Ci + TN - 1) / TN), dim3(256), 0,
-                           (hipStream_t)stream, gz, wt16, (const float*)nullptr, gx, g, Co, Ci, -1, 0);
-    else
-        hipLaunchKernelGGL((9 * Co) % 64 == 0 ? k_conv_gemm<64> : k_conv_gemm<32>,
-                           dim3((unsigned)((P + TM - 1) / TM), (Ci + TN - 1) / TN), dim3(256), 0, (hipStream_t)stream,
-                           gz, wt16, (const float*)nullptr, gx, g, Co, Ci, -1, 0);
-    BZ_CHECK_LAUNCH("bz_nn_conv3x3_dgrad");
-    return BZ_OK;
+    return conv_dgrad(gz, wt16, nullptr, gx, B, H, W, Ci, Co, stream, "bz_nn_conv3x3_dgrad");
+}
+
+extern "C" int bz_nn_conv3x3_dgrad_relu(const uint16_t* gz, const uint16_t* wt16, const uint16_t* x, uint16_t* gx, int B,
+                                        int H, int W, int Ci, int Co, void* stream) {
+    if (!x) {
+        bz::set_error("bz_nn_conv3x3_dgrad_relu: null mask");
+        return BZ_EINVAL;
+    }
+    return conv_dgrad(gz, wt16, x, gx, B, H, W, Ci, Co, stream, "bz_nn_conv3x3_dgrad_relu");
 }
 
 // Launch geometry of bz_nn_conv3x3_wgrad: pixel chunk and number of splits of the weight GEMM, and pixel
@@ -1089,11 +1154,12 @@ static void tree_sum(float* part, long Z, long stride, long count, float* scratc
     hipLaunchKernelGGL(k_sum_parts, grid1(count), dim3(256), 0, st, cur, (int)Z, sc, count, out);
 }
 
-// The row march (k_wgrad_march) serves rows of 16, 32, 64 or 128 pixels with channel counts in multiples of 32:
-// 32-wide channel blocks for 128-pixel rows, 64-wide where the counts allow otherwise (16- and 32-pixel rows
-// need 64-wide output blocks, 16-pixel rows 64-wide input blocks too), R = max(1, 64 / W) rows per step. The
-// rows are cut into nz bands (multiples of R rows): about 2048 workgroups, at least 16 rows per band, at most
-// 16 M partial floats.
+// The row march (k_wgrad_march) serves rows of 16, 32, 64 or 128 pixels with channel counts in multiples of 32,
+// R = max(1, 64 / W) rows per step: 32-wide channel blocks on 128-pixel rows, 64-wide where the counts allow
+// otherwise (16- and 32-pixel rows need 64-wide output blocks, 16-pixel rows 64-wide input blocks too; on
+// the deep levels they measured faster than 32-wide blocks with fewer bands). The rows are cut into nz bands
+// (multiples of R rows) for about 2048 workgroups, at least 16 rows per band, at most 8 M (64- and 128-pixel
+// rows) or 16 M partial floats (the ordered sums read them all).
 struct MarchPlan {
     bool ok;
     int nco, nci, rows_per_step, nz, band;
@@ -1111,7 +1177,7 @@ inline MarchPlan march_plan(int B, int H, int W, int Ci, int Co) {
     const long nw = (long)Co * 9 * Ci;
     long nz = std::max(1L, 2048 / nblk);
     nz = std::min(nz, std::max(1L, rows / 16));
-    nz = std::min(nz, std::max(1L, (16L << 20) / nw));
+    nz = std::min(nz, std::max(1L, (W >= 64 ? 8L << 20 : 16L << 20) / nw));
     long band = (rows + nz - 1) / nz;
     band = (band + R - 1) / R * R;
     m.ok = true;
@@ -1299,6 +1365,18 @@ extern "C" int bz_nn_avgpool2_bwd(const uint16_t* gy, uint16_t* gx, int B, int H
     else
         hipLaunchKernelGGL(k_avgpool2_bwd, grid1((long)B * H * W * C), dim3(256), 0, (hipStream_t)stream, gy, gx, g, C);
     BZ_CHECK_LAUNCH("bz_nn_avgpool2_bwd");
+    return BZ_OK;
+}
+
+extern "C" int bz_nn_avgpool2_bwd_relu(const uint16_t* gy, const uint16_t* y, uint16_t* gx, int B, int H, int W, int C,
+                                       void* stream) {
+    BZ_REQUIRE(gy && y && gx && B > 0 && H % 2 == 0 && W % 2 == 0 && H > 0 && W > 0 && C % 8 == 0 &&
+                   ((uintptr_t)gy & 15) == 0 && ((uintptr_t)y & 15) == 0 && ((uintptr_t)gx & 15) == 0,
+               "bz_nn_avgpool2_bwd_relu: bad arguments (C a multiple of 8, 16-byte aligned tensors)");
+    Geo g{B, H, W};
+    hipLaunchKernelGGL(k_avgpool2_bwd8_relu, grid1((long)B * H * W * (C / 8)), dim3(256), 0, (hipStream_t)stream, gy, y,
+                       gx, g, C);
+    BZ_CHECK_LAUNCH("bz_nn_avgpool2_bwd_relu");
     return BZ_OK;
 }
 

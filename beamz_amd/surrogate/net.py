@@ -150,20 +150,22 @@ class Surrogate:
         for lvl in range(len(LEVELS) - 1, -1, -1):
             ci, co = LEVELS[lvl]
             a, y0, y1, h, w = cache["acts"][lvl]
+            # the gradient at y1 through its ReLU: fused into the pooling backward below the top level
             if lvl < len(LEVELS) - 1:
                 up = self._empty16(B, h, w, co)
-                N.call("bz_nn_avgpool2_bwd", ga.data_ptr(), up.data_ptr(), B, h, w, co, st)
+                N.call("bz_nn_avgpool2_bwd_relu", ga.data_ptr(), y1.data_ptr(), up.data_ptr(), B, h, w, co, st)
                 ga = up
+            else:
+                N.call("bz_nn_relu_mask", ga.data_ptr(), y1.data_ptr(), ga.numel(), st)
             splits = self._splits(B * h * w, co, co)
             # second convolution of the level: input y0 (co channels)
-            N.call("bz_nn_relu_mask", ga.data_ptr(), y1.data_ptr(), ga.numel(), st)
             N.call("bz_nn_conv3x3_wgrad", y0.data_ptr(), ga.data_ptr(), self.grads[f"w{lvl}1"].data_ptr(),
                    self.grads[f"b{lvl}1"].data_ptr(), B, h, w, co, co, splits, wp, wn, st)
+            # its backward-data, through the ReLU of y0 (fused)
             g0 = self._empty16(B, h, w, co)
-            N.call("bz_nn_conv3x3_dgrad", ga.data_ptr(), self.wt16[f"w{lvl}1"].data_ptr(), g0.data_ptr(), B, h, w,
-                   co, co, st)
+            N.call("bz_nn_conv3x3_dgrad_relu", ga.data_ptr(), self.wt16[f"w{lvl}1"].data_ptr(), y0.data_ptr(),
+                   g0.data_ptr(), B, h, w, co, co, st)
             # first convolution of the level: input a (ci channels)
-            N.call("bz_nn_relu_mask", g0.data_ptr(), y0.data_ptr(), g0.numel(), st)
             if lvl == 0:
                 if input_grad:
                     gx = T.empty((B, H, W), dtype=T.float32, device=self.dev)

@@ -285,6 +285,10 @@ int bz_nn_conv3x3(const uint16_t* x, const uint16_t* w16, const float* bias, uin
                   int Ci, int Co, int relu, void* stream);          /* Ci % 32 == 0, Co % 32 == 0 */
 int bz_nn_conv3x3_dgrad(const uint16_t* gz, const uint16_t* wt16, uint16_t* gx, int B, int H, int W, int Ci,
                         int Co, void* stream);                      /* wt16 = [Ci][3][3][Co] */
+/* bz_nn_conv3x3_dgrad followed by bz_nn_relu_mask(gx, x): gx = 0 where x (the forward input, an activation after
+ * ReLU) is not positive; the same bits as the two calls, in one pass. */
+int bz_nn_conv3x3_dgrad_relu(const uint16_t* gz, const uint16_t* wt16, const uint16_t* x, uint16_t* gx, int B, int H,
+                             int W, int Ci, int Co, void* stream);
 /* Weight and bias gradients. With a workspace (work != NULL, nwork >= bz_nn_wgrad_work(...) floats) every
  * partial sum is written out and added in a fixed order: the result is deterministic. With work == NULL
  * the partial sums go through FP32 atomics, whose order varies from run to run. */
@@ -300,6 +304,9 @@ int bz_nn_conv_first_bwd(const uint16_t* x, const uint16_t* gz, const float* w, 
 int bz_nn_relu_mask(uint16_t* gz, const uint16_t* y, long n, void* stream);
 int bz_nn_avgpool2(const uint16_t* x, uint16_t* y, int B, int H, int W, int C, void* stream);
 int bz_nn_avgpool2_bwd(const uint16_t* gy, uint16_t* gx, int B, int H, int W, int C, void* stream);
+/* bz_nn_avgpool2_bwd followed by bz_nn_relu_mask(gx, y) (C % 8 == 0, 16-byte aligned tensors), same bits. */
+int bz_nn_avgpool2_bwd_relu(const uint16_t* gy, const uint16_t* y, uint16_t* gx, int B, int H, int W, int C,
+                            void* stream);
 int bz_nn_head(const uint16_t* a, const float* fw, const float* fb, float* gap, float* out, int B, int HW, int C,
                int nout, void* stream);                             /* global average pool + fully connected */
 int bz_nn_head_bwd(const float* gout, const float* gap, const float* fw, float* gfw, float* gfb, uint16_t* ga,

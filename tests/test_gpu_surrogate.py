@@ -308,3 +308,35 @@ def test_row_march_weight_gradients_match_oracle(hip, B, H, W, Ci, Co):
     assert rel_l2(runs[0][0], dw) < 1e-4, rel_l2(runs[0][0], dw)
     assert rel_l2(runs[0][1], db) < 1e-5, rel_l2(runs[0][1], db)
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+
+
+@pytest.mark.parametrize("B,H,W,Ci,Co", [(2, 5, 128, 32, 32), (2, 3, 64, 64, 64), (2, 8, 16, 64, 128),
+                                         (1, 4, 32, 32, 32)])
+def test_fused_relu_derivatives_equal_the_separate_passes(hip, B, H, W, Ci, Co):
+    """Backward-data with the ReLU mask of its forward input fused (row march, tiled GEMM and halo kernels), and
+    the pooling backward with the mask of the level below fused: the same bits as the separate passes."""
+    torch = hip
+    rng = np.random.default_rng(B + H + W + Ci + Co)
+    st = torch.cuda.current_stream().cuda_stream
+    w = S.bf16(rng.standard_normal((Co, 3, 3, Ci)) * 0.05)
+    w32 = torch.tensor(w, dtype=torch.float32).cuda()
+    w16 = torch.empty((Co, 3, 3, Ci), dtype=torch.int16).cuda()
+    wt16 = torch.empty((Ci, 3, 3, Co), dtype=torch.int16).cuda()
+    N.call("bz_nn_weights", w32.data_ptr(), w16.data_ptr(), wt16.data_ptr(), Co, Ci, st)
+    gzd = to16(torch, S.bf16(rng.standard_normal((B, H, W, Co))))
+    xd = to16(torch, S.bf16(np.maximum(rng.standard_normal((B, H, W, Ci)), 0.0)))     # half of it zero
+    a = torch.empty((B, H, W, Ci), dtype=torch.int16).cuda()
+    b = torch.empty_like(a)
+    N.call("bz_nn_conv3x3_dgrad", gzd.data_ptr(), wt16.data_ptr(), a.data_ptr(), B, H, W, Ci, Co, st)
+    N.call("bz_nn_relu_mask", a.data_ptr(), xd.data_ptr(), a.numel(), st)
+    N.call("bz_nn_conv3x3_dgrad_relu", gzd.data_ptr(), wt16.data_ptr(), xd.data_ptr(), b.data_ptr(), B, H, W, Ci, Co,
+           st)
+    assert torch.equal(a, b) and int((a == 0).sum()) > a.numel() // 4
+    gy = to16(torch, S.bf16(rng.standard_normal((B, H // 2 or 1, W // 2, Ci)))) if H % 2 == 0 else None
+    if gy is not None:
+        p = torch.empty((B, H, W, Ci), dtype=torch.int16).cuda()
+        q = torch.empty_like(p)
+        N.call("bz_nn_avgpool2_bwd", gy.data_ptr(), p.data_ptr(), B, H, W, Ci, st)
+        N.call("bz_nn_relu_mask", p.data_ptr(), xd.data_ptr(), p.numel(), st)
+        N.call("bz_nn_avgpool2_bwd_relu", gy.data_ptr(), xd.data_ptr(), q.data_ptr(), B, H, W, Ci, st)
+        assert torch.equal(p, q)
