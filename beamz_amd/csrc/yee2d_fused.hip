@@ -53,11 +53,29 @@ struct Gather2D {
     double* dft;            // [nfreq][npts] complex
 };
 
+// Every buffer a descriptor points at is device (global) memory. The pointers are read from the descriptor, so
+// the compiler cannot tell their address space and would issue flat instructions, which also count against
+// the LDS wait counter; these accessors issue global loads and stores (with the scalar base and a 32-bit
+// offset).
+#define BZ_GLOBAL __attribute__((address_space(1)))
 __device__ __forceinline__ float at(const float* __restrict__ base, uint32_t byte_off) {
-    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_off);
+    return *(const BZ_GLOBAL float*)((const char*)base + byte_off);
 }
 __device__ __forceinline__ void put(float* __restrict__ base, uint32_t byte_off, float v) {
-    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_off) = v;
+    *(BZ_GLOBAL float*)((char*)base + byte_off) = v;
+}
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 at4(const float* __restrict__ base, uint32_t byte_off) {
+    const f32x4 v = *(const BZ_GLOBAL f32x4*)((const char*)base + byte_off);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+template <typename T>
+__device__ __forceinline__ T gload(const T* p, long i) {
+    return ((const BZ_GLOBAL T*)p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, long i, T v) {
+    ((BZ_GLOBAL T*)p)[i] = v;
 }
 
 // Monitor sample p of step n, read from the buffer step n wrote.
@@ -357,9 +375,9 @@ __device__ __forceinline__ void snap_put(const bz_sim2d& S, int slot, int i, int
     if (sa >= 0 && sa < S.snap_sx && sb >= 0 && sb < S.snap_sy) {
         const long plane = (long)S.snap_sy * S.snap_sx;
         float* p = S.snap + (long)slot * 3 * plane + (long)sb * S.snap_sx + sa;
-        p[0] = ez;
-        p[plane] = hx;
-        p[2 * plane] = hy;
+        gstore(p, 0, ez);
+        gstore(p, plane, hx);
+        gstore(p, 2 * plane, hy);
     }
 }
 
@@ -679,7 +697,7 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
             // precomputed by k_coef2d (the same roundings), one 16-byte (fa, fb, ca, cb) per cell
 #pragma unroll
             for (int r = 0; r < B; ++r) {
-                const float4 q = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(cf) + 4u * o[r]);
+                const float4 q = at4(cf, 4u * o[r]);
                 fa[r] = q.x;
                 fb[r] = q.y;
                 ca[r] = q.z;
@@ -697,10 +715,10 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
     const int e0 = ent_start ? ent_start[t] : 0, e1 = ent_start ? ent_start[t + 1] : 0;
     const int tid = w * LANES + l;
 
-    float amp_next = sgl ? sgl[n0] : 0.0f;
+    float amp_next = sgl ? gload(sgl, n0) : 0.0f;
     for (int k = 0; k < K; ++k) {
         const float amp = amp_next;              // the next step's amplitude loads during this step
-        if (sgl && k + 1 < K) amp_next = sgl[n0 + k + 1];
+        if (sgl && k + 1 < K) amp_next = gload(sgl, n0 + k + 1);
         // H of step n0+k+1: Hy of the band's last row needs Ez of the next band's first row
         ez_top[w][l] = ez[0];
         __syncthreads();
@@ -759,7 +777,7 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
                 const int j = jb + r;
                 if (j >= sy0 && j < sy0 + ssy && src_col) {
                     const long rr = (long)(j - sy0) * ssx + (i - sx0);
-                    ez[r] = (smul ? smul[rr] : 1.0f) * ez[r] + sadd[rr] * amp;
+                    ez[r] = (smul ? gload(smul, rr) : 1.0f) * ez[r] + gload(sadd, rr) * amp;
                 }
             }
         }
@@ -771,7 +789,7 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
                 __syncthreads();
                 for (int e = e0 + tid; e < e1; e += LANES * KW) {
                     const int code = ents[2 * e + 1];
-                    if ((code & 3) == comp) stage[(long)k * npts + ents[2 * e]] = smp[code >> 10][(code >> 2) & 255];
+                    if ((code & 3) == comp) gstore(stage, (long)k * npts + ents[2 * e], smp[code >> 10][(code >> 2) & 255]);
                 }
                 __syncthreads();
             }
