@@ -355,6 +355,8 @@ class Grid:
     def _fused3d(self, n):
         """One fused step; returns True if the sources were applied inside the kernel."""
         boxed = len(self.sources) <= N.MAX_BOX and all(s.box is not None for s in self.sources)
+        if not boxed:
+            self._slow_path("3D sources are added by separate point-list launches after each step")
         arr = (N.BoxSource * N.MAX_BOX)()
         nb = 0
         if boxed:
@@ -425,13 +427,32 @@ class Grid:
 
     def _pair_ok(self):
         """The two-step kernel applies: 3D fused path, box sources only, no grid power (needs every step)."""
-        if not (self.is_3d and self.kernel == "fused" and self.tb2) or self.power is not None:
+        if not (self.is_3d and self.kernel == "fused" and self.tb2):
             return False
-        if len(self.sources) > N.MAX_BOX or any(s.box is None for s in self.sources):
+        why = None
+        if self.power is not None:
+            why = "grid power is accumulated every step"
+        elif len(self.sources) > N.MAX_BOX:
+            why = f"{len(self.sources)} sources (the two-step kernel takes {N.MAX_BOX})"
+        elif any(s.box is None for s in self.sources):
+            why = "a source spans more than 4 M cells"
+        else:
+            if self._stage is None:
+                self._stage = self._setup_stage()
+            if self._stage is False:
+                why = "the monitors span more than 8 M cells"
+        if why is not None:
+            self._slow_path(f"3D run takes one step per launch: {why}")
             return False
-        if self._stage is None:
-            self._stage = self._setup_stage()
-        return self._stage is not False
+        return True
+
+    def _slow_path(self, message):
+        """Say once per grid and reason that a run left the fastest kernel."""
+        seen = self.__dict__.setdefault("_slow_reported", set())
+        if message not in seen:
+            seen.add(message)
+            from ..helpers import display_status
+            display_status(message, "warning")
 
     def _setup_stage(self, max_cells=1 << 23):
         """Staging box covering every monitor sample, for the samples of the intermediate step."""

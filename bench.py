@@ -65,20 +65,24 @@ MEASURED_TRAFFIC = {
 }
 
 
-def cpu_baseline_3d(nx, ny, nz, dx, dt, seconds_target=15.0):
-    """FP64 NumPy restatement of fdtd.py:200-310 (complex128 fields), timed on 1 core."""
+def cpu_baseline_3d(nx, ny, nz, seconds_target=15.0):
+    """The reference-order FP64 step (oracle RefSim: complex128 fields, fdtd.py:200-310 update, the dipole source
+    and both port monitors, fdtd.py:411-429 order) on a slab of the 3D workload (nz planes, no z-PML), 1 core.
+    The ports are sampled as planes (the reference copies whole volumes for x-normal monitors, which would make
+    its step slower still)."""
     from oracle import fdtd_ref as R
-    rng = np.random.default_rng(0)
-    eps = 1 + 10 * rng.random((nz, ny, nx)); sig = rng.random((nz, ny, nx)) * 1e3
-    F = {k: np.zeros(s, dtype=np.complex128) for k, s in R.field_shapes(nx, ny, nz).items()}
-    F["Ez"][nz // 2 - 1, ny // 2, nx // 2] = 1.0
-    R.update_fields_3d(F, eps, sig, dx, dx, dx, dt)          # warm-up
+    from beamz_amd.workloads import soi_mmi_3d
+    mesh, d3, t, mons = soi_mmi_3d(nx, ny, nz, steps=400, z_pml=False, record_ports=True)
+    ref = R.RefSim(mesh.permittivity, mesh.conductivity, mesh.dx, mesh.dy, float(t[1] - t[0]), len(t), dz=mesh.dz,
+                   sources=d3.sources, monitors=mons, dtype=np.complex128, true_planes=True,
+                   design_dims=(d3.width, d3.height, d3.depth))
+    ref.step()                                                  # warm-up
     n, t0 = 0, time.perf_counter()
-    while True:
-        R.update_fields_3d(F, eps, sig, dx, dx, dx, dt)
+    while n < len(t) - 1:
+        ref.step()
         n += 1
         el = time.perf_counter() - t0
-        if el > seconds_target or n >= 50:
+        if el > seconds_target:
             break
     return nx * ny * nz * n / el, n, el
 
@@ -333,8 +337,9 @@ def build_runner(a, total):
 
         def cpu():
             nzs = max(4, nz // 8)
-            v, n, el = cpu_baseline_3d(nx, ny, nzs, 25e-9, float(t[1] - t[0]))
-            return v, f"{n} steps of the NumPy complex128 3D Yee update on {nx}x{ny}x{nzs} cells ({el:.1f} s)"
+            v, n, el = cpu_baseline_3d(nx, ny, nzs)
+            return v, (f"{n} reference-order steps (NumPy complex128 update, dipole source, both port monitors) "
+                       f"of the same MMI on a {nx}x{ny}x{nzs}-cell slab ({el:.1f} s)")
     else:
         nx, ny = a.nx or 800, a.ny or 400
         d, dx, t, mon = waveguide_2d(nx, ny, steps=max(total, 16))
