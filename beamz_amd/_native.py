@@ -67,6 +67,8 @@ SIGNATURES = {
                       i32, i32, vp],
     "bz_yee3d_step2": [P6, P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, C.POINTER(BoxSource),
                        C.POINTER(f32), i32, vp, C.POINTER(i32), i32, i32, i32, vp],
+    "bz_yee3d_step3": [P6, P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, C.POINTER(BoxSource),
+                       C.POINTER(f32), C.POINTER(f32), i32, vp, vp, C.POINTER(i32), i32, i32, i32, vp],
     "bz_yee3d_step2_noalloc": [P6, P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64,
                                C.POINTER(BoxSource), C.POINTER(f32), i32, C.POINTER(i32), vp, vp, vp, i32, i32, i32,
                                vp, vp, i32, vp, vp, vp, i32, i32, i32, vp],
@@ -75,6 +77,7 @@ SIGNATURES = {
     "bz_yee2d_run2": [vp, i32, vp, i32, i32, vp, i32, i32, vp, vp, i32, vp, vp, i32, vp, vp, vp],
     "bz_yee2d_coef": [vp, i32, i32, vp],
     "bz_gather2": [P6, vp, P6, vp, vp, i32, i32, i32, vp, vp, i32, vp, vp, vp, vp],
+    "bz_gather3": [P6, P6, vp, P6, vp, vp, i32, i32, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp],
     "bz_abi_sizes": [C.POINTER(i32), i32],
     "bz_yee2d_runk": [vp, i32, vp, i32, i32, i32, i32, vp, i32, i32, vp, vp, vp, i32, vp, vp, i32, vp, vp, vp],
     "bz_adj2d_run": [vp, i32, vp, i32, i32, i32, i32, vp],

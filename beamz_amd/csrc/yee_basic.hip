@@ -171,6 +171,49 @@ __global__ void k_gather2(CompBase base0, const int64_t* __restrict__ off0, Comp
     a[1] = im;
 }
 
+// The samples of three consecutive steps (n and n+1 from base0/base1 at off0, n+2 from base2 at off2) in one
+// launch, thread per (point, frequency); each accumulator receives the three terms in step order, the rounded
+// adds of three k_gather launches.
+__global__ void k_gather3(CompBase base0, CompBase base1, const int64_t* __restrict__ off0, CompBase base2,
+                          const int* __restrict__ comp, const int64_t* __restrict__ off2, int n, int act0, int act1,
+                          int act2, float* __restrict__ series0, float* __restrict__ series1,
+                          float* __restrict__ series2, int nfreq, const double* __restrict__ phasor0,
+                          const double* __restrict__ phasor1, const double* __restrict__ phasor2,
+                          double* __restrict__ dft) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = (int)(g % n), k = (int)(g / n);
+    if (k >= max(nfreq, 1)) return;
+    const int c = comp[p];
+    const float v0 = act0 ? base0.p[c][off0[p]] : 0.0f;
+    const float v1 = act1 ? base1.p[c][off0[p]] : 0.0f;
+    const float v2 = act2 ? base2.p[c][off2[p]] : 0.0f;
+    if (k == 0) {
+        if (act0 && series0) series0[p] = v0;
+        if (act1 && series1) series1[p] = v1;
+        if (act2 && series2) series2[p] = v2;
+    }
+    if (k >= nfreq) return;
+    double* a = dft + 2 * ((long)k * n + p);
+    double re = a[0], im = a[1];
+    if (act0) {
+        const double dv = (double)v0;
+        re += dv * phasor0[2 * k];
+        im += dv * phasor0[2 * k + 1];
+    }
+    if (act1) {
+        const double dv = (double)v1;
+        re += dv * phasor1[2 * k];
+        im += dv * phasor1[2 * k + 1];
+    }
+    if (act2) {
+        const double dv = (double)v2;
+        re += dv * phasor2[2 * k];
+        im += dv * phasor2[2 * k + 1];
+    }
+    a[0] = re;
+    a[1] = im;
+}
+
 // helper.py:152-187 (real fields)
 __global__ __launch_bounds__(TX* TY) void k_pow2d(double* __restrict__ acc, const float* __restrict__ ez,
                                                    const float* __restrict__ hx, const float* __restrict__ hy,
@@ -405,5 +448,29 @@ extern "C" int bz_gather2(const float* const base0[6], const int64_t* off0, cons
     hipLaunchKernelGGL(k_gather2, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, (hipStream_t)stream, c0, off0,
                        c1, comp, off1, n, act0, act1, series0, series1, nfreq, phasor0, phasor1, dft);
     BZ_CHECK_LAUNCH("bz_gather2");
+    return BZ_OK;
+}
+
+extern "C" int bz_gather3(const float* const base0[6], const float* const base1[6], const int64_t* off0,
+                          const float* const base2[6], const int* comp, const int64_t* off2, int n, int act0,
+                          int act1, int act2, float* series0, float* series1, float* series2, int nfreq,
+                          const double* phasor0, const double* phasor1, const double* phasor2, double* dft,
+                          void* stream) {
+    if (n == 0 || (!act0 && !act1 && !act2)) return BZ_OK;
+    BZ_REQUIRE(base0 && base1 && off0 && base2 && comp && off2 && n > 0 && nfreq >= 0, "bz_gather3: bad arguments");
+    BZ_REQUIRE(nfreq == 0 || (dft && (!act0 || phasor0) && (!act1 || phasor1) && (!act2 || phasor2)),
+               "bz_gather3: DFT requested without buffers");
+    CompBase c0, c1, c2;
+    for (int q = 0; q < 6; ++q) {
+        c0.p[q] = base0[q];
+        c1.p[q] = base1[q];
+        c2.p[q] = base2[q];
+    }
+    const long threads = (long)n * (nfreq > 0 ? nfreq : 1);
+    BZ_REQUIRE(threads / 256 < (1L << 31) - 1, "bz_gather3: too many points x frequencies");
+    hipLaunchKernelGGL(k_gather3, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, (hipStream_t)stream, c0, c1,
+                       off0, c2, comp, off2, n, act0, act1, act2, series0, series1, series2, nfreq, phasor0, phasor1,
+                       phasor2, dft);
+    BZ_CHECK_LAUNCH("bz_gather3");
     return BZ_OK;
 }

@@ -115,6 +115,11 @@ class Grid:
         self.kseg2 = None       # z-planes per segment of the two-step kernel (None: auto_kseg2)
         self.rows2 = 2          # y-rows per wave of the two-step kernel (2, 3 or 4)
         self.waves2 = 8         # waves per workgroup of the two-step kernel (8, or 16 with 2 rows)
+        self.tb3 = False        # 3D: three steps per launch where the run allows it
+        self.kseg3 = None       # z-planes per segment of the three-step kernel (None: auto_kseg3)
+        self.rows3 = 1          # y-rows per wave of the three-step kernel
+        self.waves3 = 16        # waves per workgroup of the three-step kernel: (rows, waves) (2, 8), (3, 8),
+                                # (1, 16) or (2, 16)
         self._stage = None
         self._comb = None
         self.timing = False
@@ -127,6 +132,9 @@ class Grid:
 
     def kernel_name(self):
         if self.is_3d:
+            if self.kernel == "fused" and self.tb3:
+                return (f"bz_yee3d_step3 (three steps per pass, {self.waves3} waves x {self.rows3} rows, "
+                        f"{self.kseg3_eff()}-plane segments)")
             if self.kernel == "fused" and self.tb2:
                 return (f"bz_yee3d_step2 (two steps per pass, {self.waves2} waves x {self.rows2} rows, "
                         f"{self.kseg2_eff()}-plane segments)")
@@ -398,7 +406,10 @@ class Grid:
             return
         n, end = n0, n0 + count
         while n < end:
-            if end - n >= 2 and self._pair_ok():
+            if end - n >= 3 and self.tb3 and self._pair_ok():
+                self._fused3d_triple(n)
+                n += 3
+            elif end - n >= 2 and self.tb2 and self._pair_ok():
                 self._fused3d_pair(n)
                 n += 2
             else:
@@ -425,9 +436,23 @@ class Grid:
         segs = max(1, capacity // (gx * gy))
         return max(2, -(-self.nz // segs))
 
+    def kseg3_eff(self):
+        return int(self.kseg3) if self.kseg3 else self.auto_kseg3()
+
+    def auto_kseg3(self):
+        """Segment length of the three-step kernel by the rule of auto_kseg2 (every shape of it holds one
+        workgroup per CU)."""
+        rows, waves = self.rows3, self.waves3
+        gx = -(-self.nx // 58)
+        gy = -(-self.ny // (waves * rows - 6))
+        cus = self.torch.cuda.get_device_properties(self.dev).multi_processor_count
+        segs = max(1, cus // (gx * gy))
+        return max(2, -(-self.nz // segs))
+
     def _pair_ok(self):
-        """The two-step kernel applies: 3D fused path, box sources only, no grid power (needs every step)."""
-        if not (self.is_3d and self.kernel == "fused" and self.tb2):
+        """The two- and three-step kernels apply: 3D fused path, box sources only, no grid power (needs every
+        step)."""
+        if not (self.is_3d and self.kernel == "fused" and (self.tb2 or self.tb3)):
             return False
         why = None
         if self.power is not None:
@@ -515,6 +540,52 @@ class Grid:
             N.call("bz_gather2", stg["base"], t.stage_off.data_ptr(), base1, t.comp.data_ptr(), t.off.data_ptr(),
                    t.n, int(q0 is not None), int(q1 is not None), ser(q0), ser(q1), t.nfreq, ph(q0), ph(q1),
                    t.dft.data_ptr() if t.nfreq else None, self.stream)
+
+    def _fused3d_triple(self, n):
+        """Steps n, n+1 and n+2 in one launch of the three-step kernel, then the monitor samples of all three."""
+        arr = (N.BoxSource * N.MAX_BOX)()
+        amp1 = (C.c_float * N.MAX_BOX)()
+        amp2 = (C.c_float * N.MAX_BOX)()
+        for q, s in enumerate(self.sources):
+            b = s.box[0]
+            b.amp = float(np.real(s.signal[n]))
+            arr[q] = b
+            amp1[q] = float(np.real(s.signal[n + 1]))
+            amp2[q] = float(np.real(s.signal[n + 2]))
+        stg = self._stage
+        if stg["plans"] and "buf2" not in stg:
+            stg["buf2"] = self.torch.zeros_like(stg["buf"])
+            cells = stg["buf"].numel() // 6
+            stg["base2"] = N.ptrs6([stg["buf2"].data_ptr() + 4 * q * cells for q in range(6)])
+        need1 = any(p.slot.get(n) is not None for p in stg["plans"])
+        need2 = any(p.slot.get(n + 1) is not None for p in stg["plans"])
+        src, dst = self._fb[self._cur], self._fb[1 - self._cur]
+        ps = N.ptrs6([src[k].data_ptr() for k in COMP3])
+        pd = N.ptrs6([dst[k].data_ptr() for k in COMP3])
+        st = self.stream
+        if self.timing:
+            e0, e1 = self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True)
+            e0.record()
+        N.call("bz_yee3d_step3", ps, pd, self.eps.data_ptr(), self.sig.data_ptr(), self.nx, self.ny, self.nz,
+               self.ld, self.dx, self.dy, self.dz, self.dt, MU_0, EPS_0, arr, amp1, amp2, len(self.sources),
+               stg["buf"].data_ptr() if need1 else None, stg["buf2"].data_ptr() if need2 else None,
+               stg["box"] if (need1 or need2) else None, self.kseg3_eff(), self.rows3, self.waves3, st)
+        if self.timing:
+            e1.record()
+            self._events.append((e0, e1, 3))
+        self._cur = 1 - self._cur
+        # the samples of the three steps, one launch per table: steps n and n+1 from the staging boxes,
+        # step n+2 from the fields
+        base2 = N.ptrs6([self.f[k].data_ptr() for k in self.names] + [0] * (6 - len(self.names)))
+        for t in stg["plans"]:
+            q = [t.slot.get(n + s) for s in range(3)]
+            if all(x is None for x in q):
+                continue
+            ser = lambda x: t.series[x].data_ptr() if (t.series is not None and x is not None) else None
+            ph = lambda x: t.phasor[x].data_ptr() if (t.nfreq and x is not None) else None
+            N.call("bz_gather3", stg["base"], stg["base2"], t.stage_off.data_ptr(), base2, t.comp.data_ptr(),
+                   t.off.data_ptr(), t.n, *[int(x is not None) for x in q], *[ser(x) for x in q], t.nfreq,
+                   *[ph(x) for x in q], t.dft.data_ptr() if t.nfreq else None, self.stream)
 
     # ---------------- fused, batched 2D path ----------------
     def invalidate(self):

@@ -357,13 +357,17 @@ def build_runner(a, total):
     sim.grid.rows2 = a.rows2
     sim.grid.kseg2 = a.kseg2
     sim.grid.waves2 = a.waves2
+    sim.grid.tb3 = bool(a.tb3)
+    sim.grid.rows3, sim.grid.waves3, sim.grid.kseg3 = a.rows3, a.waves3, a.kseg3
     sim.grid.seg2d = a.seg2d
     sim.grid.graph2d = a.graph
     sim.grid.kstep2d = None if a.kstep < 0 else a.kstep
     sim.grid.krows2d, sim.grid.kwaves2d = a.krows, a.kwaves
     sim.initialize_simulation(save=False, live=False)
     r = Runner(sim._advance, sim.grid, cells, desc, sim.grid.kernel_name(), cpu)
-    r.steps_per_launch = 2 if (a.workload == "mmi3d" and a.tb2) else None      # 2D: set once the engine exists
+    r.steps_per_launch = None                  # 2D: set once the engine exists
+    if a.workload == "mmi3d" and (a.tb2 or a.tb3):
+        r.steps_per_launch = 3 if a.tb3 else 2
     r.tuned = None
     return r
 
@@ -393,6 +397,10 @@ def main():
     ap.add_argument("--rows2", type=int, default=2, help="3D two-step kernel: y-rows per wave (2, 3, 4)")
     ap.add_argument("--kseg2", type=int, default=0, help="3D two-step kernel: z-planes per segment (0: auto)")
     ap.add_argument("--waves2", type=int, default=8, help="3D two-step kernel: waves per workgroup (8, 16)")
+    ap.add_argument("--tb3", type=int, default=0, help="3D: three steps per launch (1) or not (0)")
+    ap.add_argument("--rows3", type=int, default=1, help="3D three-step kernel: y-rows per wave")
+    ap.add_argument("--waves3", type=int, default=16, help="3D three-step kernel: waves per workgroup")
+    ap.add_argument("--kseg3", type=int, default=0, help="3D three-step kernel: z-planes per segment (0: auto)")
     ap.add_argument("--seg2d", type=int, default=None, help="rows per tile of the 2D engine (default: by size)")
     ap.add_argument("--xcd", type=int, default=1, help="2D engines: XCD-ordered tile lists (1) or row-major (0)")
     ap.add_argument("--pairs", type=int, default=-1, help="batch2d: two steps per launch (1), one (0), by size (-1)")
@@ -462,7 +470,9 @@ def main():
     launch_cells = r.cells if not getattr(r, "per_step_cells", False) else r.cells // (3 * a.nt)
     achieved = launch_cells * b_alg / step_kernel_s
     nx, ny = r.grid.nx, r.grid.ny
-    if a.workload == "mmi3d" and a.tb2:
+    if a.workload == "mmi3d" and a.tb3:
+        key = (a.workload, f"fused-tb3-w{a.waves3}", a.rows3, r.grid.kseg3_eff(), nx, ny, r.grid.nz)
+    elif a.workload == "mmi3d" and a.tb2:
         key = (a.workload, f"fused-tb2-w{a.waves2}", a.rows2, r.grid.kseg2_eff(), nx, ny, r.grid.nz)
     elif a.workload == "batch2d":
         key = (a.workload, r.seg_key, a.nsims)

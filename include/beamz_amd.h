@@ -20,7 +20,8 @@
  *   bz_update_3d .......... FDTD._update_3d_fields, beamz/simulation/fdtd.py:200-310
  *   bz_add_points / bz_affine_points
  *                           source injection, beamz/simulation/helper.py:5-118
- *   bz_gather / bz_gather2  monitor sampling of one step / of both steps of a two-step pass,
+ *   bz_gather / bz_gather2 / bz_gather3  monitor sampling of one step / of the steps of a two- or
+ *                           three-step pass,
  *                           beamz/simulation/helper.py:231-256 with beamz/devices/monitors.py:279-397,
  *                           plus the on-the-fly DFT (no reference counterpart)
  *   bz_grid_power_2d/3d ... grid power accumulation, beamz/simulation/helper.py:120-187
@@ -38,6 +39,7 @@
  *                           fdtd.py:200-310 restricted to d/dz = 0 (its 2D path numpy_backend.py:34-88 has
  *                           only the (Ez, Hx, Hy) family)
  *   bz_yee3d_step2 ........ two consecutive bz_yee3d_step steps in one pass (same bits)
+ *   bz_yee3d_step3 ........ three consecutive bz_yee3d_step steps in one pass (same bits)
  *   bz_yee2d_run2 ......... two consecutive bz_yee2d_run steps per pass (same bits, no grid power)
  *   bz_yee2d_runk ......... K consecutive bz_yee2d_run steps per pass (same bits, no grid power), with
  *                           bz_yee2d_coef filling the per-cell update coefficients it reads
@@ -112,6 +114,29 @@ int bz_yee3d_step2(float* const src[6], float* const dst[6], const float* eps_r,
                    int nx, int ny, int nz, int ld, double dx, double dy, double dz, double dt,
                    double mu0, double eps0, const bz_box_source* boxes, const float* amp1, int nbox,
                    float* stage, const int* stage_box, int kseg, int rows, int waves, void* stream);
+
+/* ---------------- three-step 3D kernel ----------------
+ * Steps n, n+1 and n+2 of bz_yee3d_step (fdtd.py:200-310 with the soft sources of helper.py:62-118, the
+ * H, E, sources part of the step order of fdtd.py:411-429) in one pass: reads src[6] (step n), writes every
+ * cell of dst[6] (step n+3); bit-identical to three bz_yee3d_step calls. boxes[q].amp is the amplitude
+ * of step n, amp1[q] and amp2[q] those of steps n+1 and n+2. stage1 / stage2 (each may be NULL) receive
+ * the fields of steps n+1 / n+2 inside stage_box = {x0, y0, z0, sx, sy, sz}, laid out as for
+ * bz_yee3d_step2. kseg: z-planes per segment (>= 2; <= 0: 32); (rows, waves): (2, 8), (3, 8), (1, 16)
+ * or (2, 16). */
+int bz_yee3d_step3(float* const src[6], float* const dst[6], const float* eps_r, const float* sigma,
+                   int nx, int ny, int nz, int ld, double dx, double dy, double dz, double dt,
+                   double mu0, double eps0, const bz_box_source* boxes, const float* amp1, const float* amp2,
+                   int nbox, float* stage1, float* stage2, const int* stage_box, int kseg, int rows, int waves,
+                   void* stream);
+
+/* Monitor samples of the three steps of a bz_yee3d_step3 pass (helper.py:231-256 with
+ * monitors.py:279-397, plus the on-the-fly DFT): steps n+1 and n+2 from base0 / base1 at off0 (the staging
+ * boxes), step n+3 from base2 at off2 (the fields); act0..act2 select the steps sampled. Each DFT
+ * accumulator receives its terms in step order, as from three bz_gather calls. */
+int bz_gather3(const float* const base0[6], const float* const base1[6], const int64_t* off0,
+               const float* const base2[6], const int* comp, const int64_t* off2, int n, int act0, int act1,
+               int act2, float* series0, float* series1, float* series2, int nfreq, const double* phasor0,
+               const double* phasor1, const double* phasor2, double* dft, void* stream);
 
 /* ---------------- allocation-free two-step 3D pass with monitor sampling ----------------
  * No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph

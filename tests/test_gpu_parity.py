@@ -270,6 +270,39 @@ def test_two_step_3d_kernel_bit_identical_to_single_steps(hip):
             assert np.array_equal(r[k], base[k]), (key, k)
 
 
+def test_three_step_3d_kernel_bit_identical_to_single_steps(hip):
+    """The three-step kernel (steps n, n+1, n+2 in one pass, two staging boxes, one three-step gather) must
+    give the very bits of single-step launches for every (rows, waves) shape and segment length, on a grid
+    whose sizes are not multiples of any tile, with sources and DFT monitors; 121 steps end with one single
+    step, 122 with one two-step pass."""
+    from beamz_amd.workloads import soi_mmi_3d
+    for steps in (121, 122):
+        out = {}
+        for key in (("single", 0, 0, 0), ("triple", 1, 16, 0), ("triple", 2, 8, 32), ("triple", 2, 8, 2),
+                    ("triple", 1, 16, 5), ("triple", 2, 16, 7), ("triple", 3, 8, 11)):
+            mesh, d3, t, mons = soi_mmi_3d(nx=131, ny=45, nz=37, steps=steps, npml=6, dx=40e-9, record_ports=True)
+            sim = bz.FDTD(mesh, time=t)
+            sim.grid.kernel, sim.grid.rows = "fused", 2
+            sim.grid.tb2 = key[0] == "triple"
+            sim.grid.tb3 = key[0] == "triple"
+            if sim.grid.tb3:
+                sim.grid.rows3, sim.grid.waves3, sim.grid.kseg3 = key[1], key[2], key[3] or None
+            sim.run(live=False, save=False)
+            r = {k: np.real(getattr(sim, k)) for k in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")}
+            for q, m in enumerate(mons):
+                for k, v in m.dft.items():
+                    r[f"dft{q}{k}"] = np.array(v)
+                for k, v in m.fields.items():
+                    if len(v):
+                        r[f"series{q}{k}"] = np.array(v)
+            assert any(k.startswith("dft") for k in r) and any(k.startswith("series") for k in r)
+            out[key] = r
+        base = out[("single", 0, 0, 0)]
+        for key, r in out.items():
+            for k in base:
+                assert np.array_equal(r[k], base[k]), (steps, key, k)
+
+
 def test_watchdog_counts_nonfinite_entries(hip):
     """bz_nonfinite_count counts NaN and infinite entries exactly (across several workgroups); a normal run
     reports no component, and a NaN planted in a field is reported for that component only."""
