@@ -72,14 +72,15 @@ SIGNATURES = {
     "bz_yee3d_step2_noalloc": [P6, P6, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64, f64, f64,
                                C.POINTER(BoxSource), C.POINTER(f32), i32, C.POINTER(i32), vp, vp, vp, i32, i32, i32,
                                vp, vp, i32, vp, vp, vp, i32, i32, i32, vp],
-    "bz_yee2d_run": [vp, i32, vp, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp],
-    "bz_yee2d_hz_run": [vp, i32, vp, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp],
-    "bz_yee2d_run2": [vp, i32, vp, i32, i32, vp, i32, i32, vp, vp, i32, vp, vp, i32, vp, vp, vp],
+    "bz_yee2d_run": [vp, i32, vp, i32, i32, vp, i32, vp, i32, vp, vp, i32, i32, vp, i32, vp, vp, vp],
+    "bz_yee2d_hz_run": [vp, i32, vp, i32, i32, vp, i32, vp, i32, vp, vp, i32, i32, vp, i32, vp, vp, vp],
+    "bz_yee2d_run2": [vp, i32, vp, i32, i32, vp, i32, i32, vp, vp, i32, vp, vp, i32, i32, vp, i32, vp, vp, vp],
     "bz_yee2d_coef": [vp, i32, i32, vp],
     "bz_gather2": [P6, vp, P6, vp, vp, i32, i32, i32, vp, vp, i32, vp, vp, vp, vp],
     "bz_gather3": [P6, P6, vp, P6, vp, vp, i32, i32, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp],
     "bz_abi_sizes": [C.POINTER(i32), i32],
-    "bz_yee2d_runk": [vp, i32, vp, i32, i32, i32, i32, vp, i32, i32, vp, vp, vp, i32, vp, vp, i32, vp, vp, vp],
+    "bz_yee2d_runk": [vp, i32, vp, i32, i32, i32, i32, vp, i32, i32, vp, vp, vp, i32, vp, vp, i32, i32, vp, i32, vp,
+                      vp, vp],
     "bz_adj2d_run": [vp, i32, vp, i32, i32, i32, i32, vp],
     "bz_adj2d_run2": [vp, i32, vp, i32, i32, vp, i32, i32, i32, i32, i32, vp],
     "bz_adj2d_runk": [vp, i32, vp, i32, i32, i32, i32, vp, i32, i32, i32, i32, i32, vp],
@@ -130,6 +131,10 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m beamz_amd.build`")
+    # torch first: its import maps torch's own HIP runtime, which the library's HIP dependency then resolves
+    # to. Loaded before torch (a mesh rasterised before any GPU work), the library would map the system HIP
+    # runtime and torch a second one, and the library's launches fail with "no ROCm-capable device".
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     for name, args in SIGNATURES.items():
         fn = getattr(L, name)

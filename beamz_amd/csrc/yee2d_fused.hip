@@ -46,12 +46,24 @@ constexpr int GPB = LANES * WPG;   // monitor points per gather workgroup
 struct Gather2D {
     const int* pts;      // (sim, comp, offset) per point
     int npts;
-    const int* slot;     // slot[n] = recorded row of step n, or -1
-    float* series;       // [rows][npts] or NULL
+    const int* slot;     // slot[g][n] = recorded row of step n for point group g, or -1
+    const int* grp;      // group of each point, or NULL (one group)
+    int nslot;           // steps per group row of slot (groups > 0 only)
+    int nrow;            // rows per group block of phasor (groups > 0 only)
+    float* series;       // [rows][npts] or NULL (each group numbers its rows from 0)
     int nfreq;
-    const double* phasor;   // [rows][nfreq] complex
+    const double* phasor;   // [group][rows][nfreq] complex (frequencies a group lacks hold 0)
     double* dft;            // [nfreq][npts] complex
 };
+
+// Recorded row of step n for point p (-1: not recorded), and the phasor of that row at frequency k.
+__device__ __forceinline__ int gslot(const Gather2D& G, int p, int n) {
+    return G.grp ? G.slot[(long)G.grp[p] * G.nslot + n] : G.slot[n];
+}
+__device__ __forceinline__ const double* gphasor(const Gather2D& G, int p, int row, int k) {
+    const long g = G.grp ? G.grp[p] : 0;
+    return G.phasor + 2 * ((g * G.nrow + row) * G.nfreq + k);
+}
 
 // Every buffer a descriptor points at is device (global) memory. The pointers are read from the descriptor, so
 // the compiler cannot tell their address space and would issue flat instructions, which also count against
@@ -81,7 +93,7 @@ __device__ __forceinline__ void gstore(T* p, long i, T v) {
 // Monitor sample p of step n, read from the buffer step n wrote.
 __device__ __forceinline__ void gather_point(const bz_sim2d* __restrict__ sims, const Gather2D& G, int p, int n) {
     if (p >= G.npts) return;
-    const int q = G.slot[n];
+    const int q = gslot(G, p, n);
     if (q < 0) return;
     const int par = (n + 1) & 1;
     const bz_sim2d& S = sims[G.pts[3 * p]];
@@ -90,7 +102,7 @@ __device__ __forceinline__ void gather_point(const bz_sim2d* __restrict__ sims, 
     const float v = f[G.pts[3 * p + 2]];
     if (G.series) G.series[(long)q * G.npts + p] = v;
     const double dv = v;
-    const double* ph = G.phasor + 2 * (long)q * G.nfreq;
+    const double* ph = gphasor(G, p, q, 0);
     for (int k = 0; k < G.nfreq; ++k) {
         double* a = G.dft + 2 * ((long)k * G.npts + p);
         a[0] += dv * ph[2 * k];
@@ -341,7 +353,7 @@ __device__ __forceinline__ void gather_pair(const bz_sim2d* __restrict__ sims, c
     int row[2];
     float v[2];
     for (int s = 0; s < 2; ++s) {
-        row[s] = G.slot[n + s];
+        row[s] = gslot(G, p, n + s);
         v[s] = 0.0f;
         if (row[s] < 0) continue;
         if (s == 0) {
@@ -360,7 +372,7 @@ __device__ __forceinline__ void gather_pair(const bz_sim2d* __restrict__ sims, c
     for (int s = 0; s < 2; ++s) {
         if (row[s] < 0) continue;
         const double dv = v[s];
-        const double* ph = G.phasor + 2 * ((long)row[s] * G.nfreq + k);
+        const double* ph = gphasor(G, p, row[s], k);
         re += dv * ph[0];
         im += dv * ph[1];
     }
@@ -597,14 +609,14 @@ __device__ __forceinline__ void gather_k(const Gather2D& G, const float* __restr
     double re = a ? a[0] : 0.0, im = a ? a[1] : 0.0;
     bool any = false;
     for (int k = 0; k < K; ++k) {
-        const int row = G.slot[n0 + k];
+        const int row = gslot(G, p, n0 + k);
         if (row < 0) continue;
         any = true;
         const float v = stage[(long)k * G.npts + p];
         if (f == 0 && G.series) G.series[(long)row * G.npts + p] = v;
         if (a) {
             const double dv = v;
-            const double* ph = G.phasor + 2 * ((long)row * G.nfreq + f);
+            const double* ph = gphasor(G, p, row, f);
             re += dv * ph[0];
             im += dv * ph[1];
         }
@@ -840,16 +852,20 @@ int runk(const bz_sim2d* sims, const int* tiles, int ntiles, int K, int* step, i
 }  // namespace
 
 extern "C" int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step,
-                            int nsteps, const int* pts, int npts, const int* slot, float* series, int nfreq,
-                            const double* phasor, double* dft, void* stream) {
+                            int nsteps, const int* pts, int npts, const int* slot, const int* grp, int nslot,
+                            int nrow, float* series, int nfreq, const double* phasor, double* dft, void* stream) {
     BZ_REQUIRE(sims && nsims > 0 && tiles && ntiles > 0 && step && nsteps >= 0, "bz_yee2d_run: bad arguments");
     BZ_REQUIRE(npts >= 0 && (npts == 0 || (pts && slot)), "bz_yee2d_run: monitor points without a slot table");
     BZ_REQUIRE(nfreq >= 0 && (nfreq == 0 || (phasor && dft)), "bz_yee2d_run: DFT requested without buffers");
+    BZ_REQUIRE(!grp || (nslot > 0 && nrow > 0), "bz_yee2d_run: point groups need nslot > 0 and nrow > 0");
     if (nsteps == 0) return BZ_OK;
     Gather2D G;
     G.pts = pts;
     G.npts = npts;
     G.slot = slot;
+    G.grp = grp;
+    G.nslot = nslot;
+    G.nrow = nrow;
     G.series = series;
     G.nfreq = nfreq;
     G.phasor = phasor;
@@ -865,16 +881,21 @@ extern "C" int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, i
 }
 
 extern "C" int bz_yee2d_hz_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step,
-                               int nsteps, const int* pts, int npts, const int* slot, float* series, int nfreq,
-                               const double* phasor, double* dft, void* stream) {
+                               int nsteps, const int* pts, int npts, const int* slot, const int* grp, int nslot,
+                               int nrow, float* series, int nfreq, const double* phasor, double* dft,
+                               void* stream) {
     BZ_REQUIRE(sims && nsims > 0 && tiles && ntiles > 0 && step && nsteps >= 0, "bz_yee2d_hz_run: bad arguments");
     BZ_REQUIRE(npts >= 0 && (npts == 0 || (pts && slot)), "bz_yee2d_hz_run: monitor points without a slot table");
     BZ_REQUIRE(nfreq >= 0 && (nfreq == 0 || (phasor && dft)), "bz_yee2d_hz_run: DFT requested without buffers");
+    BZ_REQUIRE(!grp || (nslot > 0 && nrow > 0), "bz_yee2d_hz_run: point groups need nslot > 0 and nrow > 0");
     if (nsteps == 0) return BZ_OK;
     Gather2D G;
     G.pts = pts;
     G.npts = npts;
     G.slot = slot;
+    G.grp = grp;
+    G.nslot = nslot;
+    G.nrow = nrow;
     G.series = series;
     G.nfreq = nfreq;
     G.phasor = phasor;
@@ -891,16 +912,21 @@ extern "C" int bz_yee2d_hz_run(const bz_sim2d* sims, int nsims, const int* tiles
 
 extern "C" int bz_yee2d_run2(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step,
                              int rd0, int npairs, const int* pts, const int* spts, int npts, const int* slot,
-                             float* series, int nfreq, const double* phasor, double* dft, void* stream) {
+                             const int* grp, int nslot, int nrow, float* series, int nfreq, const double* phasor,
+                             double* dft, void* stream) {
     BZ_REQUIRE(sims && nsims > 0 && tiles && ntiles > 0 && step && npairs >= 0 && seg >= 2 && (rd0 == 0 || rd0 == 1),
                "bz_yee2d_run2: bad arguments");
     BZ_REQUIRE(npts >= 0 && (npts == 0 || (pts && spts && slot)), "bz_yee2d_run2: monitor points without tables");
     BZ_REQUIRE(nfreq >= 0 && (nfreq == 0 || (phasor && dft)), "bz_yee2d_run2: DFT requested without buffers");
+    BZ_REQUIRE(!grp || (nslot > 0 && nrow > 0), "bz_yee2d_run2: point groups need nslot > 0 and nrow > 0");
     if (npairs == 0) return BZ_OK;
     Gather2D G;
     G.pts = pts;
     G.npts = npts;
     G.slot = slot;
+    G.grp = grp;
+    G.nslot = nslot;
+    G.nrow = nrow;
     G.series = series;
     G.nfreq = nfreq;
     G.phasor = phasor;
@@ -930,8 +956,8 @@ extern "C" int bz_yee2d_run2(const bz_sim2d* sims, int nsims, const int* tiles, 
 
 extern "C" int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int rows, int waves,
                              int ksteps, int* step, int rd0, int nlaunch, const int* ent_start, const int* ents,
-                             float* stage, int npts, const int* slot, float* series, int nfreq, const double* phasor,
-                             double* dft, void* stream) {
+                             float* stage, int npts, const int* slot, const int* grp, int nslot, int nrow,
+                             float* series, int nfreq, const double* phasor, double* dft, void* stream) {
     BZ_REQUIRE(sims && nsims > 0 && tiles && ntiles > 0 && step && nlaunch >= 0 && (rd0 == 0 || rd0 == 1),
                "bz_yee2d_runk: bad arguments");
     const int shape = rows * 100 + waves;
@@ -942,6 +968,7 @@ extern "C" int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, 
     BZ_REQUIRE(npts >= 0 && (npts == 0 || (ent_start && ents && stage && slot)),
                "bz_yee2d_runk: monitor points without tables");
     BZ_REQUIRE(nfreq >= 0 && (nfreq == 0 || (phasor && dft)), "bz_yee2d_runk: DFT requested without buffers");
+    BZ_REQUIRE(!grp || (nslot > 0 && nrow > 0), "bz_yee2d_runk: point groups need nslot > 0 and nrow > 0");
     BZ_REQUIRE((long)npts * (nfreq > 0 ? nfreq : 1) / 256 < (1L << 31) - 1,
                "bz_yee2d_runk: too many monitor points x frequencies");
     if (nlaunch == 0) return BZ_OK;
@@ -949,6 +976,9 @@ extern "C" int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, 
     G.pts = nullptr;
     G.npts = npts;
     G.slot = slot;
+    G.grp = grp;
+    G.nslot = nslot;
+    G.nrow = nrow;
     G.series = series;
     G.nfreq = nfreq;
     G.phasor = phasor;

@@ -13,11 +13,16 @@ registers. ``kstep=0`` falls back to one or two steps per launch
 monitor samples and FP64 DFT sums of every member are gathered into one table.
 Each member's step is bit-for-bit the step it takes when run alone.
 
-The members must share their time axis, their monitor record schedule and
-their DFT frequencies (one phasor table serves the batch), and each may have at
-most one source. ``autotune`` picks the kernel shape (K, rows, waves; or the
-tile height without K steps) by timing a few steps of each candidate on the
-batch itself (HIP events), then restores the initial state.
+Members may differ in everything a single run may: design, grid size,
+resolution, time step and time axis length, source and signal (wavelength),
+and monitors with their own record steps and DFT frequencies (each distinct
+schedule and frequency list is a point group with its own slot row and phasor
+block). A member whose time axis is shorter stops at its end: the engine then
+rebuilds its tile lists without it and the others go on. Each member may have
+at most one source (the engine's descriptor holds one source box). ``autotune``
+picks the kernel shape (K, rows, waves; or the tile height without K steps) by
+timing a few steps of each candidate on the batch itself (HIP events), then
+restores the initial state.
 """
 from __future__ import annotations
 
@@ -36,15 +41,12 @@ class FDTDBatch:
         for s in sims:
             if s.is_3d:
                 raise ValueError("FDTDBatch runs 2D simulations only")
-        t0 = np.asarray(sims[0].time)
-        for s in sims[1:]:
-            if len(s.time) != len(t0) or not np.array_equal(np.asarray(s.time), t0):
-                raise ValueError("all simulations of a batch must share one time axis")
         self.sims = list(sims)
+        self.lengths = np.array([len(s.time) for s in sims], np.int64)   # steps of each member's time axis
         self.seg, self.graph, self.chunk = seg, bool(graph), int(chunk)
         self.pairs, self.seg2 = pairs, seg2      # pairs None: the engine decides by batch size
         self.kstep, self.krows, self.kwaves = kstep, krows, kwaves    # K steps per pass (0: off, None: auto)
-        self.num_steps = len(t0)
+        self.num_steps = int(self.lengths.max())
         self.current_step = 0
         self.engine = None
 
@@ -83,6 +85,7 @@ class FDTDBatch:
         for buf in (self.engine.series, self.engine.dft):
             if buf is not None:
                 buf.zero_()
+        self.engine.set_active(None)
         self.current_step = 0
 
     def advance(self, count):
@@ -93,13 +96,16 @@ class FDTDBatch:
         first = True
         while self.current_step < end:
             n = self.current_step
-            c = min(self.chunk, end - n)
+            # members whose time axis has ended stay where they are; a run stops where the next one ends
+            self.engine.set_active(self.lengths > n)
+            stop = min(end, int(self.lengths[self.lengths > n].min()))
+            c = min(self.chunk, stop - n)
             self.engine.run(n, c, coef=first)      # materials can only change between calls
             first = False
             self.current_step += c
-        for s in self.sims:
-            s.current_step = self.current_step
-            s.t = self.current_step * s.dt
+        for s, ln in zip(self.sims, self.lengths):
+            s.current_step = min(self.current_step, int(ln))
+            s.t = s.current_step * s.dt
 
     def run(self, steps=None):
         """Initialise, run ``steps`` (default: the whole time axis) and finalise every member."""

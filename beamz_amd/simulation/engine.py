@@ -622,6 +622,7 @@ def kstep_owner_tables(tiles, pts, dims, K, rh):
     tiles = np.asarray(tiles, np.int64).reshape(-1, 3)
     pts = np.asarray(pts, np.int64).reshape(-1, 3)
     ix, iy = 64 - 2 * K, rh - 2 * K
+    # a point of a simulation without tiles (one the batch no longer advances) has no owner and is not staged
     sim, comp = pts[:, 0], pts[:, 1]
     nxs, nys, lds = (dims[sim, c] for c in range(3))
     j, i = np.divmod(pts[:, 2], lds)
@@ -629,9 +630,10 @@ def kstep_owner_tables(tiles, pts, dims, K, rh):
         raise ValueError("a monitor point lies outside its grid")
     tx, ty = i // ix, j // iy
     index = {(int(a), int(b), int(c)): q for q, (a, b, c) in enumerate(tiles) if a >= 0}
-    t = np.array([index[(int(a), int(b), int(c))] for a, b, c in zip(sim, tx, ty)], np.int64)
+    t = np.array([index.get((int(a), int(b), int(c)), -1) for a, b, c in zip(sim, tx, ty)], np.int64)
     code = ((j - (ty * iy - K)) << 10) | ((i - (tx * ix - K)) << 2) | comp
-    order = np.lexsort((comp, t))
+    own = np.nonzero(t >= 0)[0]
+    order = own[np.lexsort((comp[own], t[own]))]
     ents = np.stack([order, code[order]], axis=1).astype(np.int32)
     start = np.searchsorted(t[order], np.arange(len(tiles) + 1)).astype(np.int32)
     return start, ents
@@ -704,24 +706,32 @@ class Fused2D:
             if 4 * g.ny * g.ld >= 2 ** 32:
                 raise ValueError("a 2D array exceeds 4 GiB (the engine uses 32-bit byte offsets)")
         plans = [(s, p) for s, g in enumerate(grids) for p in g.monitors if p.n > 0] if monitors else []
-        if plans:
-            p0 = plans[0][1]
-            for _, p in plans[1:]:
-                if not np.array_equal(p.record_steps, p0.record_steps) or p.nfreq != p0.nfreq:
-                    raise NotImplementedError("monitors of a fused 2D run must share their record steps and DFT")
-                if p.nfreq and not T.equal(p.phasor, p0.phasor):
-                    raise NotImplementedError("monitors of a fused 2D run must share their DFT phasors")
+        # point groups: monitors with the same record steps and DFT phasors share one slot row and phasor block
+        groups, pgroup = [], []
+        for _, p in plans:
+            for gi, q in enumerate(groups):
+                if (np.array_equal(p.record_steps, q.record_steps) and p.nfreq == q.nfreq
+                        and (not p.nfreq or T.equal(p.phasor, q.phasor))):
+                    pgroup.append(gi)
+                    break
+            else:
+                pgroup.append(len(groups))
+                groups.append(p)
         self.keep = []
-        self.signal_len = 1 << 62
-        self.descs = [self._describe(g, sn) for g, sn in zip(grids, self.snapshots)]
+        self.active = None                    # None: every grid advances; else a bool per grid (set_active)
+        self.signal_lens = {}                 # grid index -> samples of its source signal
+        self.descs = []
+        for g, sn in zip(grids, self.snapshots):
+            self.descs.append(self._describe(g, sn))
         self.desc_dev = T.zeros(len(grids) * C.sizeof(N.Sim2D), dtype=T.uint8, device=self.dev)
         self._desc_cache = {}
         self.shift = None
         self.set_seg(seg)
         self.step = T.zeros(1, dtype=T.int32, device=self.dev)
         self.npts, self.nfreq = 0, 0
-        self.pts = self.slot = self.series = self.dft = self.phasor = None
-        self.nslot = 0
+        self.pts = self.slot = self.series = self.dft = self.phasor = self.grp = None
+        self.nslot = self.nrow = 0
+        self.groups = [q.record_steps for q in groups]
         if plans:
             ntot = sum(p.n for _, p in plans)
             sim = np.concatenate([np.full(p.n, s, np.int32) for s, p in plans])
@@ -731,22 +741,34 @@ class Fused2D:
                 raise ValueError("monitor offsets exceed 32 bits")
             pts = np.stack([sim, comp, off.astype(np.int32)], axis=1).astype(np.int32)
             self.pts = T.from_numpy(pts.ravel()).to(self.dev)
-            rs = self.record_steps = p0.record_steps
-            need = max(int(rs.max()) + 1 if len(rs) else 1, int(max_steps or 0), 1)
+            if len(groups) > 1:
+                self.grp = T.from_numpy(np.repeat(np.asarray(pgroup, np.int32), [p.n for _, p in plans])).to(self.dev)
+            last = max((int(rs.max()) + 1 for rs in self.groups if len(rs)), default=1)
+            need = max(last, int(max_steps or 0), 1)
             self._make_slot(need)
-            nrec = max(len(rs), 1)
-            keep = all(p.series is not None for _, p in plans)
-            self.series = T.zeros((nrec, ntot), dtype=T.float32, device=self.dev) if keep else None
-            self.dft = T.zeros((p0.nfreq, ntot, 2), dtype=T.float64, device=self.dev) if p0.nfreq else None
-            self.phasor = p0.phasor
+            self.nrow = max(max(len(rs) for rs in self.groups), 1)
+            self.nfreq = max(q.nfreq for q in groups)
+            if self.nfreq:
+                if len(groups) == 1:
+                    self.phasor = groups[0].phasor
+                else:
+                    # [group][row][frequency] complex; frequencies a group lacks stay 0
+                    self.phasor = T.zeros((len(groups), self.nrow, 2 * self.nfreq), dtype=T.float64,
+                                          device=self.dev)
+                    for gi, q in enumerate(groups):
+                        if q.nfreq:
+                            self.phasor[gi, :len(q.record_steps), :2 * q.nfreq] = q.phasor
+            keep = any(p.series is not None for _, p in plans)
+            self.series = T.zeros((self.nrow, ntot), dtype=T.float32, device=self.dev) if keep else None
+            self.dft = T.zeros((self.nfreq, ntot, 2), dtype=T.float64, device=self.dev) if self.nfreq else None
             c0 = 0
             for _, p in plans:
-                if keep:
-                    p.series = self.series[:, c0:c0 + p.n]
-                if p0.nfreq:
-                    p.dft = self.dft[:, c0:c0 + p.n, :]
+                if p.series is not None:
+                    p.series = self.series[:max(len(p.record_steps), 1), c0:c0 + p.n]
+                if p.nfreq:
+                    p.dft = self.dft[:p.nfreq, c0:c0 + p.n, :]
                 c0 += p.n
-            self.npts, self.nfreq = ntot, p0.nfreq
+            self.npts = ntot
         self.use_graph = bool(graph)
         self.graphs = {}
         if self.pol == "Hz":
@@ -841,6 +863,29 @@ class Fused2D:
         self.set_seg2(best)
         return best, timings
 
+    def is_active(self, s):
+        return self.active is None or bool(self.active[s])
+
+    def _advancing(self):
+        return [g for s, g in enumerate(self.grids) if self.is_active(s)]
+
+    def set_active(self, active):
+        """Advance only the grids whose entry is true (None: all), e.g. when batch members with shorter time
+        axes have finished: the tile lists are rebuilt without the others, whose fields, monitor sums and
+        buffer parity then stay as they are (their record schedules have no rows beyond their own steps)."""
+        active = None if active is None or all(active) else [bool(a) for a in active]
+        if active == self.active:
+            return
+        if active is not None and not any(active):
+            raise ValueError("set_active needs at least one active grid")
+        self.active = active
+        self.set_seg(self.seg)
+        if self.pairs:
+            self.set_seg2(self.seg2)
+        if self.kstep:
+            self.set_kstep(self.kstep, self.krows, self.kwaves)
+        self.graphs = {}
+
     def set_seg2(self, seg2):
         """Tile list of the two-step kernel: tiles of 60 columns x seg2 rows."""
         T = self.torch
@@ -849,6 +894,8 @@ class Fused2D:
             raise ValueError("pair tiles need at least 2 rows")
         rows = []
         for s, g in enumerate(self.grids):
+            if not self.is_active(s):
+                continue
             gx = (g.nx + 59) // 60
             gy = (g.ny + self.seg2 - 1) // self.seg2
             t = np.stack(np.meshgrid(np.arange(gx), np.arange(gy), indexing="xy"), -1).reshape(-1, 2)
@@ -915,6 +962,8 @@ class Fused2D:
         ix, iy = 64 - 2 * K, rows * waves - 2 * K
         parts = []
         for s, g in enumerate(self.grids):
+            if not self.is_active(s):
+                continue
             gx, gy = -(-g.nx // ix), -(-g.ny // iy)
             t = np.stack(np.meshgrid(np.arange(gx), np.arange(gy), indexing="xy"), -1).reshape(-1, 2)
             parts.append(np.concatenate([np.full((len(t), 1), s), t], axis=1))
@@ -970,7 +1019,7 @@ class Fused2D:
             d.src_add = dadd.data_ptr()
             d.src_mul = dmul.data_ptr() if dmul is not None else None
             d.signal = dsig.data_ptr()
-            self.signal_len = min(self.signal_len, len(s.signal))
+            self.signal_lens[len(self.descs)] = len(s.signal)
         d.power = g.power.data_ptr() if g.power is not None else None
         if snap is not None:
             x0, y0, sx, sy, buf = snap
@@ -1003,6 +1052,8 @@ class Fused2D:
         self.seg = seg
         rows = []
         for s, g in enumerate(self.grids):
+            if not self.is_active(s):
+                continue
             gx = (g.nx + self.TILE_X - 1) // self.TILE_X
             gy = (g.ny + self.seg - 1) // self.seg
             t = np.stack(np.meshgrid(np.arange(gx), np.arange(gy), indexing="xy"), -1).reshape(-1, 2)
@@ -1015,12 +1066,13 @@ class Fused2D:
         self.graphs = {}
 
     def _make_slot(self, need):
+        """slot[g][n]: the row of step n in point group g's record numbering, or -1."""
         T = self.torch
-        rs = self.record_steps
-        slot = np.full(need, -1, np.int32)
-        rs = rs[rs < need]
-        slot[rs] = np.arange(len(rs), dtype=np.int32)
-        self.slot = T.from_numpy(slot).to(self.dev)
+        slot = np.full((len(self.groups), need), -1, np.int32)
+        for g, rs in enumerate(self.groups):
+            rs = rs[rs < need]
+            slot[g, rs] = np.arange(len(rs), dtype=np.int32)
+        self.slot = T.from_numpy(slot.ravel()).to(self.dev)
         self.nslot = need
         self.graphs = {}
 
@@ -1062,15 +1114,15 @@ class Fused2D:
         ptr = lambda t: t.data_ptr() if t is not None else None
         N.call("bz_yee2d_runk", self.desc_dev.data_ptr(), len(self.grids), self.tilesk.data_ptr(), self.ntilesk,
                self.krows, self.kwaves, self.kstep, self.step.data_ptr(), rd0, nl, ptr(self.ent_start),
-               ptr(self.ents), ptr(self.stagek), self.npts, ptr(self.slot), ptr(self.series), self.nfreq,
-               ptr(self.phasor), ptr(self.dft), self.grids[0].stream)
+               ptr(self.ents), ptr(self.stagek), self.npts, ptr(self.slot), ptr(self.grp), self.nslot, self.nrow,
+               ptr(self.series), self.nfreq, ptr(self.phasor), ptr(self.dft), self.grids[0].stream)
 
     def _launch(self, count):
         ptr = lambda t: t.data_ptr() if t is not None else None
         g0 = self.grids[0]
         N.call("bz_yee2d_hz_run" if self.pol == "Hz" else "bz_yee2d_run", self.desc_dev.data_ptr(), len(self.grids), self.tiles.data_ptr(), self.ntiles,
-               self.seg, self.step.data_ptr(), count, ptr(self.pts), self.npts, ptr(self.slot), ptr(self.series),
-               self.nfreq, ptr(self.phasor), ptr(self.dft), g0.stream)
+               self.seg, self.step.data_ptr(), count, ptr(self.pts), self.npts, ptr(self.slot), ptr(self.grp),
+               self.nslot, self.nrow, ptr(self.series), self.nfreq, ptr(self.phasor), ptr(self.dft), g0.stream)
 
     def run(self, n0, count, coef=True):
         """Advance every grid by ``count`` steps starting at step index n0. coef=False skips recomputing the
@@ -1079,8 +1131,10 @@ class Fused2D:
         if count <= 0:
             return
         T = self.torch
-        if n0 < 0 or n0 + count > self.signal_len:
-            raise ValueError(f"steps {n0}..{n0 + count - 1} run past the source signal ({self.signal_len} samples)")
+        for s, ln in self.signal_lens.items():
+            if self.is_active(s) and (n0 < 0 or n0 + count > ln):
+                raise ValueError(f"steps {n0}..{n0 + count - 1} run past the source signal of simulation {s} "
+                                 f"({ln} samples)")
         if count > getattr(self, "snap_steps", 1 << 62):
             raise ValueError(f"a run of {count} steps overflows the snapshot buffers ({self.snap_steps} slots)")
         if self.npts and n0 + count > self.nslot:
@@ -1119,7 +1173,7 @@ class Fused2D:
                 gr.replay()
             else:
                 self._launch_k(nl, n0 & 1)
-            for g in self.grids:
+            for g in self._advancing():
                 g._cur = (g._cur + nl) % 2
             n0, count = n0 + nl * K, count - nl * K
             if count:
@@ -1132,9 +1186,9 @@ class Fused2D:
             ptr = lambda t: t.data_ptr() if t is not None else None
             N.call("bz_yee2d_run2", self.desc_dev.data_ptr(), len(self.grids), self.tiles2.data_ptr(),
                    self.ntiles2, self.seg2, self.step.data_ptr(), n0 & 1, npairs, ptr(self.pts), ptr(self.spts),
-                   self.npts, ptr(self.slot), ptr(self.series), self.nfreq, ptr(self.phasor), ptr(self.dft),
-                   g0.stream)
-            for g in self.grids:
+                   self.npts, ptr(self.slot), ptr(self.grp), self.nslot, self.nrow, ptr(self.series), self.nfreq,
+                   ptr(self.phasor), ptr(self.dft), g0.stream)
+            for g in self._advancing():
                 g._cur = (g._cur + npairs) % 2
             rem = count - 2 * npairs
             if rem:
@@ -1161,7 +1215,7 @@ class Fused2D:
         if g0.timing:
             e1.record()
             g0._events.append((e0, e1, total))
-        for g in self.grids:
+        for g in self._advancing():
             g._cur = (g._cur + rem) % 2
             if g.power is not None:
                 g.power_count += total

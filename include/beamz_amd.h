@@ -171,6 +171,10 @@ int bz_yee3d_step2_noalloc(float* const src[6], float* const dst[6], const float
  * pts: device int[3*npts] = (sim, comp 0 Ez / 1 Hx / 2 Hy, offset); slot: device int[steps],
  * slot[n] = recorded row of step n or -1; series [rows][npts] FP32 (may be NULL);
  * phasor [rows][nfreq] complex FP64; dft [nfreq][npts] complex FP64 accumulators.
+ * Point groups (monitors with their own record steps and DFT frequencies, in one batch): with grp (device
+ * int[npts], the group of each point) not NULL, slot is [groups][nslot] (slot[g*nslot + n]: row of step n
+ * in group g's own numbering, or -1) and phasor is [groups][nrow][nfreq] (a group with fewer frequencies
+ * holds 0 in the others, whose accumulators then stay 0). grp NULL: one group, nslot and nrow unused.
  * Snapshot box (snap_sx > 0): the q-th step of the call (q = 0..nsteps-1) stores, for every cell
  * (j, i) of the box, the Ez it read and the Hx, Hy it wrote, at
  * snap[((q*3 + c)*snap_sy + j - snap_y0)*snap_sx + i - snap_x0], c = 0 Ez, 1 Hx, 2 Hy
@@ -198,8 +202,8 @@ typedef struct bz_sim2d {
 } bz_sim2d;
 
 int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step, int nsteps,
-                 const int* pts, int npts, const int* slot, float* series, int nfreq, const double* phasor,
-                 double* dft, void* stream);
+                 const int* pts, int npts, const int* slot, const int* grp, int nslot, int nrow, float* series,
+                 int nfreq, const double* phasor, double* dft, void* stream);
 
 /* The (Hz, Ex, Ey) polarisation: nsteps steps of the z-invariant restriction of fdtd.py:200-310 (the
  * reference's 3D update with every d/dz = 0, kept to the components Hz (ny-1, nx-1), Ex (ny, nx-1),
@@ -209,8 +213,8 @@ int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, 
  * the E update; grid power accumulates Sx^2 + Sy^2 with (Sx, Sy) = (Ey Hz, -Ex Hz) on the Hz cells' box.
  * No snapshot boxes. */
 int bz_yee2d_hz_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step,
-                    int nsteps, const int* pts, int npts, const int* slot, float* series, int nfreq,
-                    const double* phasor, double* dft, void* stream);
+                    int nsteps, const int* pts, int npts, const int* slot, const int* grp, int nslot, int nrow,
+                    float* series, int nfreq, const double* phasor, double* dft, void* stream);
 
 /* Two steps per pass: npairs pairs of the bz_yee2d_run step (same results, bit for bit), pair q reading
  * buffer (rd0 + q) & 1 and writing the other, so after the call the state sits in buffer
@@ -219,8 +223,8 @@ int bz_yee2d_hz_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntile
  * simulation's staging box: spts[p] = offset of point p inside its box (-1: not in it, reads 0).
  * Snapshot boxes as for bz_yee2d_run: pair q writes slots 2q and 2q+1. */
 int bz_yee2d_run2(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step, int rd0,
-                  int npairs, const int* pts, const int* spts, int npts, const int* slot, float* series,
-                  int nfreq, const double* phasor, double* dft, void* stream);
+                  int npairs, const int* pts, const int* spts, int npts, const int* slot, const int* grp,
+                  int nslot, int nrow, float* series, int nfreq, const double* phasor, double* dft, void* stream);
 
 /* The update coefficients of every cell of every simulation whose coef is not NULL, from its eps and sig:
  * s = sig*kh, fa = (1-s)/(1+s), fb = kbh/(1+s) (H); s = sig*ke/eps, ca = (1-s)/(1+s), cb = kbe/eps/(1+s)
@@ -244,7 +248,8 @@ int bz_abi_sizes(int* out, int n);
  * step k of launch q writes slot q*ksteps + k. No grid power (use bz_yee2d_run / run2). */
 int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int rows, int waves, int ksteps,
                   int* step, int rd0, int nlaunch, const int* ent_start, const int* ents, float* stage, int npts,
-                  const int* slot, float* series, int nfreq, const double* phasor, double* dft, void* stream);
+                  const int* slot, const int* grp, int nslot, int nrow, float* series, int nfreq,
+                  const double* phasor, double* dft, void* stream);
 
 /* ---------------- reverse-time 2D adjoint ----------------
  * No reference counterpart (the reference's gradient is the overlap heuristic of

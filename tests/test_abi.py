@@ -115,3 +115,18 @@ def test_kstep_shapes_are_valid():
     out = xcd_order(tiles, per_wg=1)
     real = out[out[:, 0] >= 0]
     assert len(out) == len(tiles) and sorted(map(tuple, real)) == sorted(map(tuple, tiles))
+
+
+def test_library_shares_torch_hip_runtime():
+    """Loading the library first (a mesh rasterised before any GPU work) must not map a second HIP runtime
+    next to torch's: the library's launches would then fail with "no ROCm-capable device"."""
+    import subprocess
+    import sys
+    code = ("import os, sys; sys.path.insert(0, os.getcwd())\n"
+            "from beamz_amd import _native\n_native.lib()\nimport torch\n"
+            "print(sorted({l.split()[-1] for l in open('/proc/self/maps') if 'amdhip64' in l}))\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=root, timeout=300)
+    assert out.returncode == 0, out.stderr
+    libs = eval(out.stdout.strip().splitlines()[-1])
+    assert len(libs) == 1, libs

@@ -65,6 +65,63 @@ def test_batch_equals_single_runs_and_oracle(hip):
     assert T.shape == (8,) and np.all(np.isfinite(T))
 
 
+def varied_members(seed=5):
+    """Members that differ in everything but the polarisation: side, resolution (so time step), time axis
+    length, DFT frequency count, record interval and whether series are kept."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for n, steps, dx, nf, every, keep in ((96, 300, 25e-9, 8, 1, True), (131, 221, 30e-9, 5, 3, False),
+                                          (170, 262, 25e-9, 3, 2, True)):
+        mesh, t, mons = batch_member_2d(n, rng, steps=steps, dx=dx, n_freq=nf, design_um=1.0)
+        mons[1].record_interval = every
+        mons[1].should_record_fields = keep
+        out.append((bz.FDTD(mesh, time=t), mons))
+    return out
+
+
+def varied_state(sim, mons):
+    d = state(sim, mons)
+    for m in mons:
+        for k, v in m.fields.items():
+            if len(v):
+                d[m.name + "series" + k] = np.array(v)
+    return d
+
+
+@pytest.mark.parametrize("kstep,pairs", [(None, None), (0, True), (0, False)])
+def test_batch_members_with_their_own_time_axes_and_monitors(hip, kstep, pairs):
+    """Members with different time steps, time axis lengths (a member that ends first stops there while the
+    others go on), DFT frequency lists and record intervals take the bits they take alone, through the K-step,
+    two-step and one-step kernels; one member against the FP64 oracle."""
+    single = []
+    for sim, mons in varied_members():
+        sim.grid.kstep2d = 0
+        sim.run(live=False, save=False)
+        single.append(varied_state(sim, mons))
+    mem = varied_members()
+    batch = FDTDBatch([s for s, _ in mem], seg=8, chunk=50, kstep=kstep, pairs=pairs)
+    batch.run()
+    # point groups: member 0's two monitors share one; members 1 and 2 have their own time steps or frequency
+    # counts, and their output monitors their own record intervals
+    assert len(batch.engine.groups) == 5 and batch.engine.grp is not None
+    assert (batch.engine.kstep > 0) == (kstep is None)
+    assert sum(any("series" in k for k in ref) for ref in single) == 2     # members 0 and 2 keep series
+    for (sim, mons), ref, ln in zip(mem, single, (300, 221, 262)):
+        assert sim.current_step == ln
+        got = varied_state(sim, mons)
+        assert set(got) == set(ref), (sim.nx, sorted(set(got) ^ set(ref)))
+        for k in ref:
+            assert np.array_equal(got[k], ref[k]), (sim.nx, k)
+    sim, mons = mem[1]
+    ref = R.RefSim(sim.epsilon_r, sim.sigma, sim.dx, sim.dy, sim.dt, sim.num_steps, sources=sim.sources,
+                   monitors=list(mons)).run()
+    for k in ("Ez", "Hx", "Hy"):
+        assert rel_l2(np.real(getattr(sim, k)), ref.F[k]) < 1e-4, k
+    m = mons[0]                               # records every step, as the oracle does
+    F = R.dft_of_series(np.array(ref.mon_series[0]["Ez"]), ref.mon_series[0]["t"], 2 * np.pi * m.dft_frequencies)
+    assert rel_l2(m.dft["Ez"], F) < 1e-4
+
+
 def test_batch_graph_replays_equal_eager(hip):
     torch = hip
     a = members(steps=320)
