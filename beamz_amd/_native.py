@@ -43,7 +43,8 @@ class Sim2D(C.Structure):
                 ("rdx", f32), ("rdy", f32), ("src_x0", i32), ("src_y0", i32), ("src_sx", i32), ("src_sy", i32),
                 ("src_add", vp), ("src_mul", vp), ("signal", vp), ("power", vp),
                 ("snap_x0", i32), ("snap_y0", i32), ("snap_sx", i32), ("snap_sy", i32), ("snap", vp),
-                ("stg", vp), ("stg_x0", i32), ("stg_y0", i32), ("stg_sx", i32), ("stg_sy", i32), ("coef", vp)]
+                ("stg", vp), ("stg_x0", i32), ("stg_y0", i32), ("stg_sx", i32), ("stg_sy", i32), ("coef", vp),
+                ("src_sel", vp), ("src_sig_len", i32)]
 
 
 class Adj2D(C.Structure):

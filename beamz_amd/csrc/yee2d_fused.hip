@@ -90,6 +90,12 @@ __device__ __forceinline__ void gstore(T* p, long i, T v) {
     ((BZ_GLOBAL T*)p)[i] = v;
 }
 
+// Amplitude of step n for cell rr of the source box: that cell's own source when the box holds several
+// (disjoint) sources, else the single signal's value amp.
+__device__ __forceinline__ float box_amp(const bz_sim2d& S, long rr, int n, float amp) {
+    return S.src_sel ? gload(S.signal, (long)gload(S.src_sel, rr) * S.src_sig_len + n) : amp;
+}
+
 // Monitor sample p of step n, read from the buffer step n wrote.
 __device__ __forceinline__ void gather_point(const bz_sim2d* __restrict__ sims, const Gather2D& G, int p, int n) {
     if (p >= G.npts) return;
@@ -198,7 +204,7 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee2d_fused(const bz_sim2d* __re
             const int a = i - S.src_x0, b = j - S.src_y0;
             if (a >= 0 && a < S.src_sx && b >= 0 && b < S.src_sy) {
                 const long rr = (long)b * S.src_sx + a;
-                ezn = (S.src_mul ? S.src_mul[rr] : 1.0f) * ezn + S.src_add[rr] * amp;
+                ezn = (S.src_mul ? S.src_mul[rr] : 1.0f) * ezn + S.src_add[rr] * box_amp(S, rr, n, amp);
             }
             put(oez, ro[r], ezn);
             put(ohx, ro[r], hxn[r]);
@@ -310,7 +316,7 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee2d_hz(const bz_sim2d* __restr
             const int a = i - S.src_x0, b = j - S.src_y0;
             if (a >= 0 && a < S.src_sx && b >= 0 && b < S.src_sy) {
                 const long rr = (long)b * S.src_sx + a;
-                hzo = (S.src_mul ? S.src_mul[rr] : 1.0f) * hzo + S.src_add[rr] * amp;
+                hzo = (S.src_mul ? S.src_mul[rr] : 1.0f) * hzo + S.src_add[rr] * box_amp(S, rr, n, amp);
             }
             put(ohz, ro[r], hzo);
             put(oex, ro[r], exn);
@@ -396,11 +402,11 @@ __device__ __forceinline__ void snap_put(const bz_sim2d& S, int slot, int i, int
 // Same rounded operations as k_yee2d_fused (no contraction in this file), so the results are identical.
 constexpr int OX2 = LANES - 4;
 
-__device__ __forceinline__ void box_src(float& e, const bz_sim2d& S, int i, int j, float amp) {
+__device__ __forceinline__ void box_src(float& e, const bz_sim2d& S, int i, int j, int n, float amp) {
     const int a = i - S.src_x0, b = j - S.src_y0;
     if (a >= 0 && a < S.src_sx && b >= 0 && b < S.src_sy) {
         const long r = (long)b * S.src_sx + a;
-        e = (S.src_mul ? S.src_mul[r] : 1.0f) * e + S.src_add[r] * amp;
+        e = (S.src_mul ? S.src_mul[r] : 1.0f) * e + S.src_add[r] * box_amp(S, r, n, amp);
     }
 }
 
@@ -480,7 +486,7 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee2d_tb2(const bz_sim2d* __rest
         float ca, cb;
         e_coef(sg, ee, c, ca, cb);
         float e1 = (r >= 1 && r <= ny - 2 && xin) ? ca * ez0 + cb * (-((h1x - h1xL) * rdy) + (h1y - h1yp) * rdx) : ez0;
-        if (S.src_sx) box_src(e1, S, i, r, amp0);
+        if (S.src_sx) box_src(e1, S, i, r, n, amp0);
         // step n+2 on row m = r - 1 (step n+1 of that row is e1p, h1xp, h1yp)
         const int m = r - 1;
         const float e1pR = __shfl_down(e1p, 1, LANES);
@@ -494,7 +500,7 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee2d_tb2(const bz_sim2d* __rest
         float e2 = (m >= 1 && m <= ny - 2 && xin) ? ca2 * e1p + cb2 * (-((h2x - h2xL) * rdy) + (h2y - h2yp) * rdx)
                                                   : e1p;
         if (out && m >= j0 && m < j1) {
-            if (S.src_sx) box_src(e2, S, i, m, amp1);
+            if (S.src_sx) box_src(e2, S, i, m, n + 1, amp1);
             const uint32_t om = off(m);
             put(oez, om, e2);
             put(ohx, om, h2x);
@@ -789,7 +795,7 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
                 const int j = jb + r;
                 if (j >= sy0 && j < sy0 + ssy && src_col) {
                     const long rr = (long)(j - sy0) * ssx + (i - sx0);
-                    ez[r] = (smul ? gload(smul, rr) : 1.0f) * ez[r] + gload(sadd, rr) * amp;
+                    ez[r] = (smul ? gload(smul, rr) : 1.0f) * ez[r] + gload(sadd, rr) * box_amp(S, rr, n0 + k, amp);
                 }
             }
         }

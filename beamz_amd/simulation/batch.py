@@ -18,8 +18,9 @@ resolution, time step and time axis length, source and signal (wavelength),
 and monitors with their own record steps and DFT frequencies (each distinct
 schedule and frequency list is a point group with its own slot row and phasor
 block). A member whose time axis is shorter stops at its end: the engine then
-rebuilds its tile lists without it and the others go on. Each member may have
-at most one source (the engine's descriptor holds one source box). ``autotune``
+rebuilds its tile lists without it and the others go on. A member may have
+several sources on disjoint cells (one source box per member, each cell
+reading its own source's signal). ``autotune``
 picks the kernel shape (K, rows, waves; or the tile height without K steps) by
 timing a few steps of each candidate on the batch itself (HIP events), then
 restores the initial state.

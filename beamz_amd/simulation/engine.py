@@ -701,8 +701,8 @@ class Fused2D:
                 raise ValueError("all grids of a batch must be on one device")
             if g.pol != self.pol:
                 raise ValueError("all grids of a batch must share one polarisation")
-            if len(g.sources) > 1:
-                raise NotImplementedError("the fused 2D engine takes at most one source per simulation")
+            if len(g.sources) > 255:
+                raise NotImplementedError("the fused 2D engine takes at most 255 sources per simulation")
             if 4 * g.ny * g.ld >= 2 ** 32:
                 raise ValueError("a 2D array exceeds 4 GiB (the engine uses 32-bit byte offsets)")
         plans = [(s, p) for s, g in enumerate(grids) for p in g.monitors if p.n > 0] if monitors else []
@@ -1001,25 +1001,39 @@ class Fused2D:
         d.kbh, d.ke, d.kbe = g.dt / MU_0, g.dt / (2.0 * EPS_0), g.dt / EPS_0
         d.rdx, d.rdy = 1.0 / g.dx, 1.0 / g.dy
         d.src_sx = 0
+        d.src_sel, d.src_sig_len = None, 0
         if g.sources:
-            s = g.sources[0]
-            idx = s.idx
+            # one box holds every source of the grid; with several, each cell records whose it is (the sources
+            # must touch disjoint cells: each cell then takes the one affine update it takes alone)
+            srcs = g.sources
+            idx = np.concatenate([s.idx for s in srcs])
+            if len(srcs) > 1 and len(np.unique(idx[:, 0] * (g.nx + 1) + idx[:, 1])) != len(idx):
+                raise NotImplementedError("the fused 2D engine needs the sources of a simulation on disjoint cells")
             lo = idx.min(axis=0); hi = idx.max(axis=0) + 1
             sy, sx = (int(v) for v in (hi - lo))
             add = np.zeros((sy, sx), np.float32); mul = np.ones((sy, sx), np.float32)
-            rel = idx - lo
-            add[rel[:, 0], rel[:, 1]] = s.w_host
-            if s.mul_host is not None:
-                mul[rel[:, 0], rel[:, 1]] = s.mul_host
+            sel = np.zeros((sy, sx), np.uint8)
+            for q, s in enumerate(srcs):
+                rel = s.idx - lo
+                add[rel[:, 0], rel[:, 1]] = s.w_host
+                sel[rel[:, 0], rel[:, 1]] = q
+                if s.mul_host is not None:
+                    mul[rel[:, 0], rel[:, 1]] = s.mul_host
+            has_mul = any(s.mul_host is not None for s in srcs)
             dadd = T.from_numpy(add.ravel()).to(self.dev)
-            dmul = T.from_numpy(mul.ravel()).to(self.dev) if s.mul_host is not None else None
-            dsig = T.from_numpy(np.real(s.signal).astype(np.float32)).to(self.dev)
-            self.keep += [dadd, dmul, dsig]
+            dmul = T.from_numpy(mul.ravel()).to(self.dev) if has_mul else None
+            ln = min(len(s.signal) for s in srcs)
+            sig = np.stack([np.real(s.signal[:ln]).astype(np.float32) for s in srcs])
+            dsig = T.from_numpy(sig.ravel()).to(self.dev)
+            dsel = T.from_numpy(sel.ravel()).to(self.dev) if len(srcs) > 1 else None
+            self.keep += [dadd, dmul, dsig, dsel]
             d.src_y0, d.src_x0, d.src_sy, d.src_sx = int(lo[0]), int(lo[1]), sy, sx
             d.src_add = dadd.data_ptr()
             d.src_mul = dmul.data_ptr() if dmul is not None else None
             d.signal = dsig.data_ptr()
-            self.signal_lens[len(self.descs)] = len(s.signal)
+            if dsel is not None:
+                d.src_sel, d.src_sig_len = dsel.data_ptr(), ln
+            self.signal_lens[len(self.descs)] = ln
         d.power = g.power.data_ptr() if g.power is not None else None
         if snap is not None:
             x0, y0, sx, sy, buf = snap

@@ -199,6 +199,10 @@ typedef struct bz_sim2d {
     int stg_x0, stg_y0, stg_sx, stg_sy;   /* Ez, Hx, Hy for monitor samples, slot q & 1 for pair q (NULL: none) */
     float* coef;                          /* NULL, or [ny][ld][4] FP32 update coefficients (fa, fb, ca, cb) of every
                                            * cell as bz_yee2d_coef writes them (read by bz_yee2d_runk) */
+    const unsigned char* src_sel;         /* NULL (one source), or per source-box cell the index of the source it
+                                           * belongs to: several sources on disjoint cells in one box, whose
+                                           * amplitude at step n is signal[src_sel[cell] * src_sig_len + n] */
+    int src_sig_len;                      /* samples per source in signal when src_sel is set */
 } bz_sim2d;
 
 int bz_yee2d_run(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int seg, int* step, int nsteps,

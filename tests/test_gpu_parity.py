@@ -193,6 +193,66 @@ def test_mode_source_2d_matches_oracle(hip):
         assert rel_l2(np.real(sim.Ez), ref.F["Ez"]) < 1e-4, enforce
 
 
+def _two_sources():
+    """A 2D waveguide with a ModeSource line (with direction enforcement: its mul zeroes upstream cells) and a
+    Gaussian point source of another wavelength, on disjoint cells, and a DFT line monitor."""
+    um = 1e-6
+    d, dx = waveguide_design_2d(nx=150, ny=80, npml=10)
+    dt = 0.99 * dx / (bz.LIGHT_SPEED * np.sqrt(2))
+    t = np.arange(300) * dt
+    sig = bz.ramped_cosine(t, 1.0, bz.LIGHT_SPEED / (1.55 * um), 0.0, ramp_duration=20 * dt, t_max=t[-1])
+    mode = bz.ModeSource(design=d, start=(20 * dx, d.height / 2 - 1.0 * um), end=(20 * dx, d.height / 2 + 1.0 * um),
+                         wavelength=1.55 * um, signal=sig, num_modes=1, grid_resolution=400)
+    d += mode
+    d += bz.GaussianSource(position=(70 * dx, d.height / 2 + 0.3 * um), width=dx / 2, signal=pulse(t, lam=1.31 * um))
+    mon = bz.Monitor(design=d, start=(120 * dx, 12 * dx), end=(120 * dx, d.height - 12 * dx),
+                     dft_frequencies=bz.LIGHT_SPEED / (np.array([1.31, 1.55]) * um), name="m")
+    d += mon
+    return d, dx, t, mon
+
+
+def test_disjoint_sources_share_one_box_in_every_2d_kernel(hip):
+    """Two sources of one grid in one source box (each cell reading its own source's signal): one step per
+    launch, two steps per launch and K steps per pass give the same bits, alone and inside a batch, and the
+    run matches the FP64 oracle, which applies the sources one after the other."""
+    from beamz_amd.simulation.batch import FDTDBatch
+    out = {}
+    # complex_fields=False: the mode profile is complex, and batch members run real fields (the reference's
+    # real-backend branch, helper.py:29-33)
+    for key in ("one", "pairs", "kstep", "batch"):
+        d, dx, t, mon = _two_sources()
+        sim = bz.FDTD(d, time=t, resolution=dx, complex_fields=False)
+        if key == "one":
+            sim.grid.kstep2d = 0
+            sim.run(live=False, save=False)
+            assert sim.grid._f2d and not sim.grid._f2d.kstep and not sim.grid._f2d.pairs
+        elif key == "kstep":
+            sim.run(live=False, save=False)
+            assert sim.grid._f2d and sim.grid._f2d.kstep
+        else:
+            other = _k_case("gauss")
+            members = [sim, bz.FDTD(other[0], time=other[2], resolution=other[1])]
+            b = FDTDBatch(members, kstep=0 if key == "pairs" else None, pairs=key == "pairs")
+            b.run()
+            assert bool(b.engine.kstep) == (key == "batch")
+        assert len(sim.sources) == 2
+        out[key] = {k: np.real(getattr(sim, k)).copy() for k in ("Ez", "Hx", "Hy")}
+        out[key]["series"] = np.array(mon.fields["Ez"])
+        out[key]["dft"] = np.array(mon.dft["Ez"])
+    for key in out:
+        for k in out["one"]:
+            assert np.array_equal(out[key][k], out["one"][k]), (key, k)
+    d, dx, t, mon = _two_sources()
+    sim = bz.FDTD(d, time=t, resolution=dx, complex_fields=False)
+    sim.run(live=False, save=False)
+    ref = R.RefSim(sim.epsilon_r, sim.sigma, dx, dx, sim.dt, len(t), sources=sim.sources, monitors=[mon]).run()
+    assert np.abs(ref.F["Ez"]).max() > 0
+    for k in ("Ez", "Hx", "Hy"):
+        assert rel_l2(np.real(getattr(sim, k)), ref.F[k]) < 1e-4, k
+    F = R.dft_of_series(np.array(ref.mon_series[0]["Ez"]), ref.mon_series[0]["t"], 2 * np.pi * mon.dft_frequencies)
+    assert rel_l2(mon.dft["Ez"], F) < 1e-4
+
+
 def _k_case(kind):
     """A single 2D grid with one source and one line monitor (series and DFT): a Gaussian point source, or a
     ModeSource line that zeroes its upstream cells."""
