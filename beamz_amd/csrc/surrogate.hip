@@ -58,7 +58,10 @@ __device__ __forceinline__ void pixel(int p, const Geo& g, int& b, int& y, int& 
 // K slices of FK = 32 or 64: thread t stages the 8-wide chunks kq + 32 h (h < FK / 32) of row t / 4 of both
 // tiles (the tap of a chunk is k / Cs; Cs % 8 == 0, so a chunk never straddles two taps, and chunks past K
 // are zero). The next slice's loads are issued before the current slice's MFMAs.
-template <int FK>
+// TAP (Cs % FK == 0): a slice lies within one tap, so the tap and its pixel shift are computed once per slice
+// (uniform) and each lane adds offsets to its pixel's precomputed base, instead of two integer divisions per
+// chunk and lane (the per-lane index arithmetic outweighed the MFMAs several times over).
+template <int FK, bool TAP>
 __global__ __launch_bounds__(256) void k_conv_gemm(const uint16_t* __restrict__ src, const uint16_t* __restrict__ wk,
                                                    const float* __restrict__ bias, uint16_t* __restrict__ out,
                                                    const uint16_t* __restrict__ mask, Geo g, int Cs, int N, int sign,
@@ -76,7 +79,28 @@ __global__ __launch_bounds__(256) void k_conv_gemm(const uint16_t* __restrict__ 
     const int nrow = n0 + pr;
     const int K = 9 * Cs;
     uint4 a[NH], bw[NH];
+    // TAP: this lane's pixel offset (P * Cs < 2^31, checked by the entry points) and weight row
+    const int po = ((pb * g.H + py) * g.W + px) * Cs + kq;
+    const uint16_t* wr = wk + (long)min(nrow, N - 1) * K + kq;
     auto load = [&](int k0) {
+        if (TAP) {
+            const int tap = k0 / Cs;
+            const int ky = tap / 3, kx = tap - 3 * ky;
+            const int dy = sign * (ky - 1), dx = sign * (kx - 1);
+            const int yy = py + dy, xx = px + dx;
+            const bool ok = p < P && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
+            const int cs = k0 - tap * Cs;
+            // every lane loads from a valid address (its own pixel where the tap falls outside) and selects 0
+            const int base = po + cs + (ok ? (dy * g.W + dx) * Cs : 0);
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+                const uint4 v = *reinterpret_cast<const uint4*>(src + base + 32 * h);
+                const uint4 w = *reinterpret_cast<const uint4*>(wr + k0 + 32 * h);
+                a[h] = ok ? v : make_uint4(0, 0, 0, 0);
+                bw[h] = nrow < N ? w : make_uint4(0, 0, 0, 0);
+            }
+            return;
+        }
 #pragma unroll
         for (int h = 0; h < NH; ++h) {
             const int k = k0 + kq + 32 * h;
@@ -1070,7 +1094,8 @@ extern "C" int bz_nn_conv3x3(const uint16_t* x, const uint16_t* w16, const float
         hipLaunchKernelGGL(k_conv_halo<32>, dim3((unsigned)(P / 64), (Co + TN - 1) / TN), dim3(256), 0,
                            (hipStream_t)stream, x, w16, bias, y, (const uint16_t*)nullptr, g, Ci, Co, 1, relu);
     else
-        hipLaunchKernelGGL((9 * Ci) % 64 == 0 ? k_conv_gemm<64> : k_conv_gemm<32>,
+        hipLaunchKernelGGL((Ci % 64 == 0 ? k_conv_gemm<64, true>
+                                         : ((9 * Ci) % 64 == 0 ? k_conv_gemm<64, false> : k_conv_gemm<32, false>)),
                            dim3((unsigned)((P + TM - 1) / TM), (Co + TN - 1) / TN), dim3(256), 0, (hipStream_t)stream,
                            x, w16, bias, y, (const uint16_t*)nullptr, g, Ci, Co, 1, relu);
     BZ_CHECK_LAUNCH("bz_nn_conv3x3");
@@ -1088,6 +1113,10 @@ static int conv_dgrad(const uint16_t* gz, const uint16_t* wt16, const uint16_t* 
         return BZ_EINVAL;
     }
     const long P = (long)B * H * W;
+    if (P * (long)(Ci > Co ? Ci : Co) >= (1L << 31)) {
+        bz::set_error(std::string(name) + ": tensor too large");
+        return BZ_EINVAL;
+    }
     Geo g{B, H, W};
     if (conv_march(gz, wt16, nullptr, gx, mask, B, H, W, Co, Ci, -1, 0, (hipStream_t)stream))
         ;
@@ -1095,7 +1124,8 @@ static int conv_dgrad(const uint16_t* gz, const uint16_t* wt16, const uint16_t* 
         hipLaunchKernelGGL(k_conv_halo<32>, dim3((unsigned)(P / 64), (Ci + TN - 1) / TN), dim3(256), 0,
                            (hipStream_t)stream, gz, wt16, (const float*)nullptr, gx, mask, g, Co, Ci, -1, 0);
     else
-        hipLaunchKernelGGL((9 * Co) % 64 == 0 ? k_conv_gemm<64> : k_conv_gemm<32>,
+        hipLaunchKernelGGL((Co % 64 == 0 ? k_conv_gemm<64, true>
+                                         : ((9 * Co) % 64 == 0 ? k_conv_gemm<64, false> : k_conv_gemm<32, false>)),
                            dim3((unsigned)((P + TM - 1) / TM), (Ci + TN - 1) / TN), dim3(256), 0, (hipStream_t)stream,
                            gz, wt16, (const float*)nullptr, gx, mask, g, Co, Ci, -1, 0);
     BZ_CHECK_LAUNCH(name);
