@@ -968,6 +968,44 @@ __global__ void k_head(const uint16_t* __restrict__ a, const float* __restrict__
     }
 }
 
+// The head with 8 channels per thread: a workgroup of 256 threads per sample splits its pixels into S = 256 / (C / 8)
+// interleaved slices (thread t: channels 8 (t % (C / 8)) .. +7 by 16-byte loads, pixels t / (C / 8) + S k); the
+// slices' sums are added in slice order in LDS (deterministic). C % 8 == 0 and 256 % (C / 8) == 0.
+__global__ __launch_bounds__(256) void k_head8(const uint16_t* __restrict__ a, const float* __restrict__ fw,
+                                               const float* __restrict__ fb, float* __restrict__ gap,
+                                               float* __restrict__ out, int HW, int C, int nout) {
+    extern __shared__ float sh[];
+    const int G = C / 8, S = 256 / G;
+    float* red = sh;             // [S][C]
+    float* sg = sh + S * C;      // [C]
+    const int b = blockIdx.x, t = threadIdx.x, cg = t % G, sl = t / G;
+    float acc[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) acc[e] = 0.0f;
+    const uint16_t* ab = a + (long)b * HW * C + 8 * cg;
+    for (int p = sl; p < HW; p += S) {
+        const uint4 v = *reinterpret_cast<const uint4*>(ab + (long)p * C);
+        const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[e] += b2f(h[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) red[sl * C + 8 * cg + e] = acc[e];
+    __syncthreads();
+    for (int c = t; c < C; c += 256) {
+        float v = 0.0f;
+        for (int q = 0; q < S; ++q) v += red[q * C + c];
+        sg[c] = v / (float)HW;
+        gap[(long)b * C + c] = sg[c];
+    }
+    __syncthreads();
+    for (int j = t; j < nout; j += 256) {
+        float v = fb[j];
+        for (int c = 0; c < C; ++c) v += fw[j * C + c] * sg[c];
+        out[(long)b * nout + j] = v;
+    }
+}
+
 // Head backward: gfw += gout^T gap, gfb += sum gout, ga[b][p][c] = (gout[b] . fw[:, c]) / HW.
 __global__ void k_head_bwd(const float* __restrict__ gout, const float* __restrict__ gap, const float* __restrict__ fw,
                            float* __restrict__ gfw, float* __restrict__ gfb, uint16_t* __restrict__ ga, int HW, int C,
@@ -1414,8 +1452,12 @@ extern "C" int bz_nn_head(const uint16_t* a, const float* fw, const float* fb, f
                           int C, int nout, void* stream) {
     BZ_REQUIRE(a && fw && fb && gap && out && B > 0 && HW > 0 && C > 0 && C <= 4096 && nout > 0,
                "bz_nn_head: bad arguments");
-    hipLaunchKernelGGL(k_head, dim3(B), dim3(256), C * sizeof(float), (hipStream_t)stream, a, fw, fb, gap, out, HW, C,
-                       nout);
+    if (C % 8 == 0 && 256 % (C / 8) == 0)
+        hipLaunchKernelGGL(k_head8, dim3(B), dim3(256), (256 / (C / 8) + 1) * C * sizeof(float), (hipStream_t)stream,
+                           a, fw, fb, gap, out, HW, C, nout);
+    else
+        hipLaunchKernelGGL(k_head, dim3(B), dim3(256), C * sizeof(float), (hipStream_t)stream, a, fw, fb, gap, out, HW,
+                           C, nout);
     BZ_CHECK_LAUNCH("bz_nn_head");
     return BZ_OK;
 }
