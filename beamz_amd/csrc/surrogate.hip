@@ -612,10 +612,10 @@ constexpr int SUM_FAN = 16;
 
 __global__ void k_sum_group(const float* __restrict__ in, int Z, long sin, long count, float* __restrict__ outp,
                             long sout) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long g = idx / count, i = idx - g * count;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;   // element (x), group of rows (y)
+    const long g = blockIdx.y;
     const long z0 = g * SUM_FAN;
-    if (z0 >= Z) return;
+    if (i >= count || z0 >= Z) return;
     const long z1 = min((long)Z, z0 + SUM_FAN);
     float s = in[z0 * sin + i];
     for (long z = z0 + 1; z < z1; ++z) s += in[z * sin + i];
@@ -864,66 +864,60 @@ __global__ void k_relu_mask(uint16_t* __restrict__ gz, const uint16_t* __restric
 }
 
 // 2x2 average pooling and its backward, 8 channels per thread (C % 8 == 0).
+// Thread x of a row handles pixel x / C8 and channels 8 (x % C8) .. +7; rows (b, y) come from blockIdx.y in
+// strides of gridDim.y, so the row's image and line are uniform and no thread divides 64-bit indices.
 __global__ void k_avgpool2_8(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, Geo g, int C) {
     const int Ho = g.H / 2, Wo = g.W / 2, C8 = C / 8;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)g.B * Ho * Wo * C8) return;
-    const int c = (int)(idx % C8) * 8;
-    const long q = idx / C8;
-    const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), b = (int)(q / ((long)Wo * Ho));
-    const long base = (((long)b * g.H + 2 * yo) * g.W + 2 * xo) * C + c;
-    const uint4 v0 = *reinterpret_cast<const uint4*>(x + base);
-    const uint4 v1 = *reinterpret_cast<const uint4*>(x + base + C);
-    const uint4 v2 = *reinterpret_cast<const uint4*>(x + base + (long)g.W * C);
-    const uint4 v3 = *reinterpret_cast<const uint4*>(x + base + (long)g.W * C + C);
-    const uint16_t* h0 = reinterpret_cast<const uint16_t*>(&v0);
-    const uint16_t* h1 = reinterpret_cast<const uint16_t*>(&v1);
-    const uint16_t* h2 = reinterpret_cast<const uint16_t*>(&v2);
-    const uint16_t* h3 = reinterpret_cast<const uint16_t*>(&v3);
-    uint4 o;
-    uint16_t* oh = reinterpret_cast<uint16_t*>(&o);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Wo * C8) return;
+    const int xo = t / C8, c = (t - xo * C8) * 8;
+    for (int row = blockIdx.y; row < g.B * Ho; row += gridDim.y) {
+        const int b = row / Ho, yo = row - b * Ho;
+        const long base = (((long)b * g.H + 2 * yo) * g.W + 2 * xo) * C + c;
+        const uint4 v0 = *reinterpret_cast<const uint4*>(x + base);
+        const uint4 v1 = *reinterpret_cast<const uint4*>(x + base + C);
+        const uint4 v2 = *reinterpret_cast<const uint4*>(x + base + (long)g.W * C);
+        const uint4 v3 = *reinterpret_cast<const uint4*>(x + base + (long)g.W * C + C);
+        const uint16_t* h0 = reinterpret_cast<const uint16_t*>(&v0);
+        const uint16_t* h1 = reinterpret_cast<const uint16_t*>(&v1);
+        const uint16_t* h2 = reinterpret_cast<const uint16_t*>(&v2);
+        const uint16_t* h3 = reinterpret_cast<const uint16_t*>(&v3);
+        uint4 o;
+        uint16_t* oh = reinterpret_cast<uint16_t*>(&o);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) oh[e] = f2b(0.25f * (b2f(h0[e]) + b2f(h1[e]) + b2f(h2[e]) + b2f(h3[e])));
-    *reinterpret_cast<uint4*>(y + q * C + c) = o;
+        for (int e = 0; e < 8; ++e) oh[e] = f2b(0.25f * (b2f(h0[e]) + b2f(h1[e]) + b2f(h2[e]) + b2f(h3[e])));
+        *reinterpret_cast<uint4*>(y + ((long)row * Wo + xo) * C + c) = o;
+    }
 }
 
-__global__ void k_avgpool2_bwd8(const uint16_t* __restrict__ gy, uint16_t* __restrict__ gx, Geo g, int C) {
-    const int C8 = C / 8;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)g.B * g.H * g.W * C8) return;
-    const int c = (int)(idx % C8) * 8;
-    const long q = idx / C8;
-    const int xx = (int)(q % g.W), yy = (int)((q / g.W) % g.H), b = (int)(q / ((long)g.W * g.H));
-    const int Ho = g.H / 2, Wo = g.W / 2;
-    const uint4 v = *reinterpret_cast<const uint4*>(gy + (((long)b * Ho + yy / 2) * Wo + xx / 2) * C + c);
-    const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
-    uint4 o;
-    uint16_t* oh = reinterpret_cast<uint16_t*>(&o);
+// Backward of the pooling, full-resolution rows as in k_avgpool2_8. RELU: 0 where y (the pooled layer's own
+// output, after its ReLU) is not positive, the ReLU derivative of that layer fused; the same bits as the two
+// passes.
+template <bool RELU>
+__global__ void k_avgpool2_bwd8(const uint16_t* __restrict__ gy, const uint16_t* __restrict__ y,
+                                uint16_t* __restrict__ gx, Geo g, int C) {
+    const int Ho = g.H / 2, Wo = g.W / 2, C8 = C / 8;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.W * C8) return;
+    const int xx = t / C8, c = (t - xx * C8) * 8;
+    for (int row = blockIdx.y; row < g.B * g.H; row += gridDim.y) {
+        const int b = row / g.H, yy = row - b * g.H;
+        const long q = (long)row * g.W + xx;
+        const uint4 v = *reinterpret_cast<const uint4*>(gy + (((long)b * Ho + yy / 2) * Wo + xx / 2) * C + c);
+        const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+        uint4 o;
+        uint16_t* oh = reinterpret_cast<uint16_t*>(&o);
+        if (RELU) {
+            const uint4 m = *reinterpret_cast<const uint4*>(y + q * C + c);
+            const uint16_t* mh = reinterpret_cast<const uint16_t*>(&m);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) oh[e] = f2b(0.25f * b2f(h[e]));
-    *reinterpret_cast<uint4*>(gx + q * C + c) = o;
-}
-
-// k_avgpool2_bwd8 with the ReLU derivative of the layer below fused: 0 where y (that layer's output) is not
-// positive; the same bits as the two passes.
-__global__ void k_avgpool2_bwd8_relu(const uint16_t* __restrict__ gy, const uint16_t* __restrict__ y,
-                                     uint16_t* __restrict__ gx, Geo g, int C) {
-    const int C8 = C / 8;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)g.B * g.H * g.W * C8) return;
-    const int c = (int)(idx % C8) * 8;
-    const long q = idx / C8;
-    const int xx = (int)(q % g.W), yy = (int)((q / g.W) % g.H), b = (int)(q / ((long)g.W * g.H));
-    const int Ho = g.H / 2, Wo = g.W / 2;
-    const uint4 v = *reinterpret_cast<const uint4*>(gy + (((long)b * Ho + yy / 2) * Wo + xx / 2) * C + c);
-    const uint4 m = *reinterpret_cast<const uint4*>(y + q * C + c);
-    const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
-    const uint16_t* mh = reinterpret_cast<const uint16_t*>(&m);
-    uint4 o;
-    uint16_t* oh = reinterpret_cast<uint16_t*>(&o);
+            for (int e = 0; e < 8; ++e) oh[e] = b2f(mh[e]) > 0.0f ? f2b(0.25f * b2f(h[e])) : (uint16_t)0;
+        } else {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) oh[e] = b2f(mh[e]) > 0.0f ? f2b(0.25f * b2f(h[e])) : (uint16_t)0;
-    *reinterpret_cast<uint4*>(gx + q * C + c) = o;
+            for (int e = 0; e < 8; ++e) oh[e] = f2b(0.25f * b2f(h[e]));
+        }
+        *reinterpret_cast<uint4*>(gx + q * C + c) = o;
+    }
 }
 
 __global__ void k_avgpool2(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, Geo g, int C) {
@@ -1041,11 +1035,11 @@ __global__ void k_head_wgrad(const float* __restrict__ gout, const float* __rest
 // [Ci][9][Co] (same tap order).
 __global__ void k_weights(const float* __restrict__ w, uint16_t* __restrict__ w16, uint16_t* __restrict__ wt16, int Co,
                           int Ci) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)Co * 9 * Ci) return;
-    const int ci = (int)(idx % Ci);
-    const int tap = (int)((idx / Ci) % 9);
-    const int co = (int)(idx / (9L * Ci));
+    // 32-bit index arithmetic: a layer's Co * 9 * Ci weights are far below 2^31
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= Co * 9 * Ci) return;
+    const int q = idx / Ci, ci = idx - q * Ci;
+    const int co = q / 9, tap = q - co * 9;
     const uint16_t v = f2b(w[idx]);
     w16[idx] = v;
     if (wt16) wt16[((long)ci * 9 + tap) * Co + co] = v;   // the dgrad GEMM mirrors the tap offset itself
@@ -1069,6 +1063,10 @@ __global__ void k_f2b(const float* __restrict__ x, uint16_t* __restrict__ y, lon
 }
 
 inline dim3 grid1(long n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
+// (threads of one row, rows): x covers a row, y strides over the rows (at most 65535 workgroups in y)
+inline dim3 rows_grid(long per_row, long rows) {
+    return dim3((unsigned)((per_row + 255) / 256), (unsigned)(rows < 65535 ? rows : 65535));
+}
 
 }  // namespace
 
@@ -1213,7 +1211,8 @@ static void tree_sum(float* part, long Z, long stride, long count, float* scratc
     while (Z > SUM_FAN) {
         const long Zn = (Z + SUM_FAN - 1) / SUM_FAN;
         float* dst = in_part ? scratch : part;
-        hipLaunchKernelGGL(k_sum_group, grid1(Zn * count), dim3(256), 0, st, cur, (int)Z, sc, count, dst, count);
+        hipLaunchKernelGGL(k_sum_group, dim3(grid1(count).x, (unsigned)Zn), dim3(256), 0, st, cur, (int)Z, sc, count,
+                           dst, count);
         cur = dst;
         sc = count;
         in_part = !in_part;
@@ -1377,8 +1376,8 @@ extern "C" int bz_nn_conv_first_bwd(const uint16_t* x, const uint16_t* gz, const
             while (Z > SUM_FAN) {
                 const long Zn = (Z + SUM_FAN - 1) / SUM_FAN;
                 float* dst = in_part ? scratch : work;
-                hipLaunchKernelGGL(k_sum_group, grid1(Zn * 10L * Co), dim3(256), 0, st, cur, (int)Z, 10L * Co, 10L * Co,
-                                   dst, 10L * Co);
+                hipLaunchKernelGGL(k_sum_group, dim3(grid1(10L * Co).x, (unsigned)Zn), dim3(256), 0, st, cur, (int)Z,
+                                   10L * Co, 10L * Co, dst, 10L * Co);
                 cur = dst;
                 in_part = !in_part;
                 Z = Zn;
@@ -1414,8 +1413,8 @@ extern "C" int bz_nn_avgpool2(const uint16_t* x, uint16_t* y, int B, int H, int 
     BZ_REQUIRE(x && y && B > 0 && H % 2 == 0 && W % 2 == 0 && H > 0 && W > 0 && C > 0, "bz_nn_avgpool2: bad arguments");
     Geo g{B, H, W};
     if (C % 8 == 0 && ((uintptr_t)x & 15) == 0 && ((uintptr_t)y & 15) == 0)
-        hipLaunchKernelGGL(k_avgpool2_8, grid1((long)B * (H / 2) * (W / 2) * (C / 8)), dim3(256), 0, (hipStream_t)stream,
-                           x, y, g, C);
+        hipLaunchKernelGGL(k_avgpool2_8, rows_grid((long)(W / 2) * (C / 8), (long)B * (H / 2)), dim3(256), 0,
+                           (hipStream_t)stream, x, y, g, C);
     else
         hipLaunchKernelGGL(k_avgpool2, grid1((long)B * (H / 2) * (W / 2) * C), dim3(256), 0, (hipStream_t)stream, x, y,
                            g, C);
@@ -1428,8 +1427,8 @@ extern "C" int bz_nn_avgpool2_bwd(const uint16_t* gy, uint16_t* gx, int B, int H
                "bz_nn_avgpool2_bwd: bad arguments");
     Geo g{B, H, W};
     if (C % 8 == 0 && ((uintptr_t)gy & 15) == 0 && ((uintptr_t)gx & 15) == 0)
-        hipLaunchKernelGGL(k_avgpool2_bwd8, grid1((long)B * H * W * (C / 8)), dim3(256), 0, (hipStream_t)stream, gy, gx,
-                           g, C);
+        hipLaunchKernelGGL(k_avgpool2_bwd8<false>, rows_grid((long)W * (C / 8), (long)B * H), dim3(256), 0,
+                           (hipStream_t)stream, gy, (const uint16_t*)nullptr, gx, g, C);
     else
         hipLaunchKernelGGL(k_avgpool2_bwd, grid1((long)B * H * W * C), dim3(256), 0, (hipStream_t)stream, gy, gx, g, C);
     BZ_CHECK_LAUNCH("bz_nn_avgpool2_bwd");
@@ -1442,8 +1441,8 @@ extern "C" int bz_nn_avgpool2_bwd_relu(const uint16_t* gy, const uint16_t* y, ui
                    ((uintptr_t)gy & 15) == 0 && ((uintptr_t)y & 15) == 0 && ((uintptr_t)gx & 15) == 0,
                "bz_nn_avgpool2_bwd_relu: bad arguments (C a multiple of 8, 16-byte aligned tensors)");
     Geo g{B, H, W};
-    hipLaunchKernelGGL(k_avgpool2_bwd8_relu, grid1((long)B * H * W * (C / 8)), dim3(256), 0, (hipStream_t)stream, gy, y,
-                       gx, g, C);
+    hipLaunchKernelGGL(k_avgpool2_bwd8<true>, rows_grid((long)W * (C / 8), (long)B * H), dim3(256), 0,
+                       (hipStream_t)stream, gy, y, gx, g, C);
     BZ_CHECK_LAUNCH("bz_nn_avgpool2_bwd_relu");
     return BZ_OK;
 }
@@ -1475,7 +1474,7 @@ extern "C" int bz_nn_head_bwd(const float* gout, const float* gap, const float* 
 }
 
 extern "C" int bz_nn_weights(const float* w, uint16_t* w16, uint16_t* wt16, int Co, int Ci, void* stream) {
-    BZ_REQUIRE(w && w16 && Co > 0 && Ci > 0, "bz_nn_weights: bad arguments");
+    BZ_REQUIRE(w && w16 && Co > 0 && Ci > 0 && (long)Co * 9 * Ci < (1L << 31), "bz_nn_weights: bad arguments");
     hipLaunchKernelGGL(k_weights, grid1((long)Co * 9 * Ci), dim3(256), 0, (hipStream_t)stream, w, w16, wt16, Co, Ci);
     BZ_CHECK_LAUNCH("bz_nn_weights");
     return BZ_OK;

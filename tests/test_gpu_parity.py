@@ -253,6 +253,22 @@ def test_disjoint_sources_share_one_box_in_every_2d_kernel(hip):
     assert rel_l2(mon.dft["Ez"], F) < 1e-4
 
 
+def test_overlapping_sources_leave_the_fused_engine_and_match_oracle(hip):
+    """Two Gaussian sources on overlapping cells cannot share one source box: the grid runs the one-kernel-per-
+    update path (each source added in turn, as the reference does) and still matches the oracle."""
+    d, dx = waveguide_design_2d(nx=120, ny=60, npml=10)
+    dt = 0.99 * dx / (bz.LIGHT_SPEED * np.sqrt(2))
+    t = np.arange(200) * dt
+    d += bz.GaussianSource(position=(30 * dx, d.height / 2), width=dx, signal=pulse(t))
+    d += bz.GaussianSource(position=(30.5 * dx, d.height / 2), width=dx, signal=pulse(t, lam=1.31 * um))
+    sim = bz.FDTD(d, time=t, resolution=dx)
+    sim.run(live=False, save=False)
+    assert sim.grid._f2d is False
+    ref = R.RefSim(sim.epsilon_r, sim.sigma, dx, dx, dt, len(t), sources=sim.sources).run()
+    for k in ("Ez", "Hx", "Hy"):
+        assert rel_l2(np.real(getattr(sim, k)), ref.F[k]) < 1e-4, k
+
+
 def _k_case(kind):
     """A single 2D grid with one source and one line monitor (series and DFT): a Gaussian point source, or a
     ModeSource line that zeroes its upstream cells."""
