@@ -81,7 +81,7 @@ SIGNATURES = {
     "bz_gather3": [P6, P6, vp, P6, vp, vp, i32, i32, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp],
     "bz_abi_sizes": [C.POINTER(i32), i32],
     "bz_yee2d_runk": [vp, i32, vp, i32, i32, i32, i32, vp, i32, i32, vp, vp, vp, i32, vp, vp, i32, i32, vp, i32, vp,
-                      vp, vp],
+                      vp, i32, vp],
     "bz_adj2d_run": [vp, i32, vp, i32, i32, i32, i32, vp],
     "bz_adj2d_run2": [vp, i32, vp, i32, i32, vp, i32, i32, i32, i32, i32, vp],
     "bz_adj2d_runk": [vp, i32, vp, i32, i32, i32, i32, vp, i32, i32, i32, i32, i32, vp],

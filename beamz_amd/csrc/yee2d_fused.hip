@@ -638,7 +638,9 @@ __global__ void k_gather2d_k(Gather2D G, const float* __restrict__ stage, const 
     gather_k(G, stage, *step + K * q, K, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-template <int B, int KW>
+// SEL: some simulation's source box holds several sources (bz_sim2d.src_sel); without it the per-cell
+// source lookup is compiled out, which keeps the kernel at the register count of four waves per SIMD.
+template <int B, int KW, bool SEL>
 __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restrict__ sims,
                                                         const int* __restrict__ tiles, int ntiles, int K,
                                                         const int* __restrict__ step, int q, int rd,
@@ -795,7 +797,8 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
                 const int j = jb + r;
                 if (j >= sy0 && j < sy0 + ssy && src_col) {
                     const long rr = (long)(j - sy0) * ssx + (i - sx0);
-                    ez[r] = (smul ? gload(smul, rr) : 1.0f) * ez[r] + gload(sadd, rr) * box_amp(S, rr, n0 + k, amp);
+                    const float a = SEL ? box_amp(S, rr, n0 + k, amp) : amp;
+                    ez[r] = (smul ? gload(smul, rr) : 1.0f) * ez[r] + gload(sadd, rr) * a;
                 }
             }
         }
@@ -833,15 +836,19 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
 
 template <int B, int KW>
 int runk(const bz_sim2d* sims, const int* tiles, int ntiles, int K, int* step, int rd0, int nlaunch,
-         const int* ent_start, const int* ents, float* stage, const Gather2D& G, hipStream_t st) {
+         const int* ent_start, const int* ents, float* stage, const Gather2D& G, bool sel, hipStream_t st) {
     const dim3 b(LANES, KW);
     const long gather_threads = (long)G.npts * (G.nfreq > 0 ? G.nfreq : 1);
     const int gather_blocks = (int)((gather_threads + LANES * KW - 1) / (LANES * KW));
     // the gather of launch q-1 runs as extra workgroups of launch q; the last launch's gets a launch of its own
     for (int q = 0; q < nlaunch; ++q) {
         const dim3 g(ntiles + (q > 0 && G.npts > 0 ? gather_blocks : 0));
-        hipLaunchKernelGGL((k_yee2d_tbk<B, KW>), g, b, 0, st, sims, tiles, ntiles, K, step, q, (rd0 + q) & 1,
-                           ent_start, ents, stage, G);
+        if (sel)
+            hipLaunchKernelGGL((k_yee2d_tbk<B, KW, true>), g, b, 0, st, sims, tiles, ntiles, K, step, q, (rd0 + q) & 1,
+                               ent_start, ents, stage, G);
+        else
+            hipLaunchKernelGGL((k_yee2d_tbk<B, KW, false>), g, b, 0, st, sims, tiles, ntiles, K, step, q,
+                               (rd0 + q) & 1, ent_start, ents, stage, G);
         BZ_CHECK_LAUNCH("bz_yee2d_runk(steps)");
     }
     if (G.npts > 0) {
@@ -963,7 +970,8 @@ extern "C" int bz_yee2d_run2(const bz_sim2d* sims, int nsims, const int* tiles, 
 extern "C" int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int rows, int waves,
                              int ksteps, int* step, int rd0, int nlaunch, const int* ent_start, const int* ents,
                              float* stage, int npts, const int* slot, const int* grp, int nslot, int nrow,
-                             float* series, int nfreq, const double* phasor, double* dft, void* stream) {
+                             float* series, int nfreq, const double* phasor, double* dft, int src_sel,
+                             void* stream) {
     BZ_REQUIRE(sims && nsims > 0 && tiles && ntiles > 0 && step && nlaunch >= 0 && (rd0 == 0 || rd0 == 1),
                "bz_yee2d_runk: bad arguments");
     const int shape = rows * 100 + waves;
@@ -991,7 +999,8 @@ extern "C" int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, 
     G.dft = dft;
     const hipStream_t st = (hipStream_t)stream;
     if (npts == 0) ent_start = ents = nullptr;
-#define BZ_RUNK(R, W) runk<R, W>(sims, tiles, ntiles, ksteps, step, rd0, nlaunch, ent_start, ents, stage, G, st)
+#define BZ_RUNK(R, W) \
+    runk<R, W>(sims, tiles, ntiles, ksteps, step, rd0, nlaunch, ent_start, ents, stage, G, src_sel != 0, st)
     switch (shape) {
         case 804: return BZ_RUNK(8, 4);
         case 808: return BZ_RUNK(8, 8);

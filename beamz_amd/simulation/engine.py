@@ -723,6 +723,7 @@ class Fused2D:
         self.descs = []
         for g, sn in zip(grids, self.snapshots):
             self.descs.append(self._describe(g, sn))
+        self.multi_src = any(d.src_sel for d in self.descs)   # some grid has several sources in its box
         self.desc_dev = T.zeros(len(grids) * C.sizeof(N.Sim2D), dtype=T.uint8, device=self.dev)
         self._desc_cache = {}
         self.shift = None
@@ -1129,7 +1130,8 @@ class Fused2D:
         N.call("bz_yee2d_runk", self.desc_dev.data_ptr(), len(self.grids), self.tilesk.data_ptr(), self.ntilesk,
                self.krows, self.kwaves, self.kstep, self.step.data_ptr(), rd0, nl, ptr(self.ent_start),
                ptr(self.ents), ptr(self.stagek), self.npts, ptr(self.slot), ptr(self.grp), self.nslot, self.nrow,
-               ptr(self.series), self.nfreq, ptr(self.phasor), ptr(self.dft), self.grids[0].stream)
+               ptr(self.series), self.nfreq, ptr(self.phasor), ptr(self.dft), int(self.multi_src),
+               self.grids[0].stream)
 
     def _launch(self, count):
         ptr = lambda t: t.data_ptr() if t is not None else None

@@ -248,12 +248,14 @@ int bz_abi_sizes(int* out, int n);
  * in units of the inner size, one tile per workgroup (sim < 0: padding). Monitor point p is sampled by the
  * tile whose inner cells hold it: ent_start[t] .. ent_start[t+1]-1 index ents[2e] = p,
  * ents[2e+1] = (region row << 10) | (region column << 2) | comp, sorted by comp within a tile;
+ * src_sel: nonzero if any descriptor's src_sel is set (several sources in one box; the kernel variant that
+ * reads the per-cell selector runs only then).
  * stage: device FP32 [2][ksteps][npts] (launch q writes slot q & 1). Snapshot boxes as for bz_yee2d_run:
  * step k of launch q writes slot q*ksteps + k. No grid power (use bz_yee2d_run / run2). */
 int bz_yee2d_runk(const bz_sim2d* sims, int nsims, const int* tiles, int ntiles, int rows, int waves, int ksteps,
                   int* step, int rd0, int nlaunch, const int* ent_start, const int* ents, float* stage, int npts,
                   const int* slot, const int* grp, int nslot, int nrow, float* series, int nfreq,
-                  const double* phasor, double* dft, void* stream);
+                  const double* phasor, double* dft, int src_sel, void* stream);
 
 /* ---------------- reverse-time 2D adjoint ----------------
  * No reference counterpart (the reference's gradient is the overlap heuristic of

@@ -38,7 +38,7 @@ def test_2d_and_adjoint_checks():
     einval("bz_yee2d_run2", FAKE, 1, FAKE, 1, 8, FAKE, 2, 1, None, None, 0, None, None, 0, 0, None, 2, None, None,
            None, match="bz_yee2d_run2: bad arguments")                         # rd0 must be 0 or 1
     einval("bz_yee2d_runk", FAKE, 1, FAKE, 1, 8, 8, 4, FAKE, 0, 1, FAKE, FAKE, FAKE, 5, FAKE, FAKE, 10, 0, None, 0,
-           None, None, None, match="bz_yee2d_runk: point groups need nslot > 0 and nrow > 0")
+           None, None, 0, None, match="bz_yee2d_runk: point groups need nslot > 0 and nrow > 0")
     einval("bz_adj2d_run2", FAKE, 1, FAKE, 1, 4, None, 0, 2, 10, 3, 0, None,
            match="bz_adj2d_run2: an odd step count needs one-step tiles")
     einval("bz_adj2d_run2", FAKE, 1, FAKE, 1, 4, FAKE, 1, 2, 10, 2, 2, None, match="bz_adj2d_run2: bad arguments")
