@@ -800,6 +800,36 @@ __global__ void k_conv_first_dgrad(const uint16_t* __restrict__ gz, const float*
     gx[p] = s;
 }
 
+// k_conv_first_dgrad with 16-byte loads of each tap's channels (Co % 8 == 0, Co <= 64) and the weights rounded
+// to bf16 once, in LDS: the same products added in the same order (tap, then channel), so the same bits.
+__global__ __launch_bounds__(256) void k_conv_first_dgrad8(const uint16_t* __restrict__ gz, const float* __restrict__ w,
+                                                           float* __restrict__ gx, Geo g, int Co) {
+    __shared__ float ws[9][64];
+    for (int q = threadIdx.x; q < 9 * Co; q += blockDim.x) {
+        const int co = q / 9, tap = q - 9 * co;
+        ws[tap][co] = b2f(f2b(w[q]));
+    }
+    __syncthreads();
+    const int P = g.B * g.H * g.W;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    int pb, py, px;
+    pixel(p, g, pb, py, px);
+    float s = 0.0f;
+    for (int tap = 0; tap < 9; ++tap) {
+        const int yy = py - (tap / 3 - 1), xx = px - (tap % 3 - 1);
+        if (yy < 0 || yy >= g.H || xx < 0 || xx >= g.W) continue;
+        const uint4* gp = reinterpret_cast<const uint4*>(gz + (((long)pb * g.H + yy) * g.W + xx) * Co);
+        for (int c8 = 0; c8 < Co / 8; ++c8) {
+            const uint4 v = gp[c8];
+            const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s += b2f(h[e]) * ws[tap][8 * c8 + e];
+        }
+    }
+    gx[p] = s;
+}
+
 // Column sums (bias gradient): gb[c] += sum_p g[p][c] over the workgroup's `chunk` pixel rows. Thread t
 // reads channels 8*(t % G) .. +7 (G = ceil(C/8) groups, 16-byte loads when C % 8 == 0) of every R-th row
 // (R = 256 / G), the R partial sums of each channel are added in LDS, and one atomic per channel and
@@ -1392,7 +1422,10 @@ extern "C" int bz_nn_conv_first_bwd(const uint16_t* x, const uint16_t* gz, const
     }
     BZ_CHECK_LAUNCH("bz_nn_conv_first_bwd(w)");
     if (gx) {
-        hipLaunchKernelGGL(k_conv_first_dgrad, grid1(P), dim3(256), 0, st, gz, w, gx, g, Co);
+        if (Co % 8 == 0 && Co <= 64 && ((uintptr_t)gz & 15) == 0)
+            hipLaunchKernelGGL(k_conv_first_dgrad8, grid1(P), dim3(256), 0, st, gz, w, gx, g, Co);
+        else
+            hipLaunchKernelGGL(k_conv_first_dgrad, grid1(P), dim3(256), 0, st, gz, w, gx, g, Co);
         BZ_CHECK_LAUNCH("bz_nn_conv_first_bwd(x)");
     }
     return BZ_OK;

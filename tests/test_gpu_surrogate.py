@@ -143,6 +143,18 @@ def test_first_layer_pooling_and_mask_match_oracle(hip):
                gb.data_ptr(), None, B, H, W, Co, wk, nwk, st)
         assert rel_l2(gw.cpu().numpy(), dw.reshape(Co, 9)) < 1e-5
         assert rel_l2(gb.cpu().numpy(), db) < 1e-5
+    for C in (32, 12):                      # input gradient: 16-byte channel loads (C % 8 == 0), then per channel
+        wc = S.bf16(rng.standard_normal((C, 3, 3, 1)) * 0.3)
+        gzc = S.bf16(rng.standard_normal((B, H, W, C)))
+        gx = torch.empty((B, H, W), dtype=torch.float32).cuda()
+        wcd = torch.tensor(wc.reshape(C, 9), dtype=torch.float32).cuda()
+        gwc = torch.zeros((C, 9), dtype=torch.float32).cuda()
+        gbc = torch.zeros(C, dtype=torch.float32).cuda()
+        gzcd = to16(torch, gzc)
+        N.call("bz_nn_conv_first_bwd", xd.data_ptr(), gzcd.data_ptr(), wcd.data_ptr(), gwc.data_ptr(),
+               gbc.data_ptr(), gx.data_ptr(), B, H, W, C, None, 0, st)
+        dx, _, _ = S.conv3x3_backward(x, wc, gzc)
+        assert rel_l2(gx.cpu().numpy(), dx[..., 0]) < 1e-5, C
     for C in (64, 12):
         a = S.bf16(rng.standard_normal((2, 6, 8, C)))
         p = torch.empty((2, 3, 4, C), dtype=torch.int16).cuda()
