@@ -650,7 +650,7 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
     constexpr int RH = KW * B;
     __shared__ float ez_top[KW][LANES];
     __shared__ float hy_bot[KW][LANES];
-    __shared__ float smp[RH][LANES];
+    extern __shared__ float smp[];      // [3][RH][LANES] when there are monitor points (launch-sized)
     const int npts = G.npts;
     const int t = blockIdx.x;
     if (t >= ntiles) {
@@ -734,6 +734,11 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
     }
     const int e0 = ent_start ? ent_start[t] : 0, e1 = ent_start ? ent_start[t + 1] : 0;
     const int tid = w * LANES + l;
+    // the tile's monitor entries: thread tid stages entries e0 + tid, e0 + tid + LANES*KW, ...; the first one
+    // is read here, once for all K steps
+    const int ef = e0 + tid;
+    const int epnt = ef < e1 ? ents[2 * ef] : -1;
+    const int ecode = ef < e1 ? ents[2 * ef + 1] : 0;
 
     float amp_next = sgl ? gload(sgl, n0) : 0.0f;
     for (int k = 0; k < K; ++k) {
@@ -802,18 +807,22 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
                 }
             }
         }
-        // monitor samples of step n0+k owned by this tile: entries sorted by component
+        // monitor samples of step n0+k owned by this tile, all three components behind one barrier. The next
+        // step's writes to smp come after its two barriers, which every thread reaches only after these reads.
         if (e1 > e0) {
-            for (int comp = 0; comp < 3; ++comp) {
 #pragma unroll
-                for (int r = 0; r < B; ++r) smp[w * B + r][l] = comp == 0 ? ez[r] : (comp == 1 ? hx[r] : hy[r]);
-                __syncthreads();
-                for (int e = e0 + tid; e < e1; e += LANES * KW) {
-                    const int code = ents[2 * e + 1];
-                    if ((code & 3) == comp) gstore(stage, (long)k * npts + ents[2 * e], smp[code >> 10][(code >> 2) & 255]);
-                }
-                __syncthreads();
+            for (int r = 0; r < B; ++r) {
+                const int o = (w * B + r) * LANES + l;
+                smp[o] = ez[r];
+                smp[RH * LANES + o] = hx[r];
+                smp[2 * RH * LANES + o] = hy[r];
             }
+            __syncthreads();
+            // code = (region row << 10) | (region column << 2) | component
+            auto sample = [&](int code) { return smp[((code & 3) * RH + (code >> 10)) * LANES + ((code >> 2) & 255)]; };
+            if (epnt >= 0) gstore(stage, (long)k * npts + epnt, sample(ecode));
+            for (int e = ef + LANES * KW; e < e1; e += LANES * KW)
+                gstore(stage, (long)k * npts + ents[2 * e], sample(ents[2 * e + 1]));
         }
     }
 
@@ -838,16 +847,18 @@ template <int B, int KW>
 int runk(const bz_sim2d* sims, const int* tiles, int ntiles, int K, int* step, int rd0, int nlaunch,
          const int* ent_start, const int* ents, float* stage, const Gather2D& G, bool sel, hipStream_t st) {
     const dim3 b(LANES, KW);
+    // sample staging in LDS, [3][KW*B][LANES] floats, only when there are monitor points
+    const size_t lds = G.npts > 0 ? 3u * KW * B * LANES * sizeof(float) : 0;
     const long gather_threads = (long)G.npts * (G.nfreq > 0 ? G.nfreq : 1);
     const int gather_blocks = (int)((gather_threads + LANES * KW - 1) / (LANES * KW));
     // the gather of launch q-1 runs as extra workgroups of launch q; the last launch's gets a launch of its own
     for (int q = 0; q < nlaunch; ++q) {
         const dim3 g(ntiles + (q > 0 && G.npts > 0 ? gather_blocks : 0));
         if (sel)
-            hipLaunchKernelGGL((k_yee2d_tbk<B, KW, true>), g, b, 0, st, sims, tiles, ntiles, K, step, q, (rd0 + q) & 1,
+            hipLaunchKernelGGL((k_yee2d_tbk<B, KW, true>), g, b, lds, st, sims, tiles, ntiles, K, step, q, (rd0 + q) & 1,
                                ent_start, ents, stage, G);
         else
-            hipLaunchKernelGGL((k_yee2d_tbk<B, KW, false>), g, b, 0, st, sims, tiles, ntiles, K, step, q,
+            hipLaunchKernelGGL((k_yee2d_tbk<B, KW, false>), g, b, lds, st, sims, tiles, ntiles, K, step, q,
                                (rd0 + q) & 1, ent_start, ents, stage, G);
         BZ_CHECK_LAUNCH("bz_yee2d_runk(steps)");
     }

@@ -909,18 +909,18 @@ class Fused2D:
 
     KSHAPES = ((8, 4), (8, 8), (8, 16), (4, 8), (4, 16), (16, 4))     # (rows per wave, waves per workgroup)
     # (K, rows, waves) candidates of tune_kstep; the automatic shapes were the fastest measured on 64 members
-    # of 256..1024 cells a side (30 M cells: 0.059 ms/step for (8, 8, 8)) and on one 800 x 400 grid
-    # (2.95 us/step for (8, 4, 8))
+    # of 256..1024 cells a side (30 M cells: 0.051 ms/step for (8, 8, 8)) and on one 800 x 400 grid with a
+    # 16-wavelength DFT line (1.43 us/step for (12, 4, 16), against 1.51 for (8, 4, 8) and 1.80 for (8, 8, 8))
     KCANDIDATES = ((8, 8, 8), (6, 8, 8), (8, 8, 4), (8, 4, 8), (12, 4, 16))
 
     def kstep_applies(self):
         return all(g.power is None for g in self.grids)
 
     def auto_kstep(self):
-        """(K, rows, waves): tall 64 x 64 regions from PAIR_MIN_CELLS cells on, else 64 x 32 regions, which
-        give a small grid more workgroups."""
+        """(K, rows, waves): 64 x 64 regions of 8 waves x 8 rows from PAIR_MIN_CELLS cells on; below, 64 x 64
+        regions of 16 waves x 4 rows and 12 steps per pass, which halve each wave's rows and the launches."""
         big = sum(g.nx * g.ny for g in self.grids) >= self.PAIR_MIN_CELLS
-        return (8, 8, 8) if big else (8, 4, 8)
+        return (8, 8, 8) if big else (12, 4, 16)
 
     def tune_kstep(self, n0=0, steps=16, candidates=None):
         """Time every (K, rows, waves) of `candidates` (default KCANDIDATES) on about `steps` steps from step
