@@ -112,12 +112,13 @@ class AdjointProblem2D:
                                           "complex_fields=False or real sources")
         self.fwd = Fused2D(grids, seg=self.seg, max_steps=self.N, pairs=self.pairs, kstep=self.kstep)
         if self.fwd.kstep and self.N >= 64:
-            # region shape of the K-step sweeps, timed once, among the shapes whose K divides the checkpoint
+            # region shape of the forward sweep, timed once, among the shapes whose K divides the checkpoint
             # interval (every segment then runs as whole K-step launches); the recompute and reverse sweeps
-            # use the same one
+            # start from it and are timed on their own in the first gradient
             shapes = [c for c in self.fwd.KCANDIDATES if self.K % c[0] == 0]
             if len(shapes) > 1:
-                self.fwd.tune_kstep(steps=16, candidates=shapes)
+                # 64 steps per shape: two launches of 8 steps ranked the shapes differently from run to run
+                self.fwd.tune_kstep(steps=64, candidates=shapes)
                 self._reset()
         elif self.fwd.pairs and self.N >= 24:
             self.fwd.tune_seg2(steps=10)
@@ -287,6 +288,30 @@ class AdjointProblem2D:
             return flip
 
         nseg = len(self.ckpt)
+        if self.rec.kstep and not self._rev_tuned and nseg > 1:
+            # the recompute and reverse sweeps share the K-step tiles, and the reverse kernel ranks the region
+            # shapes differently from the forward one: time one whole segment of both per shape whose K divides
+            # the checkpoint interval and keep the fastest (the adjoint state and gradient are cleared below)
+            shapes = [c for c in self.rec.KCANDIDATES if self.K % c[0] == 0]
+            k_t = nseg - 1 if self.N - (nseg - 1) * self.K == self.K else nseg - 2
+            if len(shapes) > 1 and k_t >= 0:
+                ms = {}
+                for shape in shapes:
+                    self.rec.set_kstep(*shape)
+                    segment(k_t, 0)                                   # warm-up
+                    e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
+                    e0.record()
+                    segment(k_t, 0)
+                    e1.record()
+                    T.cuda.synchronize(self.dev)
+                    ms[shape] = e0.elapsed_time(e1)
+                self.rec.set_kstep(*min(ms, key=ms.get))
+                self.rev_timings = ms
+                for pair in lam:
+                    for t in pair:
+                        t.zero_()
+                grad.zero_()
+            self._rev_tuned = True
         if self.rec.pairs and not self.rec.kstep and not self._rev_tuned and nseg > 1:
             # the recompute and reverse sweeps share the pair tiles: time one segment of both per tile height
             # and keep the fastest (the adjoint state and gradient are cleared again below)

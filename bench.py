@@ -61,7 +61,7 @@ MEASURED_TRAFFIC = {
     ("batch2d", "pairs-seg8-xcd", 64): (2 * 474_825 + 400_315) * 1024,
     # K = 8 steps per pass, 64 x 64 regions of 8 waves x 8 rows, XCD order, the previous launch's gather inside,
     # interleaved per-cell coefficients (profiles/yee2d_tbk_batch64_kernel_and_counters.md)
-    ("batch2d", "k8-8x8-xcd", 64): (2 * 423_209 + 358_906) * 1024,
+    ("batch2d", "k8-8x8-xcd", 64): (2 * 423_055 + 360_374) * 1024,
 }
 
 
@@ -487,9 +487,10 @@ def main():
     if a.workload == "adjoint2d":             # the sweeps' engines exist once a gradient has run
         f = r.prob.fwd
         if f.kstep:
-            shape = f"{f.kstep} steps per pass, 64x{f.krows * f.kwaves} regions"
-            r.kernel_name = (f"forward sweep and recompute sweep (with design-box snapshots): bz_yee2d_runk; "
-                             f"reverse sweep: bz_adj2d_runk; {shape}")
+            rc = r.prob.rec
+            r.kernel_name = (f"forward sweep: bz_yee2d_runk, {f.kstep} steps per pass, 64x{f.krows * f.kwaves} "
+                             f"regions; recompute sweep (with design-box snapshots): bz_yee2d_runk, and reverse "
+                             f"sweep: bz_adj2d_runk, {rc.kstep} steps per pass, 64x{rc.krows * rc.kwaves} regions")
         else:
             r.kernel_name = ("forward sweep and recompute sweep (with design-box snapshots): "
                              + ("bz_yee2d_run2; reverse sweep: bz_adj2d_run2" if f.pairs
