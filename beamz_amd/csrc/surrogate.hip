@@ -630,6 +630,65 @@ __global__ void k_sum_parts(const float* __restrict__ part, int Z, long stride, 
     out[i] = s;
 }
 
+// The whole tree of k_sum_group levels and the closing k_sum_parts in one launch, for up to two segments (a
+// layer's weight and bias partials): the same additions in the same order, so the same bits. A workgroup
+// takes SE consecutive elements of one segment. Its 256 / SE lanes per element add the first level's groups
+// of SUM_FAN rows (group g on lane g mod (256 / SE)) into LDS; one lane per element then adds the further
+// levels in place and the output, in order.
+struct SumSeg {
+    float* in;        // the partial rows (workspace; the fallback through tree_sum overwrites them)
+    long stride, count;
+    float* out;
+    int Z;
+};
+constexpr int SUM_LDS_FLOATS = 16384;     // first-level sums one workgroup holds (64 KB)
+
+template <int SE>
+__global__ __launch_bounds__(256) void k_tree_sum(SumSeg a, SumSeg b, int nblk_a) {
+    extern __shared__ float lv[];     // [first-level group][SE]
+    constexpr int ST = 256 / SE;
+    const bool second = (int)blockIdx.x >= nblk_a;
+    const SumSeg S = second ? b : a;
+    const long i = (long)(second ? (int)blockIdx.x - nblk_a : (int)blockIdx.x) * SE + threadIdx.x % SE;
+    const int e = threadIdx.x % SE, t = threadIdx.x / SE;
+    const bool ok = i < S.count;
+    // rows z0 .. z1-1 (at most SUM_FAN) of element i added in order: all loads issued first, then the adds
+    // (a row past z1 is skipped, not added as 0, which would turn -0 into +0)
+    auto group = [&](int z0, int z1, float s, bool first) {
+        float v[SUM_FAN];
+#pragma unroll
+        for (int q = 0; q < SUM_FAN; ++q) v[q] = z0 + q < z1 ? S.in[(long)(z0 + q) * S.stride + i] : 0.0f;
+        if (first) s = v[0];
+#pragma unroll
+        for (int q = 0; q < SUM_FAN; ++q)
+            if ((q > 0 || !first) && z0 + q < z1) s += v[q];
+        return s;
+    };
+    if (S.Z <= SUM_FAN) {             // uniform over the workgroup
+        if (t == 0 && ok) S.out[i] = group(0, S.Z, S.out[i], false);
+        return;
+    }
+    const int G1 = (S.Z + SUM_FAN - 1) / SUM_FAN;
+    if (ok)
+        for (int g = t; g < G1; g += ST) lv[g * SE + e] = group(g * SUM_FAN, min(S.Z, (g + 1) * SUM_FAN), 0.0f, true);
+    __syncthreads();
+    if (t != 0 || !ok) return;
+    int Zc = G1;
+    while (Zc > SUM_FAN) {
+        const int Zn = (Zc + SUM_FAN - 1) / SUM_FAN;
+        for (int h = 0; h < Zn; ++h) {            // group h reads entries 16h.. and writes entry h <= 16h
+            const int z0 = h * SUM_FAN, z1 = min(Zc, z0 + SUM_FAN);
+            float s = lv[z0 * SE + e];
+            for (int z = z0 + 1; z < z1; ++z) s += lv[z * SE + e];
+            lv[h * SE + e] = s;
+        }
+        Zc = Zn;
+    }
+    float s = S.out[i];
+    for (int z = 0; z < Zc; ++z) s += lv[z * SE + e];
+    S.out[i] = s;
+}
+
 // First convolution (one input channel): y[p][co] = relu(b[co] + sum_tap x[p+tap] w[co][tap]).
 __global__ void k_conv_first(const uint16_t* __restrict__ x, const float* __restrict__ w, const float* __restrict__ b,
                              uint16_t* __restrict__ y, Geo g, int Co) {
@@ -1251,6 +1310,29 @@ static void tree_sum(float* part, long Z, long stride, long count, float* scratc
     hipLaunchKernelGGL(k_sum_parts, grid1(count), dim3(256), 0, st, cur, (int)Z, sc, count, out);
 }
 
+// out += the Z rows of each segment's partials, in tree_sum's order, in one launch (k_tree_sum); b.count may be
+// 0. First levels too large for one workgroup's LDS go through tree_sum (scratch as there).
+static void tree_sum2(const SumSeg& a, const SumSeg& b, float* scratch, hipStream_t st) {
+    auto g1 = [](const SumSeg& q) { return q.count > 0 && q.Z > SUM_FAN ? (q.Z + SUM_FAN - 1) / SUM_FAN : 0; };
+    const int G1 = std::max(g1(a), g1(b));
+    // lanes per element (256 / se) about the number of first-level groups
+    const int se = G1 >= 32 ? 8 : (G1 >= 4 ? 32 : (G1 >= 2 ? 128 : 256));
+    if ((long)G1 * se > SUM_LDS_FLOATS) {
+        tree_sum(a.in, a.Z, a.stride, a.count, scratch, a.out, st);
+        if (b.count > 0) tree_sum(b.in, b.Z, b.stride, b.count, scratch, b.out, st);
+        return;
+    }
+    const long na = (a.count + se - 1) / se, nb = b.count > 0 ? (b.count + se - 1) / se : 0;
+    const size_t lds = (size_t)G1 * se * sizeof(float);
+    const dim3 grid((unsigned)(na + nb));
+    switch (se) {
+        case 8: hipLaunchKernelGGL(k_tree_sum<8>, grid, dim3(256), lds, st, a, b, (int)na); break;
+        case 32: hipLaunchKernelGGL(k_tree_sum<32>, grid, dim3(256), lds, st, a, b, (int)na); break;
+        case 128: hipLaunchKernelGGL(k_tree_sum<128>, grid, dim3(256), lds, st, a, b, (int)na); break;
+        default: hipLaunchKernelGGL(k_tree_sum<256>, grid, dim3(256), lds, st, a, b, (int)na); break;
+    }
+}
+
 // The row march (k_wgrad_march) serves rows of 16, 32, 64 or 128 pixels with channel counts in multiples of 32,
 // R = max(1, 64 / W) rows per step: 32-wide channel blocks on 128-pixel rows, 64-wide where the counts allow
 // otherwise (16- and 32-pixel rows need 64-wide output blocks, 16-pixel rows 64-wide input blocks too; on
@@ -1334,8 +1416,7 @@ extern "C" int bz_nn_conv3x3_wgrad(const uint16_t* x, const uint16_t* gz, float*
         }
         BZ_CHECK_LAUNCH("bz_nn_conv3x3_wgrad(march)");
         float* scratch = bpart + (long)mp.nz * Co;
-        tree_sum(wpart, mp.nz, nw, nw, scratch, gw, st);
-        tree_sum(bpart, mp.nz, Co, Co, scratch, gb, st);
+        tree_sum2(SumSeg{wpart, nw, nw, gw, mp.nz}, SumSeg{bpart, Co, Co, gb, mp.nz}, scratch, st);
         BZ_CHECK_LAUNCH("bz_nn_conv3x3_wgrad(sums)");
         return BZ_OK;
     }
@@ -1350,8 +1431,7 @@ extern "C" int bz_nn_conv3x3_wgrad(const uint16_t* x, const uint16_t* gz, float*
     BZ_CHECK_LAUNCH("bz_nn_conv3x3_wgrad(bias)");
     if (work) {
         float* scratch = bpart + q.nblk * Co;
-        tree_sum(wpart, q.nz, nw, nw, scratch, gw, st);
-        tree_sum(bpart, q.nblk, Co, Co, scratch, gb, st);
+        tree_sum2(SumSeg{wpart, nw, nw, gw, q.nz}, SumSeg{bpart, Co, Co, gb, (int)q.nblk}, scratch, st);
         BZ_CHECK_LAUNCH("bz_nn_conv3x3_wgrad(sums)");
     }
     return BZ_OK;
@@ -1397,24 +1477,9 @@ extern "C" int bz_nn_conv_first_bwd(const uint16_t* x, const uint16_t* gz, const
                    "bz_nn_conv_first_bwd: workspace smaller than bz_nn_first_work");
         hipLaunchKernelGGL(k_conv_first_wgrad8, dim3((unsigned)nblk), dim3(256), 0, st, x, gz, gw, gb, work, g, Co,
                            chunk);
-        if (work) {
-            // reduce all 10 Co columns to at most SUM_FAN rows, then add the gw and gb columns into their outputs
-            float* cur = work;
-            long Z = nblk;
-            bool in_part = true;
-            float* scratch = work + nblk * 10L * Co;
-            while (Z > SUM_FAN) {
-                const long Zn = (Z + SUM_FAN - 1) / SUM_FAN;
-                float* dst = in_part ? scratch : work;
-                hipLaunchKernelGGL(k_sum_group, dim3(grid1(10L * Co).x, (unsigned)Zn), dim3(256), 0, st, cur, (int)Z,
-                                   10L * Co, 10L * Co, dst, 10L * Co);
-                cur = dst;
-                in_part = !in_part;
-                Z = Zn;
-            }
-            hipLaunchKernelGGL(k_sum_parts, grid1(9L * Co), dim3(256), 0, st, cur, (int)Z, 10L * Co, 9L * Co, gw);
-            hipLaunchKernelGGL(k_sum_parts, grid1(Co), dim3(256), 0, st, cur + 9L * Co, (int)Z, 10L * Co, (long)Co, gb);
-        }
+        if (work)   // rows of 10 Co: the 9 Co weight columns into gw, the Co bias columns into gb
+            tree_sum2(SumSeg{work, 10L * Co, 9L * Co, gw, (int)nblk},
+                      SumSeg{work + 9L * Co, 10L * Co, (long)Co, gb, (int)nblk}, work + nblk * 10L * Co, st);
     } else {
         const int chunk = 8192;
         hipLaunchKernelGGL(k_conv_first_wgrad, dim3((unsigned)((P + chunk - 1) / chunk), Co), dim3(256), 0, st, x, gz,

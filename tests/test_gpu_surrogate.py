@@ -352,3 +352,80 @@ def test_fused_relu_derivatives_equal_the_separate_passes(hip, B, H, W, Ci, Co):
         N.call("bz_nn_relu_mask", p.data_ptr(), xd.data_ptr(), p.numel(), st)
         N.call("bz_nn_avgpool2_bwd_relu", gy.data_ptr(), xd.data_ptr(), q.data_ptr(), B, H, W, Ci, st)
         assert torch.equal(p, q)
+
+
+def _tree_f32(rows, out):
+    """out + the rows in the reduction's fixed order, in FP32: groups of 16 consecutive rows summed in row order,
+    level after level while more than 16 remain, then the last rows added to out in order."""
+    cur = [np.asarray(r, np.float32) for r in rows]
+    while len(cur) > 16:
+        nxt = []
+        for g in range(0, len(cur), 16):
+            s = cur[g].copy()
+            for r in cur[g + 1:g + 16]:
+                s = (s + r).astype(np.float32)
+            nxt.append(s)
+        cur = nxt
+    s = np.asarray(out, np.float32).copy()
+    for r in cur:
+        s = (s + r).astype(np.float32)
+    return s
+
+
+def _rows_for(need, per_row, extra_per_row):
+    """Partial rows z of a workspace of `need` floats laid out as z rows of per_row floats plus ceil(z/16) scratch
+    rows of extra_per_row (when z > 16)."""
+    for z in range(1, 1 << 16):
+        if z * per_row + (-(-z // 16) if z > 16 else 0) * extra_per_row == need:
+            return z
+    raise AssertionError("no row count fits the workspace size")
+
+
+@pytest.mark.parametrize("B,H,W,Ci,Co,Z", [(2, 8, 16, 64, 128, 1), (16, 32, 32, 64, 128, 32),
+                                           (128, 32, 32, 64, 128, 205), (128, 64, 64, 32, 64, 432),
+                                           (128, 128, 128, 32, 32, 863)])
+def test_weight_gradient_sums_follow_the_tree_order(hip, B, H, W, Ci, Co, Z):
+    """The ordered reduction of the weight-gradient partials (one launch for all levels) gives the bits of the
+    fixed tree over the Z partial rows the kernel left in the workspace, recomputed in FP32: no level, one level
+    (32 and 205 rows), two levels (432 rows; 863 rows, where each element takes 32 lanes)."""
+    torch = hip
+    rng = np.random.default_rng(B + H + W + Ci + Co)
+    st = torch.cuda.current_stream().cuda_stream
+    xd = to16(torch, S.bf16(rng.standard_normal((B, H, W, Ci))))
+    gzd = to16(torch, S.bf16(rng.standard_normal((B, H, W, Co))))
+    need = N.lib().bz_nn_wgrad_work(B, H, W, Ci, Co, 3)
+    nw = Co * 9 * Ci
+    assert _rows_for(need, nw + Co, nw) == Z
+    work = torch.empty(need, dtype=torch.float32).cuda()
+    gw = torch.zeros(nw, dtype=torch.float32).cuda()
+    gb = torch.full((Co,), 0.25, dtype=torch.float32).cuda()
+    N.call("bz_nn_conv3x3_wgrad", xd.data_ptr(), gzd.data_ptr(), gw.data_ptr(), gb.data_ptr(), B, H, W, Ci, Co, 3,
+           work.data_ptr(), need, st)
+    wk = work.cpu().numpy()
+    wpart = wk[:Z * nw].reshape(Z, nw)
+    bpart = wk[Z * nw:Z * (nw + Co)].reshape(Z, Co)
+    assert np.array_equal(gw.cpu().numpy(), _tree_f32(wpart, np.zeros(nw, np.float32)))
+    assert np.array_equal(gb.cpu().numpy(), _tree_f32(bpart, np.full(Co, 0.25, np.float32)))
+
+
+def test_first_layer_weight_gradient_sums_follow_the_tree_order(hip):
+    """The same for the first layer's partial rows of 9 Co weight and Co bias columns (1024 rows, three
+    levels)."""
+    torch = hip
+    rng = np.random.default_rng(3)
+    st = torch.cuda.current_stream().cuda_stream
+    B, H, W, Co = 64, 128, 128, 32
+    xd = to16(torch, S.bf16(rng.random((B, H, W, 1))))
+    gzd = to16(torch, S.bf16(rng.standard_normal((B, H, W, Co))))
+    wd = torch.tensor(S.bf16(rng.standard_normal((Co, 9)) * 0.3), dtype=torch.float32).cuda()
+    need = N.lib().bz_nn_first_work(B, H, W, Co)
+    Z = _rows_for(need, 10 * Co, 10 * Co)
+    work = torch.empty(need, dtype=torch.float32).cuda()
+    gw = torch.zeros((Co, 9), dtype=torch.float32).cuda()
+    gb = torch.zeros(Co, dtype=torch.float32).cuda()
+    N.call("bz_nn_conv_first_bwd", xd.data_ptr(), gzd.data_ptr(), wd.data_ptr(), gw.data_ptr(), gb.data_ptr(), None,
+           B, H, W, Co, work.data_ptr(), need, st)
+    part = work.cpu().numpy()[:Z * 10 * Co].reshape(Z, 10 * Co)
+    assert Z == 1024, Z
+    assert np.array_equal(gw.cpu().numpy().ravel(), _tree_f32(part[:, :9 * Co], np.zeros(9 * Co, np.float32)))
+    assert np.array_equal(gb.cpu().numpy(), _tree_f32(part[:, 9 * Co:], np.zeros(Co, np.float32)))
