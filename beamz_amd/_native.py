@@ -104,6 +104,7 @@ SIGNATURES = {
     "bz_nn_adam": [vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp],
     "bz_nn_to_bf16": [vp, vp, i64, vp],
     "bz_adj2d_inject": [vp, i32, i32, i32, vp],
+    "bz_filter2d": [vp, vp, i32, i32, i32, vp, i32, vp, vp, vp],
     "bz_add_points": [vp, vp, vp, i32, f32, vp],
     "bz_affine_points": [vp, vp, vp, vp, i32, f32, vp],
     "bz_gather": [P6, vp, vp, i32, vp, i32, vp, vp, vp],

@@ -428,17 +428,56 @@ class FDTDObjective:
         return _F.apply(eps_box)
 
 
-def cone_filter_torch(rho, radius):
-    """Normalised cone filter with zero padding, as ``topology.cone_filter``, on (B, H, W) tensors."""
+_FILTER_TABLES = {}
+
+
+def _cone_tables(radius, H, W, dev):
+    """Device tables of the cone filter: the weights, the flipped weights and 1 / (K * 1) (FP64 on the host,
+    rounded to FP32), cached per (radius, H, W, device)."""
     import torch
-    import torch.nn.functional as F
-    r = int(radius)
-    yy, xx = np.mgrid[-r:r + 1, -r:r + 1]
-    K = np.maximum(0.0, float(r) - np.hypot(xx, yy))
-    k = torch.from_numpy(K).to(device=rho.device, dtype=rho.dtype)[None, None]
-    num = F.conv2d(rho[:, None], k, padding=r)[:, 0]
-    den = F.conv2d(torch.ones_like(rho[:1, None]), k, padding=r)[:, 0]
-    return num / den
+    key = (int(radius), H, W, str(dev))
+    t = _FILTER_TABLES.get(key)
+    if t is None:
+        from numpy.lib.stride_tricks import sliding_window_view
+        r = int(radius)
+        yy, xx = np.mgrid[-r:r + 1, -r:r + 1]
+        K = np.maximum(0.0, float(r) - np.hypot(xx, yy))
+        den = (sliding_window_view(np.pad(np.ones((H, W)), r), K.shape) * K).sum(axis=(-2, -1))
+        f32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev)
+        t = (f32(K), f32(K[::-1, ::-1]), f32(1.0 / den))
+        _FILTER_TABLES[key] = t
+    return t
+
+
+def cone_filter_torch(rho, radius):
+    """Normalised cone filter with zero padding, as ``topology.cone_filter``, on (B, H, W) FP32 device tensors,
+    differentiable: the filter and its gradient run as bz_filter2d."""
+    import torch
+
+    class _Cone(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, x):
+            B, H, W = x.shape
+            K, _, inv = _cone_tables(radius, H, W, x.device)
+            x = x.contiguous()
+            y = torch.empty_like(x)
+            N.call("bz_filter2d", x.data_ptr(), y.data_ptr(), B, H, W, K.data_ptr(), int(radius), None,
+                   inv.data_ptr(), torch.cuda.current_stream(x.device).cuda_stream)
+            return y
+
+        @staticmethod
+        def backward(ctx, g):
+            B, H, W = g.shape
+            _, Kf, inv = _cone_tables(radius, H, W, g.device)
+            g = g.contiguous()
+            gx = torch.empty_like(g)
+            N.call("bz_filter2d", g.data_ptr(), gx.data_ptr(), B, H, W, Kf.data_ptr(), int(radius), inv.data_ptr(),
+                   None, torch.cuda.current_stream(g.device).cuda_stream)
+            return gx
+
+    if rho.dim() != 3 or rho.dtype != torch.float32 or rho.device.type != "cuda":
+        raise ValueError("cone_filter_torch takes (B, H, W) float32 tensors on the GPU")
+    return _Cone.apply(rho)
 
 
 def tanh_projection_torch(x, beta, eta=0.5):

@@ -17,7 +17,7 @@ from __future__ import annotations
 import numpy as np
 
 from ..const import µm
-from ..workloads import EPS_OX, EPS_SI, batch_member_2d, binary_density
+from ..workloads import EPS_OX, EPS_SI, batch_member_2d, binary_densities
 
 PX = 128
 SIDE = 256
@@ -103,9 +103,7 @@ class TransmissionProblem:
 
 def random_densities(n, seed, px=PX):
     """The batched workload's density generator: seeded Bernoulli(0.5), 3-pixel Gaussian blur, threshold."""
-    rng = np.random.default_rng(seed)
-    return np.stack([binary_density(px, rng).astype(np.float32) for _ in range(n)]) if n else \
-        np.zeros((0, px, px), np.float32)
+    return binary_densities(n, px, np.random.default_rng(seed)).astype(np.float32)
 
 
 def generate(problem, n, seed=0):
@@ -115,7 +113,7 @@ def generate(problem, n, seed=0):
     B = len(problem.sims)
 
     def draw(m):
-        return np.stack([binary_density(problem.px, rng).astype(np.float32) for _ in range(m)])
+        return binary_densities(m, problem.px, rng).astype(np.float32)
 
     rho, T = [], []
     cur = draw(min(B, n)) if n else None

@@ -312,6 +312,14 @@ int bz_adj2d_runk(const bz_adj2d* adjs, int nadj, const int* tilesk, int ntilesk
 
 int bz_adj2d_inject(const bz_adj2d* adjs, int nadj, int max_inj, int n, void* stream);
 
+/* Density filter of the topology parametrisation (replaces the blur of examples/4_topo.py:120-129 in the
+ * product's filter -> projection chain): for each of B maps of H x W FP32 values,
+ *   out[y][x] = oscale[y][x] * sum_{|dy|,|dx| <= r} K[dy+r][dx+r] * iscale[y+dy][x+dx] * in[y+dy][x+dx]
+ * with zero padding (iscale and oscale may be NULL; K is (2r+1)^2 row-major, r <= 8; in != out). The
+ * normalised cone filter is oscale = 1 / (K * 1); its gradient is the flipped K with iscale = 1 / (K * 1). */
+int bz_filter2d(const float* in, float* out, int B, int H, int W, const float* K, int r, const float* iscale,
+                const float* oscale, void* stream);
+
 /* ---------------- CNN surrogate (BF16 MFMA, FP32 accumulate) ----------------
  * No reference counterpart: the surrogate maps a density map to port transmission spectra and,
  * by back-propagation, to d(spectrum)/d(density). Activations are NHWC bfloat16 (uint16 bit

@@ -339,3 +339,26 @@ def test_pair_sweeps_give_the_same_gradient_and_engines_are_reused(hip):
     for key in ("pairs", "k5", "k8"):
         for q in range(4):
             assert np.array_equal(out[key][q], out["one"][q]), (key, q)
+
+
+def test_cone_filter_kernel_and_its_gradient_match_oracle(hip):
+    """bz_filter2d through cone_filter_torch: the normalised cone filter and its gradient (the transpose) against
+    the oracle's FP64 cone_filter / cone_filter_T (whose adjointness the oracle tests pin), on maps that are not
+    multiples of the 32 x 8 tile, several radii and a batch of maps."""
+    torch = hip
+    from beamz_amd.optimization.adjoint import cone_filter_torch
+    rng = np.random.default_rng(9)
+    for B, H, W, r in ((3, 37, 50, 3), (1, 128, 128, 3), (2, 9, 70, 1), (1, 20, 20, 5)):
+        x = rng.random((B, H, W)).astype(np.float32)
+        g = rng.standard_normal((B, H, W)).astype(np.float32)
+        xt = torch.tensor(x, device="cuda", requires_grad=True)
+        y = cone_filter_torch(xt, r)
+        y.backward(torch.tensor(g, device="cuda"))
+        for b in range(B):
+            ref = TR.cone_filter(x[b].astype(np.float64), r)
+            gref = TR.cone_filter_T(g[b].astype(np.float64), r)
+            assert np.linalg.norm(y[b].detach().cpu().numpy() - ref) < 1e-6 * np.linalg.norm(ref), (B, H, W, r)
+            gx = xt.grad[b].cpu().numpy()
+            assert np.linalg.norm(gx - gref) < 1e-6 * np.linalg.norm(gref), (B, H, W, r)
+    with pytest.raises(ValueError, match="float32"):
+        cone_filter_torch(torch.zeros((1, 8, 8), dtype=torch.float64, device="cuda"), 3)

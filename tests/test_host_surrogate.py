@@ -25,3 +25,16 @@ def test_generate_pairs_densities_with_their_labels(n):
     assert np.array_equal(rho, random_densities(n, 3))
     assert T.shape == (n, 8) and T.dtype == np.float32
     assert np.allclose(T, rho.mean(axis=(1, 2))[:, None] / 2.0)
+
+
+def test_chunk_density_generator_equals_the_per_map_one():
+    """binary_densities (one draw for the chunk, the blur on a thread pool) gives the maps of binary_density
+    called once per map, bit for bit, and leaves the generator in the same state."""
+    from beamz_amd.workloads import binary_densities, binary_density
+    for m, n in ((0, 16), (1, 32), (37, 48)):
+        r1, r2 = np.random.default_rng(5), np.random.default_rng(5)
+        a = np.stack([binary_density(n, r1) for _ in range(m)]) if m else np.zeros((0, n, n), bool)
+        b = binary_densities(m, n, r2)
+        assert b.dtype == bool and b.shape == (m, n, n)
+        assert np.array_equal(a, b)
+        assert np.array_equal(r1.random(4), r2.random(4))
