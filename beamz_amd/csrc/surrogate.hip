@@ -842,6 +842,134 @@ __global__ __launch_bounds__(256) void k_conv_first_wgrad8(const uint16_t* __res
     }
 }
 
+// k_conv_first8 on chunks of RB whole rows of one image (blockIdx.x): the rows y0-1 .. y0+RB of x with a zero
+// column on each side are staged in LDS ([RB + 2][W + 2] floats, dynamic); thread t takes the 8 channels
+// 8 (t % G) .. of every (256 / G)-th pixel, with their 72 BF16-rounded weights in registers. Each channel adds
+// its nine taps in tap order, as k_conv_first8 does; a zero-padded tap adds a zero product to a sum that
+// starts at +0, which leaves it as k_conv_first8's skipped tap does, so the outputs are the same bits.
+__global__ __launch_bounds__(256) void k_conv_first_rows(const uint16_t* __restrict__ x, const float* __restrict__ w,
+                                                         const float* __restrict__ b, uint16_t* __restrict__ y,
+                                                         Geo g, int Co, int RB) {
+    extern __shared__ float xs[];
+    const int G = Co / 8, R = 256 / G;
+    const int t = threadIdx.x, gi = t % G, r = t / G;
+    const int c = 8 * gi;
+    const int W = g.W, WP = W + 2;
+    const int row0 = blockIdx.x * RB;
+    const int bi = row0 / g.H, y0 = row0 - bi * g.H;
+    for (int q = t; q < (RB + 2) * WP; q += 256) {
+        const int ry = q / WP, rx = q - ry * WP;
+        const int yy = y0 + ry - 1, xx = rx - 1;
+        xs[q] = (yy >= 0 && yy < g.H && xx >= 0 && xx < W) ? b2f(x[((long)bi * g.H + yy) * W + xx]) : 0.0f;
+    }
+    float wv[8][9], bv[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) wv[e][tap] = b2f(f2b(w[(c + e) * 9 + tap]));
+        bv[e] = b[c + e];
+    }
+    __syncthreads();
+    uint16_t* __restrict__ yc = y + (long)row0 * W * Co + c;
+    int ly = r / W, lx = r - (r / W) * W;
+    for (int q = r; q < RB * W; q += R) {
+        const float* xw = xs + ly * WP + lx;
+        float xv[9];
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) xv[tap] = xw[(tap / 3) * WP + tap % 3];
+        uint4 o;
+        uint16_t* oh = reinterpret_cast<uint16_t*>(&o);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float s = 0.0f;
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap) s += xv[tap] * wv[e][tap];
+            oh[e] = f2b(fmaxf(s + bv[e], 0.0f));
+        }
+        *reinterpret_cast<uint4*>(yc + (long)q * Co) = o;
+        lx += R;
+        while (lx >= W) {
+            lx -= W;
+            ++ly;
+        }
+    }
+}
+
+// k_conv_first_wgrad8 for a chunk of RB whole rows of one image: the rows y0-1 .. y0+RB of x, with a zero
+// column on each side, are staged in LDS once ([RB + 2][W + 2] floats, dynamic), so each pixel's nine taps
+// are LDS reads without bounds tests, and each thread steps through its pixels by row and column instead of
+// dividing. Partials as k_conv_first_wgrad8 (the same workgroups, layout and reduction).
+__global__ __launch_bounds__(256) void k_conv_first_wgrad_rows(const uint16_t* __restrict__ x,
+                                                               const uint16_t* __restrict__ gz,
+                                                               float* __restrict__ gw, float* __restrict__ gb,
+                                                               float* __restrict__ part, Geo g, int Co, int RB) {
+    extern __shared__ float xs[];
+    __shared__ float red[4][16][80];
+    const int G = Co / 8, R = 256 / G;
+    const int t = threadIdx.x, gi = t % G, r = t / G;
+    const int c = 8 * gi;
+    const int W = g.W, WP = W + 2;
+    const int row0 = blockIdx.x * RB;           // flattened image row b * H + y of the chunk's first row
+    const int b = row0 / g.H, y0 = row0 - b * g.H;
+    for (int q = t; q < (RB + 2) * WP; q += 256) {
+        const int ry = q / WP, rx = q - ry * WP;
+        const int yy = y0 + ry - 1, xx = rx - 1;
+        xs[q] = (yy >= 0 && yy < g.H && xx >= 0 && xx < W) ? b2f(x[((long)b * g.H + yy) * W + xx]) : 0.0f;
+    }
+    __syncthreads();
+    float acc[10][8];
+#pragma unroll
+    for (int q = 0; q < 10; ++q)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[q][e] = 0.0f;
+    const uint16_t* __restrict__ gzc = gz + (long)row0 * W * Co + c;
+    int ly = r / W, lx = r - (r / W) * W;
+    for (int q = r; q < RB * W; q += R) {
+        const uint4 v = *reinterpret_cast<const uint4*>(gzc + (long)q * Co);
+        const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+        float gv[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) gv[e] = b2f(h[e]);
+        const float* xw = xs + ly * WP + lx;    // tap (ky, kx) of this pixel is xw[ky * WP + kx]
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const float xv = xw[(tap / 3) * WP + tap % 3];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc[tap][e] += gv[e] * xv;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[9][e] += gv[e];
+        lx += R;
+        while (lx >= W) {
+            lx -= W;
+            ++ly;
+        }
+    }
+    for (int off = G; off < 64; off <<= 1)
+#pragma unroll
+        for (int q = 0; q < 10; ++q)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc[q][e] += __shfl_xor(acc[q][e], off);
+    const int wave = t >> 6, lane = t & 63;
+    if (lane < G)
+#pragma unroll
+        for (int q = 0; q < 10; ++q)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) red[wave][lane][q * 8 + e] = acc[q][e];
+    __syncthreads();
+    for (int k = t; k < G * 80; k += 256) {
+        const int gg = k / 80, qe = k % 80, q = qe / 8, e = qe % 8;
+        const float sum = red[0][gg][qe] + red[1][gg][qe] + red[2][gg][qe] + red[3][gg][qe];
+        const long o = q < 9 ? (long)(8 * gg + e) * 9 + q : 9L * Co + 8 * gg + e;
+        if (part)
+            part[(long)blockIdx.x * 10 * Co + o] = sum;
+        else if (q < 9)
+            atomicAdd(gw + o, sum);
+        else
+            atomicAdd(gb + 8 * gg + e, sum);
+    }
+}
+
 __global__ void k_conv_first_dgrad(const uint16_t* __restrict__ gz, const float* __restrict__ w, float* __restrict__ gx,
                                    Geo g, int Co) {
     const int P = g.B * g.H * g.W;
@@ -1441,7 +1569,15 @@ extern "C" int bz_nn_conv_first(const uint16_t* x, const float* w, const float* 
                                 int Co, void* stream) {
     BZ_REQUIRE(x && w && b && y && B > 0 && H > 0 && W > 0 && Co > 0, "bz_nn_conv_first: bad arguments");
     Geo g{B, H, W};
-    if (Co % 8 == 0 && Co <= 256 && ((uintptr_t)y & 15) == 0)
+    // chunks of whole rows of one image (about 2048 pixels) take the row-staged kernel
+    int rb = std::max(1, std::min(H, 2048 / W));
+    while (H % rb) --rb;
+    const size_t lds = (size_t)(rb + 2) * (W + 2) * sizeof(float);
+    const bool rows_ok = Co % 8 == 0 && 256 % (Co / 8) == 0 && lds <= 32768 && ((uintptr_t)y & 15) == 0;
+    if (rows_ok)
+        hipLaunchKernelGGL(k_conv_first_rows, dim3((unsigned)((long)B * H / rb)), dim3(256), lds, (hipStream_t)stream,
+                           x, w, b, y, g, Co, rb);
+    else if (Co % 8 == 0 && Co <= 256 && ((uintptr_t)y & 15) == 0)
         hipLaunchKernelGGL(k_conv_first8, grid1((long)B * H * W), dim3(256), 0, (hipStream_t)stream, x, w, b, y, g, Co);
     else
         hipLaunchKernelGGL(k_conv_first, grid1((long)B * H * W * Co), dim3(256), 0, (hipStream_t)stream, x, w, b, y, g,
@@ -1475,8 +1611,15 @@ extern "C" int bz_nn_conv_first_bwd(const uint16_t* x, const uint16_t* gz, const
         const long nblk = (P + chunk - 1) / chunk;
         BZ_REQUIRE(!work || nwork >= (nblk + tree_rows(nblk)) * 10 * Co,
                    "bz_nn_conv_first_bwd: workspace smaller than bz_nn_first_work");
-        hipLaunchKernelGGL(k_conv_first_wgrad8, dim3((unsigned)nblk), dim3(256), 0, st, x, gz, gw, gb, work, g, Co,
-                           chunk);
+        // chunks of whole rows of one image take the row-staged kernel
+        const long rb = chunk / W;
+        const size_t lds = (size_t)(rb + 2) * (W + 2) * sizeof(float);
+        if (rb > 0 && chunk % W == 0 && H % rb == 0 && lds <= 32768)
+            hipLaunchKernelGGL(k_conv_first_wgrad_rows, dim3((unsigned)nblk), dim3(256), lds, st, x, gz, gw, gb, work,
+                               g, Co, (int)rb);
+        else
+            hipLaunchKernelGGL(k_conv_first_wgrad8, dim3((unsigned)nblk), dim3(256), 0, st, x, gz, gw, gb, work, g,
+                               Co, chunk);
         if (work)   // rows of 10 Co: the 9 Co weight columns into gw, the Co bias columns into gb
             tree_sum2(SumSeg{work, 10L * Co, 9L * Co, gw, (int)nblk},
                       SumSeg{work + 9L * Co, 10L * Co, (long)Co, gb, (int)nblk}, work + nblk * 10L * Co, st);

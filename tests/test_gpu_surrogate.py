@@ -429,3 +429,40 @@ def test_first_layer_weight_gradient_sums_follow_the_tree_order(hip):
     assert Z == 1024, Z
     assert np.array_equal(gw.cpu().numpy().ravel(), _tree_f32(part[:, :9 * Co], np.zeros(9 * Co, np.float32)))
     assert np.array_equal(gb.cpu().numpy(), _tree_f32(part[:, 9 * Co:], np.zeros(Co, np.float32)))
+
+
+@pytest.mark.parametrize("B,H,W,Co", [(2, 32, 16, 32), (4, 16, 64, 64), (1, 32, 32, 16), (3, 8, 128, 32)])
+def test_first_layer_weight_gradient_on_staged_rows_matches_oracle(hip, B, H, W, Co):
+    """First-layer weight and bias gradients where a workgroup's chunk is whole rows of one image (the kernel that
+    stages those rows of the input in LDS): 2, 4 and 8 channel groups, one image or several per launch, chunks of
+    4 to 32 rows; through atomics and through the ordered partial sums, within 1e-5 of FP64. And the forward
+    convolution, which stages rows the same way."""
+    torch = hip
+    rng = np.random.default_rng(B * 100 + H + W + Co)
+    st = torch.cuda.current_stream().cuda_stream
+    x = S.bf16(rng.random((B, H, W, 1)))
+    w = S.bf16(rng.standard_normal((Co, 3, 3, 1)) * 0.3)
+    gz = S.bf16(rng.standard_normal((B, H, W, Co)))
+    _, dw, db = S.conv3x3_backward(x, w, gz)
+    xd, gzd = to16(torch, x), to16(torch, gz)
+    wd = torch.tensor(w.reshape(Co, 9), dtype=torch.float32).cuda()
+    need = N.lib().bz_nn_first_work(B, H, W, Co)
+    work = torch.empty(need, dtype=torch.float32).cuda()
+    for wk, nwk in ((None, 0), (work.data_ptr(), need)):
+        gw = torch.zeros((Co, 9), dtype=torch.float32).cuda()
+        gb = torch.zeros(Co, dtype=torch.float32).cuda()
+        N.call("bz_nn_conv_first_bwd", xd.data_ptr(), gzd.data_ptr(), wd.data_ptr(), gw.data_ptr(), gb.data_ptr(),
+               None, B, H, W, Co, wk, nwk, st)
+        assert rel_l2(gw.cpu().numpy(), dw.reshape(Co, 9)) < 1e-5, wk
+        assert rel_l2(gb.cpu().numpy(), db) < 1e-5, wk
+    # the forward on the same staged rows: within one bfloat16 step of FP64, and the bits of the per-element
+    # kernel (an output address that is not 16-byte aligned takes that one)
+    bias = rng.standard_normal(Co).astype(np.float32).astype(np.float64) * 0.1
+    bd = torch.tensor(bias, dtype=torch.float32).cuda()
+    y = torch.empty((B, H, W, Co), dtype=torch.int16).cuda()
+    N.call("bz_nn_conv_first", xd.data_ptr(), wd.data_ptr(), bd.data_ptr(), y.data_ptr(), B, H, W, Co, st)
+    ref = np.maximum(S.conv3x3(x, w, bias), 0.0)
+    assert np.all(np.abs(from16(y) - S.bf16(ref)) <= 2.0 ** -7 * np.abs(ref) + 1e-6)
+    y2 = torch.empty(B * H * W * Co + 8, dtype=torch.int16).cuda()
+    N.call("bz_nn_conv_first", xd.data_ptr(), wd.data_ptr(), bd.data_ptr(), y2.data_ptr() + 2, B, H, W, Co, st)
+    assert np.array_equal(y.cpu().numpy().ravel(), y2[1:1 + B * H * W * Co].cpu().numpy())
