@@ -65,9 +65,10 @@ class AdjointProblem2D:
     """
 
     REVERSE_SEGS2 = (4, 8, 16)      # pair-tile heights tried for the recompute and reverse sweeps
+    FULL_SNAPSHOT_BYTES = 48 << 30  # largest design-box history kept for every step ("auto")
 
     def __init__(self, sims, monitors, weights, box, checkpoint_every=None, seg=None, fscale=None, pairs=None,
-                 kstep=None):
+                 kstep=None, snapshots="auto"):
         self.sims = list(sims)
         self.monitors = list(monitors)
         if len(self.monitors) != len(self.sims):
@@ -92,6 +93,14 @@ class AdjointProblem2D:
         self.seg = seg
         self.pairs = pairs            # two steps per launch in all three sweeps (None: by size)
         self.kstep = kstep            # K steps per launch in all three sweeps (None: automatic, 0: off)
+        if snapshots not in ("auto", "full", "checkpoint"):
+            raise ValueError("snapshots must be 'auto', 'full' or 'checkpoint'")
+        # "full": the forward sweep keeps the design-box fields of every step and the reverse sweep reads them
+        # (two sweeps per gradient); "checkpoint": fields every K steps, each segment recomputed before its
+        # reverse (three sweeps, sqrt(N) memory); "auto": full while the history fits FULL_SNAPSHOT_BYTES and
+        # half the free device memory
+        self.snapshots = snapshots
+        self.full = None
         self._rev_tuned = False
         self.torch = self.sims[0].backend.torch
         self.dev = self.sims[0].backend.dev
@@ -110,12 +119,26 @@ class AdjointProblem2D:
             if s.grid_im is not None:
                 raise NotImplementedError("the adjoint runs real forward fields: build the simulations with "
                                           "complex_fields=False or real sources")
-        self.fwd = Fused2D(grids, seg=self.seg, max_steps=self.N, pairs=self.pairs, kstep=self.kstep)
-        if self.fwd.kstep and self.N >= 64:
-            # region shape of the forward sweep, timed once, among the shapes whose K divides the checkpoint
-            # interval (every segment then runs as whole K-step launches); the recompute and reverse sweeps
-            # start from it and are timed on their own in the first gradient
-            shapes = [c for c in self.fwd.KCANDIDATES if self.K % c[0] == 0]
+        x0, y0, sx, sy = self.box
+        B = len(grids)
+        history = B * self.N * 3 * (sy + 1) * (sx + 1) * 4
+        if self.snapshots == "auto":
+            self.full = history <= self.FULL_SNAPSHOT_BYTES and history <= T.cuda.mem_get_info(self.dev)[0] // 2
+        else:
+            self.full = self.snapshots == "full"
+        fsnaps = None
+        if self.full:
+            # the design-box fields of every step, written by the forward sweep: [design][step][3][sy+1][sx+1]
+            self.snap = T.zeros((B, self.N, 3, sy + 1, sx + 1), dtype=T.float32, device=self.dev)
+            fsnaps = [(x0 - 1, y0 - 1, sx + 1, sy + 1, self.snap[b]) for b in range(B)]
+        self.fwd = Fused2D(grids, seg=self.seg, max_steps=self.N, pairs=self.pairs, kstep=self.kstep,
+                           snapshots=fsnaps)
+        if self.kstep is None and self.fwd.kstep and self.N >= 64:
+            # region shape of the forward sweep (when the caller left it automatic), timed once, among the shapes whose K divides the checkpoint
+            # interval (every segment then runs as whole K-step launches; with the full history, among all);
+            # the reverse sweep (and the recompute sweep) start from it and are timed on their own in the first
+            # gradient
+            shapes = [c for c in self.fwd.KCANDIDATES if self.full or self.K % c[0] == 0]
             if len(shapes) > 1:
                 # 64 steps per shape: two launches of 8 steps ranked the shapes differently from run to run
                 self.fwd.tune_kstep(steps=64, candidates=shapes)
@@ -123,12 +146,14 @@ class AdjointProblem2D:
         elif self.fwd.pairs and self.N >= 24:
             self.fwd.tune_seg2(steps=10)
             self._reset()
-        x0, y0, sx, sy = self.box
-        self.snap = T.zeros((len(grids), self.K, 3, sy + 1, sx + 1), dtype=T.float32, device=self.dev)
-        snaps = [(x0 - 1, y0 - 1, sx + 1, sy + 1, self.snap[b]) for b in range(len(grids))]
+        snaps = None
+        if not self.full:
+            self.snap = T.zeros((B, self.K, 3, sy + 1, sx + 1), dtype=T.float32, device=self.dev)
+            snaps = [(x0 - 1, y0 - 1, sx + 1, sy + 1, self.snap[b]) for b in range(B)]
         # one-step tiles of the recompute engine, shared with the reverse kernel: the caller's height, else 2
         # rows (reverse sweep of 16 x 600^2 cells, 3000 steps: 107 / 110 / 128 ms at 2 / 4 / 8 rows; the
-        # 8-row kernel holds 144 VGPRs and runs 3 waves per SIMD)
+        # 8-row kernel holds 144 VGPRs and runs 3 waves per SIMD). With the full history this engine never
+        # runs: it only holds the reverse sweep's tile lists.
         seg = min(self.seg, 8) if self.seg is not None else 2
         self.rec = Fused2D(grids, seg=seg, monitors=False, snapshots=snaps, pairs=self.fwd.pairs,
                            seg2=self.fwd.seg2 if self.fwd.pairs else None, kstep=self.fwd.kstep,
@@ -158,6 +183,12 @@ class AdjointProblem2D:
         self._setup()          # first call: builds the engines; later calls: resets fields and monitor sums
         self.set_eps(eps_box)
         grids = [s.grid for s in self.sims]
+        if self.full:
+            # one run of every step; the engine writes the design-box history as it goes
+            self.fwd.run(0, self.N, coef=True)
+            for s in self.sims:
+                s.current_step = self.N
+            return self._objective()
         nseg = -(-self.N // self.K)
         if getattr(self, "_ckpt_buf", None) is None or len(self._ckpt_buf[0]) != nseg:
             # one buffer per grid, reused by every evaluation: (segment, component, ny, ld)
@@ -174,6 +205,10 @@ class AdjointProblem2D:
             n += cnt
         for s in self.sims:
             s.current_step = self.N
+        return self._objective()
+
+    def _objective(self):
+        T = self.torch
         c = T.from_numpy(self.c).to(self.dev)
         self.a = T.stack([self._overlap(s, m, c) for s, m in zip(self.sims, self.monitors)])   # (B, nf)
         fs = T.from_numpy(self.fscale).to(self.dev)
@@ -214,8 +249,10 @@ class AdjointProblem2D:
             d.inj_w = w.data_ptr()
             d.phasor = self._phasor[b].data_ptr()
             d.nf = self.nf
-            # the recompute engine's coefficient arrays (it refreshes them at the start of every segment)
-            cf = self.rec.coefs[b] if self.rec.kstep else None
+            # the coefficient arrays of the engine that last ran these materials: the forward engine with the full
+            # history, else the recompute engine (it refreshes them at the start of every segment)
+            eng = self.fwd if self.full else self.rec
+            cf = eng.coefs[b] if eng.kstep and eng.coefs is not None else None
             d.coef = cf.data_ptr() if cf is not None else None
             descs.append(d)
         return descs
@@ -239,7 +276,7 @@ class AdjointProblem2D:
     def gradient(self):
         """dJ/d eps_r over the design boxes, (B, sy, sx) FP64 torch tensor, for the last forward."""
         T = self.torch
-        if not self._ready or not hasattr(self, "ckpt"):
+        if not self._ready or not (self.full or hasattr(self, "ckpt")) or getattr(self, "J", None) is None:
             raise RuntimeError("call forward() first")
         x0, y0, sx, sy = self.box
         grids = [s.grid for s in self.sims]
@@ -261,6 +298,49 @@ class AdjointProblem2D:
 
         first = [True]
 
+        def reverse(n_hi, cnt, flip):
+            """Reverse steps n_hi-1 .. n_hi-cnt on the recompute engine's tile lists; returns flip."""
+            rec = self.rec
+            if rec.kstep:
+                N.call("bz_adj2d_runk", ddev.data_ptr(), B, rec.tilesk.data_ptr(), rec.ntilesk, rec.krows,
+                       rec.kwaves, rec.kstep, rec.tiles.data_ptr(), rec.ntiles, rec.seg, n_hi, cnt, flip, st)
+                return (flip + (cnt // rec.kstep) * (rec.kstep - 1)) & 1
+            if rec.pairs:
+                N.call("bz_adj2d_run2", ddev.data_ptr(), B, rec.tiles2.data_ptr(), rec.ntiles2,
+                       rec.seg2, rec.tiles.data_ptr(), rec.ntiles, rec.seg, n_hi, cnt, flip, st)
+                return flip ^ ((cnt // 2) & 1)
+            N.call("bz_adj2d_run", ddev.data_ptr(), B, rec.tiles.data_ptr(), rec.ntiles, rec.seg, n_hi, cnt, st)
+            return flip
+
+        def clear():
+            for pair in lam:
+                for t in pair:
+                    t.zero_()
+            grad.zero_()
+
+        if self.full:
+            # the forward sweep left the design-box fields of every step (snapshot slot n = step n): no
+            # recompute, one reverse sweep over all steps
+            if self.kstep is None and self.rec.kstep and not self._rev_tuned and self.N >= 96:
+                ms = {}
+                for shape in self.rec.KCANDIDATES:
+                    self.rec.set_kstep(*shape)
+                    reverse(self.N, 48, 0)                            # warm-up
+                    e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
+                    e0.record()
+                    reverse(self.N, 48, 0)
+                    e1.record()
+                    T.cuda.synchronize(self.dev)
+                    ms[shape] = e0.elapsed_time(e1)
+                self.rec.set_kstep(*min(ms, key=ms.get))
+                self.rev_timings = ms
+                clear()
+            self._rev_tuned = True
+            N.call("bz_adj2d_inject", ddev.data_ptr(), B, max_inj, self.N, st)
+            reverse(self.N, self.N, 0)
+            self._keep = (lam, inj, ddev)
+            return grad
+
         def segment(k, flip):
             """Recompute segment k from its checkpoint (snapshots on), then walk it backwards; returns flip."""
             n0 = k * self.K
@@ -274,21 +354,10 @@ class AdjointProblem2D:
             # stream-ordered upload from pinned memory: the host does not wait for the GPU to drain
             raw = T.frombuffer(bytearray(b"".join(bytes(d) for d in descs)), dtype=T.uint8).pin_memory()
             ddev.copy_(raw, non_blocking=True)
-            rec = self.rec
-            if rec.kstep:
-                N.call("bz_adj2d_runk", ddev.data_ptr(), B, rec.tilesk.data_ptr(), rec.ntilesk, rec.krows,
-                       rec.kwaves, rec.kstep, rec.tiles.data_ptr(), rec.ntiles, rec.seg, n0 + cnt, cnt, flip, st)
-                return (flip + (cnt // rec.kstep) * (rec.kstep - 1)) & 1
-            if rec.pairs:
-                N.call("bz_adj2d_run2", ddev.data_ptr(), B, rec.tiles2.data_ptr(), rec.ntiles2,
-                       rec.seg2, rec.tiles.data_ptr(), rec.ntiles, rec.seg, n0 + cnt, cnt, flip, st)
-                return flip ^ ((cnt // 2) & 1)
-            N.call("bz_adj2d_run", ddev.data_ptr(), B, self.rec.tiles.data_ptr(), self.rec.ntiles, self.rec.seg,
-                   n0 + cnt, cnt, st)
-            return flip
+            return reverse(n0 + cnt, cnt, flip)
 
         nseg = len(self.ckpt)
-        if self.rec.kstep and not self._rev_tuned and nseg > 1:
+        if self.kstep is None and self.rec.kstep and not self._rev_tuned and nseg > 1:
             # the recompute and reverse sweeps share the K-step tiles, and the reverse kernel ranks the region
             # shapes differently from the forward one: time one whole segment of both per shape whose K divides
             # the checkpoint interval and keep the fastest (the adjoint state and gradient are cleared below)
@@ -307,10 +376,7 @@ class AdjointProblem2D:
                     ms[shape] = e0.elapsed_time(e1)
                 self.rec.set_kstep(*min(ms, key=ms.get))
                 self.rev_timings = ms
-                for pair in lam:
-                    for t in pair:
-                        t.zero_()
-                grad.zero_()
+                clear()
             self._rev_tuned = True
         if self.rec.pairs and not self.rec.kstep and not self._rev_tuned and nseg > 1:
             # the recompute and reverse sweeps share the pair tiles: time one segment of both per tile height
@@ -327,10 +393,7 @@ class AdjointProblem2D:
                 ms[seg2] = e0.elapsed_time(e1)
             self.rec.set_seg2(min(ms, key=ms.get))
             self._rev_tuned = True
-            for pair in lam:
-                for t in pair:
-                    t.zero_()
-            grad.zero_()
+            clear()
         N.call("bz_adj2d_inject", ddev.data_ptr(), B, max_inj, self.N, st)
         flip = 0                    # multi-step launches move the adjoint state across its two buffers
         for k in range(nseg - 1, -1, -1):

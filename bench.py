@@ -270,10 +270,11 @@ def build_runner(a, total):
                 rho.grad = None
                 J = FDTDObjective.apply(density_to_eps(rho, ec, eh, radius=3, beta=8.0), prob)
                 J.sum().backward()
-        cells = 3 * B * sims[0].nx * sims[0].ny * a.nt        # forward, recompute, reverse sweeps per gradient
+        # cell-updates of one gradient, sweep count set once the engines exist (2 with the design-box history of
+        # every step, 3 with checkpoints and a recompute sweep)
+        cells = B * sims[0].nx * sims[0].ny * a.nt
         desc = (f"adjoint gradient of a mode-overlap FOM, batch of {B} designs, {sims[0].nx}x{sims[0].ny} 2D grid, "
-                f"{box[2]}x{box[3]} design px, {a.nt} steps, sqrt(N) checkpoints; a step = one batched gradient "
-                f"(forward + recompute + reverse sweeps)")
+                f"{box[2]}x{box[3]} design px, {a.nt} steps; a step = one batched gradient")
 
         def cpu():
             from oracle import fdtd_ref as R
@@ -449,6 +450,11 @@ def main():
         del src, dst
     r.advance(a.warmup)
     torch.cuda.synchronize()
+    if a.workload == "adjoint2d":             # the engines exist after the warm-up gradient
+        r.sweeps = 2 if r.prob.full else 3
+        r.cells *= r.sweeps
+        r.desc += (" (forward sweep keeping the design-box fields of every step, reverse sweep)" if r.prob.full else
+                   " (forward sweep with sqrt(N) checkpoints, recompute sweep, reverse sweep)")
     if a.workload == "wg2d":                  # the 2D engine is built on the first step
         f = r.grid._f2d
         r.kernel_name = r.grid.kernel_name()
@@ -470,7 +476,7 @@ def main():
     r.grid.timing = False
     step_kernel_s = lt if lt else gpu_s / a.steps
     b_alg = B_ALG[a.workload]
-    launch_cells = r.cells if not getattr(r, "per_step_cells", False) else r.cells // (3 * a.nt)
+    launch_cells = r.cells if not getattr(r, "per_step_cells", False) else r.cells // (r.sweeps * a.nt)
     achieved = launch_cells * b_alg / step_kernel_s
     nx, ny = r.grid.nx, r.grid.ny
     if a.workload == "mmi3d" and a.tb3:
@@ -486,7 +492,12 @@ def main():
     working_set = sum(g._flat.numel() * g._flat.element_size() for g in getattr(r, "grids", [r.grid]))
     if a.workload == "adjoint2d":             # the sweeps' engines exist once a gradient has run
         f = r.prob.fwd
-        if f.kstep:
+        if f.kstep and r.prob.full:
+            rc = r.prob.rec
+            r.kernel_name = (f"forward sweep (design-box snapshots of every step): bz_yee2d_runk, {f.kstep} steps "
+                             f"per pass, 64x{f.krows * f.kwaves} regions; reverse sweep: bz_adj2d_runk, {rc.kstep} "
+                             f"steps per pass, 64x{rc.krows * rc.kwaves} regions")
+        elif f.kstep:
             rc = r.prob.rec
             r.kernel_name = (f"forward sweep: bz_yee2d_runk, {f.kstep} steps per pass, 64x{f.krows * f.kwaves} "
                              f"regions; recompute sweep (with design-box snapshots): bz_yee2d_runk, and reverse "

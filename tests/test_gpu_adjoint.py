@@ -60,19 +60,22 @@ def oracle_problem(sim, mon, src, eps, sig, c):
 
 
 def test_adjoint_gradient_matches_fp64_oracle_and_checkpointing(hip):
+    """The gradient against the FP64 adjoint oracle; the design-box history of every step (two sweeps) and
+    checkpoints every 20 or 23 steps with a recompute sweep give the same bits."""
     torch = hip
     eps, sig, rng = materials()
     nsteps = 420
     grads = {}
-    for K in (20, 23):
+    for K, mode in ((20, "checkpoint"), (23, "checkpoint"), (None, "full")):
         sim, mon, src, t = build(eps, sig, nsteps)
         npts = len(mon.get_grid_points_2d(DX, DX))
         c = np.random.default_rng(7).standard_normal(npts) + 1j * np.random.default_rng(8).standard_normal(npts)
-        prob = AdjointProblem2D([sim], [mon], c, BOX, checkpoint_every=K)
+        prob = AdjointProblem2D([sim], [mon], c, BOX, checkpoint_every=K, snapshots=mode)
         x0, y0, sx, sy = BOX
         e_box = torch.from_numpy(eps[y0:y0 + sy, x0:x0 + sx]).float().cuda()[None]
         J = prob.forward(e_box).cpu().numpy()
         grads[K] = prob.gradient().cpu().numpy()[0]
+        assert prob.full == (mode == "full")
     P = oracle_problem(sim, mon, src, eps.astype(np.float32).astype(np.float64), sig.astype(np.float32).astype(np.float64), c)
     g_ref = P.gradient()
     assert abs(J[0] - P.J) <= 1e-4 * P.J, (J[0], P.J)
@@ -80,6 +83,7 @@ def test_adjoint_gradient_matches_fp64_oracle_and_checkpointing(hip):
     err = np.linalg.norm(grads[20] - gb) / np.linalg.norm(gb)
     assert err < 1e-3, err
     assert np.array_equal(grads[20], grads[23])          # checkpoint interval changes nothing
+    assert np.array_equal(grads[20], grads[None])        # nor keeping every step's fields
 
 
 def test_adjoint_gradient_matches_gpu_finite_differences(hip):
@@ -303,20 +307,26 @@ def test_pair_sweeps_give_the_same_gradient_and_engines_are_reused(hip):
     adjoint buffers an odd number of times) and in K-step launches (K = 5 and 8, whole launches then pairs or
     single steps) give the one-step gradient bit for bit; later evaluations on the same problem (engines
     reused, fields and monitor sums reset in place) repeat it, also after a different design was evaluated in
-    between, and that in-between design's J and gradient equal those of a freshly built problem."""
+    between, and that in-between design's J and gradient equal those of a freshly built problem. All of it with
+    checkpoints and a recompute sweep, and with the design-box history of every step kept by the forward sweep
+    (one reverse call over all 300 steps)."""
     torch = hip
     eps, sig, _ = materials(seed=2)
     x0, y0, sx, sy = BOX
     e_box = torch.from_numpy(eps[y0:y0 + sy, x0:x0 + sx]).float().cuda()[None]
     out = {}
-    for key, pairs, kstep in (("one", False, 0), ("pairs", True, 0), ("k5", True, 5), ("k8", False, 8)):
+    for mode, key, pairs, kstep in [(m, *v) for m in ("checkpoint", "full") for v in
+                                    (("one", False, 0), ("pairs", True, 0), ("k5", True, 5), ("k8", False, 8))]:
         sim, mon, src, t = build(eps, sig, 300)
         npts = len(mon.get_grid_points_2d(DX, DX))
         c = np.random.default_rng(7).standard_normal(npts) + 1j * np.random.default_rng(8).standard_normal(npts)
-        prob = AdjointProblem2D([sim], [mon], c, BOX, checkpoint_every=23, pairs=pairs, kstep=kstep)
+        prob = AdjointProblem2D([sim], [mon], c, BOX, checkpoint_every=23, pairs=pairs, kstep=kstep,
+                                snapshots=mode)
+        assert prob.snapshots == mode
         J = prob.forward(e_box).cpu().numpy()
         g = prob.gradient().cpu().numpy()
-        out[key] = (J, g)
+        out[mode, key] = (J, g)
+        assert prob.full == (mode == "full")
         assert prob.fwd.pairs == pairs and prob.rec.pairs == pairs
         assert prob.fwd.kstep == kstep and prob.rec.kstep == kstep
         fwd, rec = prob.fwd, prob.rec
@@ -326,19 +336,20 @@ def test_pair_sweeps_give_the_same_gradient_and_engines_are_reused(hip):
         Jm = prob.forward(e_box * 1.01).cpu().numpy()
         gm = prob.gradient().cpu().numpy()
         sim_f, mon_f, _, _ = build(eps, sig, 300)
-        fresh = AdjointProblem2D([sim_f], [mon_f], c, BOX, checkpoint_every=23, pairs=pairs, kstep=kstep)
+        fresh = AdjointProblem2D([sim_f], [mon_f], c, BOX, checkpoint_every=23, pairs=pairs, kstep=kstep,
+                                 snapshots=mode)
         Jf = fresh.forward(e_box * 1.01).cpu().numpy()
         gf = fresh.gradient().cpu().numpy()
-        assert np.array_equal(Jm, Jf) and np.array_equal(gm, gf), key
-        assert not np.array_equal(gm, g), key
+        assert np.array_equal(Jm, Jf) and np.array_equal(gm, gf), (mode, key)
+        assert not np.array_equal(gm, g), (mode, key)
         J2 = prob.forward(e_box).cpu().numpy()
         g2 = prob.gradient().cpu().numpy()
         assert prob.fwd is fwd and prob.rec is rec
-        assert np.array_equal(J2, J) and np.array_equal(g2, g), key
-        out[key] += (Jm, gm)
-    for key in ("pairs", "k5", "k8"):
+        assert np.array_equal(J2, J) and np.array_equal(g2, g), (mode, key)
+        out[mode, key] += (Jm, gm)
+    for mode, key in out:
         for q in range(4):
-            assert np.array_equal(out[key][q], out["one"][q]), (key, q)
+            assert np.array_equal(out[mode, key][q], out["checkpoint", "one"][q]), (mode, key, q)
 
 
 def test_cone_filter_kernel_and_its_gradient_match_oracle(hip):
