@@ -79,9 +79,14 @@ __device__ __forceinline__ void inject_at(const bz_adj2d& A, float& lnew, int i,
 // dJ/deps of reverse step n at an interior cell (j, i) of the design box: l2 (dca/deps Ez_n + dcb/deps curlH),
 // from the forward snapshot of step n (slot n - snap_step0; the snapshot box is the design box grown by one
 // cell on the low x and y sides).
+// With l2hist set, only l2 is stored (slot n - snap_step0) and k_adj2d_grad adds the term later.
 __device__ __forceinline__ void grad_at(const bz_adj2d& A, const Coef& c, int i, int j, int n, float l2, float ee,
                                         float sg, float cb, float rdx, float rdy) {
     const int a = i - A.box_x0, b = j - A.box_y0;
+    if (a >= 0 && a < A.box_sx && b >= 0 && b < A.box_sy && A.l2hist) {
+        A.l2hist[(long)(n - A.snap_step0) * A.box_sx * A.box_sy + (long)b * A.box_sx + a] = l2;
+        return;
+    }
     if (a >= 0 && a < A.box_sx && b >= 0 && b < A.box_sy) {
         const int sx = A.box_sx + 1, sy = A.box_sy + 1;
         const long plane = (long)sx * sy;
@@ -450,8 +455,14 @@ __global__ __launch_bounds__(LANES* KW) void k_adj2d_tbk(const bz_adj2d* __restr
             if (inj_band) inject_at(A, lnew, i, j, n);
             const int rr = w * B + r;
             if (grad_band && in && lane_inner && rr >= K && rr < RH - K) {
-                const uint32_t o = ((uint32_t)min(max(j, 0), ny - 1) * (uint32_t)ld + ic) * 4u;
-                grad_at(A, c, i, j, n, l2, at(A.eps, o), at(A.sig, o), cb[r], rdx, rdy);
+                if (A.l2hist) {
+                    const int a = i - A.box_x0, bb = j - A.box_y0;
+                    if (a >= 0 && a < A.box_sx && bb >= 0 && bb < A.box_sy)
+                        A.l2hist[(long)(n - A.snap_step0) * A.box_sx * A.box_sy + (long)bb * A.box_sx + a] = l2;
+                } else {
+                    const uint32_t o = ((uint32_t)min(max(j, 0), ny - 1) * (uint32_t)ld + ic) * 4u;
+                    grad_at(A, c, i, j, n, l2, at(A.eps, o), at(A.sig, o), cb[r], rdx, rdy);
+                }
             }
             le[r] = lnew;
         }
@@ -530,6 +541,57 @@ extern "C" int bz_adj2d_run2(const bz_adj2d* adjs, int nadj, const int* tiles2, 
         flip ^= 1;
     }
     if (nsteps % 2) return run_seg(adjs, tiles, ntiles, seg, n_hi - 2 * npairs, 1, flip, st);
+    return BZ_OK;
+}
+
+// The gradient terms of steps n_hi-1 .. n_lo of one design cell (x: cell of the box, y: adjoint), added in
+// that order to grad: grad_at's expressions, with the coefficient derivatives computed once per cell.
+__global__ __launch_bounds__(256) void k_adj2d_grad(const bz_adj2d* __restrict__ adjs, int n_lo, int n_hi) {
+    const bz_adj2d& A = adjs[blockIdx.y];
+    const int nbox = A.box_sx * A.box_sy;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nbox || !A.grad || !A.l2hist) return;
+    const int b = q / A.box_sx, a = q - b * A.box_sx;
+    const long o = (long)(A.box_y0 + b) * A.ld + (A.box_x0 + a);
+    const float ee = A.eps[o], sg = A.sig[o];
+    Coef c;
+    c.kh = A.kh;
+    c.kbh = A.kbh;
+    c.ke = A.ke;
+    c.kbe = A.kbe;
+    float ca, cb;
+    e_coef(sg, ee, c, ca, cb);
+    const float e = ee;
+    const float s = sg * c.ke / e;
+    const float dca = 2.0f * s / (e * (1.0f + s) * (1.0f + s));
+    const float dcb = -cb / (e * (1.0f + s));
+    const float rdx = A.rdx, rdy = A.rdy;
+    const int sx = A.box_sx + 1;
+    const long plane = (long)sx * (A.box_sy + 1);
+    const float* __restrict__ snap = A.snap + (long)(b + 1) * sx + (a + 1);
+    const float* __restrict__ l2h = A.l2hist + q;
+    double acc = A.grad[q];
+#pragma unroll 8
+    for (int n = n_hi - 1; n >= n_lo; --n) {
+        const long slot = n - A.snap_step0;
+        const float* p = snap + slot * 3 * plane;
+        const float en = p[0];
+        const float* hx = p + plane;
+        const float* hy = p + 2 * plane;
+        const float curl = -((hx[0] - hx[-1]) * rdy) + (hy[0] - hy[-sx]) * rdx;
+        const float l2 = l2h[slot * nbox];
+        acc += (double)l2 * ((double)dca * en + (double)dcb * curl);
+    }
+    A.grad[q] = acc;
+}
+
+extern "C" int bz_adj2d_grad(const bz_adj2d* adjs, int nadj, int max_box, int n_lo, int n_hi, void* stream) {
+    BZ_REQUIRE(adjs && nadj > 0 && nadj <= 65535 && max_box >= 0 && n_lo >= 0 && n_hi >= n_lo,
+               "bz_adj2d_grad: bad arguments");
+    if (max_box == 0 || n_hi == n_lo) return BZ_OK;
+    hipLaunchKernelGGL(k_adj2d_grad, dim3((max_box + 255) / 256, nadj), dim3(256), 0, (hipStream_t)stream, adjs,
+                       n_lo, n_hi);
+    BZ_CHECK_LAUNCH("bz_adj2d_grad");
     return BZ_OK;
 }
 

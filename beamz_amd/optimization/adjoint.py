@@ -68,7 +68,7 @@ class AdjointProblem2D:
     FULL_SNAPSHOT_BYTES = 48 << 30  # largest design-box history kept for every step ("auto")
 
     def __init__(self, sims, monitors, weights, box, checkpoint_every=None, seg=None, fscale=None, pairs=None,
-                 kstep=None, snapshots="auto"):
+                 kstep=None, snapshots="auto", gradient_pass=True):
         self.sims = list(sims)
         self.monitors = list(monitors)
         if len(self.monitors) != len(self.sims):
@@ -100,6 +100,10 @@ class AdjointProblem2D:
         # reverse (three sweeps, sqrt(N) memory); "auto": full while the history fits FULL_SNAPSHOT_BYTES and
         # half the free device memory
         self.snapshots = snapshots
+        # gradient_pass: the reverse kernels store each design cell's lambda value per step (l2hist) and
+        # bz_adj2d_grad adds the gradient terms afterwards, in the same order (the same bits); False adds them
+        # inside the reverse kernels
+        self.gradient_pass = bool(gradient_pass)
         self.full = None
         self._rev_tuned = False
         self.torch = self.sims[0].backend.torch
@@ -121,7 +125,7 @@ class AdjointProblem2D:
                                           "complex_fields=False or real sources")
         x0, y0, sx, sy = self.box
         B = len(grids)
-        history = B * self.N * 3 * (sy + 1) * (sx + 1) * 4
+        history = B * self.N * (3 * (sy + 1) * (sx + 1) + (sy * sx if self.gradient_pass else 0)) * 4
         if self.snapshots == "auto":
             self.full = history <= self.FULL_SNAPSHOT_BYTES and history <= T.cuda.mem_get_info(self.dev)[0] // 2
         else:
@@ -150,6 +154,9 @@ class AdjointProblem2D:
         if not self.full:
             self.snap = T.zeros((B, self.K, 3, sy + 1, sx + 1), dtype=T.float32, device=self.dev)
             snaps = [(x0 - 1, y0 - 1, sx + 1, sy + 1, self.snap[b]) for b in range(B)]
+        # the lambda history of the design cells, slot n - snap_step0 as the snapshots
+        self.l2hist = (T.empty((B, self.N if self.full else self.K, sy, sx), dtype=T.float32, device=self.dev)
+                       if self.gradient_pass else None)
         # one-step tiles of the recompute engine, shared with the reverse kernel: the caller's height, else 2
         # rows (reverse sweep of 16 x 600^2 cells, 3000 steps: 107 / 110 / 128 ms at 2 / 4 / 8 rows; the
         # 8-row kernel holds 144 VGPRs and runs 3 waves per SIMD). With the full history this engine never
@@ -244,6 +251,7 @@ class AdjointProblem2D:
             d.box_x0, d.box_y0, d.box_sx, d.box_sy = x0, y0, sx, sy
             d.snap = self.snap[b].data_ptr()
             d.grad = grad[b].data_ptr()
+            d.l2hist = self.l2hist[b].data_ptr() if self.l2hist is not None else None
             ix0, iy0, isx, isy, w = inj[b]
             d.inj_x0, d.inj_y0, d.inj_sx, d.inj_sy = ix0, iy0, isx, isy
             d.inj_w = w.data_ptr()
@@ -299,17 +307,21 @@ class AdjointProblem2D:
         first = [True]
 
         def reverse(n_hi, cnt, flip):
-            """Reverse steps n_hi-1 .. n_hi-cnt on the recompute engine's tile lists; returns flip."""
+            """Reverse steps n_hi-1 .. n_hi-cnt on the recompute engine's tile lists, then (with the gradient
+            pass) their gradient terms; returns flip."""
             rec = self.rec
             if rec.kstep:
                 N.call("bz_adj2d_runk", ddev.data_ptr(), B, rec.tilesk.data_ptr(), rec.ntilesk, rec.krows,
                        rec.kwaves, rec.kstep, rec.tiles.data_ptr(), rec.ntiles, rec.seg, n_hi, cnt, flip, st)
-                return (flip + (cnt // rec.kstep) * (rec.kstep - 1)) & 1
-            if rec.pairs:
+                flip = (flip + (cnt // rec.kstep) * (rec.kstep - 1)) & 1
+            elif rec.pairs:
                 N.call("bz_adj2d_run2", ddev.data_ptr(), B, rec.tiles2.data_ptr(), rec.ntiles2,
                        rec.seg2, rec.tiles.data_ptr(), rec.ntiles, rec.seg, n_hi, cnt, flip, st)
-                return flip ^ ((cnt // 2) & 1)
-            N.call("bz_adj2d_run", ddev.data_ptr(), B, rec.tiles.data_ptr(), rec.ntiles, rec.seg, n_hi, cnt, st)
+                flip ^= (cnt // 2) & 1
+            else:
+                N.call("bz_adj2d_run", ddev.data_ptr(), B, rec.tiles.data_ptr(), rec.ntiles, rec.seg, n_hi, cnt, st)
+            if self.l2hist is not None:
+                N.call("bz_adj2d_grad", ddev.data_ptr(), B, sx * sy, n_hi - cnt, n_hi, st)
             return flip
 
         def clear():
@@ -321,14 +333,14 @@ class AdjointProblem2D:
         if self.full:
             # the forward sweep left the design-box fields of every step (snapshot slot n = step n): no
             # recompute, one reverse sweep over all steps
-            if self.kstep is None and self.rec.kstep and not self._rev_tuned and self.N >= 96:
+            if self.kstep is None and self.rec.kstep and not self._rev_tuned and self.N >= 192:
                 ms = {}
                 for shape in self.rec.KCANDIDATES:
                     self.rec.set_kstep(*shape)
-                    reverse(self.N, 48, 0)                            # warm-up
+                    reverse(self.N, 96, 0)                            # warm-up
                     e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
                     e0.record()
-                    reverse(self.N, 48, 0)
+                    reverse(self.N, 96, 0)
                     e1.record()
                     T.cuda.synchronize(self.dev)
                     ms[shape] = e0.elapsed_time(e1)

@@ -287,6 +287,10 @@ typedef struct bz_adj2d {
     int nf;                               /* number of objective frequencies */
     const float* coef;                    /* NULL, or [ny][ld][4] update coefficients as bz_yee2d_coef writes
                                            * them for the same eps and sig (read by bz_adj2d_runk) */
+    float* l2hist;                        /* NULL, or [K][box_sy][box_sx]: the reverse kernels store there the
+                                           * lambda value each design cell's gradient term takes at step n (slot
+                                           * n - snap_step0) instead of adding the term into grad;
+                                           * bz_adj2d_grad then adds the terms, in the same order */
 } bz_adj2d;
 
 /* Reverse steps n = n_hi-1 down to n_hi-nsteps of every adjoint described in `adjs` (device
@@ -311,6 +315,12 @@ int bz_adj2d_runk(const bz_adj2d* adjs, int nadj, const int* tilesk, int ntilesk
                   const int* tiles, int ntiles, int seg, int n_hi, int nsteps, int flip, void* stream);
 
 int bz_adj2d_inject(const bz_adj2d* adjs, int nadj, int max_inj, int n, void* stream);
+
+/* grad[cell] += the gradient terms of reverse steps n = n_hi-1 down to n_lo, in that order, from the
+ * snapshots and the l2hist values the reverse kernels stored (slot n - snap_step0), for every adjoint with
+ * grad and l2hist set: the same bits as adding each term inside the reverse kernels, with the per-step
+ * read-modify-write of grad taken off the reverse sweep. max_box >= every box_sx * box_sy. */
+int bz_adj2d_grad(const bz_adj2d* adjs, int nadj, int max_box, int n_lo, int n_hi, void* stream);
 
 /* Density filter of the topology parametrisation (replaces the blur of examples/4_topo.py:120-129 in the
  * product's filter -> projection chain): for each of B maps of H x W FP32 values,

@@ -373,3 +373,29 @@ def test_cone_filter_kernel_and_its_gradient_match_oracle(hip):
             assert np.linalg.norm(gx - gref) < 1e-6 * np.linalg.norm(gref), (B, H, W, r)
     with pytest.raises(ValueError, match="float32"):
         cone_filter_torch(torch.zeros((1, 8, 8), dtype=torch.float64, device="cuda"), 3)
+
+
+def test_gradient_pass_equals_in_kernel_accumulation(hip):
+    """The reverse kernels storing each design cell's lambda value per step and bz_adj2d_grad adding the terms
+    afterwards give the bits of the terms added inside the reverse kernels: with the full history and with
+    checkpoints (23 steps, so segments end with single steps), for one-step, pair and K-step sweeps."""
+    torch = hip
+    eps, sig, _ = materials(seed=5)
+    x0, y0, sx, sy = BOX
+    e_box = torch.from_numpy(eps[y0:y0 + sy, x0:x0 + sx]).float().cuda()[None]
+    ref = None
+    for mode in ("full", "checkpoint"):
+        for pairs, kstep in ((False, 0), (True, 0), (False, 8)):
+            for gp in (False, True):
+                sim, mon, src, t = build(eps, sig, 260)
+                npts = len(mon.get_grid_points_2d(DX, DX))
+                c = np.exp(1j * np.linspace(0, 2, npts)) / np.sqrt(npts)
+                prob = AdjointProblem2D([sim], [mon], c, BOX, checkpoint_every=23, pairs=pairs, kstep=kstep,
+                                        snapshots=mode, gradient_pass=gp)
+                prob.forward(e_box)
+                g = prob.gradient().cpu().numpy()
+                assert (prob.l2hist is not None) == gp
+                if ref is None:
+                    ref = g
+                assert np.array_equal(g, ref), (mode, pairs, kstep, gp)
+    assert np.abs(ref).max() > 0
