@@ -674,6 +674,7 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
     constexpr int RH = KW * B;
     __shared__ float ez_top[KW][LANES];
     __shared__ float hy_bot[KW][LANES];
+    __shared__ float amps[32];          // the signal's K <= 31 samples of this launch
     extern __shared__ float smp[];      // [3][RH][LANES] when there are monitor points (launch-sized)
     const int npts = G.npts;
     const int t = blockIdx.x;
@@ -769,10 +770,11 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
     const int epnt = ef < e1 ? ents[2 * ef] : -1;
     const int ecode = ef < e1 ? ents[2 * ef + 1] : 0;
 
-    float amp_next = sgl ? gload(sgl, n0) : 0.0f;
+    // the launch's signal samples, staged in LDS and read after the step's barriers: a vector load inside the
+    // step loop would wait behind the snapshot and staging stores of the previous step (the vector memory
+    // counter retires in order)
+    if (sgl && tid < K) amps[tid] = gload(sgl, n0 + tid);
     for (int k = 0; k < K; ++k) {
-        const float amp = amp_next;              // the next step's amplitude loads during this step
-        if (sgl && k + 1 < K) amp_next = gload(sgl, n0 + k + 1);
         // H of step n0+k+1: Hy of the band's last row needs Ez of the next band's first row
         ez_top[w][l] = ez[0];
         __syncthreads();
@@ -831,6 +833,7 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
                 const int j = jb + r;
                 if (j >= sy0 && j < sy0 + ssy && src_col) {
                     const long rr = (long)(j - sy0) * ssx + (i - sx0);
+                    const float amp = sgl ? amps[k] : 0.0f;        // written before the first barrier
                     const float a = SEL ? box_amp(S, rr, n0 + k, amp) : amp;
                     ez[r] = (smul ? gload(smul, rr) : 1.0f) * ez[r] + gload(sadd, rr) * a;
                 }

@@ -216,9 +216,12 @@ class AdjointProblem2D:
 
     def _objective(self):
         T = self.torch
-        c = T.from_numpy(self.c).to(self.dev)
+        if getattr(self, "_c_dev", None) is None:
+            # device copies made once: a copy from pageable host memory waits for the queued sweep to finish
+            self._c_dev = T.from_numpy(self.c).to(self.dev)
+            self._fs_dev = T.from_numpy(self.fscale).to(self.dev)
+        c, fs = self._c_dev, self._fs_dev
         self.a = T.stack([self._overlap(s, m, c) for s, m in zip(self.sims, self.monitors)])   # (B, nf)
-        fs = T.from_numpy(self.fscale).to(self.dev)
         self.Jf = self.a.real ** 2 + self.a.imag ** 2
         self.J = (self.Jf * fs).sum(dim=1)
         return self.J
@@ -265,21 +268,60 @@ class AdjointProblem2D:
             descs.append(d)
         return descs
 
-    def _injection(self, b):
-        """Dense box of dJ/dEz-sample weights 2 s_f conj(a_f) c_fp per frequency over the monitor's Ez points."""
+    def _injection_geometry(self, b):
+        """The injection box of design b's objective monitor (its Ez points inside the grid) and, per box cell,
+        the indices of the points on it (padded with an index of a zero entry): (ix0, iy0, isx, isy, idx, c)
+        with idx (cells, points per cell) and c the weights c_fp of those points, both on the device."""
+        T = self.torch
         s, m = self.sims[b], self.monitors[b]
         pts = np.array(m.get_grid_points_2d(s.dx, s.dy), dtype=np.int64)
-        a = self.a[b].cpu().numpy()
-        wp = 2.0 * (self.fscale * np.conj(a))[:, None] * self.c          # (nf, npts)
         xs, ys = pts[:, 0], pts[:, 1]
         ok = (xs >= 0) & (xs < s.nx) & (ys >= 0) & (ys < s.ny)
-        xs, ys, wp = xs[ok], ys[ok], wp[:, ok]
+        xs, ys, c = xs[ok], ys[ok], self.c[:, ok]
         ix0, iy0 = int(xs.min()), int(ys.min())
         isx, isy = int(xs.max()) - ix0 + 1, int(ys.max()) - iy0 + 1
-        box = np.zeros((self.nf, isy, isx), dtype=np.complex128)
-        for f in range(self.nf):
-            np.add.at(box[f], (ys - iy0, xs - ix0), wp[f])
-        return ix0, iy0, isx, isy, self.torch.from_numpy(np.ascontiguousarray(box).view(np.float64).copy()).to(self.dev)
+        cell = (ys - iy0) * isx + (xs - ix0)
+        per = np.bincount(cell, minlength=isy * isx)
+        idx = np.full((isy * isx, max(1, int(per.max()))), len(xs), dtype=np.int64)
+        fill = np.zeros(isy * isx, dtype=np.int64)
+        for p, q in enumerate(cell):           # points of a cell in point order
+            idx[q, fill[q]] = p
+            fill[q] += 1
+        return ix0, iy0, isx, isy, T.from_numpy(idx).to(self.dev), T.from_numpy(np.ascontiguousarray(c)).to(self.dev)
+
+    def _reverse_state(self):
+        """Buffers and descriptors of the reverse sweep (adjoint state, gradient, injection boxes), built on the
+        first gradient and reused: a gradient then needs no host work that waits for the forward sweep."""
+        if getattr(self, "_rs", None) is not None:
+            return self._rs
+        T = self.torch
+        x0, y0, sx, sy = self.box
+        grids = [s.grid for s in self.sims]
+        B = len(grids)
+        lam = [[T.zeros((3,) + tuple(g.box), dtype=T.float32, device=self.dev) for _ in (0, 1)] for g in grids]
+        grad = T.zeros((B, sy, sx), dtype=T.float64, device=self.dev)
+        geo = [self._injection_geometry(b) for b in range(B)]
+        inj = [(ix0, iy0, isx, isy, T.zeros((self.nf, isy, isx, 2), dtype=T.float64, device=self.dev))
+               for ix0, iy0, isx, isy, _, _ in geo]
+        self._phasor = []
+        for b, s in enumerate(self.sims):
+            plan = next(p for p in s._plans if p.monitor is self.monitors[b])
+            if len(plan.record_steps) != self.N:
+                raise ValueError("the objective monitor must record every step")
+            self._phasor.append(plan.phasor.contiguous())     # (N, nf) complex as (N, 2 nf) FP64
+        descs = self._adjoint_descs(lam, grad, inj)
+        ddev = T.empty(len(descs) * C.sizeof(N.Adj2D), dtype=T.uint8, device=self.dev)
+        self._rs = dict(lam=lam, grad=grad, geo=geo, inj=inj, descs=descs, ddev=ddev,
+                        max_inj=max(i[2] * i[3] for i in inj), fs=T.from_numpy(self.fscale).to(self.dev))
+        self._upload_descs()
+        return self._rs
+
+    def _upload_descs(self):
+        """Stream-ordered upload of the reverse descriptors from pinned memory (the host does not wait)."""
+        T = self.torch
+        rs = self._rs
+        raw = T.frombuffer(bytearray(b"".join(bytes(d) for d in rs["descs"])), dtype=T.uint8).pin_memory()
+        rs["ddev"].copy_(raw, non_blocking=True)
 
     def gradient(self):
         """dJ/d eps_r over the design boxes, (B, sy, sx) FP64 torch tensor, for the last forward."""
@@ -289,19 +331,25 @@ class AdjointProblem2D:
         x0, y0, sx, sy = self.box
         grids = [s.grid for s in self.sims]
         B = len(grids)
-        lam = [[T.zeros((3,) + tuple(g.box), dtype=T.float32, device=self.dev) for _ in (0, 1)] for g in grids]
-        grad = T.zeros((B, sy, sx), dtype=T.float64, device=self.dev)
-        inj = [self._injection(b) for b in range(B)]
-        self._phasor = []
-        for b, s in enumerate(self.sims):
-            plan = next(p for p in s._plans if p.monitor is self.monitors[b])
-            if len(plan.record_steps) != self.N:
-                raise ValueError("the objective monitor must record every step")
-            self._phasor.append(plan.phasor.contiguous())     # (N, nf) complex as (N, 2 nf) FP64
-        descs = self._adjoint_descs(lam, grad, inj)
-        raw = b"".join(bytes(d) for d in descs)
-        ddev = T.frombuffer(bytearray(raw), dtype=T.uint8).to(self.dev)
-        max_inj = max(i[2] * i[3] for i in inj)
+        rs = self._reverse_state()
+        lam, grad, descs, ddev, max_inj = rs["lam"], rs["grad"], rs["descs"], rs["ddev"], rs["max_inj"]
+        for pair in lam:
+            for t in pair:
+                t.zero_()
+        grad.zero_()
+        # injection weights 2 s_f conj(a_f) c_fp of this forward's objective, summed per box cell in point order,
+        # on the device (no wait for the forward sweep)
+        for b, ((ix0, iy0, isx, isy, idx, c), (_, _, _, _, box)) in enumerate(zip(rs["geo"], rs["inj"])):
+            w = 2.0 * (rs["fs"] * self.a[b].conj())[:, None] * c                 # (nf, points)
+            w = T.cat([w, T.zeros_like(w[:, :1])], dim=1)
+            acc = w[:, idx[:, 0]]
+            for k in range(1, idx.shape[1]):
+                acc = acc + w[:, idx[:, k]]
+            box.copy_(T.view_as_real(acc).reshape(self.nf, isy, isx, 2))
+        if not self.full:
+            for d in descs:
+                d.snap_step0 = 0
+            self._upload_descs()
         st = grids[0].stream
 
         first = [True]
@@ -350,8 +398,7 @@ class AdjointProblem2D:
             self._rev_tuned = True
             N.call("bz_adj2d_inject", ddev.data_ptr(), B, max_inj, self.N, st)
             reverse(self.N, self.N, 0)
-            self._keep = (lam, inj, ddev)
-            return grad
+            return grad.clone()
 
         def segment(k, flip):
             """Recompute segment k from its checkpoint (snapshots on), then walk it backwards; returns flip."""
@@ -363,9 +410,7 @@ class AdjointProblem2D:
             first[0] = False
             for d in descs:
                 d.snap_step0 = n0
-            # stream-ordered upload from pinned memory: the host does not wait for the GPU to drain
-            raw = T.frombuffer(bytearray(b"".join(bytes(d) for d in descs)), dtype=T.uint8).pin_memory()
-            ddev.copy_(raw, non_blocking=True)
+            self._upload_descs()
             return reverse(n0 + cnt, cnt, flip)
 
         nseg = len(self.ckpt)
@@ -410,8 +455,7 @@ class AdjointProblem2D:
         flip = 0                    # multi-step launches move the adjoint state across its two buffers
         for k in range(nseg - 1, -1, -1):
             flip = segment(k, flip)
-        self._keep = (lam, inj, ddev)
-        return grad
+        return grad.clone()
 
 
 def reference_adjoint_source(design, monitor, dx, dy, target, signal, wavelength, transmission_weight=0.5,
