@@ -410,11 +410,21 @@ __device__ __forceinline__ void gather_pair(const bz_sim2d* __restrict__ sims, c
 
 // Snapshot of the step with call-relative index slot at cell (j, i), if inside the box: the Ez the step read
 // and the Hx, Hy it wrote (the layout of bz_yee2d_run's snapshots).
-__device__ __forceinline__ void snap_put(const bz_sim2d& S, int slot, int i, int j, float ez, float hx, float hy) {
-    const int sa = i - S.snap_x0, sb = j - S.snap_y0;
-    if (sa >= 0 && sa < S.snap_sx && sb >= 0 && sb < S.snap_sy) {
-        const long plane = (long)S.snap_sy * S.snap_sx;
-        float* p = S.snap + (long)slot * 3 * plane + (long)sb * S.snap_sx + sa;
+// The snapshot box, read from the descriptor once per kernel: read through the descriptor after the snapshot
+// stores of the previous row, the fields would be loaded again (a store through a float pointer may alias the
+// descriptor), and each such load waits until those stores have completed.
+struct SnapBox {
+    float* p;
+    int x0, y0, sx, sy;
+};
+__device__ __forceinline__ SnapBox snap_box(const bz_sim2d& S) {
+    return SnapBox{S.snap, S.snap_x0, S.snap_y0, S.snap_sx, S.snap_sy};
+}
+__device__ __forceinline__ void snap_put(const SnapBox& Q, int slot, int i, int j, float ez, float hx, float hy) {
+    const int sa = i - Q.x0, sb = j - Q.y0;
+    if (sa >= 0 && sa < Q.sx && sb >= 0 && sb < Q.sy) {
+        const long plane = (long)Q.sy * Q.sx;
+        float* p = Q.p + (long)slot * 3 * plane + (long)sb * Q.sx + sa;
         gstore(p, 0, ez);
         gstore(p, plane, hx);
         gstore(p, 2 * plane, hy);
@@ -457,6 +467,7 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee2d_tb2(const bz_sim2d* __rest
     const int s = tiles[3 * t];
     if (s < 0) return;   // padding entry of a reordered tile list
     const bz_sim2d& S = sims[s];
+    const SnapBox snq = snap_box(S);
     const int tx = tiles[3 * t + 1], ty = tiles[3 * t + 2];
     const int nx = S.nx, ny = S.ny, ld = S.ld;
     const int l = threadIdx.x;
@@ -530,11 +541,11 @@ __global__ __launch_bounds__(LANES* WPG) void k_yee2d_tb2(const bz_sim2d* __rest
             put(ohx, om, h2x);
             put(ohy, om, h2y);
             if (S.power) add_power(S, (long)m * ld + i, e2, h2x, h2y);
-            if (S.snap) snap_put(S, 2 * q + 1, i, m, e1p, h2x, h2y);
+            if (snq.p) snap_put(snq, 2 * q + 1, i, m, e1p, h2x, h2y);
         }
         if (out && r >= j0 && r < j1) {
             if (S.power) add_power(S, (long)r * ld + i, e1, h1x, h1y);
-            if (S.snap) snap_put(S, 2 * q, i, r, ez0, h1x, h1y);
+            if (snq.p) snap_put(snq, 2 * q, i, r, ez0, h1x, h1y);
             if (S.stg) {
                 const int a = i - S.stg_x0, b = r - S.stg_y0;
                 if (a >= 0 && a < S.stg_sx && b >= 0 && b < S.stg_sy) {
@@ -722,6 +733,10 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
     // rules per row or per lane
     const bool inner = jb >= 1 && jb + B - 1 <= ny - 2 && rx0 >= 1 && rx0 + LANES - 1 <= nx - 2;
     // does the snapshot box meet this wave's rows and this region's inner columns (uniform over the wave)
+    // the snapshot box in LDS (written before the first barrier, read after it): in registers it would add
+    // scalar spills to every wave
+    __shared__ SnapBox snq;
+    if (threadIdx.x == 0 && threadIdx.y == 0) snq = snap_box(S);
     const bool snap_band = S.snap && jb < S.snap_y0 + S.snap_sy && jb + B > S.snap_y0 &&
                            rx0 + K < S.snap_x0 + S.snap_sx && rx0 + LANES - K > S.snap_x0;
     const bool lane_inner = l >= K && l < LANES - K;
@@ -802,7 +817,7 @@ __global__ __launch_bounds__(LANES* KW) void k_yee2d_tbk(const bz_sim2d* __restr
 #pragma unroll
             for (int r = 0; r < B; ++r) {
                 const int rr = w * B + r;
-                if (rr >= K && rr < RH - K && lane_inner) snap_put(S, q * K + k, i, jb + r, ez[r], hx[r], hy[r]);
+                if (rr >= K && rr < RH - K && lane_inner) snap_put(snq, q * K + k, i, jb + r, ez[r], hx[r], hy[r]);
             }
         }
         // E: Ez of the band's first row needs the new Hy of the previous band's last row

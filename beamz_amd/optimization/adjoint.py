@@ -386,12 +386,13 @@ class AdjointProblem2D:
                 for shape in self.rec.KCANDIDATES:
                     self.rec.set_kstep(*shape)
                     reverse(self.N, 96, 0)                            # warm-up
-                    e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
-                    e0.record()
-                    reverse(self.N, 96, 0)
-                    e1.record()
-                    T.cuda.synchronize(self.dev)
-                    ms[shape] = e0.elapsed_time(e1)
+                    for _ in range(2):                                # the faster of two timings
+                        e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
+                        e0.record()
+                        reverse(self.N, 96, 0)
+                        e1.record()
+                        T.cuda.synchronize(self.dev)
+                        ms[shape] = min(ms.get(shape, 1e30), e0.elapsed_time(e1))
                 self.rec.set_kstep(*min(ms, key=ms.get))
                 self.rev_timings = ms
                 clear()
