@@ -138,10 +138,10 @@ class AdjointProblem2D:
         self.fwd = Fused2D(grids, seg=self.seg, max_steps=self.N, pairs=self.pairs, kstep=self.kstep,
                            snapshots=fsnaps)
         if self.kstep is None and self.fwd.kstep and self.N >= 64:
-            # region shape of the forward sweep (when the caller left it automatic), timed once, among the shapes whose K divides the checkpoint
-            # interval (every segment then runs as whole K-step launches; with the full history, among all);
-            # the reverse sweep (and the recompute sweep) start from it and are timed on their own in the first
-            # gradient
+            # region shape of the forward sweep (when the caller left it automatic), timed once, among the shapes
+            # whose K divides the checkpoint interval (every segment then runs as whole K-step launches; with the
+            # full history, among all); the reverse sweep (and the recompute sweep) start from it and are timed on
+            # their own in the first gradient
             shapes = [c for c in self.fwd.KCANDIDATES if self.full or self.K % c[0] == 0]
             if len(shapes) > 1:
                 # 64 steps per shape: two launches of 8 steps ranked the shapes differently from run to run
