@@ -224,6 +224,12 @@ __global__ __launch_bounds__(64 * (W / 32) * (CB / 32)) void k_conv_march(const 
     const uint16_t* wrow = wl + (nb * 32 + m) * WS + 8 * g;
     auto compute = [&](int r) {
         const int y = r % H;
+        const long o0 = ((long)r * W + mb * 32) * Cout + n;
+        // the row's mask values load before the MFMAs, which hide their latency
+        uint16_t mk[16];
+        if (MASK)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) mk[e] = mask[o0 + (long)(8 * (e >> 2) + 4 * g + (e & 3)) * Cout];
         f32x16 acc;
 #pragma unroll
         for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
@@ -243,13 +249,12 @@ __global__ __launch_bounds__(64 * (W / 32) * (CB / 32)) void k_conv_march(const 
                         __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(b + k)), acc, 0, 0, 0);
             }
         }
-        const long o0 = ((long)r * W + mb * 32) * Cout + n;
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             float v = acc[e] + bn;
             if (relu) v = fmaxf(v, 0.0f);
             const long o = o0 + (long)(8 * (e >> 2) + 4 * g + (e & 3)) * Cout;
-            if (MASK) out[o] = b2f(mask[o]) > 0.0f ? f2b(v) : (uint16_t)0;
+            if (MASK) out[o] = b2f(mk[e]) > 0.0f ? f2b(v) : (uint16_t)0;
             else out[o] = f2b(v);
         }
     };
