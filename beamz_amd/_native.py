@@ -54,7 +54,7 @@ class Adj2D(C.Structure):
                 ("src_mul", vp), ("box_x0", i32), ("box_y0", i32), ("box_sx", i32), ("box_sy", i32),
                 ("snap", vp), ("snap_step0", i32), ("grad", vp),
                 ("inj_x0", i32), ("inj_y0", i32), ("inj_sx", i32), ("inj_sy", i32), ("inj_w", vp), ("phasor", vp), ("nf", i32),
-                ("coef", vp), ("l2hist", vp)]
+                ("coef", vp), ("l2hist", vp), ("inj_tab", vp)]
 
 # name -> argtypes (restype is int unless listed in _RESTYPE)
 SIGNATURES = {
@@ -105,6 +105,7 @@ SIGNATURES = {
     "bz_nn_to_bf16": [vp, vp, i64, vp],
     "bz_adj2d_inject": [vp, i32, i32, i32, vp],
     "bz_adj2d_grad": [vp, i32, i32, i32, i32, vp],
+    "bz_adj2d_injtab": [vp, i32, i32, i32, i32, vp],
     "bz_filter2d": [vp, vp, i32, i32, i32, vp, i32, vp, vp, vp],
     "bz_add_points": [vp, vp, vp, i32, f32, vp],
     "bz_affine_points": [vp, vp, vp, vp, i32, f32, vp],

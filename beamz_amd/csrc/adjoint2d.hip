@@ -345,6 +345,7 @@ __global__ __launch_bounds__(LANES* KW) void k_adj2d_tbk(const bz_adj2d* __restr
     constexpr int RH = KW * B;
     __shared__ float w_top[KW][LANES];
     __shared__ float fmy_bot[KW][LANES];
+    __shared__ int ibox[4];     // the injection box (x0, y0, sx, sy), read after the first barrier
     const int t = blockIdx.x;
     if (t >= ntiles) return;
     const int s = tiles[3 * t];
@@ -372,6 +373,25 @@ __global__ __launch_bounds__(LANES* KW) void k_adj2d_tbk(const bz_adj2d* __restr
                           rx0 + LANES > A.src_x0;
     const bool inj_band = A.inj_w && jb < A.inj_y0 + A.inj_sy && jb + B > A.inj_y0 &&
                           rx0 < A.inj_x0 + A.inj_sx && rx0 + LANES > A.inj_x0;
+    // with the injection table, the next step's injection values are loaded one step ahead (their latency
+    // hides behind a whole step); the box comes from LDS, so it adds no registers
+    const float* __restrict__ itab = A.inj_tab;
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        ibox[0] = A.inj_x0;
+        ibox[1] = A.inj_y0;
+        ibox[2] = A.inj_sx;
+        ibox[3] = A.inj_sy;
+    }
+    float injv[B];
+    auto inj_fetch = [&](int nn) {       // values of step nn (0 outside the box or for nn < 1)
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const int a = i - ibox[0], bb = jb + r - ibox[1];
+            injv[r] = (nn >= 1 && a >= 0 && a < ibox[2] && bb >= 0 && bb < ibox[3])
+                          ? itab[(long)nn * ibox[2] * ibox[3] + (long)bb * ibox[2] + a]
+                          : 0.0f;
+        }
+    };
     const bool grad_band = A.grad && jb < A.box_y0 + A.box_sy && jb + B > A.box_y0 &&
                            rx0 + K < A.box_x0 + A.box_sx && rx0 + LANES - K > A.box_x0;
     const bool inner = jb >= 1 && jb + B - 1 <= ny - 2 && rx0 >= 1 && rx0 + LANES - 1 <= nx - 2;
@@ -425,6 +445,7 @@ __global__ __launch_bounds__(LANES* KW) void k_adj2d_tbk(const bz_adj2d* __restr
         // mu: muY of the band's last row needs w of the next band's first row
         w_top[w][l] = wv[0];
         __syncthreads();
+        if (itab && inj_band && k == 0) inj_fetch(n);     // the first step's values (ibox is written by now)
         const float wb = w < KW - 1 ? w_top[w + 1][l] : wv[B - 1];
         float fmx[B], fmy[B];
 #pragma unroll
@@ -452,7 +473,14 @@ __global__ __launch_bounds__(LANES* KW) void k_adj2d_tbk(const bz_adj2d* __restr
             const bool in = inner || (xin && j >= 1 && j <= ny - 2);
             const float l2 = le[r];
             float lnew = (in ? ca[r] * l2 : l2) + (fmx[r] - fmxl) * rdy + (fmyu - fmy[r]) * rdx;
-            if (inj_band) inject_at(A, lnew, i, j, n);
+            if (inj_band) {
+                if (itab) {
+                    const int a = i - ibox[0], bb = j - ibox[1];
+                    if (n >= 1 && a >= 0 && a < ibox[2] && bb >= 0 && bb < ibox[3]) lnew += injv[r];
+                } else {
+                    inject_at(A, lnew, i, j, n);
+                }
+            }
             const int rr = w * B + r;
             if (grad_band && in && lane_inner && rr >= K && rr < RH - K) {
                 if (A.l2hist) {
@@ -466,6 +494,7 @@ __global__ __launch_bounds__(LANES* KW) void k_adj2d_tbk(const bz_adj2d* __restr
             }
             le[r] = lnew;
         }
+        if (itab && inj_band && k + 1 < K) inj_fetch(n - 1);   // the next step's values load during this one
     }
 
     // the inner cells hold lam_{n_top-K+1}
@@ -592,6 +621,31 @@ extern "C" int bz_adj2d_grad(const bz_adj2d* adjs, int nadj, int max_box, int n_
     hipLaunchKernelGGL(k_adj2d_grad, dim3((max_box + 255) / 256, nadj), dim3(256), 0, (hipStream_t)stream, adjs,
                        n_lo, n_hi);
     BZ_CHECK_LAUNCH("bz_adj2d_grad");
+    return BZ_OK;
+}
+
+// inj_tab[n][cell] for steps n_lo .. n_hi-1 (n >= 1) of one adjoint (y) and injection cell (x).
+// one thread per (step, cell) of the steps n_lo .. n_hi - 1 (n_lo >= 1), flat index q = (n - n_lo) * cells + p
+__global__ __launch_bounds__(256) void k_adj2d_injtab(const bz_adj2d* __restrict__ adjs, int n_lo, int n_hi) {
+    const bz_adj2d& A = adjs[blockIdx.y];
+    const long cells = (long)A.inj_sx * A.inj_sy;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!A.inj_w || !A.inj_tab || q >= cells * (n_hi - n_lo)) return;
+    const int n = n_lo + (int)(q / cells);
+    const int p = (int)(q - (long)(n - n_lo) * cells);
+    A.inj_tab[(long)n * cells + p] = (float)injection(A, p, n);
+}
+
+extern "C" int bz_adj2d_injtab(const bz_adj2d* adjs, int nadj, int max_inj, int n_lo, int n_hi, void* stream) {
+    BZ_REQUIRE(adjs && nadj > 0 && nadj <= 65535 && max_inj >= 0 && n_lo >= 0 && n_hi >= n_lo,
+               "bz_adj2d_injtab: bad arguments");
+    n_lo = n_lo < 1 ? 1 : n_lo;
+    if (max_inj == 0 || n_hi <= n_lo) return BZ_OK;
+    const long total = (long)max_inj * (n_hi - n_lo);
+    BZ_REQUIRE((total + 255) / 256 <= 0x7fffffffL, "bz_adj2d_injtab: table too large");
+    hipLaunchKernelGGL(k_adj2d_injtab, dim3((unsigned)((total + 255) / 256), nadj), dim3(256), 0,
+                       (hipStream_t)stream, adjs, n_lo, n_hi);
+    BZ_CHECK_LAUNCH("bz_adj2d_injtab");
     return BZ_OK;
 }
 

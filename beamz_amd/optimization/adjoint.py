@@ -66,6 +66,7 @@ class AdjointProblem2D:
 
     REVERSE_SEGS2 = (4, 8, 16)      # pair-tile heights tried for the recompute and reverse sweeps
     FULL_SNAPSHOT_BYTES = 48 << 30  # largest design-box history kept for every step ("auto")
+    INJ_TABLE_BYTES = 256 << 20     # largest table of injection values of every step
 
     def __init__(self, sims, monitors, weights, box, checkpoint_every=None, seg=None, fscale=None, pairs=None,
                  kstep=None, snapshots="auto", gradient_pass=True):
@@ -255,9 +256,11 @@ class AdjointProblem2D:
             d.snap = self.snap[b].data_ptr()
             d.grad = grad[b].data_ptr()
             d.l2hist = self.l2hist[b].data_ptr() if self.l2hist is not None else None
-            ix0, iy0, isx, isy, w = inj[b]
+            ix0, iy0, isx, isy, w = inj[b][:5]
             d.inj_x0, d.inj_y0, d.inj_sx, d.inj_sy = ix0, iy0, isx, isy
             d.inj_w = w.data_ptr()
+            tab = inj[b][5] if len(inj[b]) > 5 else None
+            d.inj_tab = tab.data_ptr() if tab is not None else None
             d.phasor = self._phasor[b].data_ptr()
             d.nf = self.nf
             # the coefficient arrays of the engine that last ran these materials: the forward engine with the full
@@ -301,7 +304,13 @@ class AdjointProblem2D:
         lam = [[T.zeros((3,) + tuple(g.box), dtype=T.float32, device=self.dev) for _ in (0, 1)] for g in grids]
         grad = T.zeros((B, sy, sx), dtype=T.float64, device=self.dev)
         geo = [self._injection_geometry(b) for b in range(B)]
-        inj = [(ix0, iy0, isx, isy, T.zeros((self.nf, isy, isx, 2), dtype=T.float64, device=self.dev))
+        # the injection values of every step and cell, tabulated once per gradient (bz_adj2d_injtab) while they
+        # fit INJ_TABLE_BYTES: the K-step reverse kernel then loads one number a step ahead instead of summing
+        # weights times phasors when it needs them
+        tab_bytes = sum(self.N * g[2] * g[3] * 4 for g in geo)
+        tabled = self.INJ_TABLE_BYTES >= tab_bytes
+        inj = [(ix0, iy0, isx, isy, T.zeros((self.nf, isy, isx, 2), dtype=T.float64, device=self.dev),
+                T.zeros((self.N, isy, isx), dtype=T.float32, device=self.dev) if tabled else None)
                for ix0, iy0, isx, isy, _, _ in geo]
         self._phasor = []
         for b, s in enumerate(self.sims):
@@ -311,7 +320,7 @@ class AdjointProblem2D:
             self._phasor.append(plan.phasor.contiguous())     # (N, nf) complex as (N, 2 nf) FP64
         descs = self._adjoint_descs(lam, grad, inj)
         ddev = T.empty(len(descs) * C.sizeof(N.Adj2D), dtype=T.uint8, device=self.dev)
-        self._rs = dict(lam=lam, grad=grad, geo=geo, inj=inj, descs=descs, ddev=ddev,
+        self._rs = dict(lam=lam, grad=grad, geo=geo, inj=inj, descs=descs, ddev=ddev, tabled=tabled,
                         max_inj=max(i[2] * i[3] for i in inj), fs=T.from_numpy(self.fscale).to(self.dev))
         self._upload_descs()
         return self._rs
@@ -339,7 +348,7 @@ class AdjointProblem2D:
         grad.zero_()
         # injection weights 2 s_f conj(a_f) c_fp of this forward's objective, summed per box cell in point order,
         # on the device (no wait for the forward sweep)
-        for b, ((ix0, iy0, isx, isy, idx, c), (_, _, _, _, box)) in enumerate(zip(rs["geo"], rs["inj"])):
+        for b, ((ix0, iy0, isx, isy, idx, c), (_, _, _, _, box, _)) in enumerate(zip(rs["geo"], rs["inj"])):
             w = 2.0 * (rs["fs"] * self.a[b].conj())[:, None] * c                 # (nf, points)
             w = T.cat([w, T.zeros_like(w[:, :1])], dim=1)
             acc = w[:, idx[:, 0]]
@@ -351,6 +360,8 @@ class AdjointProblem2D:
                 d.snap_step0 = 0
             self._upload_descs()
         st = grids[0].stream
+        if rs["tabled"]:
+            N.call("bz_adj2d_injtab", ddev.data_ptr(), B, max_inj, 1, self.N, st)
 
         first = [True]
 

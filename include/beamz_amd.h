@@ -291,6 +291,9 @@ typedef struct bz_adj2d {
                                            * lambda value each design cell's gradient term takes at step n (slot
                                            * n - snap_step0) instead of adding the term into grad;
                                            * bz_adj2d_grad then adds the terms, in the same order */
+    float* inj_tab;                       /* NULL, or [num_steps][inj_sy][inj_sx]: the objective's injection
+                                           * value of each cell at step n >= 1, as bz_adj2d_injtab computes it;
+                                           * bz_adj2d_runk then reads it instead of summing inj_w * phasor */
 } bz_adj2d;
 
 /* Reverse steps n = n_hi-1 down to n_hi-nsteps of every adjoint described in `adjs` (device
@@ -321,6 +324,11 @@ int bz_adj2d_inject(const bz_adj2d* adjs, int nadj, int max_inj, int n, void* st
  * grad and l2hist set: the same bits as adding each term inside the reverse kernels, with the per-step
  * read-modify-write of grad taken off the reverse sweep. max_box >= every box_sx * box_sy. */
 int bz_adj2d_grad(const bz_adj2d* adjs, int nadj, int max_box, int n_lo, int n_hi, void* stream);
+
+/* inj_tab[n][cell] = the injection value sum_f Re(inj_w[f][cell] phasor[n-1][f]), rounded to FP32, for steps
+ * n_lo <= n < n_hi (n >= 1) of every adjoint with inj_w and inj_tab set: the value the reverse kernels add,
+ * so results do not change; max_inj >= every inj_sx * inj_sy. */
+int bz_adj2d_injtab(const bz_adj2d* adjs, int nadj, int max_inj, int n_lo, int n_hi, void* stream);
 
 /* Density filter of the topology parametrisation (replaces the blur of examples/4_topo.py:120-129 in the
  * product's filter -> projection chain): for each of B maps of H x W FP32 values,

@@ -378,7 +378,8 @@ def test_cone_filter_kernel_and_its_gradient_match_oracle(hip):
 def test_gradient_pass_equals_in_kernel_accumulation(hip):
     """The reverse kernels storing each design cell's lambda value per step and bz_adj2d_grad adding the terms
     afterwards give the bits of the terms added inside the reverse kernels: with the full history and with
-    checkpoints (23 steps, so segments end with single steps), for one-step, pair and K-step sweeps."""
+    checkpoints (23 steps, so segments end with single steps), for one-step, pair and K-step sweeps; the K-step
+    sweeps also with the injection values tabulated per step (bz_adj2d_injtab) and without the table."""
     torch = hip
     eps, sig, _ = materials(seed=5)
     x0, y0, sx, sy = BOX
@@ -386,16 +387,19 @@ def test_gradient_pass_equals_in_kernel_accumulation(hip):
     ref = None
     for mode in ("full", "checkpoint"):
         for pairs, kstep in ((False, 0), (True, 0), (False, 8)):
-            for gp in (False, True):
+            for gp, tab in ((False, True), (True, True)) + (((False, False), (True, False)) if kstep else ()):
                 sim, mon, src, t = build(eps, sig, 260)
                 npts = len(mon.get_grid_points_2d(DX, DX))
                 c = np.exp(1j * np.linspace(0, 2, npts)) / np.sqrt(npts)
                 prob = AdjointProblem2D([sim], [mon], c, BOX, checkpoint_every=23, pairs=pairs, kstep=kstep,
                                         snapshots=mode, gradient_pass=gp)
+                if not tab:
+                    prob.INJ_TABLE_BYTES = 0
                 prob.forward(e_box)
                 g = prob.gradient().cpu().numpy()
                 assert (prob.l2hist is not None) == gp
+                assert prob._rs["tabled"] == tab
                 if ref is None:
                     ref = g
-                assert np.array_equal(g, ref), (mode, pairs, kstep, gp)
+                assert np.array_equal(g, ref), (mode, pairs, kstep, gp, tab)
     assert np.abs(ref).max() > 0
