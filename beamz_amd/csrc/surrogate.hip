@@ -55,63 +55,79 @@ __device__ __forceinline__ void pixel(int p, const Geo& g, int& b, int& y, int& 
 
 // Forward (sign = +1) or backward-data (sign = -1) 3x3 convolution as an implicit GEMM.
 // src: [P][Cs] bf16 (x, or the masked upstream gradient), wk: [N][9][Cs] bf16, out: [P][N] bf16.
-// K slices of FK = 32 or 64: thread t stages the 8-wide chunks kq + 32 h (h < FK / 32) of row t / 4 of both
-// tiles (the tap of a chunk is k / Cs; Cs % 8 == 0, so a chunk never straddles two taps, and chunks past K
-// are zero). The next slice's loads are issued before the current slice's MFMAs.
+// K slices of FK = 32 or 64 elements, CPR = FK / 8 chunks of 16 bytes per tile row: thread t stages chunk t % CPR
+// of rows t / CPR + h 256 / CPR (h < NH) of both tiles (the tap of a chunk is k / Cs; Cs % 8 == 0, so a chunk
+// never straddles two taps, and chunks past K are zero). The next slice's loads are issued before the current
+// slice's MFMAs.
+// Tile rows are padded to FK + 8 elements, so the consecutive lanes of an MFMA operand read (rows r .. r + 15,
+// one chunk each) fall on distinct banks; with FK = 64 the eight lanes storing a row write its 128 contiguous
+// bytes (four lanes per row, two chunks each, put two rows on the same banks).
 // TAP (Cs % FK == 0): a slice lies within one tap, so the tap and its pixel shift are computed once per slice
-// (uniform) and each lane adds offsets to its pixel's precomputed base, instead of two integer divisions per
+// (uniform) and each lane adds offsets to its pixels' precomputed bases, instead of two integer divisions per
 // chunk and lane (the per-lane index arithmetic outweighed the MFMAs several times over).
 template <int FK, bool TAP>
-__global__ __launch_bounds__(256) void k_conv_gemm(const uint16_t* __restrict__ src, const uint16_t* __restrict__ wk,
+__global__ __launch_bounds__(256, TAP ? 8 : 1) void k_conv_gemm(const uint16_t* __restrict__ src, const uint16_t* __restrict__ wk,
                                                    const float* __restrict__ bias, uint16_t* __restrict__ out,
                                                    const uint16_t* __restrict__ mask, Geo g, int Cs, int N, int sign,
                                                    int relu) {
-    constexpr int FLD = FK + 8, NH = FK / 32;
+    constexpr int CPR = FK / 8, NH = FK / 32, RSTEP = 256 / CPR;
+    constexpr int FLD = FK + 8;
     __shared__ __attribute__((aligned(16))) uint16_t As[TM][FLD];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][FLD];
     const int P = g.B * g.H * g.W;
     const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
     const int t = threadIdx.x;
-    const int pr = t >> 2, kq = (t & 3) * 8;
-    int pb, py, px;
-    const int p = m0 + pr;
-    pixel(min(p, P - 1), g, pb, py, px);
-    const int nrow = n0 + pr;
+    const int pr = t / CPR, cq = t % CPR, kq = cq * 8;
     const int K = 9 * Cs;
+    // per staged row h: the pixel (TAP: its element offset, P * Cs < 2^31, checked by the entry points, and in
+    // bits 16 h + tap of tapok whether the tap's source pixel is inside the image; else its b, y, x)
+    int py[NH], px[NH], pofs[NH], wofs[NH];
+    unsigned tapok = 0;
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+        wofs[h] = min(n0 + pr + h * RSTEP, N - 1) * K + kq;   // N * K < 2^31 (checked by the entry points)
+        int pb;
+        pixel(min(m0 + pr + h * RSTEP, P - 1), g, pb, py[h], px[h]);
+        pofs[h] = TAP ? ((pb * g.H + py[h]) * g.W + px[h]) * Cs + kq : pb;
+        if (TAP && m0 + pr + h * RSTEP < P) {
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap) {
+                const int yy = py[h] + sign * (tap / 3 - 1), xx = px[h] + sign * (tap % 3 - 1);
+                if (yy >= 0 && yy < g.H && xx >= 0 && xx < g.W) tapok |= 1u << (16 * h + tap);
+            }
+        }
+    }
     uint4 a[NH], bw[NH];
-    // TAP: this lane's pixel offset (P * Cs < 2^31, checked by the entry points) and weight row
-    const int po = ((pb * g.H + py) * g.W + px) * Cs + kq;
-    const uint16_t* wr = wk + (long)min(nrow, N - 1) * K + kq;
     auto load = [&](int k0) {
         if (TAP) {
             const int tap = k0 / Cs;
             const int ky = tap / 3, kx = tap - 3 * ky;
             const int dy = sign * (ky - 1), dx = sign * (kx - 1);
-            const int yy = py + dy, xx = px + dx;
-            const bool ok = p < P && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
             const int cs = k0 - tap * Cs;
-            // every lane loads from a valid address (its own pixel where the tap falls outside) and selects 0
-            const int base = po + cs + (ok ? (dy * g.W + dx) * Cs : 0);
+            const int shift = (dy * g.W + dx) * Cs;
 #pragma unroll
             for (int h = 0; h < NH; ++h) {
-                const uint4 v = *reinterpret_cast<const uint4*>(src + base + 32 * h);
-                const uint4 w = *reinterpret_cast<const uint4*>(wr + k0 + 32 * h);
+                const bool ok = (tapok >> (16 * h + tap)) & 1u;
+                // every lane loads from a valid address (its own pixel where the tap falls outside) and selects 0
+                const uint4 v = *reinterpret_cast<const uint4*>(src + pofs[h] + cs + (ok ? shift : 0));
+                const uint4 w = *reinterpret_cast<const uint4*>(wk + wofs[h] + k0);
                 a[h] = ok ? v : make_uint4(0, 0, 0, 0);
-                bw[h] = nrow < N ? w : make_uint4(0, 0, 0, 0);
+                bw[h] = n0 + pr + h * RSTEP < N ? w : make_uint4(0, 0, 0, 0);
             }
             return;
         }
+        const int k = k0 + kq;
+        const int tap = k / Cs, c = k - tap * Cs;
+        const int ky = tap / 3, kx = tap - 3 * (tap / 3);
 #pragma unroll
         for (int h = 0; h < NH; ++h) {
-            const int k = k0 + kq + 32 * h;
             a[h] = make_uint4(0, 0, 0, 0);
             bw[h] = make_uint4(0, 0, 0, 0);
             if (k < K) {
-                const int tap = k / Cs, c = k - tap * Cs;
-                const int ky = tap / 3, kx = tap - 3 * (tap / 3);
-                const int yy = py + sign * (ky - 1), xx = px + sign * (kx - 1);
-                if (p < P && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W)
-                    a[h] = *reinterpret_cast<const uint4*>(src + (((long)pb * g.H + yy) * g.W + xx) * Cs + c);
+                const int yy = py[h] + sign * (ky - 1), xx = px[h] + sign * (kx - 1);
+                if (m0 + pr + h * RSTEP < P && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W)
+                    a[h] = *reinterpret_cast<const uint4*>(src + (((long)pofs[h] * g.H + yy) * g.W + xx) * Cs + c);
+                const int nrow = n0 + pr + h * RSTEP;
                 if (nrow < N) bw[h] = *reinterpret_cast<const uint4*>(wk + (long)nrow * K + k);
             }
         }
@@ -121,19 +137,21 @@ __global__ __launch_bounds__(256) void k_conv_gemm(const uint16_t* __restrict__ 
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
     const int l = t & 63, wave = t >> 6;
     const int wm = wave & 1, wn = wave >> 1;
+    const int ra = wm * 32 + (l & 31), rb = wn * 32 + (l & 31);
     load(0);
     for (int k0 = 0; k0 < K; k0 += FK) {
 #pragma unroll
         for (int h = 0; h < NH; ++h) {
-            *reinterpret_cast<uint4*>(&As[pr][kq + 32 * h]) = a[h];
-            *reinterpret_cast<uint4*>(&Bs[pr][kq + 32 * h]) = bw[h];
+            const int r = pr + h * RSTEP;
+            *reinterpret_cast<uint4*>(&As[r][kq]) = a[h];
+            *reinterpret_cast<uint4*>(&Bs[r][kq]) = bw[h];
         }
         __syncthreads();
         if (k0 + FK < K) load(k0 + FK);
 #pragma unroll
         for (int kk = 0; kk < FK; kk += 16) {
-            const uint4 av = *reinterpret_cast<const uint4*>(&As[wm * 32 + (l & 31)][kk + 8 * (l >> 5)]);
-            const uint4 bv = *reinterpret_cast<const uint4*>(&Bs[wn * 32 + (l & 31)][kk + 8 * (l >> 5)]);
+            const uint4 av = *reinterpret_cast<const uint4*>(&As[ra][kk + 8 * (l >> 5)]);
+            const uint4 bv = *reinterpret_cast<const uint4*>(&Bs[rb][kk + 8 * (l >> 5)]);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av),
                                                            __builtin_bit_cast(bf16x8, bv), acc, 0, 0, 0);
         }
@@ -469,6 +487,9 @@ __global__ __launch_bounds__(192 * NCO * NCI) void k_wgrad_march(const uint16_t*
                                                                  float* __restrict__ part, float* __restrict__ bpart,
                                                                  int H, int rows_total, int Ci, int Co, int band) {
     constexpr int CB = 32 * NCO, IB = 32 * NCI, NT = 192 * NCO * NCI;
+    // neighbour pixels by lane exchange, except where its four more live registers would spill (the 2 x 2
+    // channel blocks, 3 waves per SIMD, with more than one 16-pixel chunk per row)
+    constexpr bool XCH = !(NCO == 2 && NCI == 2 && W > 16);
     constexpr int XS = W + 24, GS = W + 8;                  // row strides (elements) of the x and gz tiles
     constexpr int XSLOT = IB * XS, GSLOT = CB * GS, NXS = 2 * R + 2, NGS = 2 * R;
     constexpr int NBX = (W / 8) * (IB / 8), NBG = (W / 8) * (CB / 8);
@@ -486,7 +507,9 @@ __global__ __launch_bounds__(192 * NCO * NCI) void k_wgrad_march(const uint16_t*
     const bool isx = t < R * NBX, isg = !isx && t < R * (NBX + NBG);
     const int u = isx ? t : t - R * NBX;
     const int nb = isx ? NBX : NBG, nch = isx ? IB / 8 : CB / 8;
-    const int rr = u / nb, cc = (u % nb) % nch, pg = (u % nb) / nch;    // row of the set, channel chunk, pixels
+    // row of the set, channel chunk, pixel group: consecutive lanes take consecutive pixel groups of one channel
+    // chunk, so the transposed stores of eight lanes write contiguous bytes of one LDS row
+    const int rr = u / nb, pg = (u % nb) % (W / 8), cc = (u % nb) / (W / 8);
     // two register sets in flight: rows loaded at the start of a step are stored after the next step
     uint4 va[8], vb[8];
     auto load = [&](uint4 (&v)[8], int q) {   // row q of x (isx) or gz (isg), zeros outside 0 .. rows_total-1
@@ -554,14 +577,34 @@ __global__ __launch_bounds__(192 * NCO * NCI) void k_wgrad_march(const uint16_t*
             // this wave's input row r + ky - 1 exists (else zero padding: nothing to add)
             if (!((ky == 0 && y == 0) || (ky == 2 && y == H - 1))) {
                 const uint16_t* xrow = xs + ((r + ky - 1 + NXS) % NXS) * XSLOT + (ci32 * 32 + row) * XS + 8 + 8 * g;
+                // the pixel pairs beside a lane's eight (p - 2, p - 1 and p + 8, p + 9) are the first or last dword
+                // of lane l +- 32's chunk, of this k or the next: exchanged with permlane32_swap (in result 0 the
+                // upper lanes hold the lower lanes' second operand, in result 1 the lower lanes hold the upper
+                // lanes' first operand) instead of two 4-byte LDS reads that put 32 rows on 8 bank groups; the
+                // row pads give the zeros at both ends
+                uint32_t carry = 0;                          // lower lanes: pixels p - 2, p - 1 of the next k
+                uint4 curn = XCH ? *reinterpret_cast<const uint4*>(xrow) : make_uint4(0, 0, 0, 0);
 #pragma unroll
                 for (int k = 0; k < W / 16; ++k) {
                     const bf16x8 a = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(grow + 16 * k));
                     if (with_bias) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, ones, accb, 0, 0, 0);
-                    const uint16_t* xr = xrow + 16 * k;
-                    const uint4 cur = *reinterpret_cast<const uint4*>(xr);
-                    const uint32_t prev = *reinterpret_cast<const uint32_t*>(xr - 2);
-                    const uint32_t next = *reinterpret_cast<const uint32_t*>(xr + 8);
+                    uint4 cur;
+                    uint32_t prev, next;
+                    if (XCH) {
+                        cur = curn;
+                        curn = k + 1 < W / 16 ? *reinterpret_cast<const uint4*>(xrow + 16 * (k + 1))
+                                              : make_uint4(0, 0, 0, 0);
+                        const auto s1 = __builtin_amdgcn_permlane32_swap(cur.x, cur.w, false, false);
+                        const auto s2 = __builtin_amdgcn_permlane32_swap(cur.w, curn.x, false, false);
+                        prev = g ? (uint32_t)s1[0] : carry;
+                        next = g ? (uint32_t)s2[0] : (uint32_t)s1[1];
+                        carry = s2[1];
+                    } else {
+                        const uint16_t* xr = xrow + 16 * k;
+                        cur = *reinterpret_cast<const uint4*>(xr);
+                        prev = *reinterpret_cast<const uint32_t*>(xr - 2);
+                        next = *reinterpret_cast<const uint32_t*>(xr + 8);
+                    }
                     const uint32_t m1 = __builtin_amdgcn_alignbyte(cur.y, cur.x, 2);   // pixels 1, 2
                     const uint32_t m2 = __builtin_amdgcn_alignbyte(cur.z, cur.y, 2);   // 3, 4
                     const uint32_t m3 = __builtin_amdgcn_alignbyte(cur.w, cur.z, 2);   // 5, 6
