@@ -101,6 +101,7 @@ SIGNATURES = {
     "bz_nn_head": [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp],
     "bz_nn_head_bwd": [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp],
     "bz_nn_weights": [vp, vp, vp, i32, i32, vp],
+    "bz_nn_weights_n": [vp, vp, vp, vp, vp, i32, vp],
     "bz_nn_adam": [vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp],
     "bz_nn_to_bf16": [vp, vp, i64, vp],
     "bz_adj2d_inject": [vp, i32, i32, i32, vp],

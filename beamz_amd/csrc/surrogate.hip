@@ -1298,16 +1298,36 @@ __global__ void k_head_wgrad(const float* __restrict__ gout, const float* __rest
 
 // bf16 copies of FP32 weights [Co][9][Ci]: w16 in the same order, wt16 with the channel axes swapped,
 // [Ci][9][Co] (same tap order).
-__global__ void k_weights(const float* __restrict__ w, uint16_t* __restrict__ w16, uint16_t* __restrict__ wt16, int Co,
-                          int Ci) {
+__device__ __forceinline__ void weight_copy(const float* __restrict__ w, uint16_t* __restrict__ w16,
+                                            uint16_t* __restrict__ wt16, int Co, int Ci, int idx) {
     // 32-bit index arithmetic: a layer's Co * 9 * Ci weights are far below 2^31
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= Co * 9 * Ci) return;
     const int q = idx / Ci, ci = idx - q * Ci;
     const int co = q / 9, tap = q - co * 9;
     const uint16_t v = f2b(w[idx]);
     w16[idx] = v;
     if (wt16) wt16[((long)ci * 9 + tap) * Co + co] = v;   // the dgrad GEMM mirrors the tap offset itself
+}
+
+__global__ void k_weights(const float* __restrict__ w, uint16_t* __restrict__ w16, uint16_t* __restrict__ wt16, int Co,
+                          int Ci) {
+    weight_copy(w, w16, wt16, Co, Ci, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// every layer's copies in one launch: layer j takes blocks start[j] .. start[j + 1] - 1
+constexpr int WMAX = 16;
+struct WCopies {
+    const float* w[WMAX];
+    uint16_t* w16[WMAX];
+    uint16_t* wt16[WMAX];
+    int co[WMAX], ci[WMAX], start[WMAX + 1];
+    int n;
+};
+
+__global__ void k_weights_n(const WCopies c) {
+    int j = 0;
+    while (j + 1 < c.n && (int)blockIdx.x >= c.start[j + 1]) ++j;
+    weight_copy(c.w[j], c.w16[j], c.wt16[j], c.co[j], c.ci[j], (blockIdx.x - c.start[j]) * blockDim.x + threadIdx.x);
 }
 
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
@@ -1766,6 +1786,30 @@ extern "C" int bz_nn_weights(const float* w, uint16_t* w16, uint16_t* wt16, int 
     BZ_REQUIRE(w && w16 && Co > 0 && Ci > 0 && (long)Co * 9 * Ci < (1L << 31), "bz_nn_weights: bad arguments");
     hipLaunchKernelGGL(k_weights, grid1((long)Co * 9 * Ci), dim3(256), 0, (hipStream_t)stream, w, w16, wt16, Co, Ci);
     BZ_CHECK_LAUNCH("bz_nn_weights");
+    return BZ_OK;
+}
+
+extern "C" int bz_nn_weights_n(const float* const* w, uint16_t* const* w16, uint16_t* const* wt16, const int* Co,
+                               const int* Ci, int n, void* stream) {
+    BZ_REQUIRE(w && w16 && wt16 && Co && Ci && n >= 1 && n <= WMAX, "bz_nn_weights_n: bad arguments (1..16 layers)");
+    WCopies c{};
+    c.n = n;
+    long blocks = 0;
+    for (int j = 0; j < n; ++j) {
+        BZ_REQUIRE(w[j] && w16[j] && Co[j] > 0 && Ci[j] > 0 && (long)Co[j] * 9 * Ci[j] < (1L << 31),
+                   "bz_nn_weights_n: bad layer");
+        c.w[j] = w[j];
+        c.w16[j] = w16[j];
+        c.wt16[j] = wt16[j];
+        c.co[j] = Co[j];
+        c.ci[j] = Ci[j];
+        c.start[j] = (int)blocks;
+        blocks += ((long)Co[j] * 9 * Ci[j] + 255) / 256;
+        BZ_REQUIRE(blocks < (1L << 31), "bz_nn_weights_n: too many weights");
+    }
+    c.start[n] = (int)blocks;
+    hipLaunchKernelGGL(k_weights_n, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, c);
+    BZ_CHECK_LAUNCH("bz_nn_weights_n");
     return BZ_OK;
 }
 

@@ -10,6 +10,8 @@ accumulation on the MFMA units, FP32 master weights and Adam state.
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 from .. import _native as N
@@ -63,10 +65,19 @@ class Surrogate:
         return self.torch.cuda.current_stream(self.dev).cuda_stream
 
     def sync_weights(self):
-        """bf16 copies (and channel-transposed copies for backward-data) of the FP32 conv weights."""
-        for k, w in self.w16.items():
-            co, _, _, ci = self.params[k].shape
-            N.call("bz_nn_weights", self.params[k].data_ptr(), w.data_ptr(), self.wt16[k].data_ptr(), co, ci, self.stream)
+        """bf16 copies (and channel-transposed copies for backward-data) of the FP32 conv weights, every layer
+        in one launch (the pointer and size arrays are built once)."""
+        if getattr(self, "_wcopy", None) is None:
+            ks = list(self.w16)
+            n = len(ks)
+            P = ctypes.c_void_p * n
+            I = ctypes.c_int * n
+            self._wcopy = (P(*[self.params[k].data_ptr() for k in ks]), P(*[self.w16[k].data_ptr() for k in ks]),
+                           P(*[self.wt16[k].data_ptr() for k in ks]), I(*[self.params[k].shape[0] for k in ks]),
+                           I(*[self.params[k].shape[3] for k in ks]), n)
+        w, w16, wt16, co, ci, n = self._wcopy
+        N.call("bz_nn_weights_n", ctypes.addressof(w), ctypes.addressof(w16), ctypes.addressof(wt16),
+               ctypes.addressof(co), ctypes.addressof(ci), n, self.stream)
 
     def load_state(self, params):
         for k, v in params.items():

@@ -374,6 +374,9 @@ int bz_nn_head(const uint16_t* a, const float* fw, const float* fb, float* gap, 
 int bz_nn_head_bwd(const float* gout, const float* gap, const float* fw, float* gfw, float* gfb, uint16_t* ga,
                    int B, int HW, int C, int nout, void* stream);
 int bz_nn_weights(const float* w, uint16_t* w16, uint16_t* wt16, int Co, int Ci, void* stream);
+/* bz_nn_weights of n <= 16 layers in one launch (host arrays of n pointers and sizes; wt16[j] may be NULL) */
+int bz_nn_weights_n(const float* const* w, uint16_t* const* w16, uint16_t* const* wt16, const int* Co, const int* Ci,
+                    int n, void* stream);
 int bz_nn_adam(float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
                int step, void* stream);
 int bz_nn_to_bf16(const float* x, uint16_t* y, long n, void* stream);

@@ -29,6 +29,36 @@ def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
 
 
+def test_batched_weight_copies_equal_per_layer_copies(hip):
+    """bz_nn_weights_n (every layer in one launch) writes the bits bz_nn_weights writes layer by layer, also for
+    ragged layer sizes and a layer without the transposed copy."""
+    import ctypes
+    torch = hip
+    rng = np.random.default_rng(7)
+    shapes = [(32, 1), (64, 32), (96, 40), (8, 8), (256, 256)]
+    st = torch.cuda.current_stream().cuda_stream
+    ws = [torch.tensor(rng.standard_normal((co, 3, 3, ci)), dtype=torch.float32).cuda() for co, ci in shapes]
+    one = [(torch.empty((co, 3, 3, ci), dtype=torch.int16).cuda(), torch.empty((ci, 3, 3, co), dtype=torch.int16).cuda())
+           for co, ci in shapes]
+    many = [(torch.full((co, 3, 3, ci), 7, dtype=torch.int16).cuda(),
+             torch.full((ci, 3, 3, co), 7, dtype=torch.int16).cuda()) for co, ci in shapes]
+    for (co, ci), w, (a, b) in zip(shapes, ws, one):
+        N.call("bz_nn_weights", w.data_ptr(), a.data_ptr(), b.data_ptr() if co != 8 else None, co, ci, st)
+    n = len(shapes)
+    P, I = ctypes.c_void_p * n, ctypes.c_int * n
+    arrs = (P(*[w.data_ptr() for w in ws]), P(*[a.data_ptr() for a, _ in many]),
+            P(*[b.data_ptr() if co != 8 else None for (co, _), (_, b) in zip(shapes, many)]),
+            I(*[co for co, _ in shapes]), I(*[ci for _, ci in shapes]))
+    N.call("bz_nn_weights_n", *[ctypes.addressof(x) for x in arrs], n, st)
+    torch.cuda.synchronize()
+    for (co, ci), (a, b), (c, d) in zip(shapes, one, many):
+        assert torch.equal(a, c), (co, ci)
+        if co != 8:
+            assert torch.equal(b, d), (co, ci)
+        else:
+            assert bool((d == 7).all())
+
+
 def test_conv_layers_match_oracle(hip):
     torch = hip
     rng = np.random.default_rng(0)
