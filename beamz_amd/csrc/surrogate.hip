@@ -1242,7 +1242,8 @@ __global__ __launch_bounds__(256) void k_head8(const uint16_t* __restrict__ a, c
 #pragma unroll
     for (int e = 0; e < 8; ++e) acc[e] = 0.0f;
     const uint16_t* ab = a + (long)b * HW * C + 8 * cg;
-    for (int p = sl; p < HW; p += S) {
+#pragma unroll 8
+    for (int p = sl; p < HW; p += S) {     // unrolled: the loads of 8 pixels in flight, added in pixel order
         const uint4 v = *reinterpret_cast<const uint4*>(ab + (long)p * C);
         const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
 #pragma unroll
@@ -1286,11 +1287,13 @@ __global__ void k_head_wgrad(const float* __restrict__ gout, const float* __rest
     if (i < (long)nout * C) {
         const int j = (int)(i / C), c = (int)(i % C);
         float s = gfw[i];
-        for (int b = 0; b < B; ++b) s += gout[(long)b * nout + j] * gap[(long)b * C + c];
+#pragma unroll 16
+        for (int b = 0; b < B; ++b) s += gout[(long)b * nout + j] * gap[(long)b * C + c];   // loads in flight together
         gfw[i] = s;
     } else if (i < (long)nout * C + nout) {
         const int j = (int)(i - (long)nout * C);
         float s = gfb[j];
+#pragma unroll 16
         for (int b = 0; b < B; ++b) s += gout[(long)b * nout + j];
         gfb[j] = s;
     }
