@@ -910,7 +910,9 @@ class Fused2D:
     KSHAPES = ((8, 4), (8, 8), (8, 16), (4, 8), (4, 16), (16, 4))     # (rows per wave, waves per workgroup)
     # (K, rows, waves) candidates of tune_kstep; the automatic shapes were the fastest measured on 64 members
     # of 256..1024 cells a side (30 M cells: 0.051 ms/step for (8, 8, 8)) and on one 800 x 400 grid with a
-    # 16-wavelength DFT line (1.43 us/step for (12, 4, 16), against 1.51 for (8, 4, 8) and 1.80 for (8, 8, 8))
+    # 16-wavelength DFT line (1.43 us/step for (12, 4, 16), against 1.51 for (8, 4, 8) and 1.80 for (8, 8, 8));
+    # 64 x 128 regions, (8, 8, 16), fewer recomputed halo cells but one workgroup per CU: 0.057 ms/step on the
+    # 64 members against 0.043 for (8, 8, 8)
     KCANDIDATES = ((8, 8, 8), (6, 8, 8), (8, 8, 4), (8, 4, 8), (12, 4, 16))
 
     def kstep_applies(self):
