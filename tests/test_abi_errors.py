@@ -59,3 +59,21 @@ def test_surrogate_and_watchdog_checks():
     assert first >= 320 and L.bz_nn_first_work(1, 8, 8, 0) == -1
     einval("bz_nn_conv_first_bwd", FAKE, FAKE, FAKE, FAKE, FAKE, None, 1, 8, 8, 32, FAKE, first - 1, None,
            match="workspace smaller than bz_nn_first_work")
+
+
+def test_gradient_pass_injection_table_filter_and_weight_copy_checks():
+    import ctypes
+    einval("bz_adj2d_grad", None, 1, 16, 0, 10, None, match="bz_adj2d_grad: bad arguments")
+    einval("bz_adj2d_grad", FAKE, 1, 16, 10, 5, None, match="bz_adj2d_grad: bad arguments")   # n_hi below n_lo
+    einval("bz_adj2d_injtab", FAKE, 0, 16, 1, 10, None, match="bz_adj2d_injtab: bad arguments")
+    einval("bz_adj2d_injtab", FAKE, 1, -1, 1, 10, None, match="bz_adj2d_injtab: bad arguments")
+    einval("bz_filter2d", FAKE, FAKE, 1, 8, 8, FAKE, 2, None, None, None,
+           match="bz_filter2d: null or aliased pointer")                        # in and out alike
+    einval("bz_filter2d", FAKE, FAKE * 2, 1, 8, 8, FAKE, 9, None, None, None, match="radius must be 0..8")
+    einval("bz_filter2d", FAKE, FAKE * 2, 0, 8, 8, FAKE, 2, None, None, None, match="bz_filter2d: bad shape")
+    P, I = ctypes.c_void_p * 2, ctypes.c_int * 2
+    w, w16, wt16 = P(FAKE, FAKE * 2), P(FAKE * 3, FAKE * 4), P(None, None)
+    einval("bz_nn_weights_n", ctypes.addressof(w), ctypes.addressof(w16), ctypes.addressof(wt16),
+           ctypes.addressof(I(8, 8)), ctypes.addressof(I(8, 8)), 17, None, match="1..16 layers")
+    einval("bz_nn_weights_n", ctypes.addressof(w), ctypes.addressof(w16), ctypes.addressof(wt16),
+           ctypes.addressof(I(8, 0)), ctypes.addressof(I(8, 8)), 2, None, match="bz_nn_weights_n: bad layer")
