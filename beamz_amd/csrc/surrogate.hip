@@ -1161,27 +1161,36 @@ __global__ void k_avgpool2_8(const uint16_t* __restrict__ x, uint16_t* __restric
 template <bool RELU>
 __global__ void k_avgpool2_bwd8(const uint16_t* __restrict__ gy, const uint16_t* __restrict__ y,
                                 uint16_t* __restrict__ gx, Geo g, int C) {
+    // one pooled row per grid row: its gradient chunk is loaded once for the two full-resolution rows below it
     const int Ho = g.H / 2, Wo = g.W / 2, C8 = C / 8;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.W * C8) return;
     const int xx = t / C8, c = (t - xx * C8) * 8;
-    for (int row = blockIdx.y; row < g.B * g.H; row += gridDim.y) {
-        const int b = row / g.H, yy = row - b * g.H;
-        const long q = (long)row * g.W + xx;
-        const uint4 v = *reinterpret_cast<const uint4*>(gy + (((long)b * Ho + yy / 2) * Wo + xx / 2) * C + c);
+    for (int prow = blockIdx.y; prow < g.B * Ho; prow += gridDim.y) {
+        const int b = prow / Ho, yo = prow - b * Ho;
+        const uint4 v = *reinterpret_cast<const uint4*>(gy + ((long)prow * Wo + xx / 2) * C + c);
         const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
-        uint4 o;
-        uint16_t* oh = reinterpret_cast<uint16_t*>(&o);
+        const long q0 = ((long)b * g.H + 2 * yo) * g.W + xx, q1 = q0 + g.W;
+        uint4 o0, o1;
+        uint16_t* oh0 = reinterpret_cast<uint16_t*>(&o0);
+        uint16_t* oh1 = reinterpret_cast<uint16_t*>(&o1);
         if (RELU) {
-            const uint4 m = *reinterpret_cast<const uint4*>(y + q * C + c);
-            const uint16_t* mh = reinterpret_cast<const uint16_t*>(&m);
+            const uint4 m0 = *reinterpret_cast<const uint4*>(y + q0 * C + c);
+            const uint4 m1 = *reinterpret_cast<const uint4*>(y + q1 * C + c);
+            const uint16_t* mh0 = reinterpret_cast<const uint16_t*>(&m0);
+            const uint16_t* mh1 = reinterpret_cast<const uint16_t*>(&m1);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) oh[e] = b2f(mh[e]) > 0.0f ? f2b(0.25f * b2f(h[e])) : (uint16_t)0;
+            for (int e = 0; e < 8; ++e) {
+                const uint16_t q = f2b(0.25f * b2f(h[e]));
+                oh0[e] = b2f(mh0[e]) > 0.0f ? q : (uint16_t)0;
+                oh1[e] = b2f(mh1[e]) > 0.0f ? q : (uint16_t)0;
+            }
         } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) oh[e] = f2b(0.25f * b2f(h[e]));
+            for (int e = 0; e < 8; ++e) oh0[e] = oh1[e] = f2b(0.25f * b2f(h[e]));
         }
-        *reinterpret_cast<uint4*>(gx + q * C + c) = o;
+        *reinterpret_cast<uint4*>(gx + q0 * C + c) = o0;
+        *reinterpret_cast<uint4*>(gx + q1 * C + c) = o1;
     }
 }
 
@@ -1739,7 +1748,7 @@ extern "C" int bz_nn_avgpool2_bwd(const uint16_t* gy, uint16_t* gx, int B, int H
                "bz_nn_avgpool2_bwd: bad arguments");
     Geo g{B, H, W};
     if (C % 8 == 0 && ((uintptr_t)gy & 15) == 0 && ((uintptr_t)gx & 15) == 0)
-        hipLaunchKernelGGL(k_avgpool2_bwd8<false>, rows_grid((long)W * (C / 8), (long)B * H), dim3(256), 0,
+        hipLaunchKernelGGL(k_avgpool2_bwd8<false>, rows_grid((long)W * (C / 8), (long)B * (H / 2)), dim3(256), 0,
                            (hipStream_t)stream, gy, (const uint16_t*)nullptr, gx, g, C);
     else
         hipLaunchKernelGGL(k_avgpool2_bwd, grid1((long)B * H * W * C), dim3(256), 0, (hipStream_t)stream, gy, gx, g, C);
@@ -1753,7 +1762,7 @@ extern "C" int bz_nn_avgpool2_bwd_relu(const uint16_t* gy, const uint16_t* y, ui
                    ((uintptr_t)gy & 15) == 0 && ((uintptr_t)y & 15) == 0 && ((uintptr_t)gx & 15) == 0,
                "bz_nn_avgpool2_bwd_relu: bad arguments (C a multiple of 8, 16-byte aligned tensors)");
     Geo g{B, H, W};
-    hipLaunchKernelGGL(k_avgpool2_bwd8<true>, rows_grid((long)W * (C / 8), (long)B * H), dim3(256), 0,
+    hipLaunchKernelGGL(k_avgpool2_bwd8<true>, rows_grid((long)W * (C / 8), (long)B * (H / 2)), dim3(256), 0,
                        (hipStream_t)stream, gy, y, gx, g, C);
     BZ_CHECK_LAUNCH("bz_nn_avgpool2_bwd_relu");
     return BZ_OK;
