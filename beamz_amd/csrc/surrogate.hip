@@ -1155,7 +1155,7 @@ __global__ void k_avgpool2_8(const uint16_t* __restrict__ x, uint16_t* __restric
     }
 }
 
-// Backward of the pooling, full-resolution rows as in k_avgpool2_8. RELU: 0 where y (the pooled layer's own
+// Backward of the pooling, on pooled rows as in k_avgpool2_8. RELU: 0 where y (the pooled layer's own
 // output, after its ReLU) is not positive, the ReLU derivative of that layer fused; the same bits as the two
 // passes.
 template <bool RELU>
